@@ -1,0 +1,188 @@
+"""Python mirror of the libssa.h API over libssa_amd.so (the MI355X build).
+
+The functions keep the reference's names, argument meaning and error behaviour
+(src/libssa.h:122-263): misuse ends the process through the library's fatal
+path, file errors print and return None.  Searches run on the GPU only; if the
+shared library is missing, importing this package raises.
+
+    import libssa_amd as ssa
+    ssa.init_score_matrix(ssa.MATRIX_BUILDIN, ssa.BLOSUM62)
+    ssa.init_gap_penalties(-10, -1)
+    ssa.init_db("db.fasta")
+    q = ssa.init_sequence_fasta(ssa.READ_FROM_STRING, "MKVLAT...")
+    hits = ssa.sw_align(q, 10, ssa.BIT_WIDTH_8, ssa.COMPUTE_SCORE)
+    ssa.free_sequence(q)
+"""
+import ctypes as _C
+import os as _os
+from dataclasses import dataclass as _dataclass
+
+_HERE = _os.path.dirname(_os.path.abspath(__file__))
+LIB_PATH = _os.path.join(_HERE, "libssa_amd.so")
+
+# ---- constants (src/libssa.h:29-72) ----
+BLOSUM45, BLOSUM50, BLOSUM62, BLOSUM80, BLOSUM90 = "blosum45", "blosum50", "blosum62", "blosum80", "blosum90"
+PAM30, PAM70, PAM250 = "pam30", "pam70", "pam250"
+NUCLEOTIDE, AMINOACID, TRANS_QUERY, TRANS_DB, TRANS_BOTH = 0, 1, 2, 3, 4
+FORWARD_STRAND, COMPLEMENTARY_STRAND, BOTH_STRANDS = 1, 2, 3
+BIT_WIDTH_8, BIT_WIDTH_16, BIT_WIDTH_64 = 8, 16, 64
+OUTPUT_SILENT, OUTPUT_ERROR, OUTPUT_WARNING, OUTPUT_INFO = 0, 1, 2, 3
+COMPUTE_SCORE, COMPUTE_ALIGNMENT = 0, 1
+READ_FROM_FILE, READ_FROM_STRING, MATRIX_BUILDIN = 0, 1, 2
+COMPUTE_ON_SSE2, COMPUTE_ON_SSE41, COMPUTE_ON_AVX2 = 0, 1, 2
+
+
+class _DbSeq(_C.Structure):
+    _fields_ = [("seq", _C.POINTER(_C.c_uint8)), ("len", _C.c_size_t), ("ID", _C.c_size_t),
+                ("strand", _C.c_int), ("frame", _C.c_int)]
+
+
+class _QSeq(_C.Structure):
+    _fields_ = [("seq", _C.POINTER(_C.c_uint8)), ("len", _C.c_size_t), ("strand", _C.c_int), ("frame", _C.c_int)]
+
+
+class _Alignment(_C.Structure):
+    _fields_ = [("db_seq", _DbSeq), ("query", _QSeq), ("alignment", _C.c_char_p), ("alignment_len", _C.c_size_t),
+                ("score", _C.c_long), ("align_q_start", _C.c_size_t), ("align_q_end", _C.c_size_t),
+                ("align_d_start", _C.c_size_t), ("align_d_end", _C.c_size_t)]
+
+
+class _AlignmentList(_C.Structure):
+    _fields_ = [("alignments", _C.POINTER(_C.POINTER(_Alignment))), ("len", _C.c_size_t)]
+
+
+class Stats(_C.Structure):
+    """ssa_amd_stats_t (include/libssa_amd.h)."""
+    _fields_ = [("chunk_count", _C.c_size_t), ("seq_count", _C.c_size_t),
+                ("overflow_8_bit_count", _C.c_size_t), ("overflow_16_bit_count", _C.c_size_t),
+                ("cells", _C.c_uint64), ("cells_computed", _C.c_uint64), ("rows_per_strip", _C.c_int),
+                ("ms_gpu_first", _C.c_double), ("ms_gpu_requeue", _C.c_double), ("ms_replay", _C.c_double),
+                ("ms_traceback", _C.c_double), ("ms_total", _C.c_double)]
+
+
+@_dataclass
+class Alignment:
+    """One entry of the returned alignment list (alignment_t, src/libssa.h:100-110)."""
+    db_id: int
+    db_seq: bytes
+    db_strand: int
+    db_frame: int
+    query: bytes
+    query_strand: int
+    query_frame: int
+    score: int
+    alignment: str
+    align_q_start: int
+    align_q_end: int
+    align_d_start: int
+    align_d_end: int
+
+
+def _load():
+    if not _os.path.exists(LIB_PATH):
+        raise ImportError("libssa_amd.so is not built; run __graft_entry__.build() (make -C libssa_amd)")
+    lib = _C.CDLL(LIB_PATH)
+    P, S, I, cp = _C.c_void_p, _C.c_size_t, _C.c_int, _C.c_char_p
+    sig = {
+        "set_output_mode": (None, [I]), "set_simd_compute_mode": (None, [I]), "set_chunk_size": (None, [S]),
+        "set_thread_count": (None, [S]), "init_score_matrix": (None, [I, cp]),
+        "init_constant_scores": (None, [_C.c_int8, _C.c_int8]), "init_gap_penalties": (None, [_C.c_int8, _C.c_int8]),
+        "init_symbol_translation": (None, [I, I, I, I]), "init_db": (None, [cp]),
+        "init_sequence_fasta": (P, [I, cp]), "free_sequence": (None, [P]),
+        "sw_align": (_C.POINTER(_AlignmentList), [P, S, I, I]), "nw_align": (_C.POINTER(_AlignmentList), [P, S, I, I]),
+        "free_alignment": (None, [_C.POINTER(_AlignmentList)]), "ssa_exit": (None, []),
+        "ssa_db_init": (I, [cp]), "ssa_db_get_sequence_count": (S, []), "ssa_db_close": (None, []),
+        "ssa_amd_last_stats": (None, [_C.POINTER(Stats)]), "ssa_amd_set_rows_per_strip": (None, [I]),
+        "ssa_amd_get_matrix": (None, [P]), "ssa_amd_device_arch": (cp, []),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(lib, name)
+        f.restype = res
+        f.argtypes = args
+    return lib
+
+
+lib = _load()
+
+
+def _b(s):
+    return s.encode() if isinstance(s, str) else s
+
+
+# ---- technical initialisation ----
+def set_output_mode(mode): lib.set_output_mode(mode)
+def set_simd_compute_mode(mode): lib.set_simd_compute_mode(mode)
+def set_chunk_size(size): lib.set_chunk_size(size)
+def set_thread_count(count): lib.set_thread_count(count)
+
+
+# ---- initialisation ----
+def init_score_matrix(mode, matrix): lib.init_score_matrix(mode, _b(matrix))
+def init_constant_scores(p, m): lib.init_constant_scores(p, m)
+def init_gap_penalties(gap_open, gap_extend): lib.init_gap_penalties(gap_open, gap_extend)
+def init_symbol_translation(type_, strands, db_gencode, q_gencode):
+    lib.init_symbol_translation(type_, strands, db_gencode, q_gencode)
+def init_db(path): lib.init_db(_b(path))
+
+
+def init_sequence_fasta(mode, text):
+    """Returns an opaque query handle, or None if the file could not be read."""
+    h = lib.init_sequence_fasta(mode, _b(text))
+    return h if h else None
+
+
+def free_sequence(q): lib.free_sequence(q)
+def ssa_exit(): lib.ssa_exit()
+
+
+def _convert(ptr):
+    out = []
+    lst = ptr.contents
+    for i in range(lst.len):
+        a = lst.alignments[i].contents
+        out.append(Alignment(
+            db_id=a.db_seq.ID, db_seq=_C.string_at(a.db_seq.seq, a.db_seq.len), db_strand=a.db_seq.strand,
+            db_frame=a.db_seq.frame, query=_C.string_at(a.query.seq, a.query.len), query_strand=a.query.strand,
+            query_frame=a.query.frame, score=a.score,
+            alignment=a.alignment.decode() if a.alignment else None,
+            align_q_start=a.align_q_start, align_q_end=a.align_q_end,
+            align_d_start=a.align_d_start, align_d_end=a.align_d_end))
+    return out
+
+
+def sw_align(query, hitcount, bit_width, align_type):
+    """Smith-Waterman search; returns the alignment list as Alignment objects."""
+    p = lib.sw_align(query, hitcount, bit_width, align_type)
+    try:
+        return _convert(p)
+    finally:
+        lib.free_alignment(p)
+
+
+def nw_align(query, hitcount, bit_width, align_type):
+    """Needleman-Wunsch search; returns the alignment list as Alignment objects."""
+    p = lib.nw_align(query, hitcount, bit_width, align_type)
+    try:
+        return _convert(p)
+    finally:
+        lib.free_alignment(p)
+
+
+# ---- extensions (include/libssa_amd.h) ----
+def last_stats():
+    s = Stats()
+    lib.ssa_amd_last_stats(_C.byref(s))
+    return s
+
+
+def set_rows_per_strip(rows): lib.ssa_amd_set_rows_per_strip(rows)
+
+
+def get_matrix():
+    buf = (_C.c_int64 * 1024)()
+    lib.ssa_amd_get_matrix(buf)
+    return list(buf)
+
+
+def device_arch():
+    return lib.ssa_amd_device_arch().decode()

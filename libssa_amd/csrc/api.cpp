@@ -1,0 +1,154 @@
+// api.cpp -- the libssa.h entry points (reference: src/libssa.c:41-271).
+#include <cstdlib>
+#include <cstring>
+
+#include "common.h"
+#include "gpu.h"
+#include "search.h"
+
+extern "C" {
+#include "libssa_amd.h"
+}
+
+namespace ssa {
+p_query query_from_string(const char* s);
+p_query query_from_file(const char* path);
+}  // namespace ssa
+
+using namespace ssa;
+
+extern "C" {
+
+// src/libssa.c:41-43
+void set_output_mode(int mode) { config().output_mode = mode; }
+
+// src/libssa.c:45-47 (set_max_compute_capability, src/cpu_config.c:29-33)
+void set_simd_compute_mode(int mode) { config().simd_mode = mode; }
+
+// src/libssa.c:49-56
+void set_chunk_size(size_t size) {
+    if (size == 0) {
+        print_error("Only non zero chunk sizes are allowed. Using the default size of 1000 sequences.");
+        size = kDefaultChunkSize;
+    }
+    config().chunk_size = size;
+}
+
+// src/libssa.c:58-60
+void set_thread_count(size_t count) { config().thread_count = count; }
+
+// src/libssa.c:81-94
+void init_score_matrix(int mode, const char* m) {
+    if (mode == READ_FROM_FILE) matrix_init_file(m);
+    else if (mode == READ_FROM_STRING) matrix_init_text(m);
+    else if (mode == MATRIX_BUILDIN) matrix_init_builtin(m);
+    else fatal("Unknown mode for reading score matrices: %d", mode);
+}
+
+// src/libssa.c:102-105
+void init_gap_penalties(const int8_t gap_open, const int8_t gap_extend) {
+    config().gap_open = gap_open;
+    config().gap_extend = gap_extend;
+}
+
+// src/libssa.c:115-117: the first argument is used as the match score
+void init_constant_scores(const int8_t p, const int8_t m) { matrix_init_constant(p, m); }
+
+// src/libssa.c:126-132
+void init_db(const char* db_file) {
+    ssa_db_close();
+    ssa_db_init(db_file);
+    db_generation_bump();
+    print_info("DB read %lu sequences\n", (unsigned long)ssa_db_get_sequence_count());
+}
+
+// src/libssa.c:139-157.  The symbol-type and strand checks test the setting in
+// force before the call, as the reference's do.
+void init_symbol_translation(int type, int strands, int d_gencode, int q_gencode) {
+    Config& c = config();
+    if (!gencode_valid(q_gencode)) fatal("Illegal query genetic code specified.");
+    if (!gencode_valid(d_gencode)) fatal("Illegal database genetic code specified.");
+    if (c.symtype < 0 || c.symtype > 4) fatal("Illegal symbol type specified.");
+    if (c.symtype < 0 || c.symtype > 2) fatal("Illegal strands specified.");
+    c.symtype = type;
+    c.strands = strands;
+    c.q_gencode = q_gencode;
+    c.d_gencode = d_gencode;
+}
+
+// src/libssa.c:165-182
+p_query init_sequence_fasta(int mode, const char* fasta) {
+    if (mode == READ_FROM_FILE) return query_from_file(fasta);
+    if (mode == READ_FROM_STRING) return query_from_string(fasta);
+    fatal("Unknown mode for reading query sequences: %d", mode);
+}
+
+// src/libssa.c:192-194
+void free_sequence(p_query p) { delete p; }
+
+// src/libssa.c:196-219
+static void test_configuration(p_query query) {
+    const Config& c = config();
+    if (!c.gap_open && !c.gap_extend) print_warning("Gap opening and gap extension cost set to zero. Possible error.");
+    if (!matrix().valid) fatal("Scoring not initialized.");
+    if (!query) fatal("Query not initialized.");
+    if (ssa_db_get_sequence_count() == 0) print_warning("Database contains zero sequences. Possible error.");
+    if (!matrix().constant && c.symtype == NUCLEOTIDE)
+        fatal("Nucleotide sequences can only be aligned using constant scores.");
+}
+
+// src/libssa.c:229-235
+p_alignment_list sw_align(p_query p, size_t hitcount, int bit_width, int align_type) {
+    test_configuration(p);
+    return run_search(kSW, p, hitcount, bit_width, align_type);
+}
+
+// src/libssa.c:245-251
+p_alignment_list nw_align(p_query p, size_t hitcount, int bit_width, int align_type) {
+    test_configuration(p);
+    return run_search(kNW, p, hitcount, bit_width, align_type);
+}
+
+// src/libssa.c:261-263, src/algo/aligner.c:101-143
+void free_alignment(p_alignment_list alist) {
+    if (!alist) return;
+    if (alist->alignments) {
+        for (size_t i = 0; i < alist->len; ++i) {
+            p_alignment a = alist->alignments[i];
+            if (!a) continue;
+            free(a->db_seq.seq);
+            free(a->alignment);
+            free(a);
+        }
+        free(alist->alignments);
+    }
+    free(alist);
+}
+
+// src/libssa.c:266-271
+void ssa_exit(void) {
+    matrix_free();
+    ssa_db_close();
+    release_device_db();
+}
+
+// ---- libssa_amd.h ----
+void ssa_amd_last_stats(ssa_amd_stats_t* out) {
+    if (out) *out = last_stats();
+}
+
+void ssa_amd_set_rows_per_strip(int rows) {
+    if (rows != 0 && rows != 8 && rows != 16 && rows != 24 && rows != 32 && rows != 48)
+        fatal("libssa_amd: rows per strip must be 0, 8, 16, 24, 32 or 48 (got %d)", rows);
+    rows_per_strip_override() = rows;
+}
+
+void ssa_amd_get_matrix(int64_t* out) { memcpy(out, matrix().s, sizeof(matrix().s)); }
+
+const char* ssa_amd_device_arch(void) {
+    static std::string arch;
+    arch = Gpu::get().arch();
+    return arch.c_str();
+}
+
+}  // extern "C"

@@ -1,0 +1,79 @@
+// common.h -- shared definitions of the libssa_amd host runtime.
+#pragma once
+
+#include <cstddef>
+#include <cstdint>
+#include <string>
+#include <vector>
+
+extern "C" {
+#include "libssa.h"
+#include "libssa_extern_db.h"
+}
+
+namespace ssa {
+
+// search types (the reference's SMITH_WATERMAN / NEEDLEMAN_WUNSCH, src/util/util.h:36-37)
+enum SearchType { kSW = 0, kNW = 1 };
+
+// DB sequences per chunk by default (src/util/util.h:50)
+constexpr size_t kDefaultChunkSize = 1000;
+// columns per lane block of the reference's kernels (src/algo/8/search_8_util.h:25)
+constexpr int kCDepth = 4;
+constexpr int kDim = 32;
+
+// ---- messages and fatal errors (src/util/util.c:36-89) ----
+[[noreturn]] void fatal(const char* fmt, ...) __attribute__((format(printf, 1, 2)));
+void print_info(const char* fmt, ...) __attribute__((format(printf, 1, 2)));
+void print_warning(const char* fmt, ...) __attribute__((format(printf, 1, 2)));
+void print_error(const char* fmt, ...) __attribute__((format(printf, 1, 2)));
+
+// ---- process-wide configuration set through libssa.h ----
+struct Config {
+    int output_mode = OUTPUT_WARNING;
+    int simd_mode = COMPUTE_ON_AVX2;  // selects the reference lane counts replayed
+    size_t chunk_size = kDefaultChunkSize;
+    size_t thread_count = (size_t)-1;
+    int8_t gap_open = 0, gap_extend = 0;
+    int symtype = AMINOACID;
+    int strands = FORWARD_STRAND;
+    int q_gencode = 1, d_gencode = 1;
+};
+Config& config();
+
+// lanes of the reference's 8-bit and 16-bit kernels for the configured ISA
+// (src/algo/8/search_8_util.h:26-35, src/algo/16/search_16_util.h:27-37)
+int lanes8();
+int lanes16();
+
+// ---- score matrix (src/matrices.c) ----
+struct Matrix {
+    int64_t s[kDim * kDim];  // [db code * 32 + query code]
+    bool constant = false;
+    bool valid = false;
+    int64_t at(int d, int q) const { return s[d * kDim + q]; }
+};
+Matrix& matrix();
+void matrix_init_constant(int match, int mismatch);
+void matrix_init_text(const char* text);
+void matrix_init_file(const char* path);
+void matrix_init_builtin(const char* name);
+void matrix_free();
+
+// ---- symbols (src/util/util_sequence.c) ----
+int map_aa(int c);
+int map_nt(int c);
+int complement_nt(int code);
+void revcomp(const uint8_t* in, size_t len, uint8_t* out);
+bool gencode_valid(int code);
+
+}  // namespace ssa
+
+// the opaque query handle of libssa.h (fields as src/libssa_datatypes.h:95-107)
+struct _query {
+    std::vector<uint8_t> nt[2];  // mapped nucleotide strands
+    std::vector<uint8_t> aa[6];  // mapped amino-acid frames
+    bool has_nt[2] = {false, false};
+    bool has_aa[6] = {false, false, false, false, false, false};
+    std::string header;
+};

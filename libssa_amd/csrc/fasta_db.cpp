@@ -1,0 +1,99 @@
+// fasta_db.cpp -- default implementation of the DB plugin interface
+// (include/libssa_extern_db.h; reference interface src/libssa_extern_db.h:28-56).
+//
+// The reference links an external library (libsdb) for this; this provider
+// reads a FASTA file: records in file order get IDs 0, 1, ...; a record's
+// sequence is its lines joined without line terminators; NULL past the end.
+// Programs that define these four functions themselves replace this provider.
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "common.h"
+#include "search.h"
+
+namespace {
+
+struct FastaDB {
+    std::vector<char> residues;   // all sequences back to back
+    std::vector<size_t> offset;   // record i occupies [offset[i], offset[i+1])
+    std::vector<seqinfo_t> infos;
+    bool open = false;
+};
+
+FastaDB& db() {
+    static FastaDB d;
+    return d;
+}
+
+std::mutex g_mu;
+
+bool load(const char* path, FastaDB& d) {
+    FILE* fp = fopen(path, "rb");
+    if (!fp) return false;
+    std::vector<char> buf(1 << 20);
+    bool in_header = false, at_line_start = true, have_record = false;
+    size_t n;
+    while ((n = fread(buf.data(), 1, buf.size(), fp)) > 0) {
+        for (size_t k = 0; k < n; ++k) {
+            const char c = buf[k];
+            if (at_line_start && c == '>') {
+                d.offset.push_back(d.residues.size());
+                have_record = true;
+                in_header = true;
+            } else if (c == '\n') {
+                in_header = false;
+            } else if (!in_header && have_record && c != '\r') {
+                d.residues.push_back(c);
+            }
+            at_line_start = (c == '\n');
+        }
+    }
+    fclose(fp);
+    d.offset.push_back(d.residues.size());
+    const size_t count = d.offset.size() - 1;
+    d.infos.resize(count);
+    for (size_t i = 0; i < count; ++i) {
+        d.infos[i].ID = i;
+        d.infos[i].seqlen = d.offset[i + 1] - d.offset[i];
+        d.infos[i].seq = d.residues.data() + d.offset[i];
+    }
+    return true;
+}
+
+}  // namespace
+
+extern "C" {
+
+int ssa_db_init(const char* db_name) {
+    std::lock_guard<std::mutex> lk(g_mu);
+    ssa::db_generation_bump();
+    FastaDB& d = db();
+    d = FastaDB();
+    if (!db_name || !load(db_name, d)) {
+        d = FastaDB();
+        d.open = true;
+        ssa::print_error("Cannot open database file: %s", db_name ? db_name : "(null)");
+        return -1;
+    }
+    d.open = true;
+    return 0;
+}
+
+size_t ssa_db_get_sequence_count(void) { return db().infos.size(); }
+
+p_seqinfo ssa_db_get_sequence(size_t id) {
+    FastaDB& d = db();
+    if (id >= d.infos.size()) return nullptr;
+    return &d.infos[id];
+}
+
+void ssa_db_close(void) {
+    std::lock_guard<std::mutex> lk(g_mu);
+    ssa::db_generation_bump();
+    db() = FastaDB();
+}
+
+}  // extern "C"

@@ -1,0 +1,78 @@
+// gpu.h -- device context of the host runtime: one HIP stream, growable
+// device arenas, pinned staging, and the kernel launchers.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+#include "kernels.h"
+
+namespace ssa {
+
+void hip_check(hipError_t e, const char* what);
+#define SSA_HIP(x) ::ssa::hip_check((x), #x)
+
+// A device allocation that only grows; contents are not preserved on growth.
+struct DeviceArena {
+    void* ptr = nullptr;
+    size_t bytes = 0;
+    void* get(size_t need);
+    void release();
+};
+
+class Gpu {
+public:
+    static Gpu& get();  // initialises the device on first use
+    hipStream_t stream() const { return stream_; }
+    void* arena(const std::string& name, size_t bytes) { return arenas_[name].get(bytes); }
+    void upload(void* dst, const void* src, size_t bytes);
+    void download(void* dst, const void* src, size_t bytes);
+    void sync();
+    void release_all();
+    int device() const { return device_; }
+    std::string arch() const { return arch_; }
+    int cu_count() const { return cus_; }
+
+private:
+    Gpu();
+    hipStream_t stream_ = nullptr;
+    int device_ = 0;
+    int cus_ = 0;
+    std::string arch_;
+    std::unordered_map<std::string, DeviceArena> arenas_;
+};
+
+// event pair timing a span of work on the context stream
+class GpuTimer {
+public:
+    GpuTimer();
+    ~GpuTimer();
+    void start();
+    void stop();
+    float ms();  // synchronises on the stop event
+
+private:
+    hipEvent_t a_ = nullptr, b_ = nullptr;
+};
+
+// ---- kernel launchers (score_kernels.hip, traceback_kernels.hip, pack_kernels.hip) ----
+hipError_t launch_score(int rows, int mode, int ngroups, hipStream_t st, const ScoreGroup* groups,
+                        const QueryDesc* queries, const int16_t* qp, const uint32_t* lb, const uint2* res, uint4* bnd,
+                        const int32_t* slot_begin, const int32_t* pair_list, PairResult* results,
+                        const ScoreParams& P);
+hipError_t launch_exact(int npairs, hipStream_t st, const ExactPair* pairs, const QueryDesc* queries,
+                        const int32_t* qp32, const uint8_t* codes, int mode, int go, int ge, int32_t* scratch,
+                        int64_t scratch_stride, int64_t* out);
+hipError_t launch_traceback(int ntasks, hipStream_t st, const TraceTask* tasks, const uint8_t* codes,
+                            const int64_t* mat, int go, int ge, uint8_t* dirs, int64_t* scratch, int32_t* ops,
+                            TraceResult* out);
+
+hipError_t launch_pack(int ngroups, hipStream_t st, const ScoreGroup* groups, const PackSlot* slots,
+                       const uint8_t* codes, uint2* res);
+
+}  // namespace ssa

@@ -1,0 +1,105 @@
+// kernels.h -- data layouts shared by the host runtime and the HIP kernels.
+#pragma once
+
+#include <cstdint>
+
+namespace ssa {
+
+// ---------------------------------------------------------------------------
+// Score kernels (score_kernels.hip)
+//
+// One wave processes one group: 64 lanes x 2 halves = 128 slots.  A slot is a
+// stream of (query, DB entry) pairs laid end to end, each padded to whole
+// 4-column blocks.  All slots of a group share the query.
+//
+// Interleaved residue layout of a group: uint2 res[T][64], .x = 4 codes of the
+// lo-half slot for that block, .y = the hi-half slot.  Byte k of a word holds
+// column k's code times two (an LDS byte offset into a profile row) in bits
+// 0-5.  Flag bits: byte 0 bit 7 = a pair starts at this block, byte 1 bit 7 =
+// a pair ends in this block, byte 2 bits 6-7 = column of the pair's last
+// residue in that block.
+// ---------------------------------------------------------------------------
+constexpr uint32_t kResStart = 0x80u;
+constexpr uint32_t kResLast = 0x8000u;
+constexpr int kResZShift = 22;  // (word >> 22) & 3
+
+constexpr int kSlotsPerGroup = 128;
+constexpr int kNoFlag = 0x7fffffff;
+
+struct ScoreGroup {
+    int32_t query;      // index into the QueryDesc table
+    int32_t nblocks;    // T
+    int64_t res_off;    // in uint2 units
+    int64_t bnd_off;    // in uint4 units; 2 x uint4 per lane per block
+    int32_t slot_base;  // index of this group's first slot in slot_begin
+    int32_t pad;
+};
+
+struct QueryDesc {
+    int32_t qlen;
+    int32_t qp_off;  // in int16 units: profile rows [qlen][32]
+    int32_t lb_off;  // in uint32 units: NW left boundary [qlen][2] packed (H, E)
+    int32_t pad;
+};
+
+// Per-launch constants in the kernel's value representation: every 16-bit
+// value is packed twice into a uint32.  The 16-bit tier holds the reference's
+// int16 lane values unchanged; the 8-bit tier holds the reference's int8 lane
+// values shifted left by 8, so int16 saturation at -32768 is int8 saturation
+// at -128 and the first value >= 127<<8 marks the int8 top saturation.
+struct ScoreParams {
+    uint32_t qr;          // gap open + extend
+    uint32_t r;           // gap extend
+    uint32_t zero;        // SW zero (biased), start value of H/E/F
+    int32_t thr_a;        // SW: first flag threshold (S >= thr_a); NW: score_min (hmin < thr_a)
+    int32_t thr_b;        // SW: second flag threshold; NW: hmax >= thr_b
+    uint32_t hb_init[4];  // NW: top boundary H of the first block's four columns
+    uint32_t fb_init[4];  // NW: top boundary F of the first block's four columns
+};
+
+// one slot of a group for the interleaving kernel (one pair per slot)
+struct PackSlot {
+    int64_t code_off;  // byte offset of the entry's codes (padded to 4) in the DB code buffer
+    int32_t nblocks;   // blocks of the entry, 0 = empty slot
+    int32_t last_col;  // (length - 1) % 4
+};
+
+// result per pair: x = score in representation (SW: running max, NW: last
+// cell), y = first flagged block (kNoFlag if none), z = second SW flag block
+struct PairResult {
+    int32_t score;
+    int32_t flag_a;
+    int32_t flag_b;
+    int32_t pad;
+};
+
+// exact 32-bit tier (score_kernels.hip): one thread per pair
+struct ExactPair {
+    int64_t res_off;  // byte offset of the entry's codes (plain, unpadded layout)
+    int32_t dlen;
+    int32_t query;    // index into QueryDesc (uses the exact profile)
+    int32_t out;      // index into the int64 score output
+    int32_t pad;
+};
+
+// ---------------------------------------------------------------------------
+// Traceback (traceback_kernels.hip): one task per hit.
+// ---------------------------------------------------------------------------
+struct TraceTask {
+    int64_t q_off;     // byte offset of the query codes
+    int64_t d_off;     // byte offset of the DB codes
+    int32_t qlen, dlen;
+    int64_t dir_off;   // byte offset of this task's direction matrix scratch
+    int64_t hh_off;    // int64 offset of this task's HH/EE scratch
+    int64_t ops_off;   // offset into the op output (pairs of int32: op char, count)
+    int32_t ops_cap;
+    int32_t type;      // 0 SW, 1 NW
+};
+
+struct TraceResult {
+    int64_t a_begin, a_end, b_begin, b_end;  // region (query = a, DB = b)
+    int32_t nops;    // runs written (back to front)
+    int32_t status;  // 0 ok, 1 region not found, 2 op buffer too small
+};
+
+}  // namespace ssa

@@ -1,0 +1,940 @@
+// search.cpp -- one database search on the GPU, with the reference's results.
+//
+// The reference (src/algo/manager.c:120-185 with one thread) walks the DB in
+// chunks; per chunk it runs the 8-bit (or 16-bit) lane kernel for each query,
+// re-runs overflowed entries at 16 bit for every query, then at 64 bit, and
+// inserts each accepted score into a bounded min-heap whose tie handling
+// depends on the insertion order.  Here every (query, entry) pair of a tier is
+// scored at once on the GPU; the kernels also report the block at which each
+// pair would have been flagged as overflowed.  From those results the host
+// replays the reference's lane schedule (which lane retires which entry in
+// which iteration) and feeds the heap in exactly the reference's order.
+#include "search.h"
+
+#include <algorithm>
+#include <chrono>
+#include <climits>
+#include <cstring>
+#include <numeric>
+#include <thread>
+
+#include "gpu.h"
+#include "kernels.h"
+
+namespace ssa {
+
+// ============================================================================
+// device-resident DB
+// ============================================================================
+namespace {
+
+uint64_t g_generation = 1;
+
+struct DeviceDB {
+    uint64_t generation = 0;
+    int symtype = -1, strands = -1;
+    size_t count = 0;
+    std::vector<DbEntry> entries;
+    std::vector<uint32_t> first_entry;  // per DB ID (count + 1 values)
+    std::vector<uint8_t> codes;         // all entries' codes, each padded to a multiple of 4
+    uint8_t* d_codes = nullptr;
+    size_t d_bytes = 0;
+    std::vector<uint32_t> by_length;    // all entries, longest first (stable)
+};
+
+DeviceDB g_db;
+
+}  // namespace
+
+void db_generation_bump() { ++g_generation; }
+
+void release_device_db() {
+    if (g_db.d_codes) (void)hipFree(g_db.d_codes);
+    g_db = DeviceDB();
+}
+
+static void map_db_codes(const char* s, size_t n, bool nucleotide, uint8_t* out) {
+    size_t unknown = 0;
+    for (size_t i = 0; i < n; ++i) {
+        const int m = nucleotide ? map_nt((unsigned char)s[i]) : map_aa((unsigned char)s[i]);
+        if (m < 0) {
+            out[i] = 0;
+            ++unknown;
+        } else {
+            out[i] = (uint8_t)m;
+        }
+    }
+    if (unknown) print_warning("%ld unknown symbols found and set to zero", (long)unknown);
+}
+
+// Reads the plugin DB once, maps it (src/db_adapter.c:47-110) and keeps the
+// codes resident on the GPU until the DB or the symbol settings change.
+static DeviceDB& device_db() {
+    const Config& cf = config();
+    size_t count = ssa_db_get_sequence_count();
+    if (g_db.generation == g_generation && g_db.symtype == cf.symtype && g_db.strands == cf.strands &&
+        g_db.count == count && g_db.d_codes)
+        return g_db;
+    if (cf.symtype != NUCLEOTIDE && cf.symtype != AMINOACID)
+        fatal("libssa_amd: translated symbol types (TRANS_QUERY, TRANS_DB, TRANS_BOTH) are not supported");
+    release_device_db();
+    DeviceDB& d = g_db;
+    d.generation = g_generation;
+    d.symtype = cf.symtype;
+    d.strands = cf.strands;
+    d.count = count;
+    const bool nuc = cf.symtype == NUCLEOTIDE;
+    const bool both = nuc && (cf.strands & 2);
+    d.first_entry.assign(count + 1, 0);
+    size_t total = 0;
+    for (size_t id = 0; id < count; ++id) {
+        p_seqinfo s = ssa_db_get_sequence(id);
+        if (!s) {
+            d.count = count = id;
+            break;
+        }
+        const size_t padded = (s->seqlen + 3) & ~(size_t)3;
+        total += padded * (both ? 2 : 1);
+    }
+    d.first_entry.resize(count + 1);
+    d.codes.assign(total + 4, 0);
+    size_t off = 0;
+    for (size_t id = 0; id < count; ++id) {
+        d.first_entry[id] = (uint32_t)d.entries.size();
+        p_seqinfo s = ssa_db_get_sequence(id);
+        if (s->seqlen == 0) continue;  // src/db_adapter.c:231-233
+        const size_t len = s->seqlen, padded = (len + 3) & ~(size_t)3;
+        DbEntry e;
+        e.id = s->ID;
+        e.len = (uint32_t)len;
+        e.strand = 0;
+        e.frame = 0;
+        e.code_off = (int64_t)off;
+        map_db_codes(s->seq, len, nuc, d.codes.data() + off);
+        d.entries.push_back(e);
+        off += padded;
+        if (both) {
+            DbEntry r = e;
+            r.strand = 1;
+            r.code_off = (int64_t)off;
+            revcomp(d.codes.data() + e.code_off, len, d.codes.data() + off);
+            d.entries.push_back(r);
+            off += padded;
+        }
+    }
+    d.first_entry[count] = (uint32_t)d.entries.size();
+    d.d_bytes = d.codes.size();
+    SSA_HIP(hipMalloc(&d.d_codes, d.d_bytes));
+    SSA_HIP(hipMemcpy(d.d_codes, d.codes.data(), d.d_bytes, hipMemcpyHostToDevice));
+    d.by_length.resize(d.entries.size());
+    std::iota(d.by_length.begin(), d.by_length.end(), 0u);
+    std::stable_sort(d.by_length.begin(), d.by_length.end(),
+                     [&](uint32_t a, uint32_t b) { return d.entries[a].len > d.entries[b].len; });
+    return d;
+}
+
+// ============================================================================
+// tier constants
+// ============================================================================
+namespace {
+
+struct Tier {
+    int bits;   // 8 or 16
+    int shift;  // 8 for the 8-bit tier (values held shifted in int16), 0 for 16
+    int64_t imin() const { return bits == 8 ? -128 : -32768; }
+    int64_t imax() const { return bits == 8 ? 127 : 32767; }
+    int64_t wrap(int64_t v) const { return bits == 8 ? (int64_t)(int8_t)v : (int64_t)(int16_t)v; }
+    int64_t sat(int64_t v) const { return v < imin() ? imin() : (v > imax() ? imax() : v); }
+    int32_t repr(int64_t v) const { return (int32_t)(v * (1 << shift)); }
+    int64_t decode(int32_t r) const { return (int64_t)(r >> shift); }
+};
+
+const Tier kTier8{8, 8}, kTier16{16, 0};
+
+uint32_t pack2(int32_t v) { return ((uint32_t)(uint16_t)v << 16) | (uint32_t)(uint16_t)v; }
+
+int64_t now_us() {
+    return std::chrono::duration_cast<std::chrono::microseconds>(
+               std::chrono::steady_clock::now().time_since_epoch())
+        .count();
+}
+
+struct MatrixInfo {
+    bool fits8 = true, fits16 = true, fits32 = true;
+    int64_t maxv = INT64_MIN, minv = INT64_MAX;
+};
+
+MatrixInfo matrix_info() {
+    MatrixInfo mi;
+    const Matrix& m = matrix();
+    for (int i = 0; i < kDim * kDim; ++i) {
+        const int64_t v = m.s[i];
+        mi.maxv = std::max(mi.maxv, v);
+        mi.minv = std::min(mi.minv, v);
+        if (v < -128 || v > 127) mi.fits8 = false;
+        if (v < -32768 || v > 32767) mi.fits16 = false;
+        if (v < INT32_MIN / 4 || v > INT32_MAX / 4) mi.fits32 = false;
+    }
+    return mi;
+}
+
+// result of one pair at one tier, decoded
+struct TierOut {
+    int32_t dur;     // blocks the pair occupies a reference lane
+    bool accept;     // heap insert (true) or overflow (false)
+    int64_t score;   // valid when accept
+};
+
+}  // namespace
+
+// ============================================================================
+// GPU score tier
+// ============================================================================
+namespace {
+
+int choose_rows(size_t qlen_max) {
+    const int ov = rows_per_strip_override();
+    if (ov) return ov;
+    // smallest padding waste over the instantiated heights, larger heights on ties
+    static const int kRows[] = {48, 32, 24, 16, 8};
+    int best = 32;
+    size_t best_waste = SIZE_MAX;
+    for (int r : kRows) {
+        const size_t strips = (qlen_max + r - 1) / r;
+        const size_t waste = strips * r - qlen_max;
+        if (waste < best_waste) {
+            best = r;
+            best_waste = waste;
+        }
+    }
+    return best;
+}
+
+struct PairSet {
+    // pairs (q, e) for all queries q and the entries listed (DB entry indices)
+    std::vector<uint32_t> entries;
+    size_t nq = 0;
+    size_t index(size_t q, size_t k) const { return q * entries.size() + k; }
+    size_t size() const { return nq * entries.size(); }
+};
+
+// Scores all pairs of the set at one tier.  out receives the raw per-pair
+// results in PairSet::index order.
+void gpu_score_tier(SearchType type, const Tier& T, const std::vector<QueryBuf>& qs, const PairSet& ps,
+                    const std::vector<uint32_t>* sorted_all, std::vector<PairResult>& out, double* ms,
+                    uint64_t* cells_computed, int* rows_used) {
+    Gpu& gpu = Gpu::get();
+    DeviceDB& db = g_db;
+    const Config& cf = config();
+    const Matrix& M = matrix();
+    const int go = cf.gap_open, ge = cf.gap_extend;
+    const size_t n = ps.entries.size(), nq = qs.size();
+    out.assign(ps.size(), PairResult{0, kNoFlag, kNoFlag, 0});
+    if (n == 0 || nq == 0) return;
+
+    size_t qmax = 0;
+    for (const QueryBuf& q : qs) qmax = std::max(qmax, q.len);
+    const int R = choose_rows(qmax);
+    *rows_used = R;
+
+    // ---- profiles, left boundaries, per-query descriptors ----
+    std::vector<QueryDesc> qd(nq);
+    std::vector<int16_t> prof;
+    std::vector<uint32_t> lbv;
+    const int64_t QRw = T.wrap(go + ge), Rw = T.wrap(ge);
+    for (size_t qi = 0; qi < nq; ++qi) {
+        const QueryBuf& q = qs[qi];
+        const size_t rows = ((q.len + R - 1) / R) * R;
+        qd[qi].qlen = (int32_t)q.len;
+        qd[qi].qp_off = (int32_t)prof.size();
+        qd[qi].lb_off = (int32_t)lbv.size();
+        for (size_t i = 0; i < rows; ++i) {
+            for (int d = 0; d < kDim; ++d) {
+                int32_t v;
+                if (i < q.len) v = T.repr(T.wrap(M.at(d, q.seq[i])));
+                else v = type == kSW ? -32768 : 0;
+                prof.push_back((int16_t)v);
+            }
+        }
+        int64_t c = QRw;
+        for (size_t i = 0; i < rows; ++i) {
+            if (i < q.len) {
+                lbv.push_back(pack2(T.repr(c)));
+                lbv.push_back(pack2(T.repr(T.sat(c + QRw))));
+                c = T.sat(c + Rw);
+            } else {
+                lbv.push_back(0u);
+                lbv.push_back(0u);
+            }
+        }
+    }
+    ScoreParams P;
+    P.qr = pack2(T.repr(QRw));
+    P.r = pack2(T.repr(Rw));
+    P.zero = pack2(-32768);
+    if (type == kSW) {
+        if (T.bits == 16) {
+            P.thr_a = 255 - 32768;  // the 8-bit tier's overflow (true maximum >= 255)
+            P.thr_b = 32767;        // the 16-bit tier's overflow
+        } else {
+            P.thr_a = T.repr(127);
+            P.thr_b = 1 << 20;
+        }
+    } else {
+        P.thr_a = T.repr(T.wrap(T.imin() - go - ge - 1));
+        P.thr_b = T.repr(T.imax());
+    }
+    P.hb_init[0] = 0;
+    for (int k = 1; k < 4; ++k) P.hb_init[k] = pack2(T.repr(T.wrap(go + k * ge)));
+    for (int k = 0; k < 4; ++k) P.fb_init[k] = pack2(T.repr(T.wrap(go + (k + 1) * ge)));
+
+    // ---- groups of 128 slots per query, entries longest first ----
+    std::vector<uint32_t> order;  // positions k into ps.entries, longest first
+    if (sorted_all) {
+        // the set is the whole DB in entry order: reuse the DB's length order
+        order = *sorted_all;
+    } else {
+        order.resize(n);
+        std::iota(order.begin(), order.end(), 0u);
+        std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) {
+            return db.entries[ps.entries[a]].len > db.entries[ps.entries[b]].len;
+        });
+    }
+    const size_t groups_per_q = (n + kSlotsPerGroup - 1) / kSlotsPerGroup;
+    const size_t ngroups = groups_per_q * nq;
+    std::vector<ScoreGroup> groups(ngroups);
+    std::vector<PackSlot> slots(groups_per_q * kSlotsPerGroup);
+    std::vector<int32_t> slot_begin(ngroups * kSlotsPerGroup);
+    std::vector<int32_t> pair_list(ngroups * kSlotsPerGroup, -1);
+    int64_t res_total = 0, bnd_total = 0;
+    std::vector<int64_t> res_off(groups_per_q);
+    std::vector<int32_t> gblocks(groups_per_q);
+    uint64_t computed = 0;
+    for (size_t g = 0; g < groups_per_q; ++g) {
+        int32_t T_g = 0;
+        for (int s = 0; s < kSlotsPerGroup; ++s) {
+            const size_t pos = g * kSlotsPerGroup + s;
+            PackSlot& sl = slots[pos];
+            if (pos < n) {
+                const DbEntry& e = db.entries[ps.entries[order[pos]]];
+                sl.code_off = e.code_off;
+                sl.nblocks = (int32_t)((e.len + 3) / 4);
+                sl.last_col = (int32_t)((e.len - 1) % 4);
+                T_g = std::max(T_g, sl.nblocks);
+            } else {
+                sl.code_off = 0;
+                sl.nblocks = 0;
+                sl.last_col = 0;
+            }
+        }
+        gblocks[g] = T_g;
+        res_off[g] = res_total;
+        res_total += (int64_t)T_g * 64;
+    }
+    for (size_t qi = 0; qi < nq; ++qi) {
+        const int nstrips = (qs[qi].len + R - 1) / R;
+        for (size_t g = 0; g < groups_per_q; ++g) {
+            const size_t gi = qi * groups_per_q + g;
+            ScoreGroup& G = groups[gi];
+            G.query = (int32_t)qi;
+            G.nblocks = gblocks[g];
+            G.res_off = res_off[g];  // the interleaved residues are shared by all queries
+            G.bnd_off = bnd_total;
+            G.slot_base = (int32_t)(gi * kSlotsPerGroup);
+            G.pad = 0;
+            if (nstrips > 1) bnd_total += (int64_t)gblocks[g] * 64 * 2;
+            computed += (uint64_t)gblocks[g] * 4 * kSlotsPerGroup * (uint64_t)nstrips * R;
+            for (int s = 0; s < kSlotsPerGroup; ++s) {
+                const size_t pos = g * kSlotsPerGroup + s;
+                slot_begin[gi * kSlotsPerGroup + s] = (int32_t)(gi * kSlotsPerGroup + s);
+                if (pos < n) pair_list[gi * kSlotsPerGroup + s] = (int32_t)ps.index(qi, order[pos]);
+            }
+        }
+    }
+    *cells_computed += computed;
+
+    // ---- device buffers ----
+    auto* d_groups = (ScoreGroup*)gpu.arena("groups", groups.size() * sizeof(ScoreGroup));
+    auto* d_qd = (QueryDesc*)gpu.arena("qdesc", qd.size() * sizeof(QueryDesc));
+    auto* d_prof = (int16_t*)gpu.arena("prof", prof.size() * sizeof(int16_t));
+    auto* d_lb = (uint32_t*)gpu.arena("lb", lbv.size() * sizeof(uint32_t));
+    auto* d_slots = (PackSlot*)gpu.arena("slots", slots.size() * sizeof(PackSlot));
+    auto* d_sb = (int32_t*)gpu.arena("slot_begin", slot_begin.size() * sizeof(int32_t));
+    auto* d_pl = (int32_t*)gpu.arena("pair_list", pair_list.size() * sizeof(int32_t));
+    auto* d_res = (uint2*)gpu.arena("res", (size_t)res_total * sizeof(uint2));
+    auto* d_bnd = (uint4*)gpu.arena("bnd", (size_t)std::max<int64_t>(bnd_total, 1) * sizeof(uint4));
+    auto* d_out = (PairResult*)gpu.arena("results", out.size() * sizeof(PairResult));
+    gpu.upload(d_groups, groups.data(), groups.size() * sizeof(ScoreGroup));
+    gpu.upload(d_qd, qd.data(), qd.size() * sizeof(QueryDesc));
+    gpu.upload(d_prof, prof.data(), prof.size() * sizeof(int16_t));
+    gpu.upload(d_lb, lbv.data(), lbv.size() * sizeof(uint32_t));
+    gpu.upload(d_slots, slots.data(), slots.size() * sizeof(PackSlot));
+    gpu.upload(d_sb, slot_begin.data(), slot_begin.size() * sizeof(int32_t));
+    gpu.upload(d_pl, pair_list.data(), pair_list.size() * sizeof(int32_t));
+    bool empty_query = false;
+    for (const QueryBuf& q : qs) empty_query |= q.len == 0;
+    if (empty_query) {  // an empty query scores 0 everywhere (no strip writes its pairs)
+        for (PairResult& r : out) r.score = -32768;
+        gpu.upload(d_out, out.data(), out.size() * sizeof(PairResult));
+    }
+
+    GpuTimer tm;
+    tm.start();
+    SSA_HIP(launch_pack((int)groups_per_q, gpu.stream(), d_groups, d_slots, db.d_codes, d_res));
+    SSA_HIP(launch_score(R, type == kSW ? 0 : 1, (int)ngroups, gpu.stream(), d_groups, d_qd, d_prof, d_lb, d_res,
+                         d_bnd, d_sb, d_pl, d_out, P));
+    tm.stop();
+    gpu.download(out.data(), d_out, out.size() * sizeof(PairResult));
+    gpu.sync();
+    *ms += tm.ms();
+}
+
+// exact 32-bit scores of the pairs (q, e) listed
+void gpu_exact(SearchType type, const std::vector<QueryBuf>& qs, const std::vector<std::pair<uint32_t, uint32_t>>& pairs,
+               std::vector<int64_t>& out, double* ms, uint64_t* cells_computed) {
+    out.assign(pairs.size(), 0);
+    if (pairs.empty()) return;
+    Gpu& gpu = Gpu::get();
+    DeviceDB& db = g_db;
+    const Config& cf = config();
+    const Matrix& M = matrix();
+    const MatrixInfo mi = matrix_info();
+    if (!mi.fits32) fatal("libssa_amd: score matrix entries exceed the exact 32-bit tier");
+    std::vector<QueryDesc> qd(qs.size());
+    std::vector<int32_t> prof;
+    size_t qmax = 0;
+    for (size_t qi = 0; qi < qs.size(); ++qi) {
+        qd[qi].qlen = (int32_t)qs[qi].len;
+        qd[qi].qp_off = (int32_t)prof.size();
+        qd[qi].lb_off = 0;
+        qmax = std::max(qmax, qs[qi].len);
+        for (size_t i = 0; i < qs[qi].len; ++i)
+            for (int d = 0; d < kDim; ++d) prof.push_back((int32_t)M.at(d, qs[qi].seq[i]));
+    }
+    // every value of a pair stays within +-(|go|+|ge|)*(qlen+dlen+4) + max|s|*min(qlen,dlen)
+    const int64_t gap = std::abs((int64_t)cf.gap_open) + std::abs((int64_t)cf.gap_extend);
+    const int64_t smax = std::max(std::abs(mi.maxv), std::abs(mi.minv));
+    std::vector<ExactPair> ep(pairs.size());
+    for (size_t k = 0; k < pairs.size(); ++k) {
+        const DbEntry& e = db.entries[pairs[k].second];
+        const size_t ql = qs[pairs[k].first].len;
+        const int64_t bound = gap * (int64_t)(ql + e.len + 4) * 3 + smax * (int64_t)std::min<size_t>(ql, e.len);
+        if (bound >= INT32_MAX / 2) fatal("libssa_amd: alignment too long for the exact 32-bit tier");
+        ep[k].res_off = e.code_off;
+        ep[k].dlen = (int32_t)e.len;
+        ep[k].query = (int32_t)pairs[k].first;
+        ep[k].out = (int32_t)k;
+        ep[k].pad = 0;
+        *cells_computed += (uint64_t)ql * e.len;
+    }
+    auto* d_ep = (ExactPair*)gpu.arena("exact_pairs", ep.size() * sizeof(ExactPair));
+    auto* d_qd = (QueryDesc*)gpu.arena("exact_qdesc", qd.size() * sizeof(QueryDesc));
+    auto* d_prof = (int32_t*)gpu.arena("exact_prof", std::max<size_t>(prof.size(), 1) * sizeof(int32_t));
+    auto* d_scr = (int32_t*)gpu.arena("exact_scratch", ep.size() * 2 * std::max<size_t>(qmax, 1) * sizeof(int32_t));
+    auto* d_out = (int64_t*)gpu.arena("exact_out", ep.size() * sizeof(int64_t));
+    gpu.upload(d_ep, ep.data(), ep.size() * sizeof(ExactPair));
+    gpu.upload(d_qd, qd.data(), qd.size() * sizeof(QueryDesc));
+    gpu.upload(d_prof, prof.data(), prof.size() * sizeof(int32_t));
+    GpuTimer tm;
+    tm.start();
+    SSA_HIP(launch_exact((int)ep.size(), gpu.stream(), d_ep, d_qd, d_prof, db.d_codes, type == kSW ? 0 : 1,
+                         cf.gap_open, cf.gap_extend, d_scr, (int64_t)(2 * std::max<size_t>(qmax, 1)), d_out));
+    tm.stop();
+    gpu.download(out.data(), d_out, out.size() * sizeof(int64_t));
+    gpu.sync();
+    *ms += tm.ms();
+}
+
+// Decodes one tier result into (duration, accept, score).
+TierOut decode(SearchType type, const Tier& T, const PairResult& r, uint32_t len, bool sw16_as_8) {
+    const int32_t nb = (int32_t)((len + 3) / 4);
+    TierOut o;
+    int32_t flag;
+    if (type == kSW) {
+        flag = sw16_as_8 ? r.flag_a : (T.bits == 16 ? r.flag_b : r.flag_a);
+        o.score = T.decode(r.score) - T.imin();
+        o.accept = flag == kNoFlag;
+    } else {
+        flag = r.flag_a;
+        o.score = T.decode(r.score);
+        o.accept = flag == kNoFlag && o.score > T.imin() && o.score < T.imax();
+    }
+    o.dur = flag == kNoFlag ? nb : std::min(nb, flag + 1);
+    return o;
+}
+
+}  // namespace
+
+// ============================================================================
+// replay of the reference's lane schedule and heap
+// ============================================================================
+namespace {
+
+struct Elem {
+    uint64_t db_id;
+    uint8_t db_frame, db_strand, query_id;
+    int64_t score;
+};
+
+// bounded min-heap with the reference's rules (src/util/minheap.c:60-91)
+class MinHeap {
+public:
+    explicit MinHeap(size_t k) : alloc_(k) { a_.reserve(k); }
+    void add(const Elem& e) {
+        if (a_.size() < alloc_) {
+            size_t i = a_.size();
+            a_.push_back(e);
+            while (i > 0) {
+                const size_t p = (i - 1) / 2;
+                if (!(e.score < a_[p].score)) break;
+                a_[i] = a_[p];
+                i = p;
+            }
+            a_[i] = e;
+        } else if (!a_.empty() && a_[0].score < e.score) {
+            size_t p = 0, c = 1;
+            const size_t cnt = a_.size();
+            while (c < cnt) {
+                if (c + 1 < cnt && a_[c + 1].score < a_[c].score) ++c;
+                if (a_[c].score < e.score) a_[p] = a_[c];
+                else break;
+                p = c;
+                c = 2 * p + 1;
+            }
+            a_[p] = e;
+        }
+    }
+    // heap order, then a stable sort by (score desc, db_id desc) as minheap_sort
+    // under a merge-sort qsort (src/util/minheap.c:93-106)
+    std::vector<Elem> sorted() const {
+        std::vector<Elem> v = a_;
+        std::stable_sort(v.begin(), v.end(), [](const Elem& x, const Elem& y) {
+            if (x.score != y.score) return x.score > y.score;
+            return x.db_id > y.db_id;
+        });
+        return v;
+    }
+
+private:
+    size_t alloc_;
+    std::vector<Elem> a_;
+};
+
+// One reference kernel call: W lanes take the items in order; a lane that
+// takes an item in iteration t retires it in iteration t + dur; all lanes
+// retiring in one iteration are served in lane order and immediately take the
+// next item (src/algo/simd/search_simd_sw.c:338-428).  Calls emit(k) for the
+// items (positions into the list) in retirement order.
+template <class Dur, class Emit>
+void lane_schedule(int W, size_t n, const Dur& dur, const Emit& emit) {
+    struct L {
+        int64_t t;
+        int lane;
+        size_t item;
+    };
+    auto later = [](const L& a, const L& b) { return a.t != b.t ? a.t > b.t : a.lane > b.lane; };
+    std::vector<L> h;
+    h.reserve(W);
+    size_t next = 0;
+    for (int c = 0; c < W && next < n; ++c, ++next) h.push_back(L{dur(next), c, next});
+    std::make_heap(h.begin(), h.end(), later);
+    while (!h.empty()) {
+        std::pop_heap(h.begin(), h.end(), later);
+        L x = h.back();
+        h.pop_back();
+        emit(x.item);
+        if (next < n) {
+            h.push_back(L{x.t + dur(next), x.lane, next});
+            ++next;
+            std::push_heap(h.begin(), h.end(), later);
+        }
+    }
+}
+
+}  // namespace
+
+// ============================================================================
+// results list and traceback
+// ============================================================================
+namespace {
+
+// the sequence the reference attaches to a hit (us_prepare_sequence,
+// src/util/util_sequence.c:384-417): mapped forward strand unless strand == 2,
+// which the adapter never produces; unknown symbols become 0
+std::vector<uint8_t> prepared_db_sequence(p_seqinfo s, int strand_field) {
+    std::vector<uint8_t> v(s->seqlen);
+    const bool nuc = config().symtype == NUCLEOTIDE;
+    map_db_codes(s->seq, s->seqlen, nuc, v.data());
+    if (nuc && strand_field == 2) {
+        std::vector<uint8_t> r(v.size());
+        revcomp(v.data(), v.size(), r.data());
+        v.swap(r);
+    }
+    return v;
+}
+
+void gpu_traceback(SearchType type, std::vector<p_alignment>& al, double* ms) {
+    if (al.empty()) return;
+    Gpu& gpu = Gpu::get();
+    const Config& cf = config();
+    std::vector<uint8_t> codes;
+    std::vector<TraceTask> tasks(al.size());
+    int64_t dir_total = 0, hh_total = 0, ops_total = 0;
+    for (size_t k = 0; k < al.size(); ++k) {
+        p_alignment a = al[k];
+        TraceTask& t = tasks[k];
+        t.q_off = (int64_t)codes.size();
+        codes.insert(codes.end(), (const uint8_t*)a->query.seq, (const uint8_t*)a->query.seq + a->query.len);
+        t.d_off = (int64_t)codes.size();
+        codes.insert(codes.end(), (const uint8_t*)a->db_seq.seq, (const uint8_t*)a->db_seq.seq + a->db_seq.len);
+        t.qlen = (int32_t)a->query.len;
+        t.dlen = (int32_t)a->db_seq.len;
+        t.dir_off = dir_total;
+        dir_total += (int64_t)t.qlen * t.dlen + 1;
+        t.hh_off = hh_total;
+        hh_total += 2 * (int64_t)std::max(t.qlen, t.dlen) + 2;
+        t.ops_off = ops_total;
+        t.ops_cap = t.qlen + t.dlen + 1;
+        ops_total += t.ops_cap;
+        t.type = type == kSW ? 0 : 1;
+    }
+    const int64_t t0 = now_us();
+    auto* d_codes = (uint8_t*)gpu.arena("tb_codes", codes.size() + 1);
+    auto* d_tasks = (TraceTask*)gpu.arena("tb_tasks", tasks.size() * sizeof(TraceTask));
+    auto* d_mat = (int64_t*)gpu.arena("tb_mat", kDim * kDim * sizeof(int64_t));
+    auto* d_dirs = (uint8_t*)gpu.arena("tb_dirs", (size_t)dir_total);
+    auto* d_hh = (int64_t*)gpu.arena("tb_hh", (size_t)hh_total * sizeof(int64_t));
+    auto* d_ops = (int32_t*)gpu.arena("tb_ops", (size_t)ops_total * 2 * sizeof(int32_t));
+    auto* d_res = (TraceResult*)gpu.arena("tb_res", tasks.size() * sizeof(TraceResult));
+    gpu.upload(d_codes, codes.data(), codes.size());
+    gpu.upload(d_tasks, tasks.data(), tasks.size() * sizeof(TraceTask));
+    gpu.upload(d_mat, matrix().s, kDim * kDim * sizeof(int64_t));
+    SSA_HIP(launch_traceback((int)tasks.size(), gpu.stream(), d_tasks, d_codes, d_mat, cf.gap_open, cf.gap_extend,
+                             d_dirs, d_hh, d_ops, d_res));
+    std::vector<TraceResult> res(tasks.size());
+    std::vector<int32_t> ops((size_t)ops_total * 2);
+    gpu.download(res.data(), d_res, res.size() * sizeof(TraceResult));
+    gpu.download(ops.data(), d_ops, ops.size() * sizeof(int32_t));
+    gpu.sync();
+    *ms += (now_us() - t0) / 1000.0;
+
+    for (size_t k = 0; k < al.size(); ++k) {
+        p_alignment a = al[k];
+        const TraceResult& r = res[k];
+        if (r.status == 2) fatal("libssa_amd: traceback buffer too small");
+        std::string cig;
+        if (r.status == 1) {
+            // no positive cell: the reference's region is undefined here
+            print_warning("no local alignment with a positive score for DB sequence %ld", (long)a->db_seq.ID);
+            a->align_q_start = a->align_q_end = a->align_d_start = a->align_d_end = 0;
+        } else {
+            const int32_t* o = ops.data() + tasks[k].ops_off * 2;
+            for (int32_t x = r.nops - 1; x >= 0; --x) {
+                const int32_t cnt = o[2 * x + 1];
+                if (cnt > 1) cig += std::to_string(cnt);
+                cig.push_back((char)o[2 * x]);
+            }
+            a->align_q_start = (size_t)r.a_begin;
+            a->align_q_end = (size_t)r.a_end;
+            a->align_d_start = (size_t)r.b_begin;
+            a->align_d_end = (size_t)r.b_end;
+        }
+        a->alignment = (char*)malloc(cig.size() + 1);
+        memcpy(a->alignment, cig.c_str(), cig.size() + 1);
+        a->alignment_len = cig.size();
+    }
+}
+
+}  // namespace
+
+// ============================================================================
+// the search
+// ============================================================================
+ssa_amd_stats_t& last_stats() {
+    static ssa_amd_stats_t s;
+    return s;
+}
+
+int& rows_per_strip_override() {
+    static int r = 0;
+    return r;
+}
+
+p_alignment_list run_search(SearchType type, p_query query, size_t hitcount, int bit_width, int align_type) {
+    const int64_t t_start = now_us();
+    const Config& cf = config();
+    ssa_amd_stats_t st;
+    memset(&st, 0, sizeof st);
+
+    if (bit_width != BIT_WIDTH_8 && bit_width != BIT_WIDTH_16 && bit_width != BIT_WIDTH_64)
+        fatal("\nunknown bit width provided: %d\n\n", bit_width);  // src/algo/searcher.c:115-117
+    if (bit_width == BIT_WIDTH_8 && cf.simd_mode < COMPUTE_ON_SSE41)
+        fatal("libssa_amd: 8-bit searches replay SSE4.1 or AVX2 lanes; COMPUTE_ON_SSE2 selects neither");
+
+    // query buffers (src/algo/searcher.c:42-90)
+    std::vector<QueryBuf> qs;
+    if (cf.symtype == NUCLEOTIDE) {
+        for (int s = 0; s < 2; ++s)
+            if (((s + 1) & cf.strands) && query->has_nt[s])
+                qs.push_back(QueryBuf{query->nt[s].data(), query->nt[s].size(), s, 0});
+    } else if (cf.symtype == AMINOACID) {
+        qs.push_back(QueryBuf{query->aa[0].data(), query->aa[0].size(), 0, 0});
+    } else {
+        fatal("libssa_amd: translated symbol types (TRANS_QUERY, TRANS_DB, TRANS_BOTH) are not supported");
+    }
+    for (const QueryBuf& q : qs)
+        if (q.len == 0 && type == kNW) fatal("libssa_amd: NW search with an empty query");
+
+    DeviceDB& db = device_db();
+    const size_t nq = qs.size();
+
+    // entries processed: whole chunks of DB IDs until the first chunk without
+    // entries (search_8/16/64 stop there, src/algo/8/search_8.c:135-143)
+    const size_t cs = cf.chunk_size ? cf.chunk_size : kDefaultChunkSize;
+    std::vector<std::pair<uint32_t, uint32_t>> chunks;  // entry ranges
+    for (size_t start = 0; start < db.count; start += cs) {
+        const size_t stop = std::min(db.count, start + cs);
+        const uint32_t e0 = db.first_entry[start], e1 = db.first_entry[stop];
+        if (e0 == e1) break;
+        chunks.push_back({e0, e1});
+    }
+    const uint32_t N = chunks.empty() ? 0 : chunks.back().second;
+    st.chunk_count = chunks.size();
+    st.seq_count = N;
+    for (const QueryBuf& q : qs)
+        for (uint32_t e = 0; e < N; ++e) st.cells += (uint64_t)q.len * db.entries[e].len;
+
+    const MatrixInfo mi = matrix_info();
+    const int go = cf.gap_open, ge = cf.gap_extend;
+    // 8-bit SW needs no separate pass when the 8-bit lane values equal the exact ones
+    const bool sw_single = type == kSW && mi.fits8 && (go + ge) >= -128;
+
+    PairSet all;
+    all.nq = nq;
+    all.entries.resize(N);
+    std::iota(all.entries.begin(), all.entries.end(), 0u);
+    const bool full_db = N == db.entries.size();
+
+    // per-(q, e) tier results, dense over the processed entries
+    std::vector<TierOut> t8, t16;
+    std::vector<int64_t> s64;
+    std::vector<uint8_t> need64;  // entries whose scores come from the exact tier
+    auto at = [N](size_t q, size_t e) { return q * N + e; };
+    double ms_first = 0, ms_re = 0;
+    uint64_t computed = 0;
+    int rows_used = 0;
+    std::vector<PairResult> raw;
+
+    std::vector<uint8_t> in16(N, 0);
+    if (bit_width == BIT_WIDTH_8) {
+        t8.resize(nq * N);
+        if (sw_single) {
+            gpu_score_tier(type, kTier16, qs, all, full_db ? &db.by_length : nullptr, raw, &ms_first, &computed,
+                           &rows_used);
+            t16.resize(nq * N);
+            for (size_t q = 0; q < nq; ++q)
+                for (uint32_t e = 0; e < N; ++e) {
+                    const PairResult& r = raw[all.index(q, e)];
+                    t8[at(q, e)] = decode(type, kTier16, r, db.entries[e].len, true);
+                    t16[at(q, e)] = decode(type, kTier16, r, db.entries[e].len, false);
+                    if (!t8[at(q, e)].accept) in16[e] = 1;
+                }
+        } else {
+            gpu_score_tier(type, kTier8, qs, all, full_db ? &db.by_length : nullptr, raw, &ms_first, &computed,
+                           &rows_used);
+            for (size_t q = 0; q < nq; ++q)
+                for (uint32_t e = 0; e < N; ++e) {
+                    t8[at(q, e)] = decode(type, kTier8, raw[all.index(q, e)], db.entries[e].len, false);
+                    if (!t8[at(q, e)].accept) in16[e] = 1;
+                }
+            PairSet sub;
+            sub.nq = nq;
+            for (uint32_t e = 0; e < N; ++e)
+                if (in16[e]) sub.entries.push_back(e);
+            t16.resize(nq * N);
+            if (!sub.entries.empty()) {
+                int r2 = 0;
+                gpu_score_tier(type, kTier16, qs, sub, nullptr, raw, &ms_re, &computed, &r2);
+                for (size_t q = 0; q < nq; ++q)
+                    for (size_t k = 0; k < sub.entries.size(); ++k) {
+                        const uint32_t e = sub.entries[k];
+                        t16[at(q, e)] = decode(type, kTier16, raw[sub.index(q, k)], db.entries[e].len, false);
+                    }
+            }
+        }
+    } else if (bit_width == BIT_WIDTH_16) {
+        gpu_score_tier(type, kTier16, qs, all, full_db ? &db.by_length : nullptr, raw, &ms_first, &computed,
+                       &rows_used);
+        t16.resize(nq * N);
+        for (size_t q = 0; q < nq; ++q)
+            for (uint32_t e = 0; e < N; ++e) {
+                t16[at(q, e)] = decode(type, kTier16, raw[all.index(q, e)], db.entries[e].len, false);
+                in16[e] = 1;
+            }
+    }
+
+    // entries that reach the exact tier
+    need64.assign(N, 0);
+    if (bit_width == BIT_WIDTH_64) {
+        // scores only: use the 16-bit pass where it is provably exact
+        std::fill(need64.begin(), need64.end(), 1);
+        if (mi.fits16) {
+            gpu_score_tier(type, kTier16, qs, all, full_db ? &db.by_length : nullptr, raw, &ms_first, &computed,
+                           &rows_used);
+            t16.resize(nq * N);
+            const int64_t gap = std::abs((int64_t)go) + std::abs((int64_t)ge);
+            std::fill(need64.begin(), need64.end(), 0);
+            for (size_t q = 0; q < nq; ++q)
+                for (uint32_t e = 0; e < N; ++e) {
+                    TierOut o = decode(type, kTier16, raw[all.index(q, e)], db.entries[e].len, false);
+                    if (type == kNW) {
+                        // exact when no value can reach either end of the int16 range
+                        const int64_t span = 3 * gap + gap * (int64_t)(qs[q].len + db.entries[e].len + 4);
+                        const int64_t up = std::max<int64_t>(mi.maxv, 0) * (int64_t)std::min<size_t>(qs[q].len, db.entries[e].len);
+                        const int64_t lowest = -span - std::max<int64_t>(-mi.minv, 0) * (int64_t)std::min<size_t>(qs[q].len, db.entries[e].len);
+                        if (!(lowest > -32768 + gap + 1 && up < 32767)) o.accept = false;
+                    }
+                    t16[at(q, e)] = o;
+                    if (!o.accept) need64[e] = 1;
+                }
+        }
+    } else {
+        for (size_t q = 0; q < nq; ++q)
+            for (uint32_t e = 0; e < N; ++e)
+                if (in16[e] && !t16[at(q, e)].accept) need64[e] = 1;
+    }
+    std::vector<std::pair<uint32_t, uint32_t>> p64;
+    for (uint32_t e = 0; e < N; ++e)
+        if (need64[e])
+            for (size_t q = 0; q < nq; ++q) p64.push_back({(uint32_t)q, e});
+    s64.assign(nq * N, 0);
+    if (!p64.empty()) {
+        std::vector<int64_t> o64;
+        gpu_exact(type, qs, p64, o64, &ms_re, &computed);
+        for (size_t k = 0; k < p64.size(); ++k) s64[at(p64[k].first, p64[k].second)] = o64[k];
+    }
+    if (bit_width == BIT_WIDTH_64)
+        for (size_t q = 0; q < nq; ++q)
+            for (uint32_t e = 0; e < N; ++e)
+                if (!need64[e]) s64[at(q, e)] = t16[at(q, e)].score;
+
+    // ---- replay ----
+    const int64_t t_replay = now_us();
+    const int W8 = lanes8(), W16 = lanes16();
+    // per chunk the ordered insertion list is built in parallel, then fed to the heap
+    struct Ins {
+        uint32_t e;
+        uint8_t q;
+        int64_t score;
+    };
+    std::vector<std::vector<Ins>> per_chunk(chunks.size());
+    std::vector<size_t> ov8(chunks.size(), 0), ov16(chunks.size(), 0);
+    auto do_chunk = [&](size_t c) {
+        const uint32_t e0 = chunks[c].first, e1 = chunks[c].second;
+        std::vector<Ins>& out = per_chunk[c];
+        std::vector<uint32_t> items(e1 - e0), ovf, ovf2;
+        std::iota(items.begin(), items.end(), e0);
+        auto tier64 = [&](const std::vector<uint32_t>& list) {
+            for (size_t q = 0; q < nq; ++q)
+                for (uint32_t e : list) out.push_back(Ins{e, (uint8_t)q, s64[at(q, e)]});
+        };
+        auto tier16 = [&](const std::vector<uint32_t>& list) {
+            ovf2.clear();
+            for (size_t q = 0; q < nq; ++q)
+                lane_schedule(
+                    W16, list.size(), [&](size_t k) { return (int64_t)t16[at(q, list[k])].dur; },
+                    [&](size_t k) {
+                        const TierOut& o = t16[at(q, list[k])];
+                        if (o.accept) out.push_back(Ins{list[k], (uint8_t)q, o.score});
+                        else ovf2.push_back(list[k]);
+                    });
+            if (!ovf2.empty()) {
+                ov16[c] += ovf2.size();
+                tier64(ovf2);
+            }
+        };
+        if (bit_width == BIT_WIDTH_8) {
+            for (size_t q = 0; q < nq; ++q)
+                lane_schedule(
+                    W8, items.size(), [&](size_t k) { return (int64_t)t8[at(q, items[k])].dur; },
+                    [&](size_t k) {
+                        const TierOut& o = t8[at(q, items[k])];
+                        if (o.accept) out.push_back(Ins{items[k], (uint8_t)q, o.score});
+                        else ovf.push_back(items[k]);
+                    });
+            if (!ovf.empty()) {
+                ov8[c] += ovf.size();
+                tier16(ovf);
+            }
+        } else if (bit_width == BIT_WIDTH_16) {
+            tier16(items);
+        } else {
+            tier64(items);
+        }
+    };
+    const size_t nthreads = std::max<size_t>(1, std::min<size_t>(16, std::thread::hardware_concurrency()));
+    if (chunks.size() > 8 && nthreads > 1) {
+        std::vector<std::thread> th;
+        for (size_t t = 0; t < nthreads; ++t)
+            th.emplace_back([&, t] {
+                for (size_t c = t; c < chunks.size(); c += nthreads) do_chunk(c);
+            });
+        for (auto& x : th) x.join();
+    } else {
+        for (size_t c = 0; c < chunks.size(); ++c) do_chunk(c);
+    }
+    MinHeap heap(hitcount);
+    for (size_t c = 0; c < chunks.size(); ++c) {
+        for (const Ins& x : per_chunk[c]) {
+            const DbEntry& e = db.entries[x.e];
+            heap.add(Elem{e.id, e.frame, e.strand, x.q, x.score});
+        }
+        st.overflow_8_bit_count += ov8[c];
+        st.overflow_16_bit_count += ov16[c];
+    }
+    std::vector<Elem> hits = heap.sorted();
+    st.ms_replay = (now_us() - t_replay) / 1000.0;
+
+    if (st.overflow_8_bit_count || st.overflow_16_bit_count)
+        print_info("Overflow occurred: %ld sequences were re-aligned with 16 bit, and %ld sequences with 64 bit\n",
+                   (long)st.overflow_8_bit_count, (long)st.overflow_16_bit_count);
+
+    // ---- alignment list (src/algo/manager.c:83-113, src/algo/aligner.c:62-99) ----
+    p_alignment_list alist = (p_alignment_list)malloc(sizeof(alignment_list_t));
+    alist->len = hits.size();
+    alist->alignments = (p_alignment*)malloc(std::max<size_t>(hits.size(), 1) * sizeof(p_alignment));
+    std::vector<p_alignment> al(hits.size());
+    for (size_t k = 0; k < hits.size(); ++k) {
+        const Elem& h = hits[k];
+        p_seqinfo info = ssa_db_get_sequence(h.db_id);
+        if (!info) fatal("Could not get sequence from DB: %ld", (long)h.db_id);
+        std::vector<uint8_t> dseq = prepared_db_sequence(info, h.db_strand);
+        p_alignment a = (p_alignment)calloc(1, sizeof(alignment_t));
+        a->db_seq.seq = (char*)malloc(dseq.size() + 1);
+        memcpy(a->db_seq.seq, dseq.data(), dseq.size());
+        a->db_seq.seq[dseq.size()] = 0;
+        a->db_seq.len = dseq.size();
+        a->db_seq.strand = h.db_frame;  // the reference stores these two swapped (src/algo/aligner.c:76-77)
+        a->db_seq.frame = h.db_strand;
+        a->db_seq.ID = info->ID;
+        const QueryBuf& q = qs[h.query_id];
+        a->query.seq = (char*)q.seq;
+        a->query.len = q.len;
+        a->query.strand = q.strand;
+        a->query.frame = q.frame;
+        a->score = h.score;
+        al[k] = a;
+        alist->alignments[k] = a;
+    }
+    if (align_type != COMPUTE_SCORE) gpu_traceback(type, al, &st.ms_traceback);
+
+    st.cells_computed = computed;
+    st.rows_per_strip = rows_used;
+    st.ms_gpu_first = ms_first;
+    st.ms_gpu_requeue = ms_re;
+    st.ms_total = (now_us() - t_start) / 1000.0;
+    last_stats() = st;
+    return alist;
+}
+
+}  // namespace ssa

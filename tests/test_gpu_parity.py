@@ -1,0 +1,231 @@
+"""GPU search (through the libssa.h drop-in API) against the oracle.
+
+Scores, hit selection and order, strands and CIGAR strings must be identical
+to the reference run with one thread (the oracle restates it and is pinned by
+the reference's goldens and its own compiled kernels).
+"""
+import os
+import random
+import tempfile
+
+import pytest
+
+import _oracle as O
+
+pytestmark = pytest.mark.gpu
+
+ssa = pytest.importorskip("libssa_amd")
+
+LANES = {"avx2": (32, 16, 2), "sse": (16, 8, 1)}  # lanes8, lanes16, COMPUTE_ON_*
+
+
+def write_fasta(path, seqs):
+    with open(path, "w") as f:
+        for i, s in enumerate(seqs):
+            f.write(">s%d\n" % i)
+            for k in range(0, len(s), 60):
+                f.write(s[k:k + 60] + "\n")
+            if not s:
+                f.write("\n")
+
+
+def configure(symtype, strands, gaps, matrix, isa="avx2", chunk=1000):
+    ssa.set_output_mode(ssa.OUTPUT_ERROR)
+    ssa.init_symbol_translation(ssa.AMINOACID, ssa.FORWARD_STRAND, 1, 1)  # reset to a checkable state first
+    ssa.init_symbol_translation(symtype, strands, 1, 1)
+    ssa.set_simd_compute_mode(LANES[isa][2])
+    ssa.set_chunk_size(chunk)
+    ssa.init_gap_penalties(*gaps)
+    kind, arg = matrix
+    if kind == "const":
+        ssa.init_constant_scores(*arg)
+    elif kind == "builtin":
+        ssa.init_score_matrix(ssa.MATRIX_BUILDIN, arg)
+    else:
+        ssa.init_score_matrix(ssa.READ_FROM_STRING, arg)
+
+
+def oracle_matrix(matrix):
+    kind, arg = matrix
+    if kind == "const":
+        return O.matrix_constant(*arg)
+    if kind == "builtin":
+        return O.matrix_builtin(arg)
+    return O.matrix_text(arg)
+
+
+def gpu_search(t, q, k, bw, align):
+    f = ssa.sw_align if t == O.SW else ssa.nw_align
+    return f(q, k, bw, align)
+
+
+def check(t, bw, symtype, strands, qraw, db, k, matrix, gaps, isa="avx2", chunk=1000, align=False):
+    """Runs the GPU search and the oracle on the same inputs and compares everything."""
+    configure(symtype, strands, gaps, matrix, isa, chunk)
+    with tempfile.TemporaryDirectory() as d:
+        path = os.path.join(d, "db.fas")
+        write_fasta(path, db)
+        ssa.init_db(path)
+        q = ssa.init_sequence_fasta(ssa.READ_FROM_STRING, qraw)
+        got = gpu_search(t, q, k, bw, ssa.COMPUTE_ALIGNMENT if align else ssa.COMPUTE_SCORE)
+        stats = ssa.last_stats()
+        ssa.free_sequence(q)
+    m = oracle_matrix(matrix)
+    qbufs = O.query_buffers(symtype, strands, qraw)
+    l8, l16, _ = LANES[isa]
+    want, st = O.search(t, bw, symtype, strands, qbufs, db, k, m, gaps[0], gaps[1], chunk=chunk, lanes8=l8,
+                        lanes16=l16)
+    qstrands = [s for s in range(2) if (s + 1) & strands] if symtype == O.NUCLEOTIDE else [0]
+    assert [(a.score, a.db_id, a.db_frame, qstrands.index(a.query_strand)) for a in got] == \
+        [(s, i, strand, qi) for s, i, strand, qi in want]
+    assert (stats.overflow_8_bit_count, stats.overflow_16_bit_count) == (st.overflow_8, st.overflow_16)
+    assert (stats.chunk_count, stats.seq_count) == (st.chunk_count, st.seq_count)
+    if align:
+        for a, (s, i, strand, qi) in zip(got, want):
+            dseq = O.map_db(symtype, db[i])
+            if t == O.SW and s <= 0:
+                continue
+            c, r = O.cigar(t, qbufs[qi], dseq, m, gaps[0], gaps[1])
+            assert a.alignment == c
+            assert (a.align_q_start, a.align_q_end, a.align_d_start, a.align_d_end) == r
+            assert a.db_seq == dseq
+    return got
+
+
+def golden_db(name):
+    return [s for _, s in O.read_fasta(O.data(name))]
+
+
+# ------------------------------------------------------------------ reference goldens
+AF_NW = [(-88, 908), (-92, 1050), (-94, 938), (-98, 378), (-102, 75), (-104, 23), (-106, 1229), (-110, 1148),
+         (-110, 612), (-112, 1016)]
+AF_SW = [(20, 1177), (18, 384), (18, 277), (18, 215), (18, 214), (18, 210), (18, 151), (18, 126), (18, 114),
+         (18, 101)]
+
+
+@pytest.mark.parametrize("bw", [64, 16, 8])
+@pytest.mark.parametrize("t,expect", [(O.NW, AF_NW), (O.SW, AF_SW)])
+def test_af091148_bigger_database(bw, t, expect):
+    # tests/test_bigger_databases.c:70-92
+    qraw = O.read_fasta(O.data("one_seq.fas"))[0][1]
+    got = check(t, bw, O.NUCLEOTIDE, 1, qraw, golden_db("AF091148.fas"), 10, ("const", (2, -2)), (-4, -2))
+    assert [(a.score, a.db_id) for a in got] == expect
+
+
+LIBSSA_SW = [(877, 91, "2MD2MIM2I3MD3MD4M2D8M5I3M5IM3I7MI7M3I5MI3M"),
+             (847, 91, "2MD2MIM2I3MD3MD4M2D8M5I3M5IM3I7MI7M3I4MI4M"),
+             (753, 91, "2MD2MIM2I3MD3MD3M2D9M5I3M5IM3I7MI7M3I5MI3M"),
+             (565, 91, "2MD2MIM2I3MD3MD4M2D8M5I3M5IM3I7MI7M3I5MI3M"),
+             (398, 91, "2MD2MIM2I3MD3MD3M2D9M5I3M5IM3I7MI7M3I5MI3M")]
+LIBSSA_NW = [(1050, 33, "2MI2MI6M3I6MD3M19I4M3IMI3M5I3M5IM3I7MI7M4IMIMI4MI2M7I"),
+             (908, 28, "2MI2MI6M3I6MD3M5IM2I2M3I2M5IMI9MD2M3I3M9I2M3I2M4I2M2IMI4MI2M7I"),
+             (938, 24, "2MI2MI6M3I6MD3M5IM2I2M3I2M5IMI9MD2M3I3M9I2M3I2M4IM3IMIMIM2I5M10I"),
+             (378, 21, "2MI2MI6M3I5M9I2M7IM6IMI3M3I6M5I3M5IM3I7MI7M4IMIMI4MI2M7I"),
+             (75, 12, "2MI2MI6M3I5M9I2M7IM6IMI3M3I6M5I3M5IM3I7MI7M4IMIMI4MI2M7I")]
+
+
+@pytest.mark.parametrize("t,expect", [(O.SW, LIBSSA_SW), (O.NW, LIBSSA_NW)])
+def test_public_api_cigars(t, expect):
+    # tests/test_libssa.c:48-93 (one thread, 64 bit, COMPUTE_ALIGNMENT)
+    configure(O.NUCLEOTIDE, 1, (-4, -2), ("const", (5, -4)))
+    ssa.init_db(O.data("AF091148.fas"))
+    q = ssa.init_sequence_fasta(ssa.READ_FROM_FILE, O.data("one_seq.fas"))
+    got = gpu_search(t, q, 5, 64, ssa.COMPUTE_ALIGNMENT)
+    ssa.free_sequence(q)
+    assert [(a.db_id, a.score, a.alignment) for a in got] == expect
+
+
+@pytest.mark.parametrize("t", [O.SW, O.NW])
+def test_overflow_to_64bit(t):
+    # tests/algo/test_searcher.c:507-546
+    qraw = O.read_fasta(O.data("NP_009305.1.fas"))[0][1]
+    got = check(t, 8, O.AMINOACID, 1, qraw, golden_db("NP_009305.1.fas"), 1, ("const", (127, -1)), (-1, -1), chunk=1)
+    assert [(a.score, a.db_id) for a in got] == [(67818, 0)]
+    st = ssa.last_stats()
+    assert st.overflow_8_bit_count == 1 and st.overflow_16_bit_count == 1
+
+
+MULTI_NW = [(-1, 0), (-3, 3), (-4, 7), (-4, 6), (-4, 4), (-5, 2), (-6, 5), (-6, 1)]
+MULTI_SW = [(4, 3), (4, 1), (4, 0), (3, 7), (3, 5), (3, 4), (3, 2), (2, 6)]
+
+
+@pytest.mark.parametrize("bw", [8, 16, 64])
+@pytest.mark.parametrize("isa", ["avx2", "sse"])
+@pytest.mark.parametrize("t,expect", [(O.SW, MULTI_SW), (O.NW, MULTI_NW)])
+def test_searcher_multiple_sequences(bw, isa, t, expect):
+    # tests/algo/test_searcher.c:139-227
+    got = check(t, bw, O.NUCLEOTIDE, 1, "ATGCAAATTT", golden_db("short_nuc_db.fas"), 8, ("const", (1, -1)),
+                (-1, -1), isa=isa, chunk=8)
+    assert [(a.score, a.db_id) for a in got] == expect
+
+
+@pytest.mark.parametrize("bw", [8, 16, 64])
+@pytest.mark.parametrize("t,score", [(O.SW, 103), (O.NW, 82)])
+def test_blosum62_short_aa(bw, t, score):
+    # tests/algo/test_searcher.c:433-505
+    got = check(t, bw, O.AMINOACID, 1, "HPEVYILIIPGFGIISHVVSTYSKKPVFGEISMVYAMASIGLLGFLVWSHHMYIVGLDADTRAYFTSATMIIAIPTGIKI",
+                golden_db("short_AA.fas"), 1, ("builtin", "blosum62"), (-1, -1), chunk=1)
+    assert got[0].score == score
+
+
+def test_nw8_test_fas_overflow_counts():
+    # tests/algo/8/test_8_simd_avx2_nw.c:78-95
+    got = check(O.NW, 8, O.NUCLEOTIDE, 1, "ATGCCCAAGCTGAATAGCGTAGAGGGGTTTTCATCATTTGAGGACGATGTATAA",
+                golden_db("test.fas"), 5, ("const", (1, -1)), (-1, -1), chunk=5)
+    assert [a.score for a in got] == [-43, -50, -52, -52, -147]
+    assert ssa.last_stats().overflow_8_bit_count == 4
+
+
+# ------------------------------------------------------------------ randomized parity
+def rand_db(rnd, symtype, n, lo, hi, homolog_of=None, frac=0.0):
+    alpha = "ACGT" if symtype == O.NUCLEOTIDE else "ARNDCQEGHILKMFPSTWYV"
+    out = []
+    for _ in range(n):
+        if homolog_of and rnd.random() < frac:
+            s = list(homolog_of[rnd.randint(0, len(homolog_of) // 3):])
+            for k in range(len(s)):
+                if rnd.random() < 0.15:
+                    s[k] = rnd.choice(alpha)
+            out.append("".join(s[:rnd.randint(1, len(s))]))
+        else:
+            out.append("".join(rnd.choice(alpha) for _ in range(rnd.randint(lo, hi))))
+    return out
+
+
+CASES = []
+for seed in range(24):
+    CASES.append(seed)
+
+
+@pytest.mark.parametrize("seed", CASES)
+def test_random_parity(seed):
+    rnd = random.Random(9000 + seed)
+    symtype = O.AMINOACID if seed % 3 else O.NUCLEOTIDE
+    strands = 1 if symtype == O.AMINOACID else rnd.choice([1, 2, 3])
+    t = O.SW if seed % 2 == 0 else O.NW
+    bw = [8, 16, 64][seed % 3 if seed < 12 else rnd.randrange(3)]
+    isa = "avx2" if seed % 4 else "sse"
+    alpha = "ACGT" if symtype == O.NUCLEOTIDE else "ARNDCQEGHILKMFPSTWYV"
+    qraw = "".join(rnd.choice(alpha) for _ in range(rnd.choice([5, 17, 33, 64, 100, 150])))
+    db = rand_db(rnd, symtype, rnd.randint(50, 700), 1, rnd.choice([40, 120, 400]), homolog_of=qraw, frac=0.1)
+    if symtype == O.NUCLEOTIDE:
+        matrix = ("const", rnd.choice([(1, -1), (2, -4), (5, -4), (2, -2)]))
+    else:
+        matrix = ("builtin", rnd.choice(["blosum62", "blosum50", "pam30"])) if seed % 5 else ("const", (3, -2))
+    gaps = rnd.choice([(-1, -1), (-10, -1), (-4, -2), (-20, -2), (-3, -1)])
+    k = rnd.choice([1, 5, 10, 50, 200])
+    chunk = rnd.choice([1000, 37, 250])
+    check(t, bw, symtype, strands, qraw, db, k, matrix, gaps, isa=isa, chunk=chunk, align=(seed % 4 == 1))
+
+
+@pytest.mark.parametrize("rows", [8, 16, 24, 32, 48])
+def test_rows_per_strip_do_not_change_results(rows):
+    rnd = random.Random(77)
+    qraw = "".join(rnd.choice("ARNDCQEGHILKMFPSTWYV") for _ in range(131))
+    db = rand_db(rnd, O.AMINOACID, 400, 1, 300, homolog_of=qraw, frac=0.2)
+    ssa.set_rows_per_strip(rows)
+    try:
+        for t, bw in [(O.SW, 8), (O.NW, 16), (O.NW, 8)]:
+            check(t, bw, O.AMINOACID, 1, qraw, db, 20, ("builtin", "blosum62"), (-10, -1))
+    finally:
+        ssa.set_rows_per_strip(0)
