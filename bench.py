@@ -1,0 +1,183 @@
+"""Benchmark of the MI355X database search (one GPU).
+
+Default workload (BASELINE configuration "SW 8-bit with the overflow cascade"):
+200,000 synthetic protein sequences (lognormal lengths, median 300, sigma 0.6,
+cap 35,000; 5 % planted query fragments at 60-95 % identity), queries of 50,
+500 and 5,000 residues, BLOSUM62, gaps open 10 / extend 1, top 50 hits, 8-bit
+start with the 16/64-bit cascade, scores only.  One step = the three queries
+searched through sw_align with the DB resident on the GPU.
+
+    python bench.py [--steps K] [--warmup W] [--workload c2|c1|c3] [--no-cpu-baseline]
+
+Prints one JSON line.  value = algorithmic cells (sum over queries and DB
+entries of |q| * |d|) per second of the whole step, in GCUPS.
+"""
+import argparse
+import ctypes as C
+import json
+import os
+import sys
+import tempfile
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+PEAK_VALU_TOPS = 256 * 64 * 2 * 2.4e9 / 1e12  # packed int16 element-ops/s: 256 CU x 64 lanes x 2 x 2.4 GHz
+OPS_PER_CELL = {"sw": 9, "nw": 8}             # algorithmic int ops per cell of the reference recurrence
+
+
+def workload(name):
+    from libssa_amd import synthetic as S
+    if name == "c2":
+        qs, db, _ = S.config_c2()
+        return dict(kind="sw", queries=qs, db=db, k=50, bw=8, symtype=1, strands=1, matrix="blosum62",
+                    gaps=(-10, -1), desc="sw 8-bit cascade, 200k lognormal protein DB, q=50/500/5000, k=50")
+    if name == "c1":
+        qs, db, _ = S.config_c1()
+        return dict(kind="sw", queries=qs, db=db, k=10, bw=16, symtype=1, strands=1, matrix="blosum62",
+                    gaps=(-10, -1), desc="sw 16-bit, 2k uniform protein DB, q=120, k=10")
+    if name == "c3":
+        qs, db, _ = S.config_c3()
+        return dict(kind="nw", queries=qs, db=db, k=100, bw=16, symtype=0, strands=3, const=(2, -4),
+                    gaps=(-20, -2), align=True,
+                    desc="nw 16-bit DNA both strands with CIGARs, 1M DB, q=250, k=100")
+    raise SystemExit("unknown workload " + name)
+
+
+def cpu_baseline(w, sample_frac):
+    """The reference's own AVX2 8-bit kernels (oracle/_ref) on the host cores."""
+    so = os.path.join(ROOT, "oracle", "_ref", "libssaref.so")
+    if not os.path.exists(so) or w["kind"] != "sw" or w["bw"] != 8:
+        return None, None
+    ref = C.CDLL(so)
+    ref.ref_bench_sw8.restype = C.c_double
+    ref.ref_bench_sw8.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t, C.c_char_p, C.c_size_t,
+                                  C.c_size_t, C.c_size_t, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]
+    ref.ref_matrix_builtin.argtypes = [C.c_char_p]
+    ref.ref_set_gaps.argtypes = [C.c_int, C.c_int]
+    ref.ref_matrix_builtin(w["matrix"].encode())
+    ref.ref_set_gaps(*w["gaps"])
+    lut = np.zeros(256, np.uint8)
+    for i, ch in enumerate("-ABCDEFGHIKLMNPQRSTVWXYZU*OJ"):
+        if ch != "-":
+            lut[ord(ch)] = i
+            lut[ord(ch.lower())] = i
+    n = max(1, int(len(w["db"]) * sample_frac))
+    sample = w["db"][:n]
+    lens = np.array([len(s) for s in sample], np.int64)
+    offs = np.concatenate([[0], np.cumsum(lens)[:-1]]).astype(np.int64)
+    codes = lut[np.frombuffer(b"".join(sample), np.uint8)]
+    threads = min(16, os.cpu_count() or 1)
+    k = w["k"]
+    cells = 0
+    secs = 0.0
+    tops = []
+    for q in w["queries"]:
+        qc = lut[np.frombuffer(q, np.uint8)].tobytes()
+        ids = np.zeros(k, np.int64)
+        sc = np.zeros(k, np.int64)
+        cnt = C.c_size_t()
+        secs += ref.ref_bench_sw8(codes.ctypes.data, offs.ctypes.data, lens.ctypes.data, n, qc, len(qc), k, 1000,
+                                  threads, ids.ctypes.data, sc.ctypes.data, C.byref(cnt))
+        cells += len(q) * int(lens.sum())
+        tops.append(sorted(sc[:cnt.value].tolist(), reverse=True))
+    return dict(value=round(cells / secs / 1e9, 2), unit="GCUPS", cores=threads, kind="reference",
+                sample="%d of %d DB sequences (the first), all %d queries, reference AVX2 8-bit kernels with "
+                       "16/64-bit cascade, %d threads, chunk 1000; %.2f s" % (n, len(w["db"]), len(w["queries"]),
+                                                                             threads, secs)), (n, tops)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", default="c2")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-sample", type=float, default=0.25)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py runs on one GPU only: this build has no multi-GPU path (--gpus %d requested)" % args.gpus)
+        sys.exit(2)
+
+    import libssa_amd as ssa
+    w = workload(args.workload)
+    ssa.set_output_mode(ssa.OUTPUT_ERROR)
+    ssa.init_symbol_translation(w["symtype"], w["strands"], 1, 1)
+    if "const" in w:
+        ssa.init_constant_scores(*w["const"])
+    else:
+        ssa.init_score_matrix(ssa.MATRIX_BUILDIN, w["matrix"])
+    ssa.init_gap_penalties(*w["gaps"])
+    tmp = tempfile.mkdtemp(prefix="libssa_bench_")
+    path = os.path.join(tmp, "db.fasta")
+    from libssa_amd import synthetic as S
+    S.write_fasta(path, w["db"])
+    ssa.init_db(path)
+    qh = [ssa.init_sequence_fasta(ssa.READ_FROM_STRING, q.decode()) for q in w["queries"]]
+    align = ssa.COMPUTE_ALIGNMENT if w.get("align") else ssa.COMPUTE_SCORE
+    search = ssa.sw_align if w["kind"] == "sw" else ssa.nw_align
+
+    def step():
+        st = []
+        tops = []
+        for h in qh:
+            hits = search(h, w["k"], w["bw"], align)
+            st.append(ssa.last_stats())
+            tops.append([a.score for a in hits])
+        return st, tops
+
+    for _ in range(args.warmup):
+        step()
+    t0 = time.perf_counter()
+    stats = []
+    for _ in range(args.steps):
+        st, tops = step()
+        stats.extend(st)
+    t1 = time.perf_counter()
+    os.remove(path)
+    os.rmdir(tmp)
+
+    secs = t1 - t0
+    cells_step = sum(s.cells for s in stats) / args.steps
+    gcups = cells_step * args.steps / secs / 1e9
+    kernel_ms = sum(s.ms_gpu_first for s in stats)
+    ops = sum(s.cells for s in stats) * OPS_PER_CELL[w["kind"]]
+    achieved = ops / (kernel_ms / 1e3) / 1e12
+    out = {
+        "metric": "SW/NW GCUPS on 1 MI355X",
+        "value": round(gcups, 2),
+        "unit": "GCUPS",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(secs / args.steps * 1e3, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "int16",
+        "data": "synthetic",
+        "config": {"workload": w["desc"], "db_sequences": len(w["db"]),
+                   "query_lengths": [len(q) for q in w["queries"]], "hits": w["k"], "bit_width": w["bw"]},
+        "roofline": {"bound": "valu", "achieved": round(achieved, 3), "peak": round(PEAK_VALU_TOPS, 1),
+                     "unit": "TOP/s", "frac": round(achieved / PEAK_VALU_TOPS, 4), "traffic": None,
+                     "kernel": "score_kernel (first tier)", "ops_per_cell": OPS_PER_CELL[w["kind"]],
+                     "kernel_ms_per_step": round(kernel_ms / args.steps, 3)},
+        "breakdown_ms_per_step": {
+            "gpu_first_tier": round(kernel_ms / args.steps, 3),
+            "gpu_requeue": round(sum(s.ms_gpu_requeue for s in stats) / args.steps, 3),
+            "host_replay": round(sum(s.ms_replay for s in stats) / args.steps, 3),
+            "traceback": round(sum(s.ms_traceback for s in stats) / args.steps, 3)},
+        "cpu_baseline": None,
+    }
+    if not args.no_cpu_baseline:
+        cb, check = cpu_baseline(w, args.cpu_sample)
+        out["cpu_baseline"] = cb
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -197,3 +197,122 @@ int ref_align(int type, const char* a, size_t alen, const char* b, size_t blen, 
     free(c);
     return (int)n;
 }
+
+/* ---- CPU baseline: the reference's AVX2 kernels over a whole DB ----
+ * T worker threads pull chunks of `chunk` sequences from a shared counter,
+ * run the 8-bit kernel for the query, the 16-bit kernel on the chunk's
+ * overflow and the 64-bit function on what overflows again, each into its own
+ * heap; the heaps are merged in thread order and sorted.  This is the
+ * structure of src/algo/manager.c:120-185, src/algo/searcher.c:155-174 and
+ * src/algo/8/search_8.c:94-124 around the reference's own kernels.  Only the
+ * SW search at 8 bit (the benchmark configuration) is driven.
+ * Returns the wall time of the search in seconds; writes the hits. */
+#include <pthread.h>
+#include <time.h>
+
+typedef struct {
+    const char* codes;
+    const int64_t* offs;
+    const int64_t* lens;
+    size_t n;
+    const char* q;
+    size_t qlen;
+    size_t k;
+    size_t chunk;
+    size_t next;
+    pthread_mutex_t mu;
+} bench_ctx;
+
+static void* bench_worker(void* arg) {
+    bench_ctx* c = (bench_ctx*)arg;
+    search_data_t sd;
+    memset(&sd, 0, sizeof sd);
+    sd.maxqlen = c->qlen;
+    sd.q_count = 1;
+    sd.queries[0].seq.seq = (char*)c->q;
+    sd.queries[0].seq.len = c->qlen;
+    p_s8info s8 = search_8_avx2_init(&sd);
+    p_s16info s16 = search_16_avx2_init(&sd);
+    p_minheap heap = minheap_init(c->k);
+    sdb_sequence_t* store = (sdb_sequence_t*)calloc(c->chunk, sizeof(sdb_sequence_t));
+    db_chunk_t chunk, ovf, ovf2;
+    chunk.seq = (p_sdb_sequence*)calloc(c->chunk, sizeof(p_sdb_sequence));
+    ovf.seq = (p_sdb_sequence*)calloc(c->chunk, sizeof(p_sdb_sequence));
+    ovf2.seq = (p_sdb_sequence*)calloc(c->chunk, sizeof(p_sdb_sequence));
+    chunk.size = ovf.size = ovf2.size = c->chunk;
+    int64_t* he = (int64_t*)calloc(2 * c->qlen + 2, sizeof(int64_t));
+    sequence_t qs = { (char*)c->q, c->qlen };
+    for (;;) {
+        pthread_mutex_lock(&c->mu);
+        size_t start = c->next;
+        c->next += c->chunk;
+        pthread_mutex_unlock(&c->mu);
+        if (start >= c->n) break;
+        size_t stop = start + c->chunk < c->n ? start + c->chunk : c->n;
+        chunk.fill_pointer = 0;
+        for (size_t i = start; i < stop; i++) {
+            if (c->lens[i] == 0) continue;
+            sdb_sequence_t* s = &store[chunk.fill_pointer];
+            s->ID = i;
+            s->seq.seq = (char*)c->codes + c->offs[i];
+            s->seq.len = (size_t)c->lens[i];
+            s->strand = s->frame = 0;
+            chunk.seq[chunk.fill_pointer++] = s;
+        }
+        ovf.fill_pointer = 0;
+        search_8_avx2_sw(s8, &chunk, heap, &ovf, 0);
+        if (ovf.fill_pointer) {
+            ovf2.fill_pointer = 0;
+            search_16_avx2_sw(s16, &ovf, heap, &ovf2, 0);
+            for (size_t i = 0; i < ovf2.fill_pointer; i++) {
+                long sc = (long)full_sw(&ovf2.seq[i]->seq, &qs, he);
+                add_to_minheap(heap, 0, ovf2.seq[i], sc);
+            }
+        }
+    }
+    free(he);
+    free(store);
+    free(chunk.seq);
+    free(ovf.seq);
+    free(ovf2.seq);
+    return heap;
+}
+
+double ref_bench_sw8(const char* codes, const int64_t* offs, const int64_t* lens, size_t n, const char* q,
+                     size_t qlen, size_t k, size_t chunk, int threads, int64_t* out_ids, int64_t* out_scores,
+                     size_t* out_n) {
+    bench_ctx c;
+    c.codes = codes;
+    c.offs = offs;
+    c.lens = lens;
+    c.n = n;
+    c.q = q;
+    c.qlen = qlen;
+    c.k = k;
+    c.chunk = chunk;
+    c.next = 0;
+    pthread_mutex_init(&c.mu, NULL);
+    pthread_t* th = (pthread_t*)calloc((size_t)threads, sizeof(pthread_t));
+    struct timespec t0, t1;
+    clock_gettime(CLOCK_MONOTONIC, &t0);
+    for (int t = 0; t < threads; t++) pthread_create(&th[t], NULL, bench_worker, &c);
+    p_minheap all = minheap_init(k);
+    for (int t = 0; t < threads; t++) {
+        void* r;
+        pthread_join(th[t], &r);
+        p_minheap h = (p_minheap)r;
+        for (size_t j = 0; j < h->count; j++) minheap_add(all, &h->array[j]);
+        minheap_exit(h);
+    }
+    minheap_sort(all);
+    clock_gettime(CLOCK_MONOTONIC, &t1);
+    *out_n = all->count;
+    for (size_t i = 0; i < all->count; i++) {
+        out_ids[i] = (int64_t)all->array[i].db_id;
+        out_scores[i] = all->array[i].score;
+    }
+    minheap_exit(all);
+    free(th);
+    pthread_mutex_destroy(&c.mu);
+    return (double)(t1.tv_sec - t0.tv_sec) + 1e-9 * (double)(t1.tv_nsec - t0.tv_nsec);
+}
