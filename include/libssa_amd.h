@@ -25,7 +25,8 @@ typedef struct {
     /* work and time of the last sw_align / nw_align call */
     uint64_t cells;           /* sum over (query, DB entry) pairs of |q| * |d| in the first tier */
     uint64_t cells_computed;  /* cells the GPU computed in all tiers, padding included */
-    int rows_per_strip;       /* query rows per VGPR strip used by the first tier */
+    int lanes_per_stream;     /* score kernel shape of the first tier: lanes per stream pair */
+    int rows_per_lane;        /* and query rows per lane */
     double ms_gpu_first;      /* first-tier kernel time (HIP events) */
     double ms_gpu_requeue;    /* re-queued tiers (16-bit pass, 32-bit exact pass) */
     double ms_replay;         /* host replay of the reference's lane schedule and heap */
@@ -36,9 +37,13 @@ typedef struct {
 /* Counters of the last search. */
 void ssa_amd_last_stats(ssa_amd_stats_t* out);
 
-/* Query rows per VGPR strip of the score kernel: 0 chooses automatically,
- * otherwise one of 8, 16, 24, 32, 48. */
-void ssa_amd_set_rows_per_strip(int rows);
+/* Shape of the score kernel: lanes per stream pair (8, 16, 32 or 64) and
+ * query rows per lane (4, 8 or 16); 0, 0 chooses from the query length. */
+void ssa_amd_set_kernel_shape(int lanes_per_stream, int rows_per_lane);
+
+/* Number of waves the DB's pair streams are cut into (0 = the default, 10240).
+ * Streams never get shorter than 16 blocks per lane of a group. */
+void ssa_amd_set_stream_waves(int waves);
 
 /* The current 32x32 score matrix as the search uses it ([db code * 32 + query code]). */
 void ssa_amd_get_matrix(int64_t* out);

@@ -55,7 +55,7 @@ class Stats(_C.Structure):
     """ssa_amd_stats_t (include/libssa_amd.h)."""
     _fields_ = [("chunk_count", _C.c_size_t), ("seq_count", _C.c_size_t),
                 ("overflow_8_bit_count", _C.c_size_t), ("overflow_16_bit_count", _C.c_size_t),
-                ("cells", _C.c_uint64), ("cells_computed", _C.c_uint64), ("rows_per_strip", _C.c_int),
+                ("cells", _C.c_uint64), ("cells_computed", _C.c_uint64), ("lanes_per_stream", _C.c_int), ("rows_per_lane", _C.c_int),
                 ("ms_gpu_first", _C.c_double), ("ms_gpu_requeue", _C.c_double), ("ms_replay", _C.c_double),
                 ("ms_traceback", _C.c_double), ("ms_total", _C.c_double)]
 
@@ -92,7 +92,7 @@ def _load():
         "sw_align": (_C.POINTER(_AlignmentList), [P, S, I, I]), "nw_align": (_C.POINTER(_AlignmentList), [P, S, I, I]),
         "free_alignment": (None, [_C.POINTER(_AlignmentList)]), "ssa_exit": (None, []),
         "ssa_db_init": (I, [cp]), "ssa_db_get_sequence_count": (S, []), "ssa_db_close": (None, []),
-        "ssa_amd_last_stats": (None, [_C.POINTER(Stats)]), "ssa_amd_set_rows_per_strip": (None, [I]),
+        "ssa_amd_last_stats": (None, [_C.POINTER(Stats)]), "ssa_amd_set_kernel_shape": (None, [I, I]), "ssa_amd_set_stream_waves": (None, [I]),
         "ssa_amd_get_matrix": (None, [P]), "ssa_amd_device_arch": (cp, []),
     }
     for name, (res, args) in sig.items():
@@ -175,7 +175,10 @@ def last_stats():
     return s
 
 
-def set_rows_per_strip(rows): lib.ssa_amd_set_rows_per_strip(rows)
+def set_kernel_shape(lanes_per_stream, rows_per_lane): lib.ssa_amd_set_kernel_shape(lanes_per_stream, rows_per_lane)
+
+
+def set_stream_waves(waves): lib.ssa_amd_set_stream_waves(waves)
 
 
 def get_matrix():

@@ -137,10 +137,20 @@ void ssa_amd_last_stats(ssa_amd_stats_t* out) {
     if (out) *out = last_stats();
 }
 
-void ssa_amd_set_rows_per_strip(int rows) {
-    if (rows != 0 && rows != 8 && rows != 16 && rows != 24 && rows != 32 && rows != 48)
-        fatal("libssa_amd: rows per strip must be 0, 8, 16, 24, 32 or 48 (got %d)", rows);
-    rows_per_strip_override() = rows;
+void ssa_amd_set_kernel_shape(int lanes_per_stream, int rows_per_lane) {
+    const bool autosel = lanes_per_stream == 0 && rows_per_lane == 0;
+    const bool lanes_ok = lanes_per_stream == 8 || lanes_per_stream == 16 || lanes_per_stream == 32 ||
+                          lanes_per_stream == 64;
+    const bool rows_ok = rows_per_lane == 4 || rows_per_lane == 8 || rows_per_lane == 16;
+    if (!autosel && !(lanes_ok && rows_ok))
+        fatal("libssa_amd: kernel shape must be 0, 0 or lanes in {8, 16, 32, 64} and rows in {4, 8, 16} (got %d, %d)",
+              lanes_per_stream, rows_per_lane);
+    set_shape_override(lanes_per_stream, rows_per_lane);
+}
+
+void ssa_amd_set_stream_waves(int waves) {
+    if (waves < 0) fatal("libssa_amd: stream waves must be >= 0 (got %d)", waves);
+    set_target_waves(waves);
 }
 
 void ssa_amd_get_matrix(int64_t* out) { memcpy(out, matrix().s, sizeof(matrix().s)); }

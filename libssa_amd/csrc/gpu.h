@@ -61,10 +61,12 @@ private:
 };
 
 // ---- kernel launchers (score_kernels.hip, traceback_kernels.hip, pack_kernels.hip) ----
-hipError_t launch_score(int rows, int mode, int ngroups, hipStream_t st, const ScoreGroup* groups,
+bool score_shape_supported(int lanes, int rows);
+hipError_t launch_score(int lanes, int rows, int mode, int nwg, hipStream_t st, const ScoreWave* waves,
                         const QueryDesc* queries, const int16_t* qp, const uint32_t* lb, const uint2* res, uint4* bnd,
                         const int32_t* slot_begin, const int32_t* pair_list, PairResult* results,
                         const ScoreParams& P);
+hipError_t launch_init_results(hipStream_t st, PairResult* r, int64_t n, int32_t score);
 hipError_t launch_exact(int npairs, hipStream_t st, const ExactPair* pairs, const QueryDesc* queries,
                         const int32_t* qp32, const uint8_t* codes, int mode, int go, int ge, int32_t* scratch,
                         int64_t scratch_stride, int64_t* out);
@@ -72,7 +74,7 @@ hipError_t launch_traceback(int ntasks, hipStream_t st, const TraceTask* tasks, 
                             const int64_t* mat, int go, int ge, uint8_t* dirs, int64_t* scratch, int32_t* ops,
                             TraceResult* out);
 
-hipError_t launch_pack(int ngroups, hipStream_t st, const ScoreGroup* groups, const PackSlot* slots,
-                       const uint8_t* codes, uint2* res);
+hipError_t launch_pack(int nentries, hipStream_t st, const PackEntry* entries, const uint8_t* codes,
+                       uint32_t* res, int64_t res_words);
 
 }  // namespace ssa

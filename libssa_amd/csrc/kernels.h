@@ -8,37 +8,41 @@ namespace ssa {
 // ---------------------------------------------------------------------------
 // Score kernels (score_kernels.hip)
 //
-// One wave processes one group: 64 lanes x 2 halves = 128 slots.  A slot is a
-// stream of (query, DB entry) pairs laid end to end, each padded to whole
-// 4-column blocks.  All slots of a group share the query.
+// A stream is a run of (query, DB entry) pairs laid end to end, each padded to
+// whole 4-column blocks.  A lane group of G lanes scores two streams at once
+// (the lo and hi int16 halves of every lane): lane g of the group owns query
+// rows [p*G*R + g*R, +R) of pass p and runs one block behind lane g-1, which
+// hands it the H and F of its bottom row.  A wave holds 64/G lane groups; a
+// workgroup holds kWavesPerWG waves of the same query, which share the pass's
+// score profile in LDS.
 //
-// Interleaved residue layout of a group: uint2 res[T][64], .x = 4 codes of the
-// lo-half slot for that block, .y = the hi-half slot.  Byte k of a word holds
-// column k's code times two (an LDS byte offset into a profile row) in bits
-// 0-5.  Flag bits: byte 0 bit 7 = a pair starts at this block, byte 1 bit 7 =
-// a pair ends in this block, byte 2 bits 6-7 = column of the pair's last
-// residue in that block.
+// Residue layout of a wave: uint2 res[nblocks][64/G], .x = 4 codes of the
+// group's lo stream for that block, .y = the hi stream.  Byte k of a word
+// holds column k's code times two in bits 0-5.  Flag bits: byte 0 bit 7 = a
+// pair starts at this block, byte 1 bit 7 = a pair ends in this block, byte 2
+// bits 6-7 = column of the pair's last residue in that block.
 // ---------------------------------------------------------------------------
 constexpr uint32_t kResStart = 0x80u;
 constexpr uint32_t kResLast = 0x8000u;
 constexpr int kResZShift = 22;  // (word >> 22) & 3
 
-constexpr int kSlotsPerGroup = 128;
+constexpr int kWavesPerWG = 4;
 constexpr int kNoFlag = 0x7fffffff;
 
-struct ScoreGroup {
-    int32_t query;      // index into the QueryDesc table
-    int32_t nblocks;    // T
+struct ScoreWave {
+    int32_t query;      // index into the QueryDesc table (the same for all waves of a workgroup)
+    int32_t nblocks;    // blocks of the wave's longest stream (0 = an idle filler wave)
     int64_t res_off;    // in uint2 units
-    int64_t bnd_off;    // in uint4 units; 2 x uint4 per lane per block
-    int32_t slot_base;  // index of this group's first slot in slot_begin
+    int64_t bnd_off;    // in uint4 units: bnd[nblocks][64/G][2], the last lane's bottom row between passes
+    int32_t slot_base;  // stream (group k, half h) of this wave: slot_begin[slot_base + 2k + h]
     int32_t pad;
 };
 
 struct QueryDesc {
     int32_t qlen;
-    int32_t qp_off;  // in int16 units: profile rows [qlen][32]
-    int32_t lb_off;  // in uint32 units: NW left boundary [qlen][2] packed (H, E)
+    int32_t qp_off;  // score kernel: int16 units, profile [pass][32 codes][G*R rows];
+                     // exact tier: int32 units, profile [qlen][32 codes]
+    int32_t lb_off;  // in uint32 units: NW left boundary [pass*G*R rows][2] packed (H, E)
     int32_t pad;
 };
 
@@ -57,15 +61,19 @@ struct ScoreParams {
     uint32_t fb_init[4];  // NW: top boundary F of the first block's four columns
 };
 
-// one slot of a group for the interleaving kernel (one pair per slot)
-struct PackSlot {
+// one (stream, entry) of the residue interleaving kernel
+struct PackEntry {
     int64_t code_off;  // byte offset of the entry's codes (padded to 4) in the DB code buffer
-    int32_t nblocks;   // blocks of the entry, 0 = empty slot
+    int64_t dst;       // uint32 index of the entry's first word in the residue array
+    int32_t nblocks;   // blocks of the entry
     int32_t last_col;  // (length - 1) % 4
+    int32_t stride;    // uint32 distance between the words of consecutive blocks
+    int32_t pad;
 };
 
-// result per pair: x = score in representation (SW: running max, NW: last
-// cell), y = first flagged block (kNoFlag if none), z = second SW flag block
+// result per pair: score in representation (SW: maximum cell, NW: last cell),
+// first flagged block (kNoFlag if none), second SW flag block.  The SW fields
+// are combined over lanes and passes with atomic max / min.
 struct PairResult {
     int32_t score;
     int32_t flag_a;

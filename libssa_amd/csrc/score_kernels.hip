@@ -1,27 +1,43 @@
 // score_kernels.hip -- gfx950 kernels of the database search.
 //
-// score_kernel<R, MODE>: inter-sequence SW (MODE 0) / NW (MODE 1) over packed
-// int16 lanes.  Restates the per-lane arithmetic of the reference's width
-// kernels (src/algo/simd/search_simd_sw.c:172-277, search_simd_nw.c:180-312):
+// score_kernel<R, G, MODE>: SW (MODE 0) / NW (MODE 1) over packed int16 lanes.
+// Restates the per-lane arithmetic of the reference's width kernels
+// (src/algo/simd/search_simd_sw.c:172-277, search_simd_nw.c:180-312):
 // saturating adds, the H/E/F recurrence and the overflow predicates, but not
-// their schedule: each lane half owns a fixed stream of pairs and the host
-// replays the reference's lane schedule from the per-pair results.
+// their schedule: the host replays the reference's lane schedule from the
+// per-pair results.
 //
-// Work shape: one wave = 64 lanes x 2 halves = 128 pair streams sharing one
-// query.  The query is cut into strips of R rows; a strip's H and E columns
-// live in VGPRs (2 x R packed registers), DB columns stream through in blocks
-// of four.  The strip's score profile is staged in LDS and read with the DB
-// code as a per-lane byte offset and the row as an immediate offset.  Strip
-// boundaries (H and F of the strip's last row, per column) go through HBM.
+// Work shape: a lane group of G lanes scores two streams of pairs (the lo and
+// hi halves of the packed registers).  The query is cut into passes of G*R
+// rows; lane g of the group owns R consecutive rows of the pass, keeps their H
+// and E in VGPRs and walks the streams one 4-column block behind lane g-1,
+// which hands it the H and F of its bottom row through a DPP wave shift.  The
+// top lane of a group takes its top boundary from the matrix border (first
+// pass) or from the bottom lane of the previous pass through HBM.  A block
+// costs each lane 4*R cells per half, so the cross-lane traffic is 9 DPP moves
+// per 8*R cells.
 //
-// The host pads the query to whole strips.  Padding rows sit below the last
+// The pass's score profile is staged in LDS as [code][row] int16; a lane reads
+// the R scores of its rows for one DB code with one 16-byte (R = 8) read per
+// half and interleaves the two halves with v_perm.  A lane runs one block per
+// step; lane g is idle for the first g steps of a pass (and the last G-1-g),
+// so a pass takes nblocks + G - 1 steps.
+//
+// Per-pair results combine over lanes and passes: SW takes the maximum cell
+// (atomic max) and the first flagged block (atomic min); a block is flagged
+// when any row of it is.  The NW score is the last real row's cell at the
+// pair's last column, written by the lane holding that row.
+//
+// The host pads the query to whole passes.  Padding rows sit below the last
 // query row, so no real cell depends on them, and their profile values keep
 // them from changing any result: SW pads with the lane minimum (a padded cell
 // never exceeds the real maximum), NW pads with 0 and resets padded E and H
 // to 0 at each pair start (a padded cell stays between min(0, earlier real
 // cells) and max(0, earlier real cells), so it raises no overflow flag the
-// real cells do not).  The NW score is read from the last real row.
+// real cells do not).
 #include <hip/hip_runtime.h>
+
+#include <algorithm>
 
 #include "kernels.h"
 
@@ -38,196 +54,275 @@ __device__ __forceinline__ s16x2 vmin(s16x2 a, s16x2 b) { return __builtin_eleme
 __device__ __forceinline__ s16x2 sel(uint32_t m, s16x2 a, s16x2 b) {
     return as_v((m & as_u(a)) | (~m & as_u(b)));
 }
+__host__ __device__ __forceinline__ uint32_t pack2d(int v) { return ((uint32_t)(uint16_t)v << 16) | (uint16_t)v; }
 __device__ __forceinline__ int lo16(s16x2 v) { return (int)(short)(as_u(v) & 0xffffu); }
 __device__ __forceinline__ int hi16(s16x2 v) { return (int)(short)(as_u(v) >> 16); }
-
-template <int MODE>
-__device__ __forceinline__ void finish_pair(PairResult* results, int pair, bool first, bool last, int score,
-                                            int fa, int fb) {
-    PairResult o;
-    if (first) {
-        o.score = score;
-        o.flag_a = fa;
-        o.flag_b = fb;
+// Moves each lane's value one lane down within its group of G lanes: lane g
+// receives lane g-1's value and the group's top lane receives `top`.  G = 64
+// uses the wave shift and G = 16 the row shift, whose vacated lane takes the
+// old value; G = 8 and 32 shift across group boundaries and fix the top lanes.
+template <int G>
+__device__ __forceinline__ uint32_t shift_down(uint32_t top, uint32_t v, bool top_lane) {
+    if constexpr (G == 64) {
+        return (uint32_t)__builtin_amdgcn_update_dpp((int)top, (int)v, 0x138, 0xf, 0xf, false);  // wave_shr:1
+    } else if constexpr (G == 16) {
+        return (uint32_t)__builtin_amdgcn_update_dpp((int)top, (int)v, 0x111, 0xf, 0xf, false);  // row_shr:1
+    } else if constexpr (G == 32) {
+        const uint32_t s = (uint32_t)__builtin_amdgcn_update_dpp((int)top, (int)v, 0x138, 0xf, 0xf, false);
+        return top_lane ? top : s;
     } else {
-        const PairResult old = results[pair];
-        o.score = MODE == 0 ? max(old.score, score) : (last ? score : old.score);
-        o.flag_a = min(old.flag_a, fa);
-        o.flag_b = min(old.flag_b, fb);
+        const uint32_t s = (uint32_t)__builtin_amdgcn_update_dpp((int)top, (int)v, 0x111, 0xf, 0xf, false);
+        return top_lane ? top : s;
     }
-    o.pad = 0;
-    results[pair] = o;
 }
 
-template <int R, int MODE>
-__global__ __launch_bounds__(64) void score_kernel(const ScoreGroup* __restrict__ groups,
-                                                   const QueryDesc* __restrict__ queries,
-                                                   const int16_t* __restrict__ qp, const uint32_t* __restrict__ lb,
-                                                   const uint2* __restrict__ res, uint4* __restrict__ bnd,
-                                                   const int32_t* __restrict__ slot_begin,
-                                                   const int32_t* __restrict__ pair_list,
-                                                   PairResult* __restrict__ results, const ScoreParams P) {
-    __shared__ int16_t sqp[R * 32];
-    __shared__ uint32_t slb[2 * R + 1];
+template <int N>
+struct Log2 {
+    static constexpr int value = 1 + Log2<N / 2>::value;
+};
+template <>
+struct Log2<1> {
+    static constexpr int value = 0;
+};
 
-    const int lane = threadIdx.x;
-    const ScoreGroup G = groups[blockIdx.x];
-    const QueryDesc Q = queries[G.query];
-    const int qlen = Q.qlen;                   // real rows
-    const int nstrips = (qlen + R - 1) / R;    // profile rows are padded to nstrips * R
-    const int last_row = qlen - 1 - (nstrips - 1) * R;  // last real row within the last strip
-    const uint2* __restrict__ gres = res + G.res_off;
-    uint4* __restrict__ gbnd = bnd + G.bnd_off;
-    const int sb_lo = slot_begin[G.slot_base + 2 * lane];
-    const int sb_hi = slot_begin[G.slot_base + 2 * lane + 1];
+// R profile scores of one lane for one DB code: R/2 dwords, rows (2m, 2m+1) in dword m
+template <int R>
+__device__ __forceinline__ void read_rows(const char* p, uint32_t* d) {
+    if constexpr (R == 4) {
+        const uint2 v = *reinterpret_cast<const uint2*>(p);
+        d[0] = v.x;
+        d[1] = v.y;
+    } else {
+#pragma unroll
+        for (int q = 0; q < R / 8; ++q) {
+            const uint4 v = *reinterpret_cast<const uint4*>(p + 16 * q);
+            d[4 * q + 0] = v.x;
+            d[4 * q + 1] = v.y;
+            d[4 * q + 2] = v.z;
+            d[4 * q + 3] = v.w;
+        }
+    }
+}
 
+template <int MODE>
+__device__ __forceinline__ void finish_pair(PairResult* results, int pair, bool score_lane, int score, int fa,
+                                            int fb) {
+    if (MODE == 0) {
+        atomicMax(&results[pair].score, score);
+    } else if (score_lane) {
+        results[pair].score = score;
+    }
+    if (fa != kNoFlag) atomicMin(&results[pair].flag_a, fa);
+    if (MODE == 0 && fb != kNoFlag) atomicMin(&results[pair].flag_b, fb);
+}
+
+// One pass of a wave over its streams: nblocks + G - 1 steps of one block per
+// lane, rounded up to an even count (the extra step is idle on every lane) so
+// that the loop runs two steps per iteration and the one-step-ahead loads
+// alternate between two register sets.
+template <int R, int G, int MODE, bool FIRST>
+__device__ __forceinline__ void run_pass(const ScoreParams& P, uint32_t gbase, const char* sp,
+                                         const uint32_t* slbh, const uint32_t* slbe,
+                                         const uint2* __restrict__ wres, uint4* __restrict__ wbnd,
+                                         const int32_t* __restrict__ pair_list, PairResult* __restrict__ results,
+                                         int sb_lo, int sb_hi, int k, int g, int nb, bool last_pass, int last_g,
+                                         int last_r) {
+    constexpr int NG = 64 / G;
+    constexpr int LPR = Log2<G * R>::value;
     const s16x2 QR = as_v(P.qr), RR = as_v(P.r), Z = as_v(P.zero);
+    // per-half thresholds of the overflow predicates; a half whose predicate
+    // cannot fire (flag taken, or no pair running) holds the neutral value
+    const uint32_t TA_on = MODE == 0 ? pack2d(P.thr_a - 1) : pack2d(P.thr_a);
+    const uint32_t TB_on = pack2d(P.thr_b - 1 > 32767 ? 32767 : P.thr_b - 1);
+    const uint32_t TA_off = MODE == 0 ? 0x7fff7fffu : 0x80008000u;
+    const uint32_t TB_off = 0x7fff7fffu;
 
-    for (int s = 0; s < nstrips; ++s) {
-        const int i0 = s * R;
-        const bool first = (s == 0), last = (s == nstrips - 1);
+    const bool top_lane = g == 0;
+    const bool top_in = top_lane && !FIRST;         // top boundary from the previous pass
+    const bool bot_out = g == G - 1 && !last_pass;  // bottom row to the next pass
+    const bool score_lane = last_pass && g == last_g;
+    const int nsteps = nb > 0 ? ((nb + G) & ~1) : 0;
 
-        __syncthreads();
-        {
-            const uint32_t* src = reinterpret_cast<const uint32_t*>(qp + Q.qp_off) + (int64_t)i0 * 16;
-            uint32_t* dst = reinterpret_cast<uint32_t*>(sqp);
-            for (int k = lane; k < R * 16; k += 64) dst[k] = src[k];
-            if (MODE == 1) {
-                for (int k = lane; k < 2 * R; k += 64) slb[k] = lb[Q.lb_off + 2 * i0 + k];
-                if (lane == 0) slb[2 * R] = (i0 > 0) ? lb[Q.lb_off + 2 * (i0 - 1)] : 0u;
+    s16x2 Hl[R], El[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        Hl[r] = Z;
+        El[r] = Z;
+    }
+    s16x2 S = Z;     // SW: running maximum of the lane's cells of the current pair
+    s16x2 htop = Z;  // H of the row above at the previous column
+    s16x2 cap = Z;   // NW: the last row's H at the pair's last column
+    s16x2 hb[4], fb[4];  // NW: the matrix's top border over the current block (top lane, first pass)
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+        hb[c] = as_v(P.hb_init[c]);
+        fb[c] = as_v(P.fb_init[c]);
+    }
+    uint32_t obh[4] = {0, 0, 0, 0}, obf[4] = {0, 0, 0, 0};  // bottom row H, F of the last block
+    uint32_t TA = TA_off, TB = TB_off;
+    int kp_lo = 0, kp_hi = 0, pair_lo = -1, pair_hi = -1, blk_lo = 0, blk_hi = 0;
+    int fa_lo = kNoFlag, fa_hi = kNoFlag, fb_lo = kNoFlag, fb_hi = kNoFlag;
+
+    const uint2* wp = wres + (int64_t)(-g) * NG + k;  // residue word of block t - g
+    const uint4* tp = wbnd + (int64_t)k * 2;          // top lane: boundary of block t
+
+    auto step = [&](int t, const uint2 w, const uint4 t0, const uint4 t1, uint2& wn, uint4& tn0, uint4& tn1) {
+        const int b = t - g;
+        const bool valid = (unsigned)b < (unsigned)nb;
+        wp += NG;
+        tp += 2 * NG;
+        if ((unsigned)(b + 1) < (unsigned)nb) {
+            wn = wp[0];
+            if (top_in) {
+                tn0 = tp[0];
+                tn1 = tp[1];
             }
+        } else {
+            wn = make_uint2(0, 0);
         }
-        __syncthreads();
 
-        s16x2 Hl[R], El[R];
-#pragma unroll
-        for (int r = 0; r < R; ++r) {
-            Hl[r] = Z;
-            El[r] = Z;
-        }
-        s16x2 S = Z;     // SW: running maximum of the strip's cells
-        s16x2 htop = Z;  // strips after the first: H(i0-1, j-1) for the next column
-        s16x2 cap = Z;   // NW: last-row H at the pair's last column
-        s16x2 hb[4], fb[4];
-#pragma unroll
-        for (int k = 0; k < 4; ++k) {
-            hb[k] = as_v(P.hb_init[k]);
-            fb[k] = as_v(P.fb_init[k]);
-        }
-        int kp_lo = 0, kp_hi = 0, pair_lo = -1, pair_hi = -1, blk_lo = 0, blk_hi = 0;
-        int fa_lo = kNoFlag, fa_hi = kNoFlag, fb_lo = kNoFlag, fb_hi = kNoFlag;
-
-        for (int b = 0; b < G.nblocks; ++b) {
-            const uint2 w = gres[(int64_t)b * 64 + lane];
-            const bool st_lo = (w.x & kResStart) != 0, st_hi = (w.y & kResStart) != 0;
-            uint4 tb0 = make_uint4(0, 0, 0, 0), tb1 = make_uint4(0, 0, 0, 0);
-            if (!first) {
-                tb0 = gbnd[((int64_t)b * 64 + lane) * 2];
-                tb1 = gbnd[((int64_t)b * 64 + lane) * 2 + 1];
+        const bool st_lo = (w.x & kResStart) != 0, st_hi = (w.y & kResStart) != 0;
+        if (__any(st_lo || st_hi)) {
+            const uint32_t m = (st_lo ? 0x0000ffffu : 0u) | (st_hi ? 0xffff0000u : 0u);
+            if (st_lo) {
+                pair_lo = pair_list[sb_lo + kp_lo];
+                kp_lo++;
+                blk_lo = 0;
+                fa_lo = kNoFlag;
+                fb_lo = kNoFlag;
             }
-            if (__any(st_lo || st_hi)) {
-                const uint32_t m = (st_lo ? 0x0000ffffu : 0u) | (st_hi ? 0xffff0000u : 0u);
-                if (st_lo) {
-                    pair_lo = pair_list[sb_lo + kp_lo];
-                    kp_lo++;
-                    blk_lo = 0;
-                    fa_lo = kNoFlag;
-                    fb_lo = kNoFlag;
-                }
-                if (st_hi) {
-                    pair_hi = pair_list[sb_hi + kp_hi];
-                    kp_hi++;
-                    blk_hi = 0;
-                    fa_hi = kNoFlag;
-                    fb_hi = kNoFlag;
-                }
-                if (MODE == 0) {
-#pragma unroll
-                    for (int r = 0; r < R; ++r) {
-                        Hl[r] = sel(m, Z, Hl[r]);
-                        El[r] = sel(m, Z, El[r]);
-                    }
-                    S = sel(m, Z, S);
-                    htop = sel(m, Z, htop);
-                } else {
-#pragma unroll
-                    for (int r = 0; r < R; ++r) {
-                        Hl[r] = sel(m, as_v(slb[2 * r]), Hl[r]);
-                        El[r] = sel(m, as_v(slb[2 * r + 1]), El[r]);
-                    }
-                    htop = sel(m, as_v(slb[2 * R]), htop);
-#pragma unroll
-                    for (int k = 0; k < 4; ++k) {
-                        hb[k] = sel(m, as_v(P.hb_init[k]), hb[k]);
-                        fb[k] = sel(m, as_v(P.fb_init[k]), fb[k]);
-                    }
-                }
+            if (st_hi) {
+                pair_hi = pair_list[sb_hi + kp_hi];
+                kp_hi++;
+                blk_hi = 0;
+                fa_hi = kNoFlag;
+                fb_hi = kNoFlag;
             }
-            const uint32_t tbh[4] = {tb0.x, tb0.z, tb1.x, tb1.z};
-            const uint32_t tbf[4] = {tb0.y, tb0.w, tb1.y, tb1.w};
-            uint32_t obh[4], obf[4];
-            s16x2 hmn = as_v(0u), hmx = as_v(0u);  // NW trackers, reset value of the reference
-            const bool ls_lo = (w.x & kResLast) != 0, ls_hi = (w.y & kResLast) != 0;
-            const int z_lo = (int)((w.x >> kResZShift) & 3u), z_hi = (int)((w.y >> kResZShift) & 3u);
-#pragma unroll
-            for (int c = 0; c < 4; ++c) {
-                const uint32_t alo = (w.x >> (8 * c)) & 0x3eu;
-                const uint32_t ahi = (w.y >> (8 * c)) & 0x3eu;
-                s16x2 hd, f;
-                if (first) {
-                    if (MODE == 0) {
-                        hd = Z;
-                        f = Z;
-                    } else {
-                        hd = hb[c];
-                        f = adds(fb[c], QR);
-                    }
-                } else {
-                    hd = htop;
-                    f = as_v(tbf[c]);
-                    htop = as_v(tbh[c]);
-                }
+            TA = (m & TA_on) | (~m & TA);
+            TB = (m & TB_on) | (~m & TB);
+            if (MODE == 0) {
 #pragma unroll
                 for (int r = 0; r < R; ++r) {
-                    const char* rowp = reinterpret_cast<const char*>(sqp) + r * 64;
-                    s16x2 v;
-                    v.x = *reinterpret_cast<const int16_t*>(rowp + alo);
-                    v.y = *reinterpret_cast<const int16_t*>(rowp + ahi);
-                    s16x2 h = adds(hd, v);
-                    h = vmax(h, f);
-                    h = vmax(h, El[r]);
-                    if (MODE == 0) {
-                        S = vmax(S, h);
-                    } else {
-                        hmn = vmin(hmn, h);
-                        hmx = vmax(hmx, h);
-                    }
-                    hd = Hl[r];
-                    Hl[r] = h;
-                    const s16x2 t = adds(h, QR);
-                    El[r] = vmax(adds(El[r], RR), t);
-                    f = vmax(adds(f, RR), t);
+                    Hl[r] = sel(m, Z, Hl[r]);
+                    El[r] = sel(m, Z, El[r]);
                 }
-                obh[c] = as_u(Hl[R - 1]);
-                obf[c] = as_u(f);
-                if (MODE == 1 && last) {
-                    const uint32_t cm = ((ls_lo && z_lo == c) ? 0x0000ffffu : 0u) |
-                                        ((ls_hi && z_hi == c) ? 0xffff0000u : 0u);
-                    if (__any(cm != 0u)) {
-                        s16x2 hq = Hl[0];
+                S = sel(m, Z, S);
+                htop = sel(m, Z, htop);
+            } else {
+                const int r0 = g * R;
 #pragma unroll
-                        for (int r = 1; r < R; ++r)
-                            if (r == last_row) hq = Hl[r];
-                        cap = sel(cm, hq, cap);
+                for (int r = 0; r < R; ++r) {
+                    Hl[r] = sel(m, as_v(slbh[1 + r0 + r]), Hl[r]);
+                    El[r] = sel(m, as_v(slbe[r0 + r]), El[r]);
+                }
+                htop = sel(m, as_v(slbh[r0]), htop);  // row -1 of the first pass: the corner, 0
+                if (FIRST) {
+#pragma unroll
+                    for (int c = 0; c < 4; ++c) {
+                        hb[c] = sel(m, as_v(P.hb_init[c]), hb[c]);
+                        fb[c] = sel(m, as_v(P.fb_init[c]), fb[c]);
                     }
                 }
             }
-            if (!last) {
-                gbnd[((int64_t)b * 64 + lane) * 2] = make_uint4(obh[0], obf[0], obh[1], obf[1]);
-                gbnd[((int64_t)b * 64 + lane) * 2 + 1] = make_uint4(obh[2], obf[2], obh[3], obf[3]);
-            }
+        }
 
-            // overflow predicates at the end of the block
+        // top boundary of this block: the lane above, one step earlier; the
+        // group's top lane takes the previous pass's bottom row or the matrix
+        // border.  tbh[3] also carries the next block's diagonal.
+        uint32_t toph[4], topf[4];
+        if (!FIRST) {
+            toph[0] = t0.x;
+            topf[0] = t0.y;
+            toph[1] = t0.z;
+            topf[1] = t0.w;
+            toph[2] = t1.x;
+            topf[2] = t1.y;
+            toph[3] = t1.z;
+            topf[3] = t1.w;
+        } else if (MODE == 0) {
+#pragma unroll
+            for (int c = 0; c < 4; ++c) {
+                toph[c] = as_u(Z);
+                topf[c] = as_u(Z);
+            }
+        } else {
+            toph[0] = as_u(hb[1]);
+            toph[1] = as_u(hb[2]);
+            toph[2] = as_u(hb[3]);
+            toph[3] = as_u(adds(hb[3], RR));
+#pragma unroll
+            for (int c = 0; c < 4; ++c) topf[c] = as_u(adds(fb[c], QR));
+        }
+        uint32_t tbh[4], tbf[4];
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+            tbh[c] = shift_down<G>(toph[c], obh[c], top_lane);
+            tbf[c] = shift_down<G>(topf[c], obf[c], top_lane);
+        }
+
+        s16x2 hmn = as_v(0u), hmx = as_v(0u);  // NW trackers, reset value of the reference
+        const bool ls_lo = (w.x & kResLast) != 0, ls_hi = (w.y & kResLast) != 0;
+        const bool any_last = __any(ls_lo || ls_hi);
+        const int z_lo = (int)((w.x >> kResZShift) & 3u), z_hi = (int)((w.y >> kResZShift) & 3u);
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+            const uint32_t alo = (((w.x >> (8 * c)) & 0x3eu) << LPR) + gbase;
+            const uint32_t ahi = (((w.y >> (8 * c)) & 0x3eu) << LPR) + gbase;
+            uint32_t dlo[R / 2], dhi[R / 2];
+            read_rows<R>(sp + alo, dlo);
+            read_rows<R>(sp + ahi, dhi);
+            s16x2 hd = htop;
+            s16x2 f = as_v(tbf[c]);
+            htop = as_v(tbh[c]);
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                const s16x2 v =
+                    as_v(__builtin_amdgcn_perm(dhi[r / 2], dlo[r / 2], (r & 1) ? 0x07060302u : 0x05040100u));
+                s16x2 h = adds(hd, v);
+                h = vmax(h, f);
+                h = vmax(h, El[r]);
+                if (MODE == 0) {
+                    S = vmax(S, h);
+                } else {
+                    hmn = vmin(hmn, h);
+                    hmx = vmax(hmx, h);
+                }
+                hd = Hl[r];
+                Hl[r] = h;
+                const s16x2 tt = adds(h, QR);
+                El[r] = vmax(adds(El[r], RR), tt);
+                f = vmax(adds(f, RR), tt);
+            }
+            obh[c] = as_u(Hl[R - 1]);
+            obf[c] = as_u(f);
+            if (MODE == 1 && any_last) {
+                if (last_pass) {
+                    uint32_t cm =
+                        ((ls_lo && z_lo == c) ? 0x0000ffffu : 0u) | ((ls_hi && z_hi == c) ? 0xffff0000u : 0u);
+                    if (g != last_g) cm = 0u;
+                    s16x2 hq = Hl[0];
+#pragma unroll
+                    for (int r = 1; r < R; ++r)
+                        if (r == last_r) hq = Hl[r];
+                    cap = sel(cm, hq, cap);
+                }
+            }
+        }
+        if (bot_out && valid) {
+            const int64_t o = ((int64_t)b * NG + k) * 2;
+            wbnd[o] = make_uint4(obh[0], obf[0], obh[1], obf[1]);
+            wbnd[o + 1] = make_uint4(obh[2], obf[2], obh[3], obf[3]);
+        }
+
+        // overflow predicates at the end of the block: a packed test, then the
+        // exact per-half bookkeeping only where a predicate fired
+        bool fire;
+        if (MODE == 0) {
+            fire = as_u(vmax(S, as_v(TA))) != TA || as_u(vmax(S, as_v(TB))) != TB;
+        } else {
+            fire = as_u(vmin(hmn, as_v(TA))) != TA || as_u(vmax(hmx, as_v(TB))) != TB;
+        }
+        if (__any(fire)) {
             if (MODE == 0) {
                 if (pair_lo >= 0) {
                     const int v = lo16(S);
@@ -240,31 +335,121 @@ __global__ __launch_bounds__(64) void score_kernel(const ScoreGroup* __restrict_
                     if (v >= P.thr_b && fb_hi == kNoFlag) fb_hi = blk_hi;
                 }
             } else {
-                if (pair_lo >= 0 && fa_lo == kNoFlag && (lo16(hmn) < P.thr_a || lo16(hmx) >= P.thr_b)) fa_lo = blk_lo;
-                if (pair_hi >= 0 && fa_hi == kNoFlag && (hi16(hmn) < P.thr_a || hi16(hmx) >= P.thr_b)) fa_hi = blk_hi;
-                if (first) {  // the reference advances the top boundary by four columns per block
-                    fb[0] = adds(fb[3], RR);
-                    fb[1] = adds(fb[0], RR);
-                    fb[2] = adds(fb[1], RR);
-                    fb[3] = adds(fb[2], RR);
-                    hb[0] = adds(hb[3], RR);
-                    hb[1] = adds(hb[0], RR);
-                    hb[2] = adds(hb[1], RR);
-                    hb[3] = adds(hb[2], RR);
-                }
+                if (pair_lo >= 0 && fa_lo == kNoFlag && (lo16(hmn) < P.thr_a || lo16(hmx) >= P.thr_b))
+                    fa_lo = blk_lo;
+                if (pair_hi >= 0 && fa_hi == kNoFlag && (hi16(hmn) < P.thr_a || hi16(hmx) >= P.thr_b))
+                    fa_hi = blk_hi;
             }
+            const uint32_t offa = (fa_lo != kNoFlag ? 0x0000ffffu : 0u) | (fa_hi != kNoFlag ? 0xffff0000u : 0u);
+            const uint32_t offb =
+                MODE == 0 ? ((fb_lo != kNoFlag ? 0x0000ffffu : 0u) | (fb_hi != kNoFlag ? 0xffff0000u : 0u)) : offa;
+            TA = (offa & TA_off) | (~offa & TA);
+            TB = (offb & TB_off) | (~offb & TB);
+        }
+        if (MODE == 1 && FIRST) {  // the reference advances the top border by four columns per block
+            fb[0] = adds(fb[3], RR);
+            fb[1] = adds(fb[0], RR);
+            fb[2] = adds(fb[1], RR);
+            fb[3] = adds(fb[2], RR);
+            hb[0] = adds(hb[3], RR);
+            hb[1] = adds(hb[0], RR);
+            hb[2] = adds(hb[1], RR);
+            hb[3] = adds(hb[2], RR);
+        }
+        if (any_last) {
+            uint32_t m = 0u;
             if (ls_lo && pair_lo >= 0) {
-                finish_pair<MODE>(results, pair_lo, first, last, MODE == 0 ? lo16(S) : lo16(cap), fa_lo, fb_lo);
+                finish_pair<MODE>(results, pair_lo, score_lane, MODE == 0 ? lo16(S) : lo16(cap), fa_lo, fb_lo);
                 pair_lo = -1;
+                m |= 0x0000ffffu;
             }
             if (ls_hi && pair_hi >= 0) {
-                finish_pair<MODE>(results, pair_hi, first, last, MODE == 0 ? hi16(S) : hi16(cap), fa_hi, fb_hi);
+                finish_pair<MODE>(results, pair_hi, score_lane, MODE == 0 ? hi16(S) : hi16(cap), fa_hi, fb_hi);
                 pair_hi = -1;
+                m |= 0xffff0000u;
             }
-            blk_lo++;
-            blk_hi++;
+            TA = (m & TA_off) | (~m & TA);
+            TB = (m & TB_off) | (~m & TB);
+        }
+        blk_lo++;
+        blk_hi++;
+    };
+
+    // block -g for step 0: only the top lanes start with a real block
+    uint2 wa = make_uint2(0, 0), wb = make_uint2(0, 0);
+    const uint4 zero4 = make_uint4(0, 0, 0, 0);
+    uint4 ta0 = zero4, ta1 = zero4, tb0 = zero4, tb1 = zero4;
+    if (nb > 0 && g == 0) {
+        wa = wp[0];
+        if (top_in) {
+            ta0 = tp[0];
+            ta1 = tp[1];
         }
     }
+    for (int t = 0; t < nsteps; t += 2) {
+        step(t, wa, ta0, ta1, wb, tb0, tb1);
+        step(t + 1, wb, tb0, tb1, wa, ta0, ta1);
+    }
+}
+
+template <int R, int G, int MODE>
+__global__ __launch_bounds__(64 * kWavesPerWG) void score_kernel(
+    const ScoreWave* __restrict__ waves, const QueryDesc* __restrict__ queries, const int16_t* __restrict__ qp,
+    const uint32_t* __restrict__ lb, const uint2* __restrict__ res, uint4* __restrict__ bnd,
+    const int32_t* __restrict__ slot_begin, const int32_t* __restrict__ pair_list, PairResult* __restrict__ results,
+    const ScoreParams P) {
+    constexpr int PR = G * R;  // rows per pass
+    static_assert((1 << Log2<PR>::value) == PR, "rows per pass must be a power of two");
+    __shared__ __align__(16) int16_t sprof[32 * PR];
+    __shared__ uint32_t slbh[MODE == 1 ? PR + 1 : 1];  // NW left boundary H of rows base-1 .. base+PR-1
+    __shared__ uint32_t slbe[MODE == 1 ? PR : 1];      // NW left boundary E of rows base .. base+PR-1
+
+    const int lane = threadIdx.x & 63;
+    const int k = lane / G, g = lane % G;
+    const ScoreWave Wv = waves[blockIdx.x * kWavesPerWG + (threadIdx.x >> 6)];
+    const QueryDesc Q = queries[Wv.query];
+    const int npass = (Q.qlen + PR - 1) / PR;
+    const int nb = Wv.nblocks;
+    const uint2* __restrict__ wres = res + Wv.res_off;
+    uint4* __restrict__ wbnd = bnd + Wv.bnd_off;
+    const int sb_lo = slot_begin[Wv.slot_base + 2 * k];
+    const int sb_hi = slot_begin[Wv.slot_base + 2 * k + 1];
+    const int last_lr = Q.qlen - 1 - (npass - 1) * PR;  // last real row within the last pass
+    const int last_g = last_lr / R, last_r = last_lr % R;
+    const uint32_t gbase = (uint32_t)(g * R * 2);  // byte offset of the lane's rows in a profile code row
+    const char* sp = reinterpret_cast<const char*>(sprof);
+
+    for (int pass = 0; pass < npass; ++pass) {
+        __syncthreads();
+        {
+            const uint4* src = reinterpret_cast<const uint4*>(qp + Q.qp_off + (int64_t)pass * 32 * PR);
+            uint4* dst = reinterpret_cast<uint4*>(sprof);
+            for (int i = threadIdx.x; i < 32 * PR / 8; i += 64 * kWavesPerWG) dst[i] = src[i];
+            if (MODE == 1) {
+                const int base = pass * PR;
+                for (int i = threadIdx.x; i < PR; i += 64 * kWavesPerWG) {
+                    slbh[1 + i] = lb[Q.lb_off + 2 * (base + i)];
+                    slbe[i] = lb[Q.lb_off + 2 * (base + i) + 1];
+                }
+                if (threadIdx.x == 0) slbh[0] = base > 0 ? lb[Q.lb_off + 2 * (base - 1)] : 0u;
+            }
+        }
+        __syncthreads();
+        const bool last_pass = pass == npass - 1;
+        if (pass == 0) {
+            run_pass<R, G, MODE, true>(P, gbase, sp, slbh, slbe, wres, wbnd, pair_list, results, sb_lo, sb_hi, k, g,
+                                       nb, last_pass, last_g, last_r);
+        } else {
+            run_pass<R, G, MODE, false>(P, gbase, sp, slbh, slbe, wres, wbnd, pair_list, results, sb_lo, sb_hi, k, g,
+                                        nb, last_pass, last_g, last_r);
+        }
+    }
+}
+
+// Sets every pair's result to the neutral value of the atomic combination.
+__global__ void init_results_kernel(PairResult* __restrict__ r, int64_t n, int32_t score) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) r[i] = PairResult{score, kNoFlag, kNoFlag, 0};
 }
 
 // ---------------------------------------------------------------------------
@@ -339,70 +524,81 @@ __global__ __launch_bounds__(64) void exact_kernel(const ExactPair* __restrict__
 }
 
 // ---------------------------------------------------------------------------
-// Interleaves the slots of each group into the score kernel's residue layout.
-// One workgroup per group, one lane per lane: the writes of a block are one
-// coalesced 512-byte row.  Codes are doubled into LDS byte offsets (codes are
-// below 32, so doubling a packed word never carries across bytes).
+// Residue interleaving: one wave per (stream, entry), lanes over the entry's
+// blocks.  Codes are doubled into LDS byte offsets (codes are below 32, so
+// doubling a packed word never carries across bytes).  Words past the end of
+// a stream stay zero (cleared by fill_u32_kernel).
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ uint32_t pack_word(const uint8_t* codes, const PackSlot& s, int b) {
-    if (b >= s.nblocks) return 0u;
-    uint32_t w = *reinterpret_cast<const uint32_t*>(codes + s.code_off + 4 * (int64_t)b) << 1;
-    if (b == 0) w |= kResStart;
-    if (b == s.nblocks - 1) w |= kResLast | ((uint32_t)s.last_col << kResZShift);
-    return w;
+__global__ __launch_bounds__(64) void pack_kernel(const PackEntry* __restrict__ entries,
+                                                  const uint8_t* __restrict__ codes, uint32_t* __restrict__ res) {
+    const PackEntry E = entries[blockIdx.x];
+    for (int b = threadIdx.x; b < E.nblocks; b += 64) {
+        uint32_t w = *reinterpret_cast<const uint32_t*>(codes + E.code_off + 4 * (int64_t)b) << 1;
+        if (b == 0) w |= kResStart;
+        if (b == E.nblocks - 1) w |= kResLast | ((uint32_t)E.last_col << kResZShift);
+        res[E.dst + (int64_t)b * E.stride] = w;
+    }
 }
 
-__global__ __launch_bounds__(64) void pack_kernel(const ScoreGroup* __restrict__ groups,
-                                                  const PackSlot* __restrict__ slots,
-                                                  const uint8_t* __restrict__ codes, uint2* __restrict__ res) {
-    const ScoreGroup G = groups[blockIdx.x];
-    const int lane = threadIdx.x;
-    const PackSlot lo = slots[G.slot_base + 2 * lane];
-    const PackSlot hi = slots[G.slot_base + 2 * lane + 1];
-    uint2* out = res + G.res_off;
-    for (int b = 0; b < G.nblocks; ++b) out[(int64_t)b * 64 + lane] = make_uint2(pack_word(codes, lo, b), pack_word(codes, hi, b));
+__global__ void fill_u32_kernel(uint32_t* __restrict__ p, int64_t n, uint32_t v) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        p[i] = v;
 }
 
 // ---------------------------------------------------------------------------
 // launch wrappers (host side)
 // ---------------------------------------------------------------------------
-template <int R, int MODE>
-static hipError_t launch_score_r(int ngroups, hipStream_t st, const ScoreGroup* groups, const QueryDesc* queries,
+template <int R, int G, int MODE>
+static hipError_t launch_score_t(int nwg, hipStream_t st, const ScoreWave* waves, const QueryDesc* queries,
                                  const int16_t* qp, const uint32_t* lb, const uint2* res, uint4* bnd,
                                  const int32_t* slot_begin, const int32_t* pair_list, PairResult* results,
                                  const ScoreParams& P) {
-    hipLaunchKernelGGL((score_kernel<R, MODE>), dim3(ngroups), dim3(64), 0, st, groups, queries, qp, lb, res, bnd,
-                       slot_begin, pair_list, results, P);
+    hipLaunchKernelGGL((score_kernel<R, G, MODE>), dim3(nwg), dim3(64 * kWavesPerWG), 0, st, waves, queries, qp, lb,
+                       res, bnd, slot_begin, pair_list, results, P);
     return hipGetLastError();
 }
 
-hipError_t launch_score(int rows, int mode, int ngroups, hipStream_t st, const ScoreGroup* groups,
+bool score_shape_supported(int lanes, int rows) {
+    return (lanes == 8 || lanes == 16 || lanes == 32 || lanes == 64) && (rows == 4 || rows == 8 || rows == 16);
+}
+
+hipError_t launch_score(int lanes, int rows, int mode, int nwg, hipStream_t st, const ScoreWave* waves,
                         const QueryDesc* queries, const int16_t* qp, const uint32_t* lb, const uint2* res, uint4* bnd,
                         const int32_t* slot_begin, const int32_t* pair_list, PairResult* results,
                         const ScoreParams& P) {
-    if (ngroups <= 0) return hipSuccess;
-#define SSA_CASE(RR)                                                                                        \
-    case RR:                                                                                                \
-        return mode == 0 ? launch_score_r<RR, 0>(ngroups, st, groups, queries, qp, lb, res, bnd, slot_begin, \
-                                                 pair_list, results, P)                                     \
-                         : launch_score_r<RR, 1>(ngroups, st, groups, queries, qp, lb, res, bnd, slot_begin, \
-                                                 pair_list, results, P);
-    switch (rows) {
-        SSA_CASE(8)
-        SSA_CASE(16)
-        SSA_CASE(24)
-        SSA_CASE(32)
-        SSA_CASE(48)
-        default:
-            return hipErrorInvalidValue;
-    }
+    if (nwg <= 0) return hipSuccess;
+#define SSA_CASE(GG, RR)                                                                                        \
+    if (lanes == GG && rows == RR)                                                                            \
+        return mode == 0 ? launch_score_t<RR, GG, 0>(nwg, st, waves, queries, qp, lb, res, bnd, slot_begin,   \
+                                                     pair_list, results, P)                                   \
+                         : launch_score_t<RR, GG, 1>(nwg, st, waves, queries, qp, lb, res, bnd, slot_begin,   \
+                                                     pair_list, results, P);
+#define SSA_ROWS(GG) SSA_CASE(GG, 4) SSA_CASE(GG, 8) SSA_CASE(GG, 16)
+    SSA_ROWS(8)
+    SSA_ROWS(16)
+    SSA_ROWS(32)
+    SSA_ROWS(64)
+#undef SSA_ROWS
 #undef SSA_CASE
+    return hipErrorInvalidValue;
 }
 
-hipError_t launch_pack(int ngroups, hipStream_t st, const ScoreGroup* groups, const PackSlot* slots,
-                       const uint8_t* codes, uint2* res) {
-    if (ngroups <= 0) return hipSuccess;
-    hipLaunchKernelGGL(pack_kernel, dim3(ngroups), dim3(64), 0, st, groups, slots, codes, res);
+hipError_t launch_init_results(hipStream_t st, PairResult* r, int64_t n, int32_t score) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(init_results_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, r, n, score);
+    return hipGetLastError();
+}
+
+hipError_t launch_pack(int nentries, hipStream_t st, const PackEntry* entries, const uint8_t* codes,
+                       uint32_t* res, int64_t res_words) {
+    if (res_words > 0) {
+        const int64_t blocks = std::min<int64_t>((res_words + 255) / 256, 65536);
+        hipLaunchKernelGGL(fill_u32_kernel, dim3((unsigned)blocks), dim3(256), 0, st, res, res_words, 0u);
+        const hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
+    if (nentries <= 0) return hipSuccess;
+    hipLaunchKernelGGL(pack_kernel, dim3(nentries), dim3(64), 0, st, entries, codes, res);
     return hipGetLastError();
 }
 

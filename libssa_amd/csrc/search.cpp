@@ -16,6 +16,7 @@
 #include <climits>
 #include <cstring>
 #include <numeric>
+#include <queue>
 #include <thread>
 
 #include "gpu.h"
@@ -48,7 +49,10 @@ DeviceDB g_db;
 
 void db_generation_bump() { ++g_generation; }
 
+void release_layouts();
+
 void release_device_db() {
+    release_layouts();
     if (g_db.d_codes) (void)hipFree(g_db.d_codes);
     g_db = DeviceDB();
 }
@@ -192,19 +196,36 @@ struct TierOut {
 // ============================================================================
 namespace {
 
-int choose_rows(size_t qlen_max) {
-    const int ov = rows_per_strip_override();
-    if (ov) return ov;
-    // smallest padding waste over the instantiated heights, larger heights on ties
-    static const int kRows[] = {48, 32, 24, 16, 8};
-    int best = 32;
-    size_t best_waste = SIZE_MAX;
-    for (int r : kRows) {
-        const size_t strips = (qlen_max + r - 1) / r;
-        const size_t waste = strips * r - qlen_max;
-        if (waste < best_waste) {
-            best = r;
-            best_waste = waste;
+struct Shape {
+    int lanes;  // G: lanes per stream pair
+    int rows;   // R: query rows per lane
+};
+
+}  // namespace
+
+Shape& shape_override_ref();
+void release_layouts();
+
+namespace {
+
+Shape shape_override() { return shape_override_ref(); }
+
+// Kernel shape for a query length: fewest padded rows per pass (weighted by
+// the per-row cost of short lanes), the wider lane group on ties.
+Shape choose_shape(size_t qlen_max) {
+    const Shape ov = shape_override();
+    if (ov.lanes && ov.rows) return ov;
+    static const Shape kShapes[] = {{64, 8}, {64, 4}, {32, 16}, {32, 8}, {32, 4}, {16, 16},
+                                    {16, 8}, {16, 4}, {8, 16},  {8, 8},  {8, 4}};
+    Shape best = kShapes[0];
+    double best_cost = 1e300;
+    for (const Shape& sh : kShapes) {
+        const size_t pr = (size_t)sh.lanes * sh.rows;
+        const size_t passes = std::max<size_t>(1, (qlen_max + pr - 1) / pr);
+        const double cost = (double)(passes * pr) * (sh.rows == 4 ? 1.15 : 1.0);
+        if (cost < best_cost) {
+            best = sh;
+            best_cost = cost;
         }
     }
     return best;
@@ -218,44 +239,180 @@ struct PairSet {
     size_t size() const { return nq * entries.size(); }
 };
 
+// Streams of one entry set for one lane-group width: entries are dealt
+// longest first to the least loaded stream (LPT), so streams come out of
+// nearly equal length; streams are then ordered longest first and taken in
+// runs of 2*64/G per wave, so the waves that start first are the longest.
+struct StreamLayout {
+    int lanes = 0;
+    int nwaves = 0;                      // a multiple of kWavesPerWG
+    std::vector<int32_t> wave_nblocks;   // per wave
+    std::vector<int64_t> wave_res_off;   // per wave, uint2 units
+    std::vector<int32_t> slot_first;     // per (wave, slot) + 1: range in slot_entries
+    std::vector<uint32_t> slot_entries;  // positions into PairSet::entries, stream by stream
+    int64_t res_words = 0;               // uint2 words of the residue array
+    uint2* d_res = nullptr;              // device residue array
+    uint64_t generation = 0;             // cached layouts: DB generation they belong to
+
+    void release() {
+        if (d_res) (void)hipFree(d_res);
+        d_res = nullptr;
+    }
+};
+
+int& target_waves() {
+    static int w = 10240;
+    return w;
+}
+
+void build_layout(StreamLayout& L, int lanes, const PairSet& ps, const std::vector<uint32_t>& order) {
+    DeviceDB& db = g_db;
+    Gpu& gpu = Gpu::get();
+    const int ng = 64 / lanes, spw = 2 * ng;
+    const size_t n = ps.entries.size();
+    L.lanes = lanes;
+    std::vector<int32_t> nbk(n);
+    int64_t total = 0;
+    for (size_t k = 0; k < n; ++k) {
+        nbk[k] = (int32_t)((db.entries[ps.entries[k]].len + 3) / 4);
+        total += nbk[k];
+    }
+    const int64_t per_wave = std::max<int64_t>(1, ((int64_t)target_waves()));
+    const int64_t B = std::max<int64_t>(16 * (int64_t)lanes, (total + spw * per_wave - 1) / (spw * per_wave));
+    int64_t ns = std::max<int64_t>(1, (total + B - 1) / B);
+    const int64_t unit = (int64_t)spw * kWavesPerWG;
+    ns = (ns + unit - 1) / unit * unit;
+
+    // LPT: longest entry to the least loaded stream
+    std::vector<int64_t> load(ns, 0);
+    std::vector<int32_t> stream_of(n);
+    {
+        typedef std::pair<int64_t, int32_t> Item;
+        std::priority_queue<Item, std::vector<Item>, std::greater<Item>> pq;
+        for (int64_t s = 0; s < ns; ++s) pq.push(Item(0, (int32_t)s));
+        for (uint32_t pos : order) {
+            Item it = pq.top();
+            pq.pop();
+            stream_of[pos] = it.second;
+            it.first += nbk[pos];
+            load[it.second] = it.first;
+            pq.push(it);
+        }
+    }
+    std::vector<int32_t> sorder(ns);
+    std::iota(sorder.begin(), sorder.end(), 0);
+    std::stable_sort(sorder.begin(), sorder.end(), [&](int32_t a, int32_t b) { return load[a] > load[b]; });
+    std::vector<int32_t> rank(ns);
+    for (int64_t i = 0; i < ns; ++i) rank[sorder[i]] = (int32_t)i;
+    // CSR of entries per stream slot (slot index = rank), entries in LPT order
+    L.slot_first.assign(ns + 1, 0);
+    for (uint32_t pos : order) L.slot_first[rank[stream_of[pos]] + 1]++;
+    for (int64_t i = 0; i < ns; ++i) L.slot_first[i + 1] += L.slot_first[i];
+    L.slot_entries.assign(n, 0);
+    {
+        std::vector<int32_t> fill(L.slot_first.begin(), L.slot_first.end() - 1);
+        for (uint32_t pos : order) L.slot_entries[fill[rank[stream_of[pos]]]++] = pos;
+    }
+    L.nwaves = (int)(ns / spw);
+    L.wave_nblocks.assign(L.nwaves, 0);
+    L.wave_res_off.assign(L.nwaves, 0);
+    int64_t off = 0;
+    for (int w = 0; w < L.nwaves; ++w) {
+        L.wave_nblocks[w] = (int32_t)load[sorder[(int64_t)w * spw]];
+        L.wave_res_off[w] = off;
+        off += (int64_t)L.wave_nblocks[w] * ng;
+    }
+    L.res_words = off;
+
+    // residue words on the device
+    std::vector<PackEntry> pe;
+    pe.reserve(n);
+    for (int w = 0; w < L.nwaves; ++w)
+        for (int sl = 0; sl < spw; ++sl) {
+            const int k = sl / 2, h = sl % 2;
+            const int64_t slot = (int64_t)w * spw + sl;
+            int64_t pos = 0;
+            for (int32_t j = L.slot_first[slot]; j < L.slot_first[slot + 1]; ++j) {
+                const uint32_t e = ps.entries[L.slot_entries[j]];
+                const DbEntry& de = db.entries[e];
+                PackEntry p;
+                p.code_off = de.code_off;
+                p.dst = (L.wave_res_off[w] + pos * ng + k) * 2 + h;
+                p.nblocks = (int32_t)((de.len + 3) / 4);
+                p.last_col = (int32_t)((de.len - 1) % 4);
+                p.stride = 2 * ng;
+                p.pad = 0;
+                pe.push_back(p);
+                pos += p.nblocks;
+            }
+        }
+    L.release();
+    SSA_HIP(hipMalloc(&L.d_res, (size_t)std::max<int64_t>(L.res_words, 1) * sizeof(uint2)));
+    auto* d_pe = (PackEntry*)gpu.arena("pack", std::max<size_t>(pe.size(), 1) * sizeof(PackEntry));
+    gpu.upload(d_pe, pe.data(), pe.size() * sizeof(PackEntry));
+    SSA_HIP(launch_pack((int)pe.size(), gpu.stream(), d_pe, db.d_codes, reinterpret_cast<uint32_t*>(L.d_res),
+                        L.res_words * 2));
+    gpu.sync();  // pe is staged from pageable memory
+}
+
+// layouts of the whole DB, kept while the DB stays loaded (one per lane width)
+std::vector<StreamLayout>& layout_cache() {
+    static std::vector<StreamLayout> c;
+    return c;
+}
+
+}  // namespace
+
+void release_layouts() {
+    for (StreamLayout& L : layout_cache()) L.release();
+    layout_cache().clear();
+}
+
+namespace {
+
 // Scores all pairs of the set at one tier.  out receives the raw per-pair
 // results in PairSet::index order.
 void gpu_score_tier(SearchType type, const Tier& T, const std::vector<QueryBuf>& qs, const PairSet& ps,
                     const std::vector<uint32_t>* sorted_all, std::vector<PairResult>& out, double* ms,
-                    uint64_t* cells_computed, int* rows_used) {
+                    uint64_t* cells_computed, Shape* shape_used) {
     Gpu& gpu = Gpu::get();
     DeviceDB& db = g_db;
     const Config& cf = config();
     const Matrix& M = matrix();
     const int go = cf.gap_open, ge = cf.gap_extend;
     const size_t n = ps.entries.size(), nq = qs.size();
-    out.assign(ps.size(), PairResult{0, kNoFlag, kNoFlag, 0});
+    out.assign(ps.size(), PairResult{-32768, kNoFlag, kNoFlag, 0});
     if (n == 0 || nq == 0) return;
 
     size_t qmax = 0;
     for (const QueryBuf& q : qs) qmax = std::max(qmax, q.len);
-    const int R = choose_rows(qmax);
-    *rows_used = R;
+    const Shape sh = choose_shape(qmax);
+    if (!score_shape_supported(sh.lanes, sh.rows)) fatal("libssa_amd: unsupported kernel shape");
+    *shape_used = sh;
+    const int PR = sh.lanes * sh.rows, ng = 64 / sh.lanes, spw = 2 * ng;
 
-    // ---- profiles, left boundaries, per-query descriptors ----
+    // ---- profiles [pass][code][row], left boundaries, per-query descriptors ----
     std::vector<QueryDesc> qd(nq);
     std::vector<int16_t> prof;
     std::vector<uint32_t> lbv;
+    std::vector<int> npass(nq);
     const int64_t QRw = T.wrap(go + ge), Rw = T.wrap(ge);
     for (size_t qi = 0; qi < nq; ++qi) {
         const QueryBuf& q = qs[qi];
-        const size_t rows = ((q.len + R - 1) / R) * R;
+        npass[qi] = (int)((q.len + PR - 1) / PR);
+        const size_t rows = (size_t)npass[qi] * PR;
         qd[qi].qlen = (int32_t)q.len;
         qd[qi].qp_off = (int32_t)prof.size();
         qd[qi].lb_off = (int32_t)lbv.size();
-        for (size_t i = 0; i < rows; ++i) {
-            for (int d = 0; d < kDim; ++d) {
-                int32_t v;
-                if (i < q.len) v = T.repr(T.wrap(M.at(d, q.seq[i])));
-                else v = type == kSW ? -32768 : 0;
-                prof.push_back((int16_t)v);
-            }
-        }
+        for (int p = 0; p < npass[qi]; ++p)
+            for (int d = 0; d < kDim; ++d)
+                for (int r = 0; r < PR; ++r) {
+                    const size_t i = (size_t)p * PR + r;
+                    int32_t v;
+                    if (i < q.len) v = T.repr(T.wrap(M.at(d, q.seq[i])));
+                    else v = type == kSW ? -32768 : 0;
+                    prof.push_back((int16_t)v);
+                }
         int64_t c = QRw;
         for (size_t i = 0; i < rows; ++i) {
             if (i < q.len) {
@@ -288,104 +445,78 @@ void gpu_score_tier(SearchType type, const Tier& T, const std::vector<QueryBuf>&
     for (int k = 1; k < 4; ++k) P.hb_init[k] = pack2(T.repr(T.wrap(go + k * ge)));
     for (int k = 0; k < 4; ++k) P.fb_init[k] = pack2(T.repr(T.wrap(go + (k + 1) * ge)));
 
-    // ---- groups of 128 slots per query, entries longest first ----
-    std::vector<uint32_t> order;  // positions k into ps.entries, longest first
+    // ---- streams: cached for the whole DB, built per call for subsets ----
+    StreamLayout local;
+    StreamLayout* L = nullptr;
     if (sorted_all) {
-        // the set is the whole DB in entry order: reuse the DB's length order
-        order = *sorted_all;
+        for (StreamLayout& c : layout_cache())
+            if (c.lanes == sh.lanes && c.generation == db.generation) L = &c;
+        if (!L) {
+            layout_cache().emplace_back();
+            L = &layout_cache().back();
+            build_layout(*L, sh.lanes, ps, *sorted_all);
+            L->generation = db.generation;
+        }
     } else {
-        order.resize(n);
+        std::vector<uint32_t> order(n);
         std::iota(order.begin(), order.end(), 0u);
         std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) {
             return db.entries[ps.entries[a]].len > db.entries[ps.entries[b]].len;
         });
+        build_layout(local, sh.lanes, ps, order);
+        L = &local;
     }
-    const size_t groups_per_q = (n + kSlotsPerGroup - 1) / kSlotsPerGroup;
-    const size_t ngroups = groups_per_q * nq;
-    std::vector<ScoreGroup> groups(ngroups);
-    std::vector<PackSlot> slots(groups_per_q * kSlotsPerGroup);
-    std::vector<int32_t> slot_begin(ngroups * kSlotsPerGroup);
-    std::vector<int32_t> pair_list(ngroups * kSlotsPerGroup, -1);
-    int64_t res_total = 0, bnd_total = 0;
-    std::vector<int64_t> res_off(groups_per_q);
-    std::vector<int32_t> gblocks(groups_per_q);
+
+    // ---- waves of every query ----
+    const int nw = L->nwaves;
+    std::vector<ScoreWave> waves((size_t)nq * nw);
+    std::vector<int32_t> slot_begin((size_t)nq * nw * spw);
+    std::vector<int32_t> pair_list(nq * n);
+    int64_t bnd_total = 0;
     uint64_t computed = 0;
-    for (size_t g = 0; g < groups_per_q; ++g) {
-        int32_t T_g = 0;
-        for (int s = 0; s < kSlotsPerGroup; ++s) {
-            const size_t pos = g * kSlotsPerGroup + s;
-            PackSlot& sl = slots[pos];
-            if (pos < n) {
-                const DbEntry& e = db.entries[ps.entries[order[pos]]];
-                sl.code_off = e.code_off;
-                sl.nblocks = (int32_t)((e.len + 3) / 4);
-                sl.last_col = (int32_t)((e.len - 1) % 4);
-                T_g = std::max(T_g, sl.nblocks);
-            } else {
-                sl.code_off = 0;
-                sl.nblocks = 0;
-                sl.last_col = 0;
-            }
-        }
-        gblocks[g] = T_g;
-        res_off[g] = res_total;
-        res_total += (int64_t)T_g * 64;
-    }
     for (size_t qi = 0; qi < nq; ++qi) {
-        const int nstrips = (qs[qi].len + R - 1) / R;
-        for (size_t g = 0; g < groups_per_q; ++g) {
-            const size_t gi = qi * groups_per_q + g;
-            ScoreGroup& G = groups[gi];
-            G.query = (int32_t)qi;
-            G.nblocks = gblocks[g];
-            G.res_off = res_off[g];  // the interleaved residues are shared by all queries
-            G.bnd_off = bnd_total;
-            G.slot_base = (int32_t)(gi * kSlotsPerGroup);
-            G.pad = 0;
-            if (nstrips > 1) bnd_total += (int64_t)gblocks[g] * 64 * 2;
-            computed += (uint64_t)gblocks[g] * 4 * kSlotsPerGroup * (uint64_t)nstrips * R;
-            for (int s = 0; s < kSlotsPerGroup; ++s) {
-                const size_t pos = g * kSlotsPerGroup + s;
-                slot_begin[gi * kSlotsPerGroup + s] = (int32_t)(gi * kSlotsPerGroup + s);
-                if (pos < n) pair_list[gi * kSlotsPerGroup + s] = (int32_t)ps.index(qi, order[pos]);
-            }
+        for (int w = 0; w < nw; ++w) {
+            ScoreWave& W = waves[qi * nw + w];
+            W.query = (int32_t)qi;
+            W.nblocks = L->wave_nblocks[w];
+            W.res_off = L->wave_res_off[w];
+            W.bnd_off = bnd_total;
+            W.slot_base = (int32_t)((qi * nw + w) * spw);
+            W.pad = 0;
+            if (npass[qi] > 1) bnd_total += (int64_t)W.nblocks * ng * 2;
+            computed += (uint64_t)W.nblocks * 4 * spw * (uint64_t)npass[qi] * PR;
+            for (int sl = 0; sl < spw; ++sl)
+                slot_begin[(qi * nw + w) * spw + sl] = (int32_t)(qi * n) + L->slot_first[(size_t)w * spw + sl];
         }
+        for (size_t j = 0; j < n; ++j) pair_list[qi * n + j] = (int32_t)ps.index(qi, L->slot_entries[j]);
     }
     *cells_computed += computed;
 
     // ---- device buffers ----
-    auto* d_groups = (ScoreGroup*)gpu.arena("groups", groups.size() * sizeof(ScoreGroup));
+    auto* d_waves = (ScoreWave*)gpu.arena("waves", waves.size() * sizeof(ScoreWave));
     auto* d_qd = (QueryDesc*)gpu.arena("qdesc", qd.size() * sizeof(QueryDesc));
-    auto* d_prof = (int16_t*)gpu.arena("prof", prof.size() * sizeof(int16_t));
-    auto* d_lb = (uint32_t*)gpu.arena("lb", lbv.size() * sizeof(uint32_t));
-    auto* d_slots = (PackSlot*)gpu.arena("slots", slots.size() * sizeof(PackSlot));
+    auto* d_prof = (int16_t*)gpu.arena("prof", std::max<size_t>(prof.size(), 8) * sizeof(int16_t));
+    auto* d_lb = (uint32_t*)gpu.arena("lb", std::max<size_t>(lbv.size(), 1) * sizeof(uint32_t));
     auto* d_sb = (int32_t*)gpu.arena("slot_begin", slot_begin.size() * sizeof(int32_t));
     auto* d_pl = (int32_t*)gpu.arena("pair_list", pair_list.size() * sizeof(int32_t));
-    auto* d_res = (uint2*)gpu.arena("res", (size_t)res_total * sizeof(uint2));
     auto* d_bnd = (uint4*)gpu.arena("bnd", (size_t)std::max<int64_t>(bnd_total, 1) * sizeof(uint4));
     auto* d_out = (PairResult*)gpu.arena("results", out.size() * sizeof(PairResult));
-    gpu.upload(d_groups, groups.data(), groups.size() * sizeof(ScoreGroup));
+    gpu.upload(d_waves, waves.data(), waves.size() * sizeof(ScoreWave));
     gpu.upload(d_qd, qd.data(), qd.size() * sizeof(QueryDesc));
     gpu.upload(d_prof, prof.data(), prof.size() * sizeof(int16_t));
     gpu.upload(d_lb, lbv.data(), lbv.size() * sizeof(uint32_t));
-    gpu.upload(d_slots, slots.data(), slots.size() * sizeof(PackSlot));
     gpu.upload(d_sb, slot_begin.data(), slot_begin.size() * sizeof(int32_t));
     gpu.upload(d_pl, pair_list.data(), pair_list.size() * sizeof(int32_t));
-    bool empty_query = false;
-    for (const QueryBuf& q : qs) empty_query |= q.len == 0;
-    if (empty_query) {  // an empty query scores 0 everywhere (no strip writes its pairs)
-        for (PairResult& r : out) r.score = -32768;
-        gpu.upload(d_out, out.data(), out.size() * sizeof(PairResult));
-    }
 
     GpuTimer tm;
     tm.start();
-    SSA_HIP(launch_pack((int)groups_per_q, gpu.stream(), d_groups, d_slots, db.d_codes, d_res));
-    SSA_HIP(launch_score(R, type == kSW ? 0 : 1, (int)ngroups, gpu.stream(), d_groups, d_qd, d_prof, d_lb, d_res,
-                         d_bnd, d_sb, d_pl, d_out, P));
+    SSA_HIP(launch_init_results(gpu.stream(), d_out, (int64_t)out.size(), -32768));
+    SSA_HIP(launch_score(sh.lanes, sh.rows, type == kSW ? 0 : 1, (int)((size_t)nq * nw / kWavesPerWG), gpu.stream(),
+                         d_waves, d_qd, d_prof, d_lb, L->d_res, d_bnd, d_sb, d_pl, d_out, P));
     tm.stop();
     gpu.download(out.data(), d_out, out.size() * sizeof(PairResult));
     gpu.sync();
+    local.release();
     *ms += tm.ms();
 }
 
@@ -655,9 +786,16 @@ ssa_amd_stats_t& last_stats() {
     return s;
 }
 
-int& rows_per_strip_override() {
-    static int r = 0;
-    return r;
+Shape& shape_override_ref() {
+    static Shape s{0, 0};
+    return s;
+}
+
+void set_shape_override(int lanes, int rows) { shape_override_ref() = Shape{lanes, rows}; }
+
+void set_target_waves(int waves) {
+    target_waves() = waves > 0 ? waves : 10240;
+    release_layouts();
 }
 
 p_alignment_list run_search(SearchType type, p_query query, size_t hitcount, int bit_width, int align_type) {
@@ -722,7 +860,7 @@ p_alignment_list run_search(SearchType type, p_query query, size_t hitcount, int
     auto at = [N](size_t q, size_t e) { return q * N + e; };
     double ms_first = 0, ms_re = 0;
     uint64_t computed = 0;
-    int rows_used = 0;
+    Shape shape_used{0, 0};
     std::vector<PairResult> raw;
 
     std::vector<uint8_t> in16(N, 0);
@@ -730,7 +868,7 @@ p_alignment_list run_search(SearchType type, p_query query, size_t hitcount, int
         t8.resize(nq * N);
         if (sw_single) {
             gpu_score_tier(type, kTier16, qs, all, full_db ? &db.by_length : nullptr, raw, &ms_first, &computed,
-                           &rows_used);
+                           &shape_used);
             t16.resize(nq * N);
             for (size_t q = 0; q < nq; ++q)
                 for (uint32_t e = 0; e < N; ++e) {
@@ -741,7 +879,7 @@ p_alignment_list run_search(SearchType type, p_query query, size_t hitcount, int
                 }
         } else {
             gpu_score_tier(type, kTier8, qs, all, full_db ? &db.by_length : nullptr, raw, &ms_first, &computed,
-                           &rows_used);
+                           &shape_used);
             for (size_t q = 0; q < nq; ++q)
                 for (uint32_t e = 0; e < N; ++e) {
                     t8[at(q, e)] = decode(type, kTier8, raw[all.index(q, e)], db.entries[e].len, false);
@@ -753,8 +891,8 @@ p_alignment_list run_search(SearchType type, p_query query, size_t hitcount, int
                 if (in16[e]) sub.entries.push_back(e);
             t16.resize(nq * N);
             if (!sub.entries.empty()) {
-                int r2 = 0;
-                gpu_score_tier(type, kTier16, qs, sub, nullptr, raw, &ms_re, &computed, &r2);
+                Shape s2{0, 0};
+                gpu_score_tier(type, kTier16, qs, sub, nullptr, raw, &ms_re, &computed, &s2);
                 for (size_t q = 0; q < nq; ++q)
                     for (size_t k = 0; k < sub.entries.size(); ++k) {
                         const uint32_t e = sub.entries[k];
@@ -764,7 +902,7 @@ p_alignment_list run_search(SearchType type, p_query query, size_t hitcount, int
         }
     } else if (bit_width == BIT_WIDTH_16) {
         gpu_score_tier(type, kTier16, qs, all, full_db ? &db.by_length : nullptr, raw, &ms_first, &computed,
-                       &rows_used);
+                       &shape_used);
         t16.resize(nq * N);
         for (size_t q = 0; q < nq; ++q)
             for (uint32_t e = 0; e < N; ++e) {
@@ -780,7 +918,7 @@ p_alignment_list run_search(SearchType type, p_query query, size_t hitcount, int
         std::fill(need64.begin(), need64.end(), 1);
         if (mi.fits16) {
             gpu_score_tier(type, kTier16, qs, all, full_db ? &db.by_length : nullptr, raw, &ms_first, &computed,
-                           &rows_used);
+                           &shape_used);
             t16.resize(nq * N);
             const int64_t gap = std::abs((int64_t)go) + std::abs((int64_t)ge);
             std::fill(need64.begin(), need64.end(), 0);
@@ -929,7 +1067,8 @@ p_alignment_list run_search(SearchType type, p_query query, size_t hitcount, int
     if (align_type != COMPUTE_SCORE) gpu_traceback(type, al, &st.ms_traceback);
 
     st.cells_computed = computed;
-    st.rows_per_strip = rows_used;
+    st.lanes_per_stream = shape_used.lanes;
+    st.rows_per_lane = shape_used.rows;
     st.ms_gpu_first = ms_first;
     st.ms_gpu_requeue = ms_re;
     st.ms_total = (now_us() - t_start) / 1000.0;

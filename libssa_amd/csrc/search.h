@@ -37,6 +37,7 @@ p_alignment_list run_search(SearchType type, p_query query, size_t hitcount, int
 void release_device_db();
 
 ssa_amd_stats_t& last_stats();
-int& rows_per_strip_override();
+void set_shape_override(int lanes, int rows);  // 0, 0 = chosen per query length
+void set_target_waves(int waves);             // 0 = default
 
 }  // namespace ssa

@@ -218,14 +218,30 @@ def test_random_parity(seed):
     check(t, bw, symtype, strands, qraw, db, k, matrix, gaps, isa=isa, chunk=chunk, align=(seed % 4 == 1))
 
 
-@pytest.mark.parametrize("rows", [8, 16, 24, 32, 48])
-def test_rows_per_strip_do_not_change_results(rows):
+@pytest.mark.parametrize("shape", [(8, 4), (8, 16), (16, 8), (32, 4), (32, 16), (64, 4), (64, 8), (64, 16)])
+def test_kernel_shapes_do_not_change_results(shape):
     rnd = random.Random(77)
     qraw = "".join(rnd.choice("ARNDCQEGHILKMFPSTWYV") for _ in range(131))
     db = rand_db(rnd, O.AMINOACID, 400, 1, 300, homolog_of=qraw, frac=0.2)
-    ssa.set_rows_per_strip(rows)
+    ssa.set_kernel_shape(*shape)
     try:
         for t, bw in [(O.SW, 8), (O.NW, 16), (O.NW, 8)]:
             check(t, bw, O.AMINOACID, 1, qraw, db, 20, ("builtin", "blosum62"), (-10, -1))
     finally:
-        ssa.set_rows_per_strip(0)
+        ssa.set_kernel_shape(0, 0)
+
+
+@pytest.mark.parametrize("waves", [1, 4, 37])
+def test_stream_packing_does_not_change_results(waves):
+    """Few waves put many entries end to end in each stream."""
+    rnd = random.Random(78 + waves)
+    qraw = "".join(rnd.choice("ARNDCQEGHILKMFPSTWYV") for _ in range(300))
+    db = rand_db(rnd, O.AMINOACID, 500, 1, 400, homolog_of=qraw, frac=0.2)
+    ssa.set_stream_waves(waves)
+    ssa.set_kernel_shape(32, 4)
+    try:
+        for t, bw in [(O.SW, 8), (O.SW, 16), (O.NW, 16), (O.NW, 8)]:
+            check(t, bw, O.AMINOACID, 1, qraw, db, 20, ("builtin", "blosum62"), (-10, -1), align=(bw == 16))
+    finally:
+        ssa.set_stream_waves(0)
+        ssa.set_kernel_shape(0, 0)
