@@ -90,6 +90,23 @@ def cpu_baseline(w, sample_frac):
                                                                              threads, secs)), (n, tops)
 
 
+def copy_probe(mib):
+    """One hipMemcpy device-to-device of a known size: the reference point that
+    turns the FETCH_SIZE / WRITE_SIZE counters of a profile into bytes."""
+    hip = C.CDLL("libamdhip64.so")
+    n = C.c_size_t(mib << 20)
+    a, b = C.c_void_p(), C.c_void_p()
+    for p in (a, b):
+        if hip.hipMalloc(C.byref(p), n) != 0:
+            raise SystemExit("hipMalloc failed")
+    hip.hipMemset(a, 1, n)
+    hip.hipDeviceSynchronize()
+    hip.hipMemcpy(b, a, n, 3)  # hipMemcpyDeviceToDevice
+    hip.hipDeviceSynchronize()
+    hip.hipFree(a)
+    hip.hipFree(b)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -98,6 +115,8 @@ def main():
     ap.add_argument("--workload", default="c2")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-sample", type=float, default=0.25)
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="also run one device-to-device copy of MIB MiB (calibrates HBM byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU only: this build has no multi-GPU path (--gpus %d requested)" % args.gpus)
@@ -173,6 +192,8 @@ def main():
             "traceback": round(sum(s.ms_traceback for s in stats) / args.steps, 3)},
         "cpu_baseline": None,
     }
+    if args.copy_probe:
+        copy_probe(args.copy_probe)
     if not args.no_cpu_baseline:
         cb, check = cpu_baseline(w, args.cpu_sample)
         out["cpu_baseline"] = cb

@@ -85,32 +85,49 @@ struct Log2<1> {
     static constexpr int value = 0;
 };
 
-// R profile scores of one lane for one DB code: R/2 dwords, rows (2m, 2m+1) in dword m
-template <int R>
-__device__ __forceinline__ void read_rows(const char* p, uint32_t* d) {
+// Part q of the profile scores of one lane for one DB code: rows 8q .. 8q+7
+// (R >= 8) or rows 0 .. 3 (R = 4), two rows per dword.
+template <int R, int Q>
+__device__ __forceinline__ void read_part(const char* p, uint32_t* d) {
     if constexpr (R == 4) {
         const uint2 v = *reinterpret_cast<const uint2*>(p);
         d[0] = v.x;
         d[1] = v.y;
     } else {
-#pragma unroll
-        for (int q = 0; q < R / 8; ++q) {
-            const uint4 v = *reinterpret_cast<const uint4*>(p + 16 * q);
-            d[4 * q + 0] = v.x;
-            d[4 * q + 1] = v.y;
-            d[4 * q + 2] = v.z;
-            d[4 * q + 3] = v.w;
-        }
+        const uint4 v = *reinterpret_cast<const uint4*>(p + 16 * Q);
+        d[4 * Q + 0] = v.x;
+        d[4 * Q + 1] = v.y;
+        d[4 * Q + 2] = v.z;
+        d[4 * Q + 3] = v.w;
     }
 }
 
+template <int R>
+__device__ __forceinline__ void read_rows(const char* p, uint32_t* d) {
+    read_part<R, 0>(p, d);
+    if constexpr (R == 16) read_part<R, 1>(p, d);
+}
+
+// the packed (lo, hi) score pair of row r from the two halves' profile dwords
+__device__ __forceinline__ s16x2 pair_row(const uint32_t* lo, const uint32_t* hi, int r) {
+    return as_v(__builtin_amdgcn_perm(hi[r / 2], lo[r / 2], (r & 1) ? 0x07060302u : 0x05040100u));
+}
+
+// Pins the instruction order of the cell loop.  gfx950 stalls a packed VALU
+// op that reads the result of the packed op issued just before it (the
+// compiler pads such pairs with s_nop); the cell loop below is written so
+// that every dependent op has an independent one in front of it.
+#define SSA_PIN() __builtin_amdgcn_sched_barrier(0)
+
+// SW: the group's bottom lane adds the maximum over the group's lanes (handed
+// down with the pair's last block); NW: the lane holding the last query row
+// writes the score.
 template <int MODE>
 __device__ __forceinline__ void finish_pair(PairResult* results, int pair, bool score_lane, int score, int fa,
                                             int fb) {
-    if (MODE == 0) {
-        atomicMax(&results[pair].score, score);
-    } else if (score_lane) {
-        results[pair].score = score;
+    if (score_lane) {
+        if (MODE == 0) atomicMax(&results[pair].score, score);
+        else results[pair].score = score;
     }
     if (fa != kNoFlag) atomicMin(&results[pair].flag_a, fa);
     if (MODE == 0 && fb != kNoFlag) atomicMin(&results[pair].flag_b, fb);
@@ -129,6 +146,8 @@ __device__ __forceinline__ void run_pass(const ScoreParams& P, uint32_t gbase, c
                                          int last_r) {
     constexpr int NG = 64 / G;
     constexpr int LPR = Log2<G * R>::value;
+    // byte offset of the lane's rows for column c's DB code in the staged profile
+    auto colofs = [gbase](uint32_t word, int c) { return (((word >> (8 * c)) & 0x3eu) << LPR) + gbase; };
     const s16x2 QR = as_v(P.qr), RR = as_v(P.r), Z = as_v(P.zero);
     // per-half thresholds of the overflow predicates; a half whose predicate
     // cannot fire (flag taken, or no pair running) holds the neutral value
@@ -140,7 +159,7 @@ __device__ __forceinline__ void run_pass(const ScoreParams& P, uint32_t gbase, c
     const bool top_lane = g == 0;
     const bool top_in = top_lane && !FIRST;         // top boundary from the previous pass
     const bool bot_out = g == G - 1 && !last_pass;  // bottom row to the next pass
-    const bool score_lane = last_pass && g == last_g;
+    const bool score_lane = MODE == 0 ? g == G - 1 : (last_pass && g == last_g);
     const int nsteps = nb > 0 ? ((nb + G) & ~1) : 0;
 
     s16x2 Hl[R], El[R];
@@ -160,6 +179,7 @@ __device__ __forceinline__ void run_pass(const ScoreParams& P, uint32_t gbase, c
     }
     uint32_t obh[4] = {0, 0, 0, 0}, obf[4] = {0, 0, 0, 0};  // bottom row H, F of the last block
     uint32_t TA = TA_off, TB = TB_off;
+    uint32_t scout = as_u(Z);  // SW: max over this lane and the lanes above for the pair it finished last
     int kp_lo = 0, kp_hi = 0, pair_lo = -1, pair_hi = -1, blk_lo = 0, blk_hi = 0;
     int fa_lo = kNoFlag, fa_hi = kNoFlag, fb_lo = kNoFlag, fb_hi = kNoFlag;
 
@@ -167,6 +187,10 @@ __device__ __forceinline__ void run_pass(const ScoreParams& P, uint32_t gbase, c
     const uint4* tp = wbnd + (int64_t)k * 2;          // top lane: boundary of block t
 
     auto step = [&](int t, const uint2 w, const uint4 t0, const uint4 t1, uint2& wn, uint4& tn0, uint4& tn1) {
+        // profile dwords of the current column (lo, hi halves); column 0's now
+        uint32_t pl[R / 2], ph[R / 2];
+        read_rows<R>(sp + colofs(w.x, 0), pl);
+        read_rows<R>(sp + colofs(w.y, 0), ph);
         const int b = t - g;
         const bool valid = (unsigned)b < (unsigned)nb;
         wp += NG;
@@ -266,32 +290,55 @@ __device__ __forceinline__ void run_pass(const ScoreParams& P, uint32_t gbase, c
         const int z_lo = (int)((w.x >> kResZShift) & 3u), z_hi = (int)((w.y >> kResZShift) & 3u);
 #pragma unroll
         for (int c = 0; c < 4; ++c) {
-            const uint32_t alo = (((w.x >> (8 * c)) & 0x3eu) << LPR) + gbase;
-            const uint32_t ahi = (((w.y >> (8 * c)) & 0x3eu) << LPR) + gbase;
-            uint32_t dlo[R / 2], dhi[R / 2];
-            read_rows<R>(sp + alo, dlo);
-            read_rows<R>(sp + ahi, dhi);
-            s16x2 hd = htop;
             s16x2 f = as_v(tbf[c]);
+            s16x2 fR = adds(f, RR);
+            SSA_PIN();
+            s16x2 v = pair_row(pl, ph, 0);
+            SSA_PIN();
+            s16x2 h1 = adds(htop, v);  // row 0: diagonal H from the row above
+            SSA_PIN();
             htop = as_v(tbh[c]);
 #pragma unroll
             for (int r = 0; r < R; ++r) {
-                const s16x2 v =
-                    as_v(__builtin_amdgcn_perm(dhi[r / 2], dlo[r / 2], (r & 1) ? 0x07060302u : 0x05040100u));
-                s16x2 h = adds(hd, v);
-                h = vmax(h, f);
+                const bool more = r + 1 < R;
+                if (more) v = pair_row(pl, ph, r + 1);
+                SSA_PIN();
+                // the next column's profile rows, once this column's dwords are consumed
+                if (c < 3) {
+                    if ((R == 4 && r == 3) || (R >= 8 && r == 7)) {
+                        read_part<R, 0>(sp + colofs(w.x, c + 1), pl);
+                        read_part<R, 0>(sp + colofs(w.y, c + 1), ph);
+                    }
+                    if (R == 16 && r == 15) {
+                        read_part<R, 1>(sp + colofs(w.x, c + 1), pl);
+                        read_part<R, 1>(sp + colofs(w.y, c + 1), ph);
+                    }
+                }
+                s16x2 h = vmax(h1, f);
+                SSA_PIN();
+                const s16x2 eR = adds(El[r], RR);
+                SSA_PIN();
                 h = vmax(h, El[r]);
+                SSA_PIN();
+                if (more) h1 = adds(Hl[r], v);  // next row's diagonal term
+                SSA_PIN();
+                const s16x2 tt = adds(h, QR);
+                SSA_PIN();
                 if (MODE == 0) {
                     S = vmax(S, h);
                 } else {
                     hmn = vmin(hmn, h);
+                    SSA_PIN();
                     hmx = vmax(hmx, h);
                 }
-                hd = Hl[r];
+                SSA_PIN();
+                f = vmax(fR, tt);
+                SSA_PIN();
+                El[r] = vmax(eR, tt);
+                SSA_PIN();
+                if (more) fR = adds(f, RR);
+                SSA_PIN();
                 Hl[r] = h;
-                const s16x2 tt = adds(h, QR);
-                El[r] = vmax(adds(El[r], RR), tt);
-                f = vmax(adds(f, RR), tt);
             }
             obh[c] = as_u(Hl[R - 1]);
             obf[c] = as_u(f);
@@ -308,14 +355,9 @@ __device__ __forceinline__ void run_pass(const ScoreParams& P, uint32_t gbase, c
                 }
             }
         }
-        if (bot_out && valid) {
-            const int64_t o = ((int64_t)b * NG + k) * 2;
-            wbnd[o] = make_uint4(obh[0], obf[0], obh[1], obf[1]);
-            wbnd[o + 1] = make_uint4(obh[2], obf[2], obh[3], obf[3]);
-        }
-
-        // overflow predicates at the end of the block: a packed test, then the
-        // exact per-half bookkeeping only where a predicate fired
+        // overflow predicates at the end of the block: a packed test here (in
+        // the cell loop's block, so the running reductions stay in the loop),
+        // the exact per-half bookkeeping below only where a predicate fired
         bool fire;
         if (MODE == 0) {
             fire = as_u(vmax(S, as_v(TA))) != TA || as_u(vmax(S, as_v(TB))) != TB;
@@ -346,6 +388,12 @@ __device__ __forceinline__ void run_pass(const ScoreParams& P, uint32_t gbase, c
             TA = (offa & TA_off) | (~offa & TA);
             TB = (offb & TB_off) | (~offb & TB);
         }
+        if (bot_out && valid) {
+            const int64_t o = ((int64_t)b * NG + k) * 2;
+            wbnd[o] = make_uint4(obh[0], obf[0], obh[1], obf[1]);
+            wbnd[o + 1] = make_uint4(obh[2], obf[2], obh[3], obf[3]);
+        }
+
         if (MODE == 1 && FIRST) {  // the reference advances the top border by four columns per block
             fb[0] = adds(fb[3], RR);
             fb[1] = adds(fb[0], RR);
@@ -357,14 +405,19 @@ __device__ __forceinline__ void run_pass(const ScoreParams& P, uint32_t gbase, c
             hb[3] = adds(hb[2], RR);
         }
         if (any_last) {
+            if (MODE == 0) {  // the lane above finished the same pair one step earlier
+                const uint32_t scin = shift_down<G>(as_u(Z), scout, top_lane);
+                scout = as_u(vmax(S, as_v(scin)));
+            }
+            const s16x2 fin = MODE == 0 ? as_v(scout) : cap;
             uint32_t m = 0u;
             if (ls_lo && pair_lo >= 0) {
-                finish_pair<MODE>(results, pair_lo, score_lane, MODE == 0 ? lo16(S) : lo16(cap), fa_lo, fb_lo);
+                finish_pair<MODE>(results, pair_lo, score_lane, lo16(fin), fa_lo, fb_lo);
                 pair_lo = -1;
                 m |= 0x0000ffffu;
             }
             if (ls_hi && pair_hi >= 0) {
-                finish_pair<MODE>(results, pair_hi, score_lane, MODE == 0 ? hi16(S) : hi16(cap), fa_hi, fb_hi);
+                finish_pair<MODE>(results, pair_hi, score_lane, hi16(fin), fa_hi, fb_hi);
                 pair_hi = -1;
                 m |= 0xffff0000u;
             }

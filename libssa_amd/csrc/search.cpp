@@ -210,19 +210,39 @@ namespace {
 
 Shape shape_override() { return shape_override_ref(); }
 
-// Kernel shape for a query length: fewest padded rows per pass (weighted by
-// the per-row cost of short lanes), the wider lane group on ties.
-Shape choose_shape(size_t qlen_max) {
+int& target_waves() {
+    static int w = 10240;
+    return w;
+}
+
+// Kernel shape from a cost model in wave-step units (one wave-step = one
+// block of 4 columns on every lane of a wave, R rows per lane).  The chip
+// works through passes * sum(wave blocks + G - 1) wave-steps on 1024 SIMDs;
+// the wave holding the longest stream needs passes * (longest + G - 1) steps
+// on its own and shares its SIMD for part of that time (factor 2).  A
+// wave-step costs R rows times a per-row factor that charges the fixed
+// per-step work (DPP moves, addressing, pair bookkeeping) to short lanes.
+Shape choose_shape(size_t qlen_max, int64_t total_blocks, int64_t max_blocks) {
     const Shape ov = shape_override();
     if (ov.lanes && ov.rows) return ov;
-    static const Shape kShapes[] = {{64, 8}, {64, 4}, {32, 16}, {32, 8}, {32, 4}, {16, 16},
-                                    {16, 8}, {16, 4}, {8, 16},  {8, 8},  {8, 4}};
+    static const Shape kShapes[] = {{64, 16}, {32, 16}, {16, 16}, {8, 16}, {64, 8}, {32, 8},
+                                    {16, 8},  {8, 8},   {64, 4},  {32, 4}, {16, 4}, {8, 4}};
     Shape best = kShapes[0];
     double best_cost = 1e300;
     for (const Shape& sh : kShapes) {
-        const size_t pr = (size_t)sh.lanes * sh.rows;
-        const size_t passes = std::max<size_t>(1, (qlen_max + pr - 1) / pr);
-        const double cost = (double)(passes * pr) * (sh.rows == 4 ? 1.15 : 1.0);
+        const int64_t pr = (int64_t)sh.lanes * sh.rows;
+        const int64_t passes = std::max<int64_t>(1, ((int64_t)qlen_max + pr - 1) / pr);
+        // measured per-row costs: short lanes pay the fixed per-step work; G = 8
+        // and 32 fix up their top lanes after the DPP shift; G = 64 at R = 16
+        // stages 64 KiB of profile per workgroup (two workgroups per CU)
+        double per_row = sh.rows == 4 ? 1.32 : (sh.rows == 8 ? 1.10 : 1.0);
+        if (sh.lanes == 8 || sh.lanes == 32) per_row *= 1.005;
+        if (sh.lanes == 64 && sh.rows == 16) per_row *= 1.035;
+        const double waves = std::max<double>(target_waves(), 1.0);
+        const double work =
+            (double)passes * ((double)total_blocks * sh.lanes / 128.0 + waves * (sh.lanes - 1)) / 1024.0;
+        const double tail = 2.0 * (double)passes * (double)(max_blocks + sh.lanes - 1);
+        const double cost = std::max(work, tail) * sh.rows * per_row;
         if (cost < best_cost) {
             best = sh;
             best_cost = cost;
@@ -259,11 +279,6 @@ struct StreamLayout {
         d_res = nullptr;
     }
 };
-
-int& target_waves() {
-    static int w = 10240;
-    return w;
-}
 
 void build_layout(StreamLayout& L, int lanes, const PairSet& ps, const std::vector<uint32_t>& order) {
     DeviceDB& db = g_db;
@@ -386,7 +401,13 @@ void gpu_score_tier(SearchType type, const Tier& T, const std::vector<QueryBuf>&
 
     size_t qmax = 0;
     for (const QueryBuf& q : qs) qmax = std::max(qmax, q.len);
-    const Shape sh = choose_shape(qmax);
+    int64_t total_blocks = 0, max_blocks = 0;
+    for (uint32_t e : ps.entries) {
+        const int64_t nb = (db.entries[e].len + 3) / 4;
+        total_blocks += nb;
+        max_blocks = std::max(max_blocks, nb);
+    }
+    const Shape sh = choose_shape(qmax, total_blocks, max_blocks);
     if (!score_shape_supported(sh.lanes, sh.rows)) fatal("libssa_amd: unsupported kernel shape");
     *shape_used = sh;
     const int PR = sh.lanes * sh.rows, ng = 64 / sh.lanes, spw = 2 * ng;
