@@ -27,6 +27,20 @@ sys.path.insert(0, ROOT)
 
 PEAK_VALU_TOPS = 256 * 64 * 2 * 2.4e9 / 1e12  # packed int16 element-ops/s: 256 CU x 64 lanes x 2 x 2.4 GHz
 OPS_PER_CELL = {"sw": 9, "nw": 8}             # algorithmic int ops per cell of the reference recurrence
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "score_kernel_hbm_traffic.txt")
+
+
+def measured_traffic(workload):
+    """HBM bytes per step of the score kernels, as measured by the committed
+    rocprofv3 FETCH_SIZE / WRITE_SIZE runs of this workload (None if absent)."""
+    if not os.path.exists(TRAFFIC_FILE):
+        return None
+    with open(TRAFFIC_FILE) as f:
+        for line in f:
+            parts = line.split()
+            if len(parts) == 3 and parts[0] == "bytes_per_step" and parts[1] == workload:
+                return int(parts[2])
+    return None
 
 
 def workload(name):
@@ -114,7 +128,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="c2")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-sample", type=float, default=0.25)
+    ap.add_argument("--cpu-sample", type=float, default=1.0)
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="also run one device-to-device copy of MIB MiB (calibrates HBM byte counters)")
     args = ap.parse_args()
@@ -182,8 +196,10 @@ def main():
         "config": {"workload": w["desc"], "db_sequences": len(w["db"]),
                    "query_lengths": [len(q) for q in w["queries"]], "hits": w["k"], "bit_width": w["bw"]},
         "roofline": {"bound": "valu", "achieved": round(achieved, 3), "peak": round(PEAK_VALU_TOPS, 1),
-                     "unit": "TOP/s", "frac": round(achieved / PEAK_VALU_TOPS, 4), "traffic": None,
-                     "kernel": "score_kernel (first tier)", "ops_per_cell": OPS_PER_CELL[w["kind"]],
+                     "unit": "TOP/s", "frac": round(achieved / PEAK_VALU_TOPS, 4),
+                     "traffic": measured_traffic(args.workload),
+                     "kernel": "score_kernel, all first-tier launches of a step",
+                     "ops_per_cell": OPS_PER_CELL[w["kind"]],
                      "kernel_ms_per_step": round(kernel_ms / args.steps, 3)},
         "breakdown_ms_per_step": {
             "gpu_first_tier": round(kernel_ms / args.steps, 3),
