@@ -45,6 +45,66 @@ void ssa_amd_set_kernel_shape(int lanes_per_stream, int rows_per_lane);
  * Streams never get shorter than 16 blocks per lane of a group. */
 void ssa_amd_set_stream_waves(int waves);
 
+/* ---- batched searches ---- */
+
+/* One search of a batch: the arguments of sw_align (search_type 0) or
+ * nw_align (search_type 1). */
+typedef struct {
+    int search_type;
+    p_query query;
+    size_t hitcount;
+    int bit_width;
+    int align_type;
+} ssa_amd_request_t;
+
+/* Runs n searches as one batch with the current settings: the first-tier
+ * launches of all of them go to the GPU as one captured HIP graph, searches
+ * with the same type, tier and kernel shape share one kernel launch, and the
+ * host replays run in parallel.  out[i] receives exactly the list that
+ * sw_align / nw_align returns for reqs[i] alone (free each with
+ * free_alignment, before freeing its query); stats (NULL allowed) receives one
+ * record per search.  Returns 0; -1 for an invalid request, -2 when scoring is
+ * not initialised (nothing runs in either case).  Not re-entrant, like the
+ * libssa.h entry points. */
+int ssa_amd_batch_align(const ssa_amd_request_t* reqs, size_t n, p_alignment_list* out, ssa_amd_stats_t* stats);
+
+/* ---- serving: continuous batching over the resident DB ---- */
+
+typedef struct ssa_amd_server ssa_amd_server;
+
+typedef struct {
+    size_t batches;         /* batches run so far */
+    size_t requests;        /* requests taken into batches */
+    size_t max_batch_seen;  /* largest batch */
+    size_t queued;          /* requests waiting now */
+    size_t device_budget;   /* device bytes the admission works with */
+} ssa_amd_server_counters_t;
+
+/* Starts a request loop with `slots` workers (1-8), each with its own device
+ * stream and buffers; a free worker takes the waiting requests, in arrival
+ * order and at most `max_batch` of them, as one batch, admitting requests
+ * while the batch's estimated device footprint fits its share of
+ * `device_bytes` (0 = half the free device memory).  Uses the settings current
+ * at the time (scoring, gaps, symbol type, DB); they must not change while the
+ * server runs.  Returns NULL if the arguments or the settings are invalid. */
+ssa_amd_server* ssa_amd_server_start(int slots, size_t max_batch, size_t device_bytes);
+
+/* Queues one search; returns its ticket (>= 0) or -1 for an invalid request.
+ * The query must stay alive until the result has been taken and freed. */
+int64_t ssa_amd_server_submit(ssa_amd_server* server, const ssa_amd_request_t* request);
+
+/* Waits for a ticket's result: the list sw_align / nw_align returns for the
+ * request alone (the caller frees it with free_alignment).  stats, latency_ms
+ * (submit to result) and queue_ms (submit to batch start) may be NULL.
+ * Returns NULL for an unknown ticket. */
+p_alignment_list ssa_amd_server_wait(ssa_amd_server* server, int64_t ticket, ssa_amd_stats_t* stats,
+                                     double* latency_ms, double* queue_ms);
+
+void ssa_amd_server_counters(ssa_amd_server* server, ssa_amd_server_counters_t* out);
+
+/* Finishes every queued request, then stops the workers; results not taken are freed. */
+void ssa_amd_server_stop(ssa_amd_server* server);
+
 /* The current 32x32 score matrix as the search uses it ([db code * 32 + query code]). */
 void ssa_amd_get_matrix(int64_t* out);
 

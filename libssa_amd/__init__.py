@@ -51,6 +51,21 @@ class _AlignmentList(_C.Structure):
     _fields_ = [("alignments", _C.POINTER(_C.POINTER(_Alignment))), ("len", _C.c_size_t)]
 
 
+class _Request(_C.Structure):
+    """ssa_amd_request_t (include/libssa_amd.h)."""
+    _fields_ = [("search_type", _C.c_int), ("query", _C.c_void_p), ("hitcount", _C.c_size_t),
+                ("bit_width", _C.c_int), ("align_type", _C.c_int)]
+
+
+SEARCH_SW, SEARCH_NW = 0, 1
+
+
+class ServerCounters(_C.Structure):
+    """ssa_amd_server_counters_t (include/libssa_amd.h)."""
+    _fields_ = [("batches", _C.c_size_t), ("requests", _C.c_size_t), ("max_batch_seen", _C.c_size_t),
+                ("queued", _C.c_size_t), ("device_budget", _C.c_size_t)]
+
+
 class Stats(_C.Structure):
     """ssa_amd_stats_t (include/libssa_amd.h)."""
     _fields_ = [("chunk_count", _C.c_size_t), ("seq_count", _C.c_size_t),
@@ -94,6 +109,14 @@ def _load():
         "ssa_db_init": (I, [cp]), "ssa_db_get_sequence_count": (S, []), "ssa_db_close": (None, []),
         "ssa_amd_last_stats": (None, [_C.POINTER(Stats)]), "ssa_amd_set_kernel_shape": (None, [I, I]), "ssa_amd_set_stream_waves": (None, [I]),
         "ssa_amd_get_matrix": (None, [P]), "ssa_amd_device_arch": (cp, []),
+        "ssa_amd_batch_align": (I, [_C.POINTER(_Request), S, _C.POINTER(_C.POINTER(_AlignmentList)),
+                                    _C.POINTER(Stats)]),
+        "ssa_amd_server_start": (P, [I, S, S]),
+        "ssa_amd_server_submit": (_C.c_int64, [P, _C.POINTER(_Request)]),
+        "ssa_amd_server_wait": (_C.POINTER(_AlignmentList), [P, _C.c_int64, _C.POINTER(Stats),
+                                                             _C.POINTER(_C.c_double), _C.POINTER(_C.c_double)]),
+        "ssa_amd_server_counters": (None, [P, _C.POINTER(ServerCounters)]),
+        "ssa_amd_server_stop": (None, [P]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -189,3 +212,64 @@ def get_matrix():
 
 def device_arch():
     return lib.ssa_amd_device_arch().decode()
+
+
+def batch_align(requests):
+    """Runs searches as one batch.  requests: (search_type, query, hitcount,
+    bit_width, align_type) tuples with search_type SEARCH_SW or SEARCH_NW.
+    Returns (alignment lists, Stats records), one of each per request."""
+    n = len(requests)
+    if n == 0:
+        return [], []
+    arr = (_Request * n)(*[_Request(t, q, k, bw, al) for (t, q, k, bw, al) in requests])
+    outs = (_C.POINTER(_AlignmentList) * n)()
+    stats = (Stats * n)()
+    rc = lib.ssa_amd_batch_align(arr, n, outs, stats)
+    if rc != 0:
+        raise ValueError("ssa_amd_batch_align rejected the batch (code %d)" % rc)
+    lists = []
+    for i in range(n):
+        try:
+            lists.append(_convert(outs[i]))
+        finally:
+            lib.free_alignment(outs[i])
+    return lists, list(stats)
+
+
+class Server:
+    """Continuous-batching request loop (ssa_amd_server_*): submit() queues a
+    search and returns a ticket at once; wait(ticket) returns (alignments,
+    Stats, latency_ms, queue_ms).  Queries must outlive their results."""
+
+    def __init__(self, slots=2, max_batch=64, device_bytes=0):
+        self.h = lib.ssa_amd_server_start(slots, max_batch, device_bytes)
+        if not self.h:
+            raise ValueError("ssa_amd_server_start failed (arguments or settings invalid)")
+
+    def submit(self, search_type, query, hitcount, bit_width, align_type):
+        r = _Request(search_type, query, hitcount, bit_width, align_type)
+        t = lib.ssa_amd_server_submit(self.h, _C.byref(r))
+        if t < 0:
+            raise ValueError("invalid request")
+        return t
+
+    def wait(self, ticket):
+        st = Stats()
+        lat, q = _C.c_double(), _C.c_double()
+        p = lib.ssa_amd_server_wait(self.h, ticket, _C.byref(st), _C.byref(lat), _C.byref(q))
+        if not p:
+            raise KeyError(ticket)
+        try:
+            return _convert(p), st, lat.value, q.value
+        finally:
+            lib.free_alignment(p)
+
+    def counters(self):
+        c = ServerCounters()
+        lib.ssa_amd_server_counters(self.h, _C.byref(c))
+        return c
+
+    def stop(self):
+        if self.h:
+            lib.ssa_amd_server_stop(self.h)
+            self.h = None

@@ -1,6 +1,7 @@
 // api.cpp -- the libssa.h entry points (reference: src/libssa.c:41-271).
 #include <cstdlib>
 #include <cstring>
+#include <vector>
 
 #include "common.h"
 #include "gpu.h"
@@ -151,6 +152,32 @@ void ssa_amd_set_kernel_shape(int lanes_per_stream, int rows_per_lane) {
 void ssa_amd_set_stream_waves(int waves) {
     if (waves < 0) fatal("libssa_amd: stream waves must be >= 0 (got %d)", waves);
     set_target_waves(waves);
+}
+
+int ssa_amd_batch_align(const ssa_amd_request_t* reqs, size_t n, p_alignment_list* out, ssa_amd_stats_t* stats) {
+    if (n == 0) return 0;
+    if (!reqs || !out) return -1;
+    const Config& c = config();
+    if (!matrix().valid) return -2;
+    if (!matrix().constant && c.symtype == NUCLEOTIDE) return -2;
+    std::vector<SearchRequest> rq(n);
+    for (size_t i = 0; i < n; ++i) {
+        const ssa_amd_request_t& r = reqs[i];
+        if (r.search_type != 0 && r.search_type != 1) return -1;
+        if (!r.query) return -1;
+        if (r.bit_width != BIT_WIDTH_8 && r.bit_width != BIT_WIDTH_16 && r.bit_width != BIT_WIDTH_64) return -1;
+        if (r.align_type != COMPUTE_SCORE && r.align_type != COMPUTE_ALIGNMENT) return -1;
+        if (r.bit_width == BIT_WIDTH_8 && c.simd_mode < COMPUTE_ON_SSE41) return -1;
+        rq[i] = SearchRequest{r.search_type == 0 ? kSW : kNW, r.query, r.hitcount, r.bit_width, r.align_type};
+    }
+    std::vector<SearchResult> res;
+    run_batch(Gpu::get().exec(), rq, res, true);
+    for (size_t i = 0; i < n; ++i) {
+        out[i] = res[i].list;
+        if (stats) stats[i] = res[i].stats;
+    }
+    last_stats() = res[n - 1].stats;
+    return 0;
 }
 
 void ssa_amd_get_matrix(int64_t* out) { memcpy(out, matrix().s, sizeof(matrix().s)); }

@@ -39,7 +39,7 @@ Gpu::Gpu() {
     SSA_HIP(hipGetDeviceProperties(&prop, device_));
     cus_ = prop.multiProcessorCount;
     arch_ = prop.gcnArchName;
-    SSA_HIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
+    exec_ = new Exec();
 }
 
 Gpu& Gpu::get() {
@@ -47,23 +47,38 @@ Gpu& Gpu::get() {
     return g;
 }
 
-void Gpu::upload(void* dst, const void* src, size_t bytes) {
+Exec::Exec() { SSA_HIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking)); }
+
+Exec::~Exec() {
+    release_all();
+    for (hipStream_t& a : aux_)
+        if (a) (void)hipStreamDestroy(a);
+    if (stream_) (void)hipStreamDestroy(stream_);
+}
+
+hipStream_t Exec::aux(size_t i) {
+    hipStream_t& a = aux_[i % 4];
+    if (!a) SSA_HIP(hipStreamCreateWithFlags(&a, hipStreamNonBlocking));
+    return a;
+}
+
+void Exec::upload(void* dst, const void* src, size_t bytes) {
     if (bytes) SSA_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, stream_));
 }
 
-void Gpu::download(void* dst, const void* src, size_t bytes) {
+void Exec::download(void* dst, const void* src, size_t bytes) {
     if (bytes) SSA_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, stream_));
 }
 
-void Gpu::sync() { SSA_HIP(hipStreamSynchronize(stream_)); }
+void Exec::sync() { SSA_HIP(hipStreamSynchronize(stream_)); }
 
-void Gpu::release_all() {
-    sync();
+void Exec::release_all() {
+    if (stream_) (void)hipStreamSynchronize(stream_);
     for (auto& kv : arenas_) kv.second.release();
     arenas_.clear();
 }
 
-GpuTimer::GpuTimer() {
+GpuTimer::GpuTimer(hipStream_t st) : st_(st) {
     SSA_HIP(hipEventCreate(&a_));
     SSA_HIP(hipEventCreate(&b_));
 }
@@ -73,8 +88,8 @@ GpuTimer::~GpuTimer() {
     if (b_) (void)hipEventDestroy(b_);
 }
 
-void GpuTimer::start() { SSA_HIP(hipEventRecord(a_, Gpu::get().stream())); }
-void GpuTimer::stop() { SSA_HIP(hipEventRecord(b_, Gpu::get().stream())); }
+void GpuTimer::start() { SSA_HIP(hipEventRecord(a_, st_)); }
+void GpuTimer::stop() { SSA_HIP(hipEventRecord(b_, st_)); }
 
 float GpuTimer::ms() {
     SSA_HIP(hipEventSynchronize(b_));

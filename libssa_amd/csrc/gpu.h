@@ -25,38 +25,56 @@ struct DeviceArena {
     void release();
 };
 
-class Gpu {
+// A device work context: one stream and the growable buffers of the work
+// issued on it.  Work on different contexts may run concurrently; one context
+// is used by one host thread at a time.
+class Exec {
 public:
-    static Gpu& get();  // initialises the device on first use
+    Exec();
+    ~Exec();
+    Exec(const Exec&) = delete;
+    Exec& operator=(const Exec&) = delete;
     hipStream_t stream() const { return stream_; }
+    hipStream_t aux(size_t i);  // extra streams for graph branches (4, created on first use)
     void* arena(const std::string& name, size_t bytes) { return arenas_[name].get(bytes); }
     void upload(void* dst, const void* src, size_t bytes);
     void download(void* dst, const void* src, size_t bytes);
     void sync();
     void release_all();
+
+private:
+    hipStream_t stream_ = nullptr;
+    hipStream_t aux_[4] = {nullptr, nullptr, nullptr, nullptr};
+    std::unordered_map<std::string, DeviceArena> arenas_;
+};
+
+class Gpu {
+public:
+    static Gpu& get();  // initialises the device on first use
+    Exec& exec() { return *exec_; }  // the context of the libssa.h entry points
     int device() const { return device_; }
     std::string arch() const { return arch_; }
     int cu_count() const { return cus_; }
 
 private:
     Gpu();
-    hipStream_t stream_ = nullptr;
     int device_ = 0;
     int cus_ = 0;
     std::string arch_;
-    std::unordered_map<std::string, DeviceArena> arenas_;
+    Exec* exec_ = nullptr;
 };
 
-// event pair timing a span of work on the context stream
+// event pair timing a span of work on one stream
 class GpuTimer {
 public:
-    GpuTimer();
+    explicit GpuTimer(hipStream_t st);
     ~GpuTimer();
     void start();
     void stop();
     float ms();  // synchronises on the stop event
 
 private:
+    hipStream_t st_ = nullptr;
     hipEvent_t a_ = nullptr, b_ = nullptr;
 };
 

@@ -16,6 +16,9 @@
 #include <climits>
 #include <cstring>
 #include <numeric>
+#include <list>
+#include <memory>
+#include <mutex>
 #include <queue>
 #include <thread>
 
@@ -280,9 +283,8 @@ struct StreamLayout {
     }
 };
 
-void build_layout(StreamLayout& L, int lanes, const PairSet& ps, const std::vector<uint32_t>& order) {
+void build_layout(Exec& ex, StreamLayout& L, int lanes, const PairSet& ps, const std::vector<uint32_t>& order) {
     DeviceDB& db = g_db;
-    Gpu& gpu = Gpu::get();
     const int ng = 64 / lanes, spw = 2 * ng;
     const size_t n = ps.entries.size();
     L.lanes = lanes;
@@ -363,94 +365,139 @@ void build_layout(StreamLayout& L, int lanes, const PairSet& ps, const std::vect
         }
     L.release();
     SSA_HIP(hipMalloc(&L.d_res, (size_t)std::max<int64_t>(L.res_words, 1) * sizeof(uint2)));
-    auto* d_pe = (PackEntry*)gpu.arena("pack", std::max<size_t>(pe.size(), 1) * sizeof(PackEntry));
-    gpu.upload(d_pe, pe.data(), pe.size() * sizeof(PackEntry));
-    SSA_HIP(launch_pack((int)pe.size(), gpu.stream(), d_pe, db.d_codes, reinterpret_cast<uint32_t*>(L.d_res),
+    auto* d_pe = (PackEntry*)ex.arena("pack", std::max<size_t>(pe.size(), 1) * sizeof(PackEntry));
+    ex.upload(d_pe, pe.data(), pe.size() * sizeof(PackEntry));
+    SSA_HIP(launch_pack((int)pe.size(), ex.stream(), d_pe, db.d_codes, reinterpret_cast<uint32_t*>(L.d_res),
                         L.res_words * 2));
-    gpu.sync();  // pe is staged from pageable memory
+    ex.sync();  // pe is staged from pageable memory
 }
 
-// layouts of the whole DB, kept while the DB stays loaded (one per lane width)
-std::vector<StreamLayout>& layout_cache() {
-    static std::vector<StreamLayout> c;
+// layouts of the whole DB, kept while the DB stays loaded (one per lane width);
+// std::list keeps the addresses of cached layouts stable
+std::list<StreamLayout>& layout_cache() {
+    static std::list<StreamLayout> c;
     return c;
+}
+
+std::mutex& layout_mutex() {
+    static std::mutex m;
+    return m;
 }
 
 }  // namespace
 
 void release_layouts() {
+    std::lock_guard<std::mutex> lk(layout_mutex());
     for (StreamLayout& L : layout_cache()) L.release();
     layout_cache().clear();
 }
 
 namespace {
 
-// Scores all pairs of the set at one tier.  out receives the raw per-pair
-// results in PairSet::index order.
-void gpu_score_tier(SearchType type, const Tier& T, const std::vector<QueryBuf>& qs, const PairSet& ps,
-                    const std::vector<uint32_t>* sorted_all, std::vector<PairResult>& out, double* ms,
-                    uint64_t* cells_computed, Shape* shape_used) {
-    Gpu& gpu = Gpu::get();
+// One launch of the score kernel over the pairs (q, e) for all queries q of
+// qs and the entries of ps, in three parts: prepare() builds and uploads the
+// inputs, launch() issues the kernels (no host synchronisation, so a graph
+// can capture it), finish() queues the copy of the results back.  Results are
+// in PairSet::index order over qs.
+struct TierRun {
+    SearchType type = kSW;
+    Tier T = kTier16;
+    std::vector<QueryBuf> qs;
+    const PairSet* ps = nullptr;
+    const std::vector<uint32_t>* sorted_all = nullptr;  // ps is the whole DB in this length order
+    Shape sh{0, 0};       // 0, 0 = chosen by the cost model
+    std::string pfx;      // arena names of this run
+    std::vector<PairResult> out;
+    uint64_t computed = 0;
+
+    StreamLayout local;
+    StreamLayout* L = nullptr;
+    int nwg = 0;
+    ScoreParams P;
+    ScoreWave* d_waves = nullptr;
+    QueryDesc* d_qd = nullptr;
+    int16_t* d_prof = nullptr;
+    uint32_t* d_lb = nullptr;
+    int32_t* d_sb = nullptr;
+    int32_t* d_pl = nullptr;
+    uint4* d_bnd = nullptr;
+    PairResult* d_out = nullptr;
+    // host staging, kept until finish
+    std::vector<ScoreWave> waves;
+    std::vector<QueryDesc> qd;
+    std::vector<int16_t> prof;
+    std::vector<uint32_t> lbv;
+    std::vector<int32_t> slot_begin, pair_list;
+
+    ~TierRun() { local.release(); }
+    bool empty() const { return ps == nullptr || ps->entries.empty() || qs.empty(); }
+};
+
+void tier_prepare(Exec& ex, TierRun& R) {
     DeviceDB& db = g_db;
     const Config& cf = config();
     const Matrix& M = matrix();
+    const Tier& T = R.T;
     const int go = cf.gap_open, ge = cf.gap_extend;
-    const size_t n = ps.entries.size(), nq = qs.size();
-    out.assign(ps.size(), PairResult{-32768, kNoFlag, kNoFlag, 0});
-    if (n == 0 || nq == 0) return;
+    const PairSet& ps = *R.ps;
+    const size_t n = ps.entries.size(), nq = R.qs.size();
+    R.out.assign(ps.size(), PairResult{-32768, kNoFlag, kNoFlag, 0});
+    if (R.empty()) return;
 
     size_t qmax = 0;
-    for (const QueryBuf& q : qs) qmax = std::max(qmax, q.len);
-    int64_t total_blocks = 0, max_blocks = 0;
-    for (uint32_t e : ps.entries) {
-        const int64_t nb = (db.entries[e].len + 3) / 4;
-        total_blocks += nb;
-        max_blocks = std::max(max_blocks, nb);
+    for (const QueryBuf& q : R.qs) qmax = std::max(qmax, q.len);
+    if (R.sh.lanes == 0) {
+        int64_t total_blocks = 0, max_blocks = 0;
+        for (uint32_t e : ps.entries) {
+            const int64_t nb = (db.entries[e].len + 3) / 4;
+            total_blocks += nb;
+            max_blocks = std::max(max_blocks, nb);
+        }
+        R.sh = choose_shape(qmax, total_blocks, max_blocks);
     }
-    const Shape sh = choose_shape(qmax, total_blocks, max_blocks);
+    const Shape sh = R.sh;
     if (!score_shape_supported(sh.lanes, sh.rows)) fatal("libssa_amd: unsupported kernel shape");
-    *shape_used = sh;
     const int PR = sh.lanes * sh.rows, ng = 64 / sh.lanes, spw = 2 * ng;
 
     // ---- profiles [pass][code][row], left boundaries, per-query descriptors ----
-    std::vector<QueryDesc> qd(nq);
-    std::vector<int16_t> prof;
-    std::vector<uint32_t> lbv;
+    R.qd.assign(nq, QueryDesc{});
+    R.prof.clear();
+    R.lbv.clear();
     std::vector<int> npass(nq);
     const int64_t QRw = T.wrap(go + ge), Rw = T.wrap(ge);
     for (size_t qi = 0; qi < nq; ++qi) {
-        const QueryBuf& q = qs[qi];
+        const QueryBuf& q = R.qs[qi];
         npass[qi] = (int)((q.len + PR - 1) / PR);
         const size_t rows = (size_t)npass[qi] * PR;
-        qd[qi].qlen = (int32_t)q.len;
-        qd[qi].qp_off = (int32_t)prof.size();
-        qd[qi].lb_off = (int32_t)lbv.size();
+        R.qd[qi].qlen = (int32_t)q.len;
+        R.qd[qi].qp_off = (int32_t)R.prof.size();
+        R.qd[qi].lb_off = (int32_t)R.lbv.size();
         for (int p = 0; p < npass[qi]; ++p)
             for (int d = 0; d < kDim; ++d)
                 for (int r = 0; r < PR; ++r) {
                     const size_t i = (size_t)p * PR + r;
                     int32_t v;
                     if (i < q.len) v = T.repr(T.wrap(M.at(d, q.seq[i])));
-                    else v = type == kSW ? -32768 : 0;
-                    prof.push_back((int16_t)v);
+                    else v = R.type == kSW ? -32768 : 0;
+                    R.prof.push_back((int16_t)v);
                 }
         int64_t c = QRw;
         for (size_t i = 0; i < rows; ++i) {
             if (i < q.len) {
-                lbv.push_back(pack2(T.repr(c)));
-                lbv.push_back(pack2(T.repr(T.sat(c + QRw))));
+                R.lbv.push_back(pack2(T.repr(c)));
+                R.lbv.push_back(pack2(T.repr(T.sat(c + QRw))));
                 c = T.sat(c + Rw);
             } else {
-                lbv.push_back(0u);
-                lbv.push_back(0u);
+                R.lbv.push_back(0u);
+                R.lbv.push_back(0u);
             }
         }
     }
-    ScoreParams P;
+    ScoreParams& P = R.P;
     P.qr = pack2(T.repr(QRw));
     P.r = pack2(T.repr(Rw));
     P.zero = pack2(-32768);
-    if (type == kSW) {
+    if (R.type == kSW) {
         if (T.bits == 16) {
             P.thr_a = 255 - 32768;  // the 8-bit tier's overflow (true maximum >= 255)
             P.thr_b = 32767;        // the 16-bit tier's overflow
@@ -467,16 +514,15 @@ void gpu_score_tier(SearchType type, const Tier& T, const std::vector<QueryBuf>&
     for (int k = 0; k < 4; ++k) P.fb_init[k] = pack2(T.repr(T.wrap(go + (k + 1) * ge)));
 
     // ---- streams: cached for the whole DB, built per call for subsets ----
-    StreamLayout local;
-    StreamLayout* L = nullptr;
-    if (sorted_all) {
+    if (R.sorted_all) {
+        std::lock_guard<std::mutex> lk(layout_mutex());
         for (StreamLayout& c : layout_cache())
-            if (c.lanes == sh.lanes && c.generation == db.generation) L = &c;
-        if (!L) {
+            if (c.lanes == sh.lanes && c.generation == db.generation) R.L = &c;
+        if (!R.L) {
             layout_cache().emplace_back();
-            L = &layout_cache().back();
-            build_layout(*L, sh.lanes, ps, *sorted_all);
-            L->generation = db.generation;
+            R.L = &layout_cache().back();
+            build_layout(ex, *R.L, sh.lanes, ps, *R.sorted_all);
+            R.L->generation = db.generation;
         }
     } else {
         std::vector<uint32_t> order(n);
@@ -484,20 +530,20 @@ void gpu_score_tier(SearchType type, const Tier& T, const std::vector<QueryBuf>&
         std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) {
             return db.entries[ps.entries[a]].len > db.entries[ps.entries[b]].len;
         });
-        build_layout(local, sh.lanes, ps, order);
-        L = &local;
+        build_layout(ex, R.local, sh.lanes, ps, order);
+        R.L = &R.local;
     }
+    const StreamLayout* L = R.L;
 
     // ---- waves of every query ----
     const int nw = L->nwaves;
-    std::vector<ScoreWave> waves((size_t)nq * nw);
-    std::vector<int32_t> slot_begin((size_t)nq * nw * spw);
-    std::vector<int32_t> pair_list(nq * n);
+    R.waves.assign((size_t)nq * nw, ScoreWave{});
+    R.slot_begin.assign((size_t)nq * nw * spw, 0);
+    R.pair_list.assign(nq * n, 0);
     int64_t bnd_total = 0;
-    uint64_t computed = 0;
     for (size_t qi = 0; qi < nq; ++qi) {
         for (int w = 0; w < nw; ++w) {
-            ScoreWave& W = waves[qi * nw + w];
+            ScoreWave& W = R.waves[qi * nw + w];
             W.query = (int32_t)qi;
             W.nblocks = L->wave_nblocks[w];
             W.res_off = L->wave_res_off[w];
@@ -505,48 +551,62 @@ void gpu_score_tier(SearchType type, const Tier& T, const std::vector<QueryBuf>&
             W.slot_base = (int32_t)((qi * nw + w) * spw);
             W.pad = 0;
             if (npass[qi] > 1) bnd_total += (int64_t)W.nblocks * ng * 2;
-            computed += (uint64_t)W.nblocks * 4 * spw * (uint64_t)npass[qi] * PR;
+            R.computed += (uint64_t)W.nblocks * 4 * spw * (uint64_t)npass[qi] * PR;
             for (int sl = 0; sl < spw; ++sl)
-                slot_begin[(qi * nw + w) * spw + sl] = (int32_t)(qi * n) + L->slot_first[(size_t)w * spw + sl];
+                R.slot_begin[(qi * nw + w) * spw + sl] = (int32_t)(qi * n) + L->slot_first[(size_t)w * spw + sl];
         }
-        for (size_t j = 0; j < n; ++j) pair_list[qi * n + j] = (int32_t)ps.index(qi, L->slot_entries[j]);
+        for (size_t j = 0; j < n; ++j) R.pair_list[qi * n + j] = (int32_t)ps.index(qi, L->slot_entries[j]);
     }
-    *cells_computed += computed;
+    R.nwg = (int)((size_t)nq * nw / kWavesPerWG);
 
     // ---- device buffers ----
-    auto* d_waves = (ScoreWave*)gpu.arena("waves", waves.size() * sizeof(ScoreWave));
-    auto* d_qd = (QueryDesc*)gpu.arena("qdesc", qd.size() * sizeof(QueryDesc));
-    auto* d_prof = (int16_t*)gpu.arena("prof", std::max<size_t>(prof.size(), 8) * sizeof(int16_t));
-    auto* d_lb = (uint32_t*)gpu.arena("lb", std::max<size_t>(lbv.size(), 1) * sizeof(uint32_t));
-    auto* d_sb = (int32_t*)gpu.arena("slot_begin", slot_begin.size() * sizeof(int32_t));
-    auto* d_pl = (int32_t*)gpu.arena("pair_list", pair_list.size() * sizeof(int32_t));
-    auto* d_bnd = (uint4*)gpu.arena("bnd", (size_t)std::max<int64_t>(bnd_total, 1) * sizeof(uint4));
-    auto* d_out = (PairResult*)gpu.arena("results", out.size() * sizeof(PairResult));
-    gpu.upload(d_waves, waves.data(), waves.size() * sizeof(ScoreWave));
-    gpu.upload(d_qd, qd.data(), qd.size() * sizeof(QueryDesc));
-    gpu.upload(d_prof, prof.data(), prof.size() * sizeof(int16_t));
-    gpu.upload(d_lb, lbv.data(), lbv.size() * sizeof(uint32_t));
-    gpu.upload(d_sb, slot_begin.data(), slot_begin.size() * sizeof(int32_t));
-    gpu.upload(d_pl, pair_list.data(), pair_list.size() * sizeof(int32_t));
+    auto name = [&](const char* s) { return R.pfx + s; };
+    R.d_waves = (ScoreWave*)ex.arena(name("waves"), R.waves.size() * sizeof(ScoreWave));
+    R.d_qd = (QueryDesc*)ex.arena(name("qdesc"), R.qd.size() * sizeof(QueryDesc));
+    R.d_prof = (int16_t*)ex.arena(name("prof"), std::max<size_t>(R.prof.size(), 8) * sizeof(int16_t));
+    R.d_lb = (uint32_t*)ex.arena(name("lb"), std::max<size_t>(R.lbv.size(), 1) * sizeof(uint32_t));
+    R.d_sb = (int32_t*)ex.arena(name("slot_begin"), R.slot_begin.size() * sizeof(int32_t));
+    R.d_pl = (int32_t*)ex.arena(name("pair_list"), R.pair_list.size() * sizeof(int32_t));
+    R.d_bnd = (uint4*)ex.arena(name("bnd"), (size_t)std::max<int64_t>(bnd_total, 1) * sizeof(uint4));
+    R.d_out = (PairResult*)ex.arena(name("results"), R.out.size() * sizeof(PairResult));
+    ex.upload(R.d_waves, R.waves.data(), R.waves.size() * sizeof(ScoreWave));
+    ex.upload(R.d_qd, R.qd.data(), R.qd.size() * sizeof(QueryDesc));
+    ex.upload(R.d_prof, R.prof.data(), R.prof.size() * sizeof(int16_t));
+    ex.upload(R.d_lb, R.lbv.data(), R.lbv.size() * sizeof(uint32_t));
+    ex.upload(R.d_sb, R.slot_begin.data(), R.slot_begin.size() * sizeof(int32_t));
+    ex.upload(R.d_pl, R.pair_list.data(), R.pair_list.size() * sizeof(int32_t));
+}
 
-    GpuTimer tm;
+void tier_launch(hipStream_t st, TierRun& R) {
+    if (R.empty()) return;
+    SSA_HIP(launch_init_results(st, R.d_out, (int64_t)R.out.size(), -32768));
+    SSA_HIP(launch_score(R.sh.lanes, R.sh.rows, R.type == kSW ? 0 : 1, R.nwg, st, R.d_waves, R.d_qd, R.d_prof,
+                         R.d_lb, R.L->d_res, R.d_bnd, R.d_sb, R.d_pl, R.d_out, R.P));
+}
+
+void tier_finish(Exec& ex, TierRun& R) {
+    if (R.empty()) return;
+    ex.download(R.out.data(), R.d_out, R.out.size() * sizeof(PairResult));
+}
+
+// One tier run on its own, timed.
+void gpu_score_tier(Exec& ex, TierRun& R, double* ms) {
+    tier_prepare(ex, R);
+    GpuTimer tm(ex.stream());
     tm.start();
-    SSA_HIP(launch_init_results(gpu.stream(), d_out, (int64_t)out.size(), -32768));
-    SSA_HIP(launch_score(sh.lanes, sh.rows, type == kSW ? 0 : 1, (int)((size_t)nq * nw / kWavesPerWG), gpu.stream(),
-                         d_waves, d_qd, d_prof, d_lb, L->d_res, d_bnd, d_sb, d_pl, d_out, P));
+    tier_launch(ex.stream(), R);
     tm.stop();
-    gpu.download(out.data(), d_out, out.size() * sizeof(PairResult));
-    gpu.sync();
-    local.release();
+    tier_finish(ex, R);
+    ex.sync();
     *ms += tm.ms();
 }
 
 // exact 32-bit scores of the pairs (q, e) listed
-void gpu_exact(SearchType type, const std::vector<QueryBuf>& qs, const std::vector<std::pair<uint32_t, uint32_t>>& pairs,
-               std::vector<int64_t>& out, double* ms, uint64_t* cells_computed) {
+void gpu_exact(Exec& gpu, SearchType type, const std::vector<QueryBuf>& qs,
+               const std::vector<std::pair<uint32_t, uint32_t>>& pairs, std::vector<int64_t>& out, double* ms,
+               uint64_t* cells_computed) {
     out.assign(pairs.size(), 0);
     if (pairs.empty()) return;
-    Gpu& gpu = Gpu::get();
     DeviceDB& db = g_db;
     const Config& cf = config();
     const Matrix& M = matrix();
@@ -587,7 +647,7 @@ void gpu_exact(SearchType type, const std::vector<QueryBuf>& qs, const std::vect
     gpu.upload(d_ep, ep.data(), ep.size() * sizeof(ExactPair));
     gpu.upload(d_qd, qd.data(), qd.size() * sizeof(QueryDesc));
     gpu.upload(d_prof, prof.data(), prof.size() * sizeof(int32_t));
-    GpuTimer tm;
+    GpuTimer tm(gpu.stream());
     tm.start();
     SSA_HIP(launch_exact((int)ep.size(), gpu.stream(), d_ep, d_qd, d_prof, db.d_codes, type == kSW ? 0 : 1,
                          cf.gap_open, cf.gap_extend, d_scr, (int64_t)(2 * std::max<size_t>(qmax, 1)), d_out));
@@ -725,9 +785,9 @@ std::vector<uint8_t> prepared_db_sequence(p_seqinfo s, int strand_field) {
     return v;
 }
 
-void gpu_traceback(SearchType type, std::vector<p_alignment>& al, double* ms) {
+// Regions and CIGARs of the hits listed (types[k] = the search type of al[k]).
+void gpu_traceback(Exec& gpu, const std::vector<SearchType>& types, std::vector<p_alignment>& al, double* ms) {
     if (al.empty()) return;
-    Gpu& gpu = Gpu::get();
     const Config& cf = config();
     std::vector<uint8_t> codes;
     std::vector<TraceTask> tasks(al.size());
@@ -748,7 +808,7 @@ void gpu_traceback(SearchType type, std::vector<p_alignment>& al, double* ms) {
         t.ops_off = ops_total;
         t.ops_cap = t.qlen + t.dlen + 1;
         ops_total += t.ops_cap;
-        t.type = type == kSW ? 0 : 1;
+        t.type = types[k] == kSW ? 0 : 1;
     }
     const int64_t t0 = now_us();
     auto* d_codes = (uint8_t*)gpu.arena("tb_codes", codes.size() + 1);
@@ -800,7 +860,7 @@ void gpu_traceback(SearchType type, std::vector<p_alignment>& al, double* ms) {
 }  // namespace
 
 // ============================================================================
-// the search
+// the search: one or many sw_align / nw_align calls
 // ============================================================================
 ssa_amd_stats_t& last_stats() {
     static ssa_amd_stats_t s;
@@ -819,168 +879,66 @@ void set_target_waves(int waves) {
     release_layouts();
 }
 
-p_alignment_list run_search(SearchType type, p_query query, size_t hitcount, int bit_width, int align_type) {
-    const int64_t t_start = now_us();
-    const Config& cf = config();
-    ssa_amd_stats_t st;
-    memset(&st, 0, sizeof st);
+namespace {
 
-    if (bit_width != BIT_WIDTH_8 && bit_width != BIT_WIDTH_16 && bit_width != BIT_WIDTH_64)
-        fatal("\nunknown bit width provided: %d\n\n", bit_width);  // src/algo/searcher.c:115-117
-    if (bit_width == BIT_WIDTH_8 && cf.simd_mode < COMPUTE_ON_SSE41)
-        fatal("libssa_amd: 8-bit searches replay SSE4.1 or AVX2 lanes; COMPUTE_ON_SSE2 selects neither");
+std::mutex& db_mutex() {
+    static std::mutex m;
+    return m;
+}
 
-    // query buffers (src/algo/searcher.c:42-90)
+// per-request state of a batch
+struct Job {
+    SearchRequest rq;
     std::vector<QueryBuf> qs;
+    bool sw_single = false;  // SW 8-bit whose 8-bit results come from one int16 pass
+    int first_bits = 0;      // first tier: 8, 16, or 0 (exact tier only)
+    std::vector<TierOut> t8, t16;
+    std::vector<int64_t> s64;
+    std::vector<uint8_t> in16, need64;
+    ssa_amd_stats_t st;
+    Shape shape{0, 0};
+    size_t q_off = 0;  // first query of this job in its first-tier group
+    int group = -1;
+    std::vector<Elem> hits;
+    p_alignment_list list = nullptr;
+};
+
+// Requests that share the search type, the first tier and the kernel shape
+// share one first-tier launch.
+struct Group {
+    SearchType type;
+    int bits;
+    Shape sh;
+    std::vector<size_t> jobs;
+};
+
+void job_queries(Job& J) {
+    const Config& cf = config();
+    p_query query = J.rq.query;
+    // query buffers (src/algo/searcher.c:42-90)
     if (cf.symtype == NUCLEOTIDE) {
         for (int s = 0; s < 2; ++s)
             if (((s + 1) & cf.strands) && query->has_nt[s])
-                qs.push_back(QueryBuf{query->nt[s].data(), query->nt[s].size(), s, 0});
+                J.qs.push_back(QueryBuf{query->nt[s].data(), query->nt[s].size(), s, 0});
     } else if (cf.symtype == AMINOACID) {
-        qs.push_back(QueryBuf{query->aa[0].data(), query->aa[0].size(), 0, 0});
+        J.qs.push_back(QueryBuf{query->aa[0].data(), query->aa[0].size(), 0, 0});
     } else {
         fatal("libssa_amd: translated symbol types (TRANS_QUERY, TRANS_DB, TRANS_BOTH) are not supported");
     }
-    for (const QueryBuf& q : qs)
-        if (q.len == 0 && type == kNW) fatal("libssa_amd: NW search with an empty query");
+    for (const QueryBuf& q : J.qs)
+        if (q.len == 0 && J.rq.type == kNW) fatal("libssa_amd: NW search with an empty query");
+}
 
-    DeviceDB& db = device_db();
-    const size_t nq = qs.size();
-
-    // entries processed: whole chunks of DB IDs until the first chunk without
-    // entries (search_8/16/64 stop there, src/algo/8/search_8.c:135-143)
-    const size_t cs = cf.chunk_size ? cf.chunk_size : kDefaultChunkSize;
-    std::vector<std::pair<uint32_t, uint32_t>> chunks;  // entry ranges
-    for (size_t start = 0; start < db.count; start += cs) {
-        const size_t stop = std::min(db.count, start + cs);
-        const uint32_t e0 = db.first_entry[start], e1 = db.first_entry[stop];
-        if (e0 == e1) break;
-        chunks.push_back({e0, e1});
-    }
-    const uint32_t N = chunks.empty() ? 0 : chunks.back().second;
-    st.chunk_count = chunks.size();
-    st.seq_count = N;
-    for (const QueryBuf& q : qs)
-        for (uint32_t e = 0; e < N; ++e) st.cells += (uint64_t)q.len * db.entries[e].len;
-
-    const MatrixInfo mi = matrix_info();
-    const int go = cf.gap_open, ge = cf.gap_extend;
-    // 8-bit SW needs no separate pass when the 8-bit lane values equal the exact ones
-    const bool sw_single = type == kSW && mi.fits8 && (go + ge) >= -128;
-
-    PairSet all;
-    all.nq = nq;
-    all.entries.resize(N);
-    std::iota(all.entries.begin(), all.entries.end(), 0u);
-    const bool full_db = N == db.entries.size();
-
-    // per-(q, e) tier results, dense over the processed entries
-    std::vector<TierOut> t8, t16;
-    std::vector<int64_t> s64;
-    std::vector<uint8_t> need64;  // entries whose scores come from the exact tier
-    auto at = [N](size_t q, size_t e) { return q * N + e; };
-    double ms_first = 0, ms_re = 0;
-    uint64_t computed = 0;
-    Shape shape_used{0, 0};
-    std::vector<PairResult> raw;
-
-    std::vector<uint8_t> in16(N, 0);
-    if (bit_width == BIT_WIDTH_8) {
-        t8.resize(nq * N);
-        if (sw_single) {
-            gpu_score_tier(type, kTier16, qs, all, full_db ? &db.by_length : nullptr, raw, &ms_first, &computed,
-                           &shape_used);
-            t16.resize(nq * N);
-            for (size_t q = 0; q < nq; ++q)
-                for (uint32_t e = 0; e < N; ++e) {
-                    const PairResult& r = raw[all.index(q, e)];
-                    t8[at(q, e)] = decode(type, kTier16, r, db.entries[e].len, true);
-                    t16[at(q, e)] = decode(type, kTier16, r, db.entries[e].len, false);
-                    if (!t8[at(q, e)].accept) in16[e] = 1;
-                }
-        } else {
-            gpu_score_tier(type, kTier8, qs, all, full_db ? &db.by_length : nullptr, raw, &ms_first, &computed,
-                           &shape_used);
-            for (size_t q = 0; q < nq; ++q)
-                for (uint32_t e = 0; e < N; ++e) {
-                    t8[at(q, e)] = decode(type, kTier8, raw[all.index(q, e)], db.entries[e].len, false);
-                    if (!t8[at(q, e)].accept) in16[e] = 1;
-                }
-            PairSet sub;
-            sub.nq = nq;
-            for (uint32_t e = 0; e < N; ++e)
-                if (in16[e]) sub.entries.push_back(e);
-            t16.resize(nq * N);
-            if (!sub.entries.empty()) {
-                Shape s2{0, 0};
-                gpu_score_tier(type, kTier16, qs, sub, nullptr, raw, &ms_re, &computed, &s2);
-                for (size_t q = 0; q < nq; ++q)
-                    for (size_t k = 0; k < sub.entries.size(); ++k) {
-                        const uint32_t e = sub.entries[k];
-                        t16[at(q, e)] = decode(type, kTier16, raw[sub.index(q, k)], db.entries[e].len, false);
-                    }
-            }
-        }
-    } else if (bit_width == BIT_WIDTH_16) {
-        gpu_score_tier(type, kTier16, qs, all, full_db ? &db.by_length : nullptr, raw, &ms_first, &computed,
-                       &shape_used);
-        t16.resize(nq * N);
-        for (size_t q = 0; q < nq; ++q)
-            for (uint32_t e = 0; e < N; ++e) {
-                t16[at(q, e)] = decode(type, kTier16, raw[all.index(q, e)], db.entries[e].len, false);
-                in16[e] = 1;
-            }
-    }
-
-    // entries that reach the exact tier
-    need64.assign(N, 0);
-    if (bit_width == BIT_WIDTH_64) {
-        // scores only: use the 16-bit pass where it is provably exact
-        std::fill(need64.begin(), need64.end(), 1);
-        if (mi.fits16) {
-            gpu_score_tier(type, kTier16, qs, all, full_db ? &db.by_length : nullptr, raw, &ms_first, &computed,
-                           &shape_used);
-            t16.resize(nq * N);
-            const int64_t gap = std::abs((int64_t)go) + std::abs((int64_t)ge);
-            std::fill(need64.begin(), need64.end(), 0);
-            for (size_t q = 0; q < nq; ++q)
-                for (uint32_t e = 0; e < N; ++e) {
-                    TierOut o = decode(type, kTier16, raw[all.index(q, e)], db.entries[e].len, false);
-                    if (type == kNW) {
-                        // exact when no value can reach either end of the int16 range
-                        const int64_t span = 3 * gap + gap * (int64_t)(qs[q].len + db.entries[e].len + 4);
-                        const int64_t up = std::max<int64_t>(mi.maxv, 0) * (int64_t)std::min<size_t>(qs[q].len, db.entries[e].len);
-                        const int64_t lowest = -span - std::max<int64_t>(-mi.minv, 0) * (int64_t)std::min<size_t>(qs[q].len, db.entries[e].len);
-                        if (!(lowest > -32768 + gap + 1 && up < 32767)) o.accept = false;
-                    }
-                    t16[at(q, e)] = o;
-                    if (!o.accept) need64[e] = 1;
-                }
-        }
-    } else {
-        for (size_t q = 0; q < nq; ++q)
-            for (uint32_t e = 0; e < N; ++e)
-                if (in16[e] && !t16[at(q, e)].accept) need64[e] = 1;
-    }
-    std::vector<std::pair<uint32_t, uint32_t>> p64;
-    for (uint32_t e = 0; e < N; ++e)
-        if (need64[e])
-            for (size_t q = 0; q < nq; ++q) p64.push_back({(uint32_t)q, e});
-    s64.assign(nq * N, 0);
-    if (!p64.empty()) {
-        std::vector<int64_t> o64;
-        gpu_exact(type, qs, p64, o64, &ms_re, &computed);
-        for (size_t k = 0; k < p64.size(); ++k) s64[at(p64[k].first, p64[k].second)] = o64[k];
-    }
-    if (bit_width == BIT_WIDTH_64)
-        for (size_t q = 0; q < nq; ++q)
-            for (uint32_t e = 0; e < N; ++e)
-                if (!need64[e]) s64[at(q, e)] = t16[at(q, e)].score;
-
-    // ---- replay ----
+// The host replay of one search: the reference's chunk loop, lane schedules,
+// cascade and heap (src/algo/8/search_8.c:94-146, src/algo/16/search_16.c:92-134,
+// src/algo/64/search_64.c:44-79, src/algo/manager.c:120-185).
+void job_replay(Job& J, const std::vector<std::pair<uint32_t, uint32_t>>& chunks, uint32_t N, bool threads) {
+    DeviceDB& db = g_db;
     const int64_t t_replay = now_us();
     const int W8 = lanes8(), W16 = lanes16();
-    // per chunk the ordered insertion list is built in parallel, then fed to the heap
+    const size_t nq = J.qs.size();
+    const int bit_width = J.rq.bit_width;
+    auto at = [N](size_t q, size_t e) { return q * N + e; };
     struct Ins {
         uint32_t e;
         uint8_t q;
@@ -995,15 +953,15 @@ p_alignment_list run_search(SearchType type, p_query query, size_t hitcount, int
         std::iota(items.begin(), items.end(), e0);
         auto tier64 = [&](const std::vector<uint32_t>& list) {
             for (size_t q = 0; q < nq; ++q)
-                for (uint32_t e : list) out.push_back(Ins{e, (uint8_t)q, s64[at(q, e)]});
+                for (uint32_t e : list) out.push_back(Ins{e, (uint8_t)q, J.s64[at(q, e)]});
         };
         auto tier16 = [&](const std::vector<uint32_t>& list) {
             ovf2.clear();
             for (size_t q = 0; q < nq; ++q)
                 lane_schedule(
-                    W16, list.size(), [&](size_t k) { return (int64_t)t16[at(q, list[k])].dur; },
+                    W16, list.size(), [&](size_t k) { return (int64_t)J.t16[at(q, list[k])].dur; },
                     [&](size_t k) {
-                        const TierOut& o = t16[at(q, list[k])];
+                        const TierOut& o = J.t16[at(q, list[k])];
                         if (o.accept) out.push_back(Ins{list[k], (uint8_t)q, o.score});
                         else ovf2.push_back(list[k]);
                     });
@@ -1015,9 +973,9 @@ p_alignment_list run_search(SearchType type, p_query query, size_t hitcount, int
         if (bit_width == BIT_WIDTH_8) {
             for (size_t q = 0; q < nq; ++q)
                 lane_schedule(
-                    W8, items.size(), [&](size_t k) { return (int64_t)t8[at(q, items[k])].dur; },
+                    W8, items.size(), [&](size_t k) { return (int64_t)J.t8[at(q, items[k])].dur; },
                     [&](size_t k) {
-                        const TierOut& o = t8[at(q, items[k])];
+                        const TierOut& o = J.t8[at(q, items[k])];
                         if (o.accept) out.push_back(Ins{items[k], (uint8_t)q, o.score});
                         else ovf.push_back(items[k]);
                     });
@@ -1032,7 +990,7 @@ p_alignment_list run_search(SearchType type, p_query query, size_t hitcount, int
         }
     };
     const size_t nthreads = std::max<size_t>(1, std::min<size_t>(16, std::thread::hardware_concurrency()));
-    if (chunks.size() > 8 && nthreads > 1) {
+    if (threads && chunks.size() > 8 && nthreads > 1) {
         std::vector<std::thread> th;
         for (size_t t = 0; t < nthreads; ++t)
             th.emplace_back([&, t] {
@@ -1042,27 +1000,25 @@ p_alignment_list run_search(SearchType type, p_query query, size_t hitcount, int
     } else {
         for (size_t c = 0; c < chunks.size(); ++c) do_chunk(c);
     }
-    MinHeap heap(hitcount);
+    MinHeap heap(J.rq.hitcount);
     for (size_t c = 0; c < chunks.size(); ++c) {
         for (const Ins& x : per_chunk[c]) {
             const DbEntry& e = db.entries[x.e];
             heap.add(Elem{e.id, e.frame, e.strand, x.q, x.score});
         }
-        st.overflow_8_bit_count += ov8[c];
-        st.overflow_16_bit_count += ov16[c];
+        J.st.overflow_8_bit_count += ov8[c];
+        J.st.overflow_16_bit_count += ov16[c];
     }
-    std::vector<Elem> hits = heap.sorted();
-    st.ms_replay = (now_us() - t_replay) / 1000.0;
+    J.hits = heap.sorted();
+    J.st.ms_replay = (now_us() - t_replay) / 1000.0;
+}
 
-    if (st.overflow_8_bit_count || st.overflow_16_bit_count)
-        print_info("Overflow occurred: %ld sequences were re-aligned with 16 bit, and %ld sequences with 64 bit\n",
-                   (long)st.overflow_8_bit_count, (long)st.overflow_16_bit_count);
-
-    // ---- alignment list (src/algo/manager.c:83-113, src/algo/aligner.c:62-99) ----
+// the alignment list of one search (src/algo/manager.c:83-113, src/algo/aligner.c:62-99)
+void job_list(Job& J, std::vector<p_alignment>& al_out, std::vector<SearchType>& types_out) {
+    const std::vector<Elem>& hits = J.hits;
     p_alignment_list alist = (p_alignment_list)malloc(sizeof(alignment_list_t));
     alist->len = hits.size();
     alist->alignments = (p_alignment*)malloc(std::max<size_t>(hits.size(), 1) * sizeof(p_alignment));
-    std::vector<p_alignment> al(hits.size());
     for (size_t k = 0; k < hits.size(); ++k) {
         const Elem& h = hits[k];
         p_seqinfo info = ssa_db_get_sequence(h.db_id);
@@ -1076,25 +1032,330 @@ p_alignment_list run_search(SearchType type, p_query query, size_t hitcount, int
         a->db_seq.strand = h.db_frame;  // the reference stores these two swapped (src/algo/aligner.c:76-77)
         a->db_seq.frame = h.db_strand;
         a->db_seq.ID = info->ID;
-        const QueryBuf& q = qs[h.query_id];
+        const QueryBuf& q = J.qs[h.query_id];
         a->query.seq = (char*)q.seq;
         a->query.len = q.len;
         a->query.strand = q.strand;
         a->query.frame = q.frame;
         a->score = h.score;
-        al[k] = a;
         alist->alignments[k] = a;
+        if (J.rq.align_type != COMPUTE_SCORE) {
+            al_out.push_back(a);
+            types_out.push_back(J.rq.type);
+        }
     }
-    if (align_type != COMPUTE_SCORE) gpu_traceback(type, al, &st.ms_traceback);
+    J.list = alist;
+}
 
-    st.cells_computed = computed;
-    st.lanes_per_stream = shape_used.lanes;
-    st.rows_per_lane = shape_used.rows;
-    st.ms_gpu_first = ms_first;
-    st.ms_gpu_requeue = ms_re;
-    st.ms_total = (now_us() - t_start) / 1000.0;
-    last_stats() = st;
-    return alist;
+}  // namespace
+
+void run_batch(Exec& ex, const std::vector<SearchRequest>& reqs, std::vector<SearchResult>& results,
+               bool one_graph) {
+    const int64_t t_start = now_us();
+    const Config& cf = config();
+    results.assign(reqs.size(), SearchResult{});
+    if (reqs.empty()) return;
+
+    std::vector<Job> jobs(reqs.size());
+    for (size_t j = 0; j < reqs.size(); ++j) {
+        Job& J = jobs[j];
+        J.rq = reqs[j];
+        memset(&J.st, 0, sizeof J.st);
+        const int bw = J.rq.bit_width;
+        if (bw != BIT_WIDTH_8 && bw != BIT_WIDTH_16 && bw != BIT_WIDTH_64)
+            fatal("\nunknown bit width provided: %d\n\n", bw);  // src/algo/searcher.c:115-117
+        if (bw == BIT_WIDTH_8 && cf.simd_mode < COMPUTE_ON_SSE41)
+            fatal("libssa_amd: 8-bit searches replay SSE4.1 or AVX2 lanes; COMPUTE_ON_SSE2 selects neither");
+        job_queries(J);
+    }
+
+    DeviceDB* dbp;
+    {
+        std::lock_guard<std::mutex> lk(db_mutex());
+        dbp = &device_db();
+    }
+    DeviceDB& db = *dbp;
+
+    // entries processed: whole chunks of DB IDs until the first chunk without
+    // entries (search_8/16/64 stop there, src/algo/8/search_8.c:135-143)
+    const size_t cs = cf.chunk_size ? cf.chunk_size : kDefaultChunkSize;
+    std::vector<std::pair<uint32_t, uint32_t>> chunks;  // entry ranges
+    for (size_t start = 0; start < db.count; start += cs) {
+        const size_t stop = std::min(db.count, start + cs);
+        const uint32_t e0 = db.first_entry[start], e1 = db.first_entry[stop];
+        if (e0 == e1) break;
+        chunks.push_back({e0, e1});
+    }
+    const uint32_t N = chunks.empty() ? 0 : chunks.back().second;
+    auto at = [N](size_t q, size_t e) { return q * N + e; };
+    PairSet all;
+    all.entries.resize(N);
+    std::iota(all.entries.begin(), all.entries.end(), 0u);
+    const bool full_db = N == db.entries.size();
+    int64_t total_blocks = 0, max_blocks = 0;
+    for (uint32_t e = 0; e < N; ++e) {
+        const int64_t nb = (db.entries[e].len + 3) / 4;
+        total_blocks += nb;
+        max_blocks = std::max(max_blocks, nb);
+    }
+
+    const MatrixInfo mi = matrix_info();
+    const int go = cf.gap_open, ge = cf.gap_extend;
+
+    // ---- first tiers: grouped by (type, tier, shape), one launch per group ----
+    std::vector<Group> groups;
+    for (size_t j = 0; j < jobs.size(); ++j) {
+        Job& J = jobs[j];
+        J.st.chunk_count = chunks.size();
+        J.st.seq_count = N;
+        for (const QueryBuf& q : J.qs)
+            for (uint32_t e = 0; e < N; ++e) J.st.cells += (uint64_t)q.len * db.entries[e].len;
+        const size_t nq = J.qs.size();
+        J.in16.assign(N, 0);
+        J.need64.assign(N, 0);
+        J.s64.assign(nq * N, 0);
+        J.sw_single = J.rq.type == kSW && mi.fits8 && (go + ge) >= -128;
+        if (J.rq.bit_width == BIT_WIDTH_8) J.first_bits = J.sw_single ? 16 : 8;
+        else if (J.rq.bit_width == BIT_WIDTH_16) J.first_bits = 16;
+        else J.first_bits = mi.fits16 ? 16 : 0;
+        if (J.first_bits == 0 || N == 0 || nq == 0) continue;
+        size_t qmax = 0;
+        for (const QueryBuf& q : J.qs) qmax = std::max(qmax, q.len);
+        J.shape = choose_shape(qmax, total_blocks, max_blocks);
+        int gi = -1;
+        for (size_t g = 0; g < groups.size(); ++g)
+            if (groups[g].type == J.rq.type && groups[g].bits == J.first_bits &&
+                groups[g].sh.lanes == J.shape.lanes && groups[g].sh.rows == J.shape.rows)
+                gi = (int)g;
+        if (gi < 0) {
+            groups.push_back(Group{J.rq.type, J.first_bits, J.shape, {}});
+            gi = (int)groups.size() - 1;
+        }
+        J.group = gi;
+        groups[gi].jobs.push_back(j);
+    }
+    std::vector<std::unique_ptr<TierRun>> runs(groups.size());
+    for (size_t g = 0; g < groups.size(); ++g) {
+        runs[g].reset(new TierRun());
+        TierRun& R = *runs[g];
+        R.type = groups[g].type;
+        R.T = groups[g].bits == 8 ? kTier8 : kTier16;
+        R.sh = groups[g].sh;
+        R.pfx = "g" + std::to_string(g) + "_";
+        R.sorted_all = full_db ? &db.by_length : nullptr;
+        for (size_t j : groups[g].jobs) {
+            jobs[j].q_off = R.qs.size();
+            R.qs.insert(R.qs.end(), jobs[j].qs.begin(), jobs[j].qs.end());
+        }
+        PairSet* ps = new PairSet(all);
+        ps->nq = R.qs.size();
+        R.ps = ps;
+        tier_prepare(ex, R);
+    }
+    double ms_first = 0;
+    if (!groups.empty()) {
+        GpuTimer tm(ex.stream());
+        if (one_graph) {
+            // every first-tier launch of the batch in one graph; groups run on
+            // parallel branches
+            hipGraph_t graph;
+            hipGraphExec_t gexec;
+            const hipStream_t st = ex.stream();
+            SSA_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+            hipEvent_t fork;
+            SSA_HIP(hipEventCreateWithFlags(&fork, hipEventDisableTiming));
+            SSA_HIP(hipEventRecord(fork, st));
+            std::vector<hipEvent_t> joins(groups.size());
+            for (size_t g = 0; g < groups.size(); ++g) {
+                const hipStream_t br = ex.aux(g);
+                SSA_HIP(hipStreamWaitEvent(br, fork, 0));
+                tier_launch(br, *runs[g]);
+                SSA_HIP(hipEventCreateWithFlags(&joins[g], hipEventDisableTiming));
+                SSA_HIP(hipEventRecord(joins[g], br));
+                SSA_HIP(hipStreamWaitEvent(st, joins[g], 0));
+            }
+            SSA_HIP(hipStreamEndCapture(st, &graph));
+            SSA_HIP(hipGraphInstantiate(&gexec, graph, nullptr, nullptr, 0));
+            tm.start();
+            SSA_HIP(hipGraphLaunch(gexec, st));
+            tm.stop();
+            for (size_t g = 0; g < groups.size(); ++g) tier_finish(ex, *runs[g]);
+            ex.sync();
+            ms_first = tm.ms();
+            SSA_HIP(hipGraphExecDestroy(gexec));
+            SSA_HIP(hipGraphDestroy(graph));
+            SSA_HIP(hipEventDestroy(fork));
+            for (hipEvent_t e : joins) SSA_HIP(hipEventDestroy(e));
+        } else {
+            tm.start();
+            for (size_t g = 0; g < groups.size(); ++g) tier_launch(ex.stream(), *runs[g]);
+            tm.stop();
+            for (size_t g = 0; g < groups.size(); ++g) tier_finish(ex, *runs[g]);
+            ex.sync();
+            ms_first = tm.ms();
+        }
+    }
+
+    // ---- decode the first tiers; re-queued and exact tiers per search ----
+    for (size_t j = 0; j < jobs.size(); ++j) {
+        Job& J = jobs[j];
+        const SearchType type = J.rq.type;
+        const size_t nq = J.qs.size();
+        J.st.ms_gpu_first = J.group >= 0 ? ms_first : 0.0;
+        double ms_re = 0;
+        uint64_t computed = 0;
+        if (J.group >= 0) {
+            TierRun& R = *runs[J.group];
+            computed += R.computed * nq / std::max<size_t>(R.qs.size(), 1);
+            J.st.lanes_per_stream = R.sh.lanes;
+            J.st.rows_per_lane = R.sh.rows;
+        }
+        auto raw = [&](size_t q, uint32_t e) -> const PairResult& {
+            const TierRun& R = *runs[J.group];
+            return R.out[R.ps->index(J.q_off + q, e)];
+        };
+        const int bw = J.rq.bit_width;
+        if (bw == BIT_WIDTH_8) {
+            J.t8.resize(nq * N);
+            if (J.sw_single) {
+                J.t16.resize(nq * N);
+                for (size_t q = 0; q < nq; ++q)
+                    for (uint32_t e = 0; e < N; ++e) {
+                        const PairResult& r = raw(q, e);
+                        J.t8[at(q, e)] = decode(type, kTier16, r, db.entries[e].len, true);
+                        J.t16[at(q, e)] = decode(type, kTier16, r, db.entries[e].len, false);
+                        if (!J.t8[at(q, e)].accept) J.in16[e] = 1;
+                    }
+            } else {
+                for (size_t q = 0; q < nq; ++q)
+                    for (uint32_t e = 0; e < N; ++e) {
+                        J.t8[at(q, e)] = decode(type, kTier8, raw(q, e), db.entries[e].len, false);
+                        if (!J.t8[at(q, e)].accept) J.in16[e] = 1;
+                    }
+                PairSet sub;
+                sub.nq = nq;
+                for (uint32_t e = 0; e < N; ++e)
+                    if (J.in16[e]) sub.entries.push_back(e);
+                J.t16.resize(nq * N);
+                if (!sub.entries.empty()) {
+                    TierRun R2;
+                    R2.type = type;
+                    R2.T = kTier16;
+                    R2.qs = J.qs;
+                    R2.ps = &sub;
+                    R2.pfx = "re_";
+                    gpu_score_tier(ex, R2, &ms_re);
+                    computed += R2.computed;
+                    for (size_t q = 0; q < nq; ++q)
+                        for (size_t k = 0; k < sub.entries.size(); ++k) {
+                            const uint32_t e = sub.entries[k];
+                            J.t16[at(q, e)] =
+                                decode(type, kTier16, R2.out[sub.index(q, k)], db.entries[e].len, false);
+                        }
+                }
+            }
+        } else if (bw == BIT_WIDTH_16) {
+            J.t16.resize(nq * N);
+            for (size_t q = 0; q < nq; ++q)
+                for (uint32_t e = 0; e < N; ++e) {
+                    J.t16[at(q, e)] = decode(type, kTier16, raw(q, e), db.entries[e].len, false);
+                    J.in16[e] = 1;
+                }
+        }
+        // entries that reach the exact tier
+        if (bw == BIT_WIDTH_64) {
+            // scores only: the 16-bit pass where it is provably exact
+            std::fill(J.need64.begin(), J.need64.end(), 1);
+            if (J.group >= 0) {
+                J.t16.resize(nq * N);
+                const int64_t gap = std::abs((int64_t)go) + std::abs((int64_t)ge);
+                std::fill(J.need64.begin(), J.need64.end(), 0);
+                for (size_t q = 0; q < nq; ++q)
+                    for (uint32_t e = 0; e < N; ++e) {
+                        TierOut o = decode(type, kTier16, raw(q, e), db.entries[e].len, false);
+                        if (type == kNW) {
+                            // exact when no value can reach either end of the int16 range
+                            const int64_t ml = (int64_t)std::min<size_t>(J.qs[q].len, db.entries[e].len);
+                            const int64_t span = 3 * gap + gap * (int64_t)(J.qs[q].len + db.entries[e].len + 4);
+                            const int64_t up = std::max<int64_t>(mi.maxv, 0) * ml;
+                            const int64_t lowest = -span - std::max<int64_t>(-mi.minv, 0) * ml;
+                            if (!(lowest > -32768 + gap + 1 && up < 32767)) o.accept = false;
+                        }
+                        J.t16[at(q, e)] = o;
+                        if (!o.accept) J.need64[e] = 1;
+                    }
+            }
+        } else {
+            for (size_t q = 0; q < nq; ++q)
+                for (uint32_t e = 0; e < N; ++e)
+                    if (J.in16[e] && !J.t16[at(q, e)].accept) J.need64[e] = 1;
+        }
+        std::vector<std::pair<uint32_t, uint32_t>> p64;
+        for (uint32_t e = 0; e < N; ++e)
+            if (J.need64[e])
+                for (size_t q = 0; q < nq; ++q) p64.push_back({(uint32_t)q, e});
+        if (!p64.empty()) {
+            std::vector<int64_t> o64;
+            gpu_exact(ex, type, J.qs, p64, o64, &ms_re, &computed);
+            for (size_t k = 0; k < p64.size(); ++k) J.s64[at(p64[k].first, p64[k].second)] = o64[k];
+        }
+        if (bw == BIT_WIDTH_64)
+            for (size_t q = 0; q < nq; ++q)
+                for (uint32_t e = 0; e < N; ++e)
+                    if (!J.need64[e]) J.s64[at(q, e)] = J.t16[at(q, e)].score;
+        J.st.ms_gpu_requeue = ms_re;
+        J.st.cells_computed = computed;
+    }
+    for (auto& R : runs) {
+        delete R->ps;
+        R->ps = nullptr;
+    }
+    runs.clear();
+
+    // ---- replay: searches in parallel, or chunks in parallel for few searches ----
+    const size_t nthreads = std::max<size_t>(1, std::min<size_t>(16, std::thread::hardware_concurrency()));
+    if (jobs.size() >= 4 && nthreads > 1) {
+        std::vector<std::thread> th;
+        const size_t nt = std::min(nthreads, jobs.size());
+        for (size_t t = 0; t < nt; ++t)
+            th.emplace_back([&, t] {
+                for (size_t j = t; j < jobs.size(); j += nt) job_replay(jobs[j], chunks, N, false);
+            });
+        for (auto& x : th) x.join();
+    } else {
+        for (Job& J : jobs) job_replay(J, chunks, N, true);
+    }
+    for (Job& J : jobs)
+        if (J.st.overflow_8_bit_count || J.st.overflow_16_bit_count)
+            print_info("Overflow occurred: %ld sequences were re-aligned with 16 bit, and %ld sequences with 64 bit\n",
+                       (long)J.st.overflow_8_bit_count, (long)J.st.overflow_16_bit_count);
+
+    // ---- alignment lists; one traceback launch for every hit of the batch ----
+    std::vector<p_alignment> al;
+    std::vector<SearchType> types;
+    std::vector<size_t> al_job;
+    for (size_t j = 0; j < jobs.size(); ++j) {
+        const size_t before = al.size();
+        job_list(jobs[j], al, types);
+        for (size_t k = before; k < al.size(); ++k) al_job.push_back(j);
+    }
+    double ms_tb = 0;
+    gpu_traceback(ex, types, al, &ms_tb);
+    for (size_t j = 0; j < jobs.size(); ++j) {
+        Job& J = jobs[j];
+        if (J.rq.align_type != COMPUTE_SCORE) J.st.ms_traceback = ms_tb;
+        J.st.ms_total = (now_us() - t_start) / 1000.0;
+        results[j].list = J.list;
+        results[j].stats = J.st;
+    }
+}
+
+p_alignment_list run_search(SearchType type, p_query query, size_t hitcount, int bit_width, int align_type) {
+    std::vector<SearchRequest> rq(1, SearchRequest{type, query, hitcount, bit_width, align_type});
+    std::vector<SearchResult> res;
+    run_batch(Gpu::get().exec(), rq, res, false);
+    last_stats() = res[0].stats;
+    return res[0].list;
 }
 
 }  // namespace ssa

@@ -34,6 +34,26 @@ struct DbEntry {
 void db_generation_bump();
 
 p_alignment_list run_search(SearchType type, p_query query, size_t hitcount, int bit_width, int align_type);
+
+// one sw_align / nw_align call of a batch
+struct SearchRequest {
+    SearchType type;
+    p_query query;
+    size_t hitcount;
+    int bit_width;
+    int align_type;
+};
+
+struct SearchResult {
+    p_alignment_list list = nullptr;
+    ssa_amd_stats_t stats{};
+};
+
+class Exec;
+// Runs the searches as one batch on the context's stream: every result is
+// what the search returns alone.  one_graph: the first-tier launches of all
+// searches go into one captured HIP graph.
+void run_batch(Exec& ex, const std::vector<SearchRequest>& reqs, std::vector<SearchResult>& results, bool one_graph);
 void release_device_db();
 
 ssa_amd_stats_t& last_stats();

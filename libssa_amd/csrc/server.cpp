@@ -1,0 +1,233 @@
+// server.cpp -- continuous-batching request loop over the resident DB.
+//
+// Requests (sw_align / nw_align arguments) are queued as they arrive.  Each of
+// `slots` workers owns a device context (stream and buffers); a free worker
+// takes the queued requests as one batch (run_batch: one captured graph for
+// the batch's first tiers), admitting requests in arrival order while the
+// batch's estimated device footprint stays within the worker's share of the
+// memory budget.  Batches of different workers run on the GPU at the same
+// time, so requests that arrive while others are on the GPU start as soon as
+// a worker and memory free up.  Every result is what the search returns alone
+// (run_batch's contract).
+#include <algorithm>
+#include <chrono>
+#include <condition_variable>
+#include <cstring>
+#include <deque>
+#include <mutex>
+#include <thread>
+#include <unordered_map>
+#include <vector>
+
+#include "common.h"
+#include "gpu.h"
+#include "search.h"
+
+extern "C" {
+#include "libssa_amd.h"
+}
+
+namespace ssa {
+namespace {
+
+double now_ms() {
+    return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+struct Ticket {
+    SearchRequest rq;
+    SearchResult res;
+    size_t footprint = 0;
+    double t_submit = 0, t_start = 0, t_done = 0;
+    bool done = false;
+};
+
+}  // namespace
+
+class Server {
+public:
+    Server(int slots, size_t max_batch, size_t device_bytes) : max_batch_(std::max<size_t>(1, max_batch)) {
+        size_t free_b = 0, total_b = 0;
+        SSA_HIP(hipMemGetInfo(&free_b, &total_b));
+        budget_ = device_bytes ? device_bytes : free_b / 2;
+        slot_budget_ = budget_ / (size_t)slots;
+        for (int i = 0; i < slots; ++i) workers_.emplace_back([this] { work(); });
+    }
+
+    ~Server() {
+        {
+            std::lock_guard<std::mutex> lk(mu_);
+            stopping_ = true;
+        }
+        cv_work_.notify_all();
+        for (auto& t : workers_) t.join();
+        for (auto& kv : tickets_)
+            if (kv.second.res.list) free_alignment(kv.second.res.list);
+    }
+
+    int64_t submit(const SearchRequest& rq, size_t footprint) {
+        std::lock_guard<std::mutex> lk(mu_);
+        const int64_t id = next_id_++;
+        Ticket& t = tickets_[id];
+        t.rq = rq;
+        t.footprint = footprint;
+        t.t_submit = now_ms();
+        queue_.push_back(id);
+        cv_work_.notify_one();
+        return id;
+    }
+
+    bool wait(int64_t id, SearchResult* out, double* latency_ms, double* queue_ms) {
+        std::unique_lock<std::mutex> lk(mu_);
+        auto it = tickets_.find(id);
+        if (it == tickets_.end()) return false;
+        cv_done_.wait(lk, [&] { return it->second.done; });
+        *out = it->second.res;
+        if (latency_ms) *latency_ms = it->second.t_done - it->second.t_submit;
+        if (queue_ms) *queue_ms = it->second.t_start - it->second.t_submit;
+        tickets_.erase(it);
+        return true;
+    }
+
+    void counters(ssa_amd_server_counters_t* c) {
+        std::lock_guard<std::mutex> lk(mu_);
+        c->batches = batches_;
+        c->requests = requests_;
+        c->max_batch_seen = max_batch_seen_;
+        c->queued = queue_.size();
+        c->device_budget = budget_;
+    }
+
+private:
+    void work() {
+        Exec ex;
+        for (;;) {
+            std::vector<int64_t> batch;
+            {
+                std::unique_lock<std::mutex> lk(mu_);
+                cv_work_.wait(lk, [&] { return stopping_ || !queue_.empty(); });
+                if (queue_.empty()) return;  // stopping with nothing left
+                size_t bytes = 0;
+                while (!queue_.empty() && batch.size() < max_batch_) {
+                    const Ticket& t = tickets_[queue_.front()];
+                    // admit in arrival order while the footprint fits; a request
+                    // larger than the share runs alone
+                    if (!batch.empty() && bytes + t.footprint > slot_budget_) break;
+                    bytes += t.footprint;
+                    batch.push_back(queue_.front());
+                    queue_.pop_front();
+                }
+                const double t0 = now_ms();
+                for (int64_t id : batch) tickets_[id].t_start = t0;
+                ++batches_;
+                requests_ += batch.size();
+                max_batch_seen_ = std::max<size_t>(max_batch_seen_, batch.size());
+                if (!queue_.empty()) cv_work_.notify_one();
+            }
+            std::vector<SearchRequest> rq;
+            {
+                std::lock_guard<std::mutex> lk(mu_);
+                for (int64_t id : batch) rq.push_back(tickets_[id].rq);
+            }
+            std::vector<SearchResult> res;
+            run_batch(ex, rq, res, true);
+            {
+                std::lock_guard<std::mutex> lk(mu_);
+                const double t1 = now_ms();
+                for (size_t i = 0; i < batch.size(); ++i) {
+                    Ticket& t = tickets_[batch[i]];
+                    t.res = res[i];
+                    t.t_done = t1;
+                    t.done = true;
+                }
+            }
+            cv_done_.notify_all();
+        }
+    }
+
+    std::mutex mu_;
+    std::condition_variable cv_work_, cv_done_;
+    std::deque<int64_t> queue_;
+    std::unordered_map<int64_t, Ticket> tickets_;
+    std::vector<std::thread> workers_;
+    size_t max_batch_;
+    size_t budget_ = 0, slot_budget_ = 0;
+    int64_t next_id_ = 0;
+    bool stopping_ = false;
+    size_t batches_ = 0, requests_ = 0, max_batch_seen_ = 0;
+};
+
+// device bytes one request needs in a batch: pair results and pass boundaries
+// of its first tier, and the traceback matrices of its hits
+size_t request_footprint(const SearchRequest& rq) {
+    const Config& cf = config();
+    const size_t n = ssa_db_get_sequence_count();
+    const size_t entries = n * ((cf.symtype == NUCLEOTIDE && (cf.strands & 2)) ? 2 : 1);
+    size_t qlen = 0, nq = 1;
+    if (cf.symtype == NUCLEOTIDE) {
+        nq = 0;
+        for (int s = 0; s < 2; ++s)
+            if (((s + 1) & cf.strands) && rq.query->has_nt[s]) {
+                ++nq;
+                qlen = std::max(qlen, rq.query->nt[s].size());
+            }
+    } else {
+        qlen = rq.query->aa[0].size();
+    }
+    const size_t mean_len = 400;  // a planning figure, not a limit
+    size_t bytes = nq * entries * sizeof(PairResult) + nq * entries * (mean_len / 4) * 16;
+    if (rq.align_type != COMPUTE_SCORE) bytes += rq.hitcount * qlen * mean_len * 4;
+    return bytes;
+}
+
+}  // namespace ssa
+
+using namespace ssa;
+
+extern "C" {
+
+struct ssa_amd_server {
+    Server* s;
+};
+
+ssa_amd_server* ssa_amd_server_start(int slots, size_t max_batch, size_t device_bytes) {
+    if (slots < 1 || slots > 8) return nullptr;
+    if (!matrix().valid) return nullptr;
+    const Config& c = config();
+    if (!matrix().constant && c.symtype == NUCLEOTIDE) return nullptr;
+    ssa_amd_server* h = new ssa_amd_server;
+    h->s = new Server(slots, max_batch, device_bytes);
+    return h;
+}
+
+int64_t ssa_amd_server_submit(ssa_amd_server* h, const ssa_amd_request_t* r) {
+    if (!h || !r) return -1;
+    if (r->search_type != 0 && r->search_type != 1) return -1;
+    if (!r->query) return -1;
+    if (r->bit_width != BIT_WIDTH_8 && r->bit_width != BIT_WIDTH_16 && r->bit_width != BIT_WIDTH_64) return -1;
+    if (r->align_type != COMPUTE_SCORE && r->align_type != COMPUTE_ALIGNMENT) return -1;
+    if (r->bit_width == BIT_WIDTH_8 && config().simd_mode < COMPUTE_ON_SSE41) return -1;
+    const SearchRequest rq{r->search_type == 0 ? kSW : kNW, r->query, r->hitcount, r->bit_width, r->align_type};
+    return h->s->submit(rq, request_footprint(rq));
+}
+
+p_alignment_list ssa_amd_server_wait(ssa_amd_server* h, int64_t ticket, ssa_amd_stats_t* stats, double* latency_ms,
+                                     double* queue_ms) {
+    if (!h) return nullptr;
+    SearchResult res;
+    if (!h->s->wait(ticket, &res, latency_ms, queue_ms)) return nullptr;
+    if (stats) *stats = res.stats;
+    return res.list;
+}
+
+void ssa_amd_server_counters(ssa_amd_server* h, ssa_amd_server_counters_t* out) {
+    if (h && out) h->s->counters(out);
+}
+
+void ssa_amd_server_stop(ssa_amd_server* h) {
+    if (!h) return;
+    delete h->s;
+    delete h;
+}
+
+}  // extern "C"

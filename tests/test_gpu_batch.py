@@ -1,0 +1,153 @@
+"""Batched searches (ssa_amd_batch_align) and the serving loop (ssa_amd_server_*).
+
+Every result of a batch or of the server must be identical to the same search
+run alone through sw_align / nw_align (which test_gpu_parity pins against the
+oracle); a subset is also compared with the oracle directly.
+"""
+import os
+import random
+import tempfile
+import threading
+import time
+
+import pytest
+
+import _oracle as O
+from test_gpu_parity import check, configure, rand_db, write_fasta
+
+pytestmark = pytest.mark.gpu
+
+ssa = pytest.importorskip("libssa_amd")
+
+AA = "ARNDCQEGHILKMFPSTWYV"
+
+
+def key(hits):
+    return [(a.score, a.db_id, a.db_strand, a.db_frame, a.query_strand, a.alignment, a.align_q_start,
+             a.align_q_end, a.align_d_start, a.align_d_end, a.db_seq) for a in hits]
+
+
+def alone(req):
+    t, q, k, bw, al = req
+    f = ssa.sw_align if t == ssa.SEARCH_SW else ssa.nw_align
+    return f(q, k, bw, al)
+
+
+def make_requests(rnd, queries, types=(ssa.SEARCH_SW, ssa.SEARCH_NW), widths=(8, 16, 64)):
+    reqs = []
+    for i, q in enumerate(queries):
+        t = types[i % len(types)]
+        bw = widths[(i // len(types)) % len(widths)]
+        al = ssa.COMPUTE_ALIGNMENT if i % 3 == 0 else ssa.COMPUTE_SCORE
+        reqs.append((t, q, rnd.choice([1, 5, 20]), bw, al))
+    return reqs
+
+
+@pytest.fixture
+def protein_db():
+    rnd = random.Random(2024)
+    base = "".join(rnd.choice(AA) for _ in range(700))
+    db = rand_db(rnd, O.AMINOACID, 350, 1, 450, homolog_of=base, frac=0.15)
+    configure(O.AMINOACID, 1, (-10, -1), ("builtin", "blosum62"))
+    with tempfile.TemporaryDirectory() as d:
+        path = os.path.join(d, "db.fas")
+        write_fasta(path, db)
+        ssa.init_db(path)
+        qraws = [base[:7], base[10:43], "".join(rnd.choice(AA) for _ in range(64)), base[100:220], base[:250],
+                 base[150:551], "".join(rnd.choice(AA) for _ in range(512)), base[:700], base[300:330],
+                 base[5:600], "".join(rnd.choice(AA) for _ in range(90)), base[400:480]]
+        qs = [ssa.init_sequence_fasta(ssa.READ_FROM_STRING, s) for s in qraws]
+        yield rnd, db, qraws, qs
+        for q in qs:
+            ssa.free_sequence(q)
+
+
+def test_batch_equals_alone(protein_db):
+    rnd, db, qraws, qs = protein_db
+    reqs = make_requests(rnd, qs)
+    want = [key(alone(r)) for r in reqs]
+    lists, stats = ssa.batch_align(reqs)
+    assert [key(x) for x in lists] == want
+    assert all(s.seq_count == len(db) for s in stats)
+
+
+def test_batch_matches_oracle(protein_db):
+    rnd, db, qraws, qs = protein_db
+    reqs = [(ssa.SEARCH_SW, qs[3], 20, 8, ssa.COMPUTE_ALIGNMENT), (ssa.SEARCH_NW, qs[5], 10, 16, ssa.COMPUTE_SCORE),
+            (ssa.SEARCH_SW, qs[7], 10, 16, ssa.COMPUTE_SCORE)]
+    lists, _ = ssa.batch_align(reqs)
+    m = O.matrix_builtin("blosum62")
+    for (t, _, k, bw, al), got, qi in zip(reqs, lists, (3, 5, 7)):
+        qb = [O.map_query(O.AMINOACID, qraws[qi])]
+        want, _ = O.search(O.SW if t == ssa.SEARCH_SW else O.NW, bw, O.AMINOACID, 1, qb, db, k, m, -10, -1)
+        assert [(a.score, a.db_id) for a in got] == [(s, i) for s, i, _, _ in want]
+        if al == ssa.COMPUTE_ALIGNMENT:
+            for a, (s, i, _, _) in zip(got, want):
+                c, r = O.cigar(O.SW, qb[0], O.map_db(O.AMINOACID, db[i]), m, -10, -1)
+                assert a.alignment == c
+
+
+def test_batch_nucleotide_both_strands():
+    rnd = random.Random(99)
+    base = "".join(rnd.choice("ACGT") for _ in range(300))
+    db = rand_db(rnd, O.NUCLEOTIDE, 250, 20, 320, homolog_of=base, frac=0.2)
+    configure(O.NUCLEOTIDE, 3, (-20, -2), ("const", (2, -4)))
+    with tempfile.TemporaryDirectory() as d:
+        path = os.path.join(d, "db.fas")
+        write_fasta(path, db)
+        ssa.init_db(path)
+        qraws = [base[:250], base[40:140], base, base[::-1][:77]]
+        qs = [ssa.init_sequence_fasta(ssa.READ_FROM_STRING, s) for s in qraws]
+        reqs = [(ssa.SEARCH_NW, qs[0], 20, 16, ssa.COMPUTE_ALIGNMENT), (ssa.SEARCH_SW, qs[1], 10, 8, ssa.COMPUTE_SCORE),
+                (ssa.SEARCH_NW, qs[2], 5, 8, ssa.COMPUTE_SCORE), (ssa.SEARCH_SW, qs[3], 15, 16, ssa.COMPUTE_ALIGNMENT)]
+        want = [key(alone(r)) for r in reqs]
+        lists, _ = ssa.batch_align(reqs)
+        assert [key(x) for x in lists] == want
+        for q in qs:
+            ssa.free_sequence(q)
+    # the alone path is the oracle-checked one: spot-check one request against the oracle as well
+    check(O.NW, 16, O.NUCLEOTIDE, 3, qraws[0], db, 20, ("const", (2, -4)), (-20, -2), align=True)
+
+
+@pytest.mark.parametrize("slots,max_batch", [(1, 1), (2, 3), (3, 16)])
+def test_server_results_equal_alone(protein_db, slots, max_batch):
+    rnd, db, qraws, qs = protein_db
+    reqs = make_requests(rnd, qs) * 2
+    want = [key(alone(r)) for r in reqs]
+    srv = ssa.Server(slots=slots, max_batch=max_batch)
+    try:
+        tickets = [None] * len(reqs)
+
+        def submit():
+            for i, r in enumerate(reqs):
+                tickets[i] = srv.submit(*r)
+                time.sleep(rnd.choice([0.0, 0.0, 0.002]))
+
+        th = threading.Thread(target=submit)
+        th.start()
+        th.join()
+        got = []
+        for t in tickets:
+            hits, st, lat, qms = srv.wait(t)
+            assert lat >= qms >= 0.0
+            got.append(key(hits))
+        c = srv.counters()
+        assert c.requests == len(reqs) and c.queued == 0
+        assert c.max_batch_seen <= max_batch
+    finally:
+        srv.stop()
+    assert got == want
+
+
+def test_server_rejects_invalid_requests(protein_db):
+    rnd, db, qraws, qs = protein_db
+    srv = ssa.Server(slots=1, max_batch=4)
+    try:
+        with pytest.raises(ValueError):
+            srv.submit(7, qs[0], 5, 8, ssa.COMPUTE_SCORE)
+        with pytest.raises(ValueError):
+            srv.submit(ssa.SEARCH_SW, qs[0], 5, 12, ssa.COMPUTE_SCORE)
+        with pytest.raises(KeyError):
+            srv.wait(12345)
+    finally:
+        srv.stop()
