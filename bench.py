@@ -7,10 +7,18 @@ cap 35,000; 5 % planted query fragments at 60-95 % identity), queries of 50,
 start with the 16/64-bit cascade, scores only.  One step = the three queries
 searched through sw_align with the DB resident on the GPU.
 
-    python bench.py [--steps K] [--warmup W] [--workload c2|c1|c3] [--no-cpu-baseline]
+    python bench.py [--steps K] [--warmup W] [--workload c2|c1|c3|c4|c5] [--no-cpu-baseline]
 
 Prints one JSON line.  value = algorithmic cells (sum over queries and DB
 entries of |q| * |d|) per second of the whole step, in GCUPS.
+
+c4 (batched): 256 queries (log-uniform 30-2000) against 500,000 sequences,
+sw_align(q, 20, BIT_WIDTH_8, COMPUTE_SCORE) each, run one query per call and
+then as one batch (ssa_amd_batch_align); value = batched GCUPS.
+c5 (served): 1,000,000 sequences resident; Poisson arrivals of an 80/20 SW/NW
+mix with CIGARs (top 32, 8-bit start), query lengths 70 % U(50,400) + 30 %
+log-U(400,5000); the same arrival trace is served one request per launch and
+with continuous batching; p50/p99 latency per rate.
 """
 import argparse
 import ctypes as C
@@ -121,6 +129,131 @@ def copy_probe(mib):
     hip.hipFree(b)
 
 
+def setup_protein(db):
+    import libssa_amd as ssa
+    from libssa_amd import synthetic as S
+    ssa.set_output_mode(ssa.OUTPUT_ERROR)
+    ssa.init_symbol_translation(ssa.AMINOACID, ssa.FORWARD_STRAND, 1, 1)
+    ssa.init_score_matrix(ssa.MATRIX_BUILDIN, ssa.BLOSUM62)
+    ssa.init_gap_penalties(-10, -1)
+    tmp = tempfile.mkdtemp(prefix="libssa_bench_")
+    path = os.path.join(tmp, "db.fasta")
+    S.write_fasta(path, db)
+    ssa.init_db(path)
+    return path, tmp
+
+
+def hits_key(hits):
+    return [(a.score, a.db_id, a.alignment) for a in hits]
+
+
+def bench_c4(args):
+    import libssa_amd as ssa
+    from libssa_amd import synthetic as S
+    qs, db, _ = S.config_c4(scale=args.scale)
+    path, tmp = setup_protein(db)
+    hs = [ssa.init_sequence_fasta(ssa.READ_FROM_STRING, q.decode()) for q in qs]
+    reqs = [(ssa.SEARCH_SW, h, 20, 8, ssa.COMPUTE_SCORE) for h in hs]
+    ssa.batch_align(reqs)  # warm-up: builds the DB's streams for every lane width used
+    t0 = time.perf_counter()
+    plain = [ssa.sw_align(h, 20, 8, ssa.COMPUTE_SCORE) for h in hs]
+    t1 = time.perf_counter()
+    lists, stats = ssa.batch_align(reqs)
+    t2 = time.perf_counter()
+    cells = sum(st.cells for st in stats)
+    equal = all(hits_key(a) == hits_key(b) for a, b in zip(plain, lists))
+    os.remove(path)
+    os.rmdir(tmp)
+    return {
+        "metric": "batched SW GCUPS on 1 MI355X (vs one query per launch)",
+        "value": round(cells / (t2 - t1) / 1e9, 2), "unit": "GCUPS", "n_gpus": 1, "higher_is_better": True,
+        "dtype": "int16", "data": "synthetic",
+        "config": {"workload": "256 queries log-uniform 30-2000 vs 500k lognormal protein DB, 2% planted, "
+                               "sw_align(q, 20, 8, scores), BLOSUM62 10/1", "db_sequences": len(db),
+                   "queries": len(qs), "query_residues": sum(len(q) for q in qs)},
+        "batched": {"seconds": round(t2 - t1, 3), "gcups": round(cells / (t2 - t1) / 1e9, 2),
+                    "gpu_first_tier_ms": round(stats[0].ms_gpu_first, 3)},
+        "one_query_per_launch": {"seconds": round(t1 - t0, 3), "gcups": round(cells / (t1 - t0) / 1e9, 2)},
+        "speedup": round((t1 - t0) / (t2 - t1), 3), "results_equal": equal, "cells": cells,
+    }
+
+
+def serve_trace(ssa, reqs, handles, slots, max_batch):
+    """Replays an arrival trace against a server; returns per-request
+    (latency_ms, queue_ms, cells) and the wall time from first submit to last result."""
+    import threading
+    srv = ssa.Server(slots=slots, max_batch=max_batch)
+    tickets = [None] * len(reqs)
+    t0 = time.perf_counter()
+
+    def submitter():
+        for i, r in enumerate(reqs):
+            d = t0 + r["t"] - time.perf_counter()
+            if d > 0:
+                time.sleep(d)
+            kind = ssa.SEARCH_SW if r["kind"] == "sw" else ssa.SEARCH_NW
+            tickets[i] = srv.submit(kind, handles[i], 32, 8, ssa.COMPUTE_ALIGNMENT)
+
+    th = threading.Thread(target=submitter)
+    th.start()
+    out = []
+    for i in range(len(reqs)):
+        while tickets[i] is None:
+            time.sleep(0.0005)
+        hits, st, lat, qms = srv.wait(tickets[i])
+        out.append((lat, qms, st.cells, hits_key(hits)))
+    th.join()
+    wall = time.perf_counter() - t0
+    c = srv.counters()
+    srv.stop()
+    return out, wall, c
+
+
+def bench_c5(args):
+    import numpy as np
+    import libssa_amd as ssa
+    from libssa_amd import synthetic as S
+    db = S.config_c5(scale=args.scale)
+    path, tmp = setup_protein(db)
+    warm = [ssa.init_sequence_fasta(ssa.READ_FROM_STRING, q.decode()) for q in
+            (S.make_query(1, 120), S.make_query(2, 700), S.make_query(3, 3000))]
+    for h in warm:  # builds the DB's streams for the lane widths the mix uses
+        ssa.sw_align(h, 32, 8, ssa.COMPUTE_ALIGNMENT)
+        ssa.nw_align(h, 32, 8, ssa.COMPUTE_ALIGNMENT)
+    rates = [float(x) for x in args.rates.split(",")]
+    rows = []
+    for rate in rates:
+        reqs = S.c5_requests(args.requests, rate)
+        handles = [ssa.init_sequence_fasta(ssa.READ_FROM_STRING, r["query"].decode()) for r in reqs]
+        row = {"rate_per_s": rate, "requests": len(reqs)}
+        results = {}
+        for name, slots, mb in (("one_request_per_launch", 1, 1), ("continuous_batching", 2, 16)):
+            out, wall, c = serve_trace(ssa, reqs, handles, slots, mb)
+            lat = np.array([o[0] for o in out])
+            cells = sum(o[2] for o in out)
+            results[name] = [o[3] for o in out]
+            row[name] = {"p50_ms": round(float(np.percentile(lat, 50)), 1),
+                         "p99_ms": round(float(np.percentile(lat, 99)), 1),
+                         "mean_queue_ms": round(float(np.mean([o[1] for o in out])), 1),
+                         "throughput_per_s": round(len(out) / wall, 2), "gcups": round(cells / wall / 1e9, 1),
+                         "batches": c.batches, "max_batch": c.max_batch_seen}
+        row["results_equal"] = results["one_request_per_launch"] == results["continuous_batching"]
+        rows.append(row)
+        for h in handles:
+            ssa.free_sequence(h)
+        print(json.dumps(row), file=sys.stderr, flush=True)
+    os.remove(path)
+    os.rmdir(tmp)
+    return {
+        "metric": "served p50/p99 latency and GCUPS on 1 MI355X (vs one request per launch)",
+        "unit": "ms", "n_gpus": 1, "higher_is_better": False, "dtype": "int16", "data": "synthetic",
+        "config": {"workload": "1M lognormal protein DB resident; Poisson arrivals; 80% SW / 20% NW, top 32 "
+                               "with CIGARs, 8-bit start; |q| 70% U(50,400) + 30% log-U(400,5000)",
+                   "db_sequences": len(db)},
+        "rates": rows,
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -129,12 +262,19 @@ def main():
     ap.add_argument("--workload", default="c2")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-sample", type=float, default=1.0)
+    ap.add_argument("--scale", type=float, default=1.0, help="c4/c5: DB size factor")
+    ap.add_argument("--rates", default="4,8,12,16", help="c5: arrival rates per second")
+    ap.add_argument("--requests", type=int, default=60, help="c5: requests per rate")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="also run one device-to-device copy of MIB MiB (calibrates HBM byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU only: this build has no multi-GPU path (--gpus %d requested)" % args.gpus)
         sys.exit(2)
+
+    if args.workload in ("c4", "c5"):
+        print(json.dumps(bench_c4(args) if args.workload == "c4" else bench_c5(args)))
+        return
 
     import libssa_amd as ssa
     w = workload(args.workload)

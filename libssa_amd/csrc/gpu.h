@@ -80,10 +80,9 @@ private:
 
 // ---- kernel launchers (score_kernels.hip, traceback_kernels.hip, pack_kernels.hip) ----
 bool score_shape_supported(int lanes, int rows);
-hipError_t launch_score(int lanes, int rows, int mode, int nwg, hipStream_t st, const ScoreWave* waves,
-                        const QueryDesc* queries, const int16_t* qp, const uint32_t* lb, const uint2* res, uint4* bnd,
-                        const int32_t* slot_begin, const int32_t* pair_list, PairResult* results,
-                        const ScoreParams& P);
+int score_resident_workgroups(int lanes, int rows, int mode);  // persistent grid size of a shape
+hipError_t launch_score(int lanes, int rows, int mode, hipStream_t st, const ScoreLaunch& L, const ScoreParams& P,
+                        int max_wg);
 hipError_t launch_init_results(hipStream_t st, PairResult* r, int64_t n, int32_t score);
 hipError_t launch_exact(int npairs, hipStream_t st, const ExactPair* pairs, const QueryDesc* queries,
                         const int32_t* qp32, const uint8_t* codes, int mode, int go, int ge, int32_t* scratch,

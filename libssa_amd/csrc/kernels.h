@@ -30,20 +30,23 @@ constexpr int kWavesPerWG = 4;
 constexpr int kNoFlag = 0x7fffffff;
 
 struct ScoreWave {
-    int32_t query;      // index into the QueryDesc table (the same for all waves of a workgroup)
     int32_t nblocks;    // blocks of the wave's longest stream (0 = an idle filler wave)
+    int32_t slot_base;  // stream (group k, half h): entries slot_first[slot_base + 2k + h] .. of slot_entries
     int64_t res_off;    // in uint2 units
-    int64_t bnd_off;    // in uint4 units: bnd[nblocks][64/G][2], the last lane's bottom row between passes
-    int32_t slot_base;  // stream (group k, half h) of this wave: slot_begin[slot_base + 2k + h]
-    int32_t pad;
+};
+
+// One workgroup task: a query against kWavesPerWG consecutive waves of the layout.
+struct ScoreTask {
+    int32_t query;
+    int32_t wave0;
 };
 
 struct QueryDesc {
     int32_t qlen;
-    int32_t qp_off;  // score kernel: int16 units, profile [pass][32 codes][G*R rows];
-                     // exact tier: int32 units, profile [qlen][32 codes]
-    int32_t lb_off;  // in uint32 units: NW left boundary [pass*G*R rows][2] packed (H, E)
-    int32_t pad;
+    int32_t qp_off;    // score kernel: int16 units, profile [pass][32 codes][G*R rows];
+                       // exact tier: int32 units, profile [qlen][32 codes]
+    int32_t lb_off;    // in uint32 units: NW left boundary [pass*G*R rows][2] packed (H, E)
+    int32_t pair_off;  // result index of (this query, entry position 0)
 };
 
 // Per-launch constants in the kernel's value representation: every 16-bit
@@ -61,16 +64,6 @@ struct ScoreParams {
     uint32_t fb_init[4];  // NW: top boundary F of the first block's four columns
 };
 
-// one (stream, entry) of the residue interleaving kernel
-struct PackEntry {
-    int64_t code_off;  // byte offset of the entry's codes (padded to 4) in the DB code buffer
-    int64_t dst;       // uint32 index of the entry's first word in the residue array
-    int32_t nblocks;   // blocks of the entry
-    int32_t last_col;  // (length - 1) % 4
-    int32_t stride;    // uint32 distance between the words of consecutive blocks
-    int32_t pad;
-};
-
 // result per pair: score in representation (SW: maximum cell, NW: last cell),
 // first flagged block (kNoFlag if none), second SW flag block.  The SW fields
 // are combined over lanes and passes with atomic max / min.
@@ -78,6 +71,37 @@ struct PairResult {
     int32_t score;
     int32_t flag_a;
     int32_t flag_b;
+    int32_t pad;
+};
+
+// Arguments of one score-kernel launch.  The grid is persistent: each
+// workgroup takes tasks from *counter (zeroed before the launch) until they
+// run out, and keeps the pass boundaries of its 4 waves in its own slice of
+// bnd (bnd_per_wave uint4 per wave).
+struct ScoreLaunch {
+    const ScoreWave* waves;
+    const ScoreTask* tasks;
+    int32_t ntasks;
+    int32_t pad;
+    int32_t* counter;
+    const QueryDesc* queries;
+    const int16_t* qp;
+    const uint32_t* lb;
+    const uint2* res;
+    uint4* bnd;
+    int64_t bnd_per_wave;
+    const int32_t* slot_first;
+    const int32_t* slot_entries;
+    PairResult* results;
+};
+
+// one (stream, entry) of the residue interleaving kernel
+struct PackEntry {
+    int64_t code_off;  // byte offset of the entry's codes (padded to 4) in the DB code buffer
+    int64_t dst;       // uint32 index of the entry's first word in the residue array
+    int32_t nblocks;   // blocks of the entry
+    int32_t last_col;  // (length - 1) % 4
+    int32_t stride;    // uint32 distance between the words of consecutive blocks
     int32_t pad;
 };
 

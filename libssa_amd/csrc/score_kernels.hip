@@ -141,7 +141,8 @@ template <int R, int G, int MODE, bool FIRST>
 __device__ __forceinline__ void run_pass(const ScoreParams& P, uint32_t gbase, const char* sp,
                                          const uint32_t* slbh, const uint32_t* slbe,
                                          const uint2* __restrict__ wres, uint4* __restrict__ wbnd,
-                                         const int32_t* __restrict__ pair_list, PairResult* __restrict__ results,
+                                         const int32_t* __restrict__ slot_entries, int pair_off,
+                                         PairResult* __restrict__ results,
                                          int sb_lo, int sb_hi, int k, int g, int nb, bool last_pass, int last_g,
                                          int last_r) {
     constexpr int NG = 64 / G;
@@ -209,14 +210,14 @@ __device__ __forceinline__ void run_pass(const ScoreParams& P, uint32_t gbase, c
         if (__any(st_lo || st_hi)) {
             const uint32_t m = (st_lo ? 0x0000ffffu : 0u) | (st_hi ? 0xffff0000u : 0u);
             if (st_lo) {
-                pair_lo = pair_list[sb_lo + kp_lo];
+                pair_lo = pair_off + slot_entries[sb_lo + kp_lo];
                 kp_lo++;
                 blk_lo = 0;
                 fa_lo = kNoFlag;
                 fb_lo = kNoFlag;
             }
             if (st_hi) {
-                pair_hi = pair_list[sb_hi + kp_hi];
+                pair_hi = pair_off + slot_entries[sb_hi + kp_hi];
                 kp_hi++;
                 blk_hi = 0;
                 fa_hi = kNoFlag;
@@ -446,57 +447,68 @@ __device__ __forceinline__ void run_pass(const ScoreParams& P, uint32_t gbase, c
 }
 
 template <int R, int G, int MODE>
-__global__ __launch_bounds__(64 * kWavesPerWG) void score_kernel(
-    const ScoreWave* __restrict__ waves, const QueryDesc* __restrict__ queries, const int16_t* __restrict__ qp,
-    const uint32_t* __restrict__ lb, const uint2* __restrict__ res, uint4* __restrict__ bnd,
-    const int32_t* __restrict__ slot_begin, const int32_t* __restrict__ pair_list, PairResult* __restrict__ results,
-    const ScoreParams P) {
+__global__ __launch_bounds__(64 * kWavesPerWG) void score_kernel(const ScoreLaunch L, const ScoreParams P) {
     constexpr int PR = G * R;  // rows per pass
     static_assert((1 << Log2<PR>::value) == PR, "rows per pass must be a power of two");
     __shared__ __align__(16) int16_t sprof[32 * PR];
     __shared__ uint32_t slbh[MODE == 1 ? PR + 1 : 1];  // NW left boundary H of rows base-1 .. base+PR-1
     __shared__ uint32_t slbe[MODE == 1 ? PR : 1];      // NW left boundary E of rows base .. base+PR-1
+    __shared__ int s_task;
 
-    const int lane = threadIdx.x & 63;
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     const int k = lane / G, g = lane % G;
-    const ScoreWave Wv = waves[blockIdx.x * kWavesPerWG + (threadIdx.x >> 6)];
-    const QueryDesc Q = queries[Wv.query];
-    const int npass = (Q.qlen + PR - 1) / PR;
-    const int nb = Wv.nblocks;
-    const uint2* __restrict__ wres = res + Wv.res_off;
-    uint4* __restrict__ wbnd = bnd + Wv.bnd_off;
-    const int sb_lo = slot_begin[Wv.slot_base + 2 * k];
-    const int sb_hi = slot_begin[Wv.slot_base + 2 * k + 1];
-    const int last_lr = Q.qlen - 1 - (npass - 1) * PR;  // last real row within the last pass
-    const int last_g = last_lr / R, last_r = last_lr % R;
     const uint32_t gbase = (uint32_t)(g * R * 2);  // byte offset of the lane's rows in a profile code row
     const char* sp = reinterpret_cast<const char*>(sprof);
+    uint4* __restrict__ wbnd = L.bnd + ((int64_t)blockIdx.x * kWavesPerWG + wv) * L.bnd_per_wave;
 
-    for (int pass = 0; pass < npass; ++pass) {
+    for (;;) {
+        __syncthreads();  // the previous task is done with s_task and the staged profile
+        if (threadIdx.x == 0) s_task = atomicAdd(L.counter, 1);
         __syncthreads();
-        {
-            const uint4* src = reinterpret_cast<const uint4*>(qp + Q.qp_off + (int64_t)pass * 32 * PR);
-            uint4* dst = reinterpret_cast<uint4*>(sprof);
-            for (int i = threadIdx.x; i < 32 * PR / 8; i += 64 * kWavesPerWG) dst[i] = src[i];
-            if (MODE == 1) {
-                const int base = pass * PR;
-                for (int i = threadIdx.x; i < PR; i += 64 * kWavesPerWG) {
-                    slbh[1 + i] = lb[Q.lb_off + 2 * (base + i)];
-                    slbe[i] = lb[Q.lb_off + 2 * (base + i) + 1];
+        const int task = s_task;
+        if (task >= L.ntasks) break;  // the same for the whole workgroup
+        const ScoreTask T = L.tasks[task];
+        const ScoreWave Wv = L.waves[T.wave0 + wv];
+        const QueryDesc Q = L.queries[T.query];
+        const int npass = (Q.qlen + PR - 1) / PR;
+        const int nb = Wv.nblocks;
+        const uint2* __restrict__ wres = L.res + Wv.res_off;
+        const int sb_lo = L.slot_first[Wv.slot_base + 2 * k];
+        const int sb_hi = L.slot_first[Wv.slot_base + 2 * k + 1];
+        const int last_lr = Q.qlen - 1 - (npass - 1) * PR;  // last real row within the last pass
+        const int last_g = last_lr / R, last_r = last_lr % R;
+
+        for (int pass = 0; pass < npass; ++pass) {
+            __syncthreads();
+            {
+                const uint4* src = reinterpret_cast<const uint4*>(L.qp + Q.qp_off + (int64_t)pass * 32 * PR);
+                uint4* dst = reinterpret_cast<uint4*>(sprof);
+                for (int i = threadIdx.x; i < 32 * PR / 8; i += 64 * kWavesPerWG) dst[i] = src[i];
+                if (MODE == 1) {
+                    const int base = pass * PR;
+                    for (int i = threadIdx.x; i < PR; i += 64 * kWavesPerWG) {
+                        slbh[1 + i] = L.lb[Q.lb_off + 2 * (base + i)];
+                        slbe[i] = L.lb[Q.lb_off + 2 * (base + i) + 1];
+                    }
+                    if (threadIdx.x == 0) slbh[0] = base > 0 ? L.lb[Q.lb_off + 2 * (base - 1)] : 0u;
                 }
-                if (threadIdx.x == 0) slbh[0] = base > 0 ? lb[Q.lb_off + 2 * (base - 1)] : 0u;
+            }
+            __syncthreads();
+            const bool last_pass = pass == npass - 1;
+            if (pass == 0) {
+                run_pass<R, G, MODE, true>(P, gbase, sp, slbh, slbe, wres, wbnd, L.slot_entries, Q.pair_off,
+                                           L.results, sb_lo, sb_hi, k, g, nb, last_pass, last_g, last_r);
+            } else {
+                run_pass<R, G, MODE, false>(P, gbase, sp, slbh, slbe, wres, wbnd, L.slot_entries, Q.pair_off,
+                                            L.results, sb_lo, sb_hi, k, g, nb, last_pass, last_g, last_r);
             }
         }
-        __syncthreads();
-        const bool last_pass = pass == npass - 1;
-        if (pass == 0) {
-            run_pass<R, G, MODE, true>(P, gbase, sp, slbh, slbe, wres, wbnd, pair_list, results, sb_lo, sb_hi, k, g,
-                                       nb, last_pass, last_g, last_r);
-        } else {
-            run_pass<R, G, MODE, false>(P, gbase, sp, slbh, slbe, wres, wbnd, pair_list, results, sb_lo, sb_hi, k, g,
-                                        nb, last_pass, last_g, last_r);
-        }
     }
+}
+
+__global__ void fill_u32_kernel(uint32_t* __restrict__ p, int64_t n, uint32_t v) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        p[i] = v;
 }
 
 // Sets every pair's result to the neutral value of the atomic combination.
@@ -593,45 +605,56 @@ __global__ __launch_bounds__(64) void pack_kernel(const PackEntry* __restrict__ 
     }
 }
 
-__global__ void fill_u32_kernel(uint32_t* __restrict__ p, int64_t n, uint32_t v) {
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
-        p[i] = v;
-}
 
 // ---------------------------------------------------------------------------
 // launch wrappers (host side)
 // ---------------------------------------------------------------------------
 template <int R, int G, int MODE>
-static hipError_t launch_score_t(int nwg, hipStream_t st, const ScoreWave* waves, const QueryDesc* queries,
-                                 const int16_t* qp, const uint32_t* lb, const uint2* res, uint4* bnd,
-                                 const int32_t* slot_begin, const int32_t* pair_list, PairResult* results,
-                                 const ScoreParams& P) {
-    hipLaunchKernelGGL((score_kernel<R, G, MODE>), dim3(nwg), dim3(64 * kWavesPerWG), 0, st, waves, queries, qp, lb,
-                       res, bnd, slot_begin, pair_list, results, P);
+static hipError_t launch_score_t(hipStream_t st, const ScoreLaunch& L, const ScoreParams& P, int max_wg) {
+    const int nwg = std::max(1, std::min(L.ntasks, max_wg));
+    hipLaunchKernelGGL(fill_u32_kernel, dim3(1), dim3(64), 0, st, reinterpret_cast<uint32_t*>(L.counter),
+                       (int64_t)1, 0u);
+    hipLaunchKernelGGL((score_kernel<R, G, MODE>), dim3(nwg), dim3(64 * kWavesPerWG), 0, st, L, P);
     return hipGetLastError();
+}
+
+template <int R, int G, int MODE>
+static int resident_wg_t() {
+    static int cap = 0;
+    if (!cap) {
+        int per_cu = 0, dev = 0;
+        hipDeviceProp_t prop;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, score_kernel<R, G, MODE>, 64 * kWavesPerWG, 0) !=
+                hipSuccess ||
+            hipGetDevice(&dev) != hipSuccess || hipGetDeviceProperties(&prop, dev) != hipSuccess)
+            return 0;
+        cap = std::max(1, per_cu) * prop.multiProcessorCount;
+    }
+    return cap;
 }
 
 bool score_shape_supported(int lanes, int rows) {
     return (lanes == 8 || lanes == 16 || lanes == 32 || lanes == 64) && (rows == 4 || rows == 8 || rows == 16);
 }
 
-hipError_t launch_score(int lanes, int rows, int mode, int nwg, hipStream_t st, const ScoreWave* waves,
-                        const QueryDesc* queries, const int16_t* qp, const uint32_t* lb, const uint2* res, uint4* bnd,
-                        const int32_t* slot_begin, const int32_t* pair_list, PairResult* results,
-                        const ScoreParams& P) {
-    if (nwg <= 0) return hipSuccess;
-#define SSA_CASE(GG, RR)                                                                                        \
-    if (lanes == GG && rows == RR)                                                                            \
-        return mode == 0 ? launch_score_t<RR, GG, 0>(nwg, st, waves, queries, qp, lb, res, bnd, slot_begin,   \
-                                                     pair_list, results, P)                                   \
-                         : launch_score_t<RR, GG, 1>(nwg, st, waves, queries, qp, lb, res, bnd, slot_begin,   \
-                                                     pair_list, results, P);
-#define SSA_ROWS(GG) SSA_CASE(GG, 4) SSA_CASE(GG, 8) SSA_CASE(GG, 16)
-    SSA_ROWS(8)
-    SSA_ROWS(16)
-    SSA_ROWS(32)
-    SSA_ROWS(64)
-#undef SSA_ROWS
+#define SSA_SHAPES(X) \
+    X(8, 4) X(8, 8) X(8, 16) X(16, 4) X(16, 8) X(16, 16) X(32, 4) X(32, 8) X(32, 16) X(64, 4) X(64, 8) X(64, 16)
+
+int score_resident_workgroups(int lanes, int rows, int mode) {
+#define SSA_CASE(GG, RR) \
+    if (lanes == GG && rows == RR) return mode == 0 ? resident_wg_t<RR, GG, 0>() : resident_wg_t<RR, GG, 1>();
+    SSA_SHAPES(SSA_CASE)
+#undef SSA_CASE
+    return 0;
+}
+
+hipError_t launch_score(int lanes, int rows, int mode, hipStream_t st, const ScoreLaunch& L, const ScoreParams& P,
+                        int max_wg) {
+    if (L.ntasks <= 0) return hipSuccess;
+#define SSA_CASE(GG, RR)                                                                               \
+    if (lanes == GG && rows == RR)                                                                   \
+        return mode == 0 ? launch_score_t<RR, GG, 0>(st, L, P, max_wg) : launch_score_t<RR, GG, 1>(st, L, P, max_wg);
+    SSA_SHAPES(SSA_CASE)
 #undef SSA_CASE
     return hipErrorInvalidValue;
 }

@@ -16,6 +16,7 @@
 #include <climits>
 #include <cstring>
 #include <numeric>
+#include <functional>
 #include <list>
 #include <memory>
 #include <mutex>
@@ -275,11 +276,20 @@ struct StreamLayout {
     std::vector<uint32_t> slot_entries;  // positions into PairSet::entries, stream by stream
     int64_t res_words = 0;               // uint2 words of the residue array
     uint2* d_res = nullptr;              // device residue array
+    ScoreWave* d_waves = nullptr;        // device copies of the wave table and the stream CSR
+    int32_t* d_slot_first = nullptr;
+    int32_t* d_slot_entries = nullptr;
     uint64_t generation = 0;             // cached layouts: DB generation they belong to
 
     void release() {
         if (d_res) (void)hipFree(d_res);
+        if (d_waves) (void)hipFree(d_waves);
+        if (d_slot_first) (void)hipFree(d_slot_first);
+        if (d_slot_entries) (void)hipFree(d_slot_entries);
         d_res = nullptr;
+        d_waves = nullptr;
+        d_slot_first = nullptr;
+        d_slot_entries = nullptr;
     }
 };
 
@@ -364,6 +374,14 @@ void build_layout(Exec& ex, StreamLayout& L, int lanes, const PairSet& ps, const
             }
         }
     L.release();
+    std::vector<ScoreWave> wt(L.nwaves);
+    for (int w = 0; w < L.nwaves; ++w) wt[w] = ScoreWave{L.wave_nblocks[w], w * spw, L.wave_res_off[w]};
+    SSA_HIP(hipMalloc(&L.d_waves, std::max<size_t>(wt.size(), 1) * sizeof(ScoreWave)));
+    SSA_HIP(hipMalloc(&L.d_slot_first, L.slot_first.size() * sizeof(int32_t)));
+    SSA_HIP(hipMalloc(&L.d_slot_entries, std::max<size_t>(L.slot_entries.size(), 1) * sizeof(int32_t)));
+    ex.upload(L.d_waves, wt.data(), wt.size() * sizeof(ScoreWave));
+    ex.upload(L.d_slot_first, L.slot_first.data(), L.slot_first.size() * sizeof(int32_t));
+    ex.upload(L.d_slot_entries, L.slot_entries.data(), L.slot_entries.size() * sizeof(int32_t));
     SSA_HIP(hipMalloc(&L.d_res, (size_t)std::max<int64_t>(L.res_words, 1) * sizeof(uint2)));
     auto* d_pe = (PackEntry*)ex.arena("pack", std::max<size_t>(pe.size(), 1) * sizeof(PackEntry));
     ex.upload(d_pe, pe.data(), pe.size() * sizeof(PackEntry));
@@ -412,22 +430,15 @@ struct TierRun {
 
     StreamLayout local;
     StreamLayout* L = nullptr;
-    int nwg = 0;
+    int grid = 0;
     ScoreParams P;
-    ScoreWave* d_waves = nullptr;
-    QueryDesc* d_qd = nullptr;
-    int16_t* d_prof = nullptr;
-    uint32_t* d_lb = nullptr;
-    int32_t* d_sb = nullptr;
-    int32_t* d_pl = nullptr;
-    uint4* d_bnd = nullptr;
+    ScoreLaunch K{};
     PairResult* d_out = nullptr;
     // host staging, kept until finish
-    std::vector<ScoreWave> waves;
+    std::vector<ScoreTask> tasks;
     std::vector<QueryDesc> qd;
     std::vector<int16_t> prof;
     std::vector<uint32_t> lbv;
-    std::vector<int32_t> slot_begin, pair_list;
 
     ~TierRun() { local.release(); }
     bool empty() const { return ps == nullptr || ps->entries.empty() || qs.empty(); }
@@ -535,53 +546,73 @@ void tier_prepare(Exec& ex, TierRun& R) {
     }
     const StreamLayout* L = R.L;
 
-    // ---- waves of every query ----
+    // ---- tasks: every query against every group of 4 waves, costliest first ----
     const int nw = L->nwaves;
-    R.waves.assign((size_t)nq * nw, ScoreWave{});
-    R.slot_begin.assign((size_t)nq * nw * spw, 0);
-    R.pair_list.assign(nq * n, 0);
-    int64_t bnd_total = 0;
+    if ((uint64_t)nq * n >= (uint64_t)INT32_MAX) fatal("libssa_amd: too many (query, entry) pairs in one launch");
+    const int nt_q = nw / kWavesPerWG;
+    R.tasks.clear();
+    R.tasks.reserve((size_t)nq * nt_q);
+    std::vector<int64_t> cost;
+    cost.reserve((size_t)nq * nt_q);
+    bool multipass = false;
     for (size_t qi = 0; qi < nq; ++qi) {
-        for (int w = 0; w < nw; ++w) {
-            ScoreWave& W = R.waves[qi * nw + w];
-            W.query = (int32_t)qi;
-            W.nblocks = L->wave_nblocks[w];
-            W.res_off = L->wave_res_off[w];
-            W.bnd_off = bnd_total;
-            W.slot_base = (int32_t)((qi * nw + w) * spw);
-            W.pad = 0;
-            if (npass[qi] > 1) bnd_total += (int64_t)W.nblocks * ng * 2;
-            R.computed += (uint64_t)W.nblocks * 4 * spw * (uint64_t)npass[qi] * PR;
-            for (int sl = 0; sl < spw; ++sl)
-                R.slot_begin[(qi * nw + w) * spw + sl] = (int32_t)(qi * n) + L->slot_first[(size_t)w * spw + sl];
+        R.qd[qi].pair_off = (int32_t)(qi * n);
+        multipass |= npass[qi] > 1;
+        for (int t = 0; t < nt_q; ++t) {
+            R.tasks.push_back(ScoreTask{(int32_t)qi, t * kWavesPerWG});
+            cost.push_back((int64_t)(L->wave_nblocks[t * kWavesPerWG] + sh.lanes) * npass[qi]);
         }
-        for (size_t j = 0; j < n; ++j) R.pair_list[qi * n + j] = (int32_t)ps.index(qi, L->slot_entries[j]);
+        for (int w = 0; w < nw; ++w) R.computed += (uint64_t)L->wave_nblocks[w] * 4 * spw * (uint64_t)npass[qi] * PR;
     }
-    R.nwg = (int)((size_t)nq * nw / kWavesPerWG);
+    if (nq > 1) {
+        std::vector<uint32_t> ord(R.tasks.size());
+        std::iota(ord.begin(), ord.end(), 0u);
+        std::stable_sort(ord.begin(), ord.end(), [&](uint32_t a, uint32_t b) { return cost[a] > cost[b]; });
+        std::vector<ScoreTask> sorted(R.tasks.size());
+        for (size_t i = 0; i < ord.size(); ++i) sorted[i] = R.tasks[ord[i]];
+        R.tasks.swap(sorted);
+    }
+    const int cap = score_resident_workgroups(sh.lanes, sh.rows, R.type == kSW ? 0 : 1);
+    R.grid = std::max(1, std::min<int>((int)R.tasks.size(), cap > 0 ? cap : (int)R.tasks.size()));
+    int32_t max_nb = 0;
+    for (int w = 0; w < nw; ++w) max_nb = std::max(max_nb, L->wave_nblocks[w]);
+    const int64_t bnd_per_wave = multipass ? (int64_t)max_nb * ng * 2 : 0;
 
     // ---- device buffers ----
     auto name = [&](const char* s) { return R.pfx + s; };
-    R.d_waves = (ScoreWave*)ex.arena(name("waves"), R.waves.size() * sizeof(ScoreWave));
-    R.d_qd = (QueryDesc*)ex.arena(name("qdesc"), R.qd.size() * sizeof(QueryDesc));
-    R.d_prof = (int16_t*)ex.arena(name("prof"), std::max<size_t>(R.prof.size(), 8) * sizeof(int16_t));
-    R.d_lb = (uint32_t*)ex.arena(name("lb"), std::max<size_t>(R.lbv.size(), 1) * sizeof(uint32_t));
-    R.d_sb = (int32_t*)ex.arena(name("slot_begin"), R.slot_begin.size() * sizeof(int32_t));
-    R.d_pl = (int32_t*)ex.arena(name("pair_list"), R.pair_list.size() * sizeof(int32_t));
-    R.d_bnd = (uint4*)ex.arena(name("bnd"), (size_t)std::max<int64_t>(bnd_total, 1) * sizeof(uint4));
+    auto* d_tasks = (ScoreTask*)ex.arena(name("tasks"), std::max<size_t>(R.tasks.size(), 1) * sizeof(ScoreTask));
+    auto* d_qd = (QueryDesc*)ex.arena(name("qdesc"), R.qd.size() * sizeof(QueryDesc));
+    auto* d_prof = (int16_t*)ex.arena(name("prof"), std::max<size_t>(R.prof.size(), 8) * sizeof(int16_t));
+    auto* d_lb = (uint32_t*)ex.arena(name("lb"), std::max<size_t>(R.lbv.size(), 1) * sizeof(uint32_t));
+    auto* d_bnd = (uint4*)ex.arena(
+        name("bnd"), (size_t)std::max<int64_t>((int64_t)R.grid * kWavesPerWG * bnd_per_wave, 1) * sizeof(uint4));
+    auto* d_counter = (int32_t*)ex.arena(name("counter"), sizeof(int32_t));
     R.d_out = (PairResult*)ex.arena(name("results"), R.out.size() * sizeof(PairResult));
-    ex.upload(R.d_waves, R.waves.data(), R.waves.size() * sizeof(ScoreWave));
-    ex.upload(R.d_qd, R.qd.data(), R.qd.size() * sizeof(QueryDesc));
-    ex.upload(R.d_prof, R.prof.data(), R.prof.size() * sizeof(int16_t));
-    ex.upload(R.d_lb, R.lbv.data(), R.lbv.size() * sizeof(uint32_t));
-    ex.upload(R.d_sb, R.slot_begin.data(), R.slot_begin.size() * sizeof(int32_t));
-    ex.upload(R.d_pl, R.pair_list.data(), R.pair_list.size() * sizeof(int32_t));
+    ex.upload(d_tasks, R.tasks.data(), R.tasks.size() * sizeof(ScoreTask));
+    ex.upload(d_qd, R.qd.data(), R.qd.size() * sizeof(QueryDesc));
+    ex.upload(d_prof, R.prof.data(), R.prof.size() * sizeof(int16_t));
+    ex.upload(d_lb, R.lbv.data(), R.lbv.size() * sizeof(uint32_t));
+    ScoreLaunch& K = R.K;
+    K.waves = L->d_waves;
+    K.tasks = d_tasks;
+    K.ntasks = (int32_t)R.tasks.size();
+    K.pad = 0;
+    K.counter = d_counter;
+    K.queries = d_qd;
+    K.qp = d_prof;
+    K.lb = d_lb;
+    K.res = L->d_res;
+    K.bnd = d_bnd;
+    K.bnd_per_wave = bnd_per_wave;
+    K.slot_first = L->d_slot_first;
+    K.slot_entries = L->d_slot_entries;
+    K.results = R.d_out;
 }
 
 void tier_launch(hipStream_t st, TierRun& R) {
     if (R.empty()) return;
     SSA_HIP(launch_init_results(st, R.d_out, (int64_t)R.out.size(), -32768));
-    SSA_HIP(launch_score(R.sh.lanes, R.sh.rows, R.type == kSW ? 0 : 1, R.nwg, st, R.d_waves, R.d_qd, R.d_prof,
-                         R.d_lb, R.L->d_res, R.d_bnd, R.d_sb, R.d_pl, R.d_out, R.P));
+    SSA_HIP(launch_score(R.sh.lanes, R.sh.rows, R.type == kSW ? 0 : 1, st, R.K, R.P, R.grid));
 }
 
 void tier_finish(Exec& ex, TierRun& R) {
@@ -899,6 +930,10 @@ struct Job {
     Shape shape{0, 0};
     size_t q_off = 0;  // first query of this job in its first-tier group
     int group = -1;
+    bool requeue_full = false;  // 16-bit re-run over the whole DB (most entries overflowed at 8 bit)
+    bool requeue_sub = false;   // 16-bit re-run over the overflowed entries only
+    size_t rq_off = 0;          // first query of this job in its re-run group
+    int rgroup = -1;
     std::vector<Elem> hits;
     p_alignment_list list = nullptr;
 };
@@ -1093,10 +1128,12 @@ void run_batch(Exec& ex, const std::vector<SearchRequest>& reqs, std::vector<Sea
     std::iota(all.entries.begin(), all.entries.end(), 0u);
     const bool full_db = N == db.entries.size();
     int64_t total_blocks = 0, max_blocks = 0;
+    uint64_t total_residues = 0;
     for (uint32_t e = 0; e < N; ++e) {
         const int64_t nb = (db.entries[e].len + 3) / 4;
         total_blocks += nb;
         max_blocks = std::max(max_blocks, nb);
+        total_residues += db.entries[e].len;
     }
 
     const MatrixInfo mi = matrix_info();
@@ -1108,8 +1145,7 @@ void run_batch(Exec& ex, const std::vector<SearchRequest>& reqs, std::vector<Sea
         Job& J = jobs[j];
         J.st.chunk_count = chunks.size();
         J.st.seq_count = N;
-        for (const QueryBuf& q : J.qs)
-            for (uint32_t e = 0; e < N; ++e) J.st.cells += (uint64_t)q.len * db.entries[e].len;
+        for (const QueryBuf& q : J.qs) J.st.cells += (uint64_t)q.len * total_residues;
         const size_t nq = J.qs.size();
         J.in16.assign(N, 0);
         J.need64.assign(N, 0);
@@ -1197,13 +1233,29 @@ void run_batch(Exec& ex, const std::vector<SearchRequest>& reqs, std::vector<Sea
     }
 
     // ---- decode the first tiers; re-queued and exact tiers per search ----
-    for (size_t j = 0; j < jobs.size(); ++j) {
+    std::vector<double> ms_re_job(jobs.size(), 0.0);
+    std::vector<uint64_t> computed_job(jobs.size(), 0);
+    const size_t nthreads = std::max<size_t>(1, std::min<size_t>(16, std::thread::hardware_concurrency()));
+    // host work per search, searches spread over threads
+    auto parallel_jobs = [&](const std::function<void(size_t)>& fn) {
+        const size_t nt = std::min(nthreads, jobs.size());
+        if (nt <= 1) {
+            for (size_t j = 0; j < jobs.size(); ++j) fn(j);
+            return;
+        }
+        std::vector<std::thread> th;
+        for (size_t t = 0; t < nt; ++t)
+            th.emplace_back([&, t] {
+                for (size_t j = t; j < jobs.size(); j += nt) fn(j);
+            });
+        for (auto& x : th) x.join();
+    };
+    parallel_jobs([&](size_t j) {
         Job& J = jobs[j];
         const SearchType type = J.rq.type;
         const size_t nq = J.qs.size();
         J.st.ms_gpu_first = J.group >= 0 ? ms_first : 0.0;
-        double ms_re = 0;
-        uint64_t computed = 0;
+        uint64_t& computed = computed_job[j];
         if (J.group >= 0) {
             TierRun& R = *runs[J.group];
             computed += R.computed * nq / std::max<size_t>(R.qs.size(), 1);
@@ -1237,21 +1289,12 @@ void run_batch(Exec& ex, const std::vector<SearchRequest>& reqs, std::vector<Sea
                 for (uint32_t e = 0; e < N; ++e)
                     if (J.in16[e]) sub.entries.push_back(e);
                 J.t16.resize(nq * N);
-                if (!sub.entries.empty()) {
-                    TierRun R2;
-                    R2.type = type;
-                    R2.T = kTier16;
-                    R2.qs = J.qs;
-                    R2.ps = &sub;
-                    R2.pfx = "re_";
-                    gpu_score_tier(ex, R2, &ms_re);
-                    computed += R2.computed;
-                    for (size_t q = 0; q < nq; ++q)
-                        for (size_t k = 0; k < sub.entries.size(); ++k) {
-                            const uint32_t e = sub.entries[k];
-                            J.t16[at(q, e)] =
-                                decode(type, kTier16, R2.out[sub.index(q, k)], db.entries[e].len, false);
-                        }
+                if (sub.entries.size() * 2 >= N && N > 0) {
+                    // most entries overflowed: re-run the whole DB on its cached
+                    // streams together with the batch's other such searches
+                    J.requeue_full = true;
+                } else if (!sub.entries.empty()) {
+                    J.requeue_sub = true;
                 }
             }
         } else if (bw == BIT_WIDTH_16) {
@@ -1262,6 +1305,91 @@ void run_batch(Exec& ex, const std::vector<SearchRequest>& reqs, std::vector<Sea
                     J.in16[e] = 1;
                 }
         }
+    });
+
+    // ---- 16-bit re-runs over the 8-bit overflows of single searches ----
+    for (size_t j = 0; j < jobs.size(); ++j) {
+        Job& J = jobs[j];
+        if (!J.requeue_sub) continue;
+        const size_t nq = J.qs.size();
+        PairSet sub;
+        sub.nq = nq;
+        for (uint32_t e = 0; e < N; ++e)
+            if (J.in16[e]) sub.entries.push_back(e);
+        TierRun R2;
+        R2.type = J.rq.type;
+        R2.T = kTier16;
+        R2.qs = J.qs;
+        R2.ps = &sub;
+        R2.pfx = "re_";
+        gpu_score_tier(ex, R2, &ms_re_job[j]);
+        computed_job[j] += R2.computed;
+        for (size_t q = 0; q < nq; ++q)
+            for (size_t k = 0; k < sub.entries.size(); ++k) {
+                const uint32_t e = sub.entries[k];
+                J.t16[at(q, e)] = decode(J.rq.type, kTier16, R2.out[sub.index(q, k)], db.entries[e].len, false);
+            }
+    }
+
+    // ---- 16-bit re-runs over the whole DB, grouped like the first tiers ----
+    {
+        std::vector<Group> rgroups;
+        for (size_t j = 0; j < jobs.size(); ++j) {
+            Job& J = jobs[j];
+            if (!J.requeue_full) continue;
+            size_t qmax = 0;
+            for (const QueryBuf& q : J.qs) qmax = std::max(qmax, q.len);
+            const Shape sh = choose_shape(qmax, total_blocks, max_blocks);
+            int gi = -1;
+            for (size_t g = 0; g < rgroups.size(); ++g)
+                if (rgroups[g].type == J.rq.type && rgroups[g].sh.lanes == sh.lanes && rgroups[g].sh.rows == sh.rows)
+                    gi = (int)g;
+            if (gi < 0) {
+                rgroups.push_back(Group{J.rq.type, 16, sh, {}});
+                gi = (int)rgroups.size() - 1;
+            }
+            J.rgroup = gi;
+            rgroups[gi].jobs.push_back(j);
+        }
+        for (size_t g = 0; g < rgroups.size(); ++g) {
+            TierRun R;
+            R.type = rgroups[g].type;
+            R.T = kTier16;
+            R.sh = rgroups[g].sh;
+            R.pfx = "rg_";
+            R.sorted_all = full_db ? &db.by_length : nullptr;
+            for (size_t j : rgroups[g].jobs) {
+                jobs[j].rq_off = R.qs.size();
+                R.qs.insert(R.qs.end(), jobs[j].qs.begin(), jobs[j].qs.end());
+            }
+            PairSet ps(all);
+            ps.nq = R.qs.size();
+            R.ps = &ps;
+            double ms = 0;
+            gpu_score_tier(ex, R, &ms);
+            for (size_t j : rgroups[g].jobs) {
+                Job& J = jobs[j];
+                const size_t nq = J.qs.size();
+                ms_re_job[j] += ms;
+                computed_job[j] += R.computed * nq / std::max<size_t>(R.qs.size(), 1);
+                for (size_t q = 0; q < nq; ++q)
+                    for (uint32_t e = 0; e < N; ++e)
+                        if (J.in16[e])
+                            J.t16[at(q, e)] =
+                                decode(J.rq.type, kTier16, R.out[ps.index(J.rq_off + q, e)], db.entries[e].len, false);
+            }
+        }
+    }
+
+    parallel_jobs([&](size_t j) {
+        Job& J = jobs[j];
+        const SearchType type = J.rq.type;
+        const size_t nq = J.qs.size();
+        const int bw = J.rq.bit_width;
+        auto raw = [&](size_t q, uint32_t e) -> const PairResult& {
+            const TierRun& R = *runs[J.group];
+            return R.out[R.ps->index(J.q_off + q, e)];
+        };
         // entries that reach the exact tier
         if (bw == BIT_WIDTH_64) {
             // scores only: the 16-bit pass where it is provably exact
@@ -1290,13 +1418,20 @@ void run_batch(Exec& ex, const std::vector<SearchRequest>& reqs, std::vector<Sea
                 for (uint32_t e = 0; e < N; ++e)
                     if (J.in16[e] && !J.t16[at(q, e)].accept) J.need64[e] = 1;
         }
+    });
+    for (size_t j = 0; j < jobs.size(); ++j) {
+        Job& J = jobs[j];
+        const size_t nq = J.qs.size();
+        const int bw = J.rq.bit_width;
+        double& ms_re = ms_re_job[j];
+        uint64_t& computed = computed_job[j];
         std::vector<std::pair<uint32_t, uint32_t>> p64;
         for (uint32_t e = 0; e < N; ++e)
             if (J.need64[e])
                 for (size_t q = 0; q < nq; ++q) p64.push_back({(uint32_t)q, e});
         if (!p64.empty()) {
             std::vector<int64_t> o64;
-            gpu_exact(ex, type, J.qs, p64, o64, &ms_re, &computed);
+            gpu_exact(ex, J.rq.type, J.qs, p64, o64, &ms_re, &computed);
             for (size_t k = 0; k < p64.size(); ++k) J.s64[at(p64[k].first, p64[k].second)] = o64[k];
         }
         if (bw == BIT_WIDTH_64)
@@ -1313,7 +1448,6 @@ void run_batch(Exec& ex, const std::vector<SearchRequest>& reqs, std::vector<Sea
     runs.clear();
 
     // ---- replay: searches in parallel, or chunks in parallel for few searches ----
-    const size_t nthreads = std::max<size_t>(1, std::min<size_t>(16, std::thread::hardware_concurrency()));
     if (jobs.size() >= 4 && nthreads > 1) {
         std::vector<std::thread> th;
         const size_t nt = std::min(nthreads, jobs.size());

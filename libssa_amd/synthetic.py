@@ -134,3 +134,37 @@ def config_c3(scale=1.0):
         if k % 2:
             db[i] = db[i].translate(comp)[::-1]
     return [q], db, planted
+
+
+def log_uniform_lengths(rng, n, lo, hi):
+    return np.rint(np.exp(rng.uniform(np.log(lo), np.log(hi), size=n))).astype(np.int64)
+
+
+def config_c4(scale=1.0, nq=256):
+    """Batched SW: 256 queries with lengths log-uniform in [30, 2000], 500,000
+    lognormal protein sequences (cap 35,000), 2 % planted homologs of the
+    queries (round robin)."""
+    rng = np.random.default_rng(0x5EED0004)
+    lens = log_uniform_lengths(rng, nq, 30, 2000)
+    queries = [make_query(0x5EED0400 + i, int(L)) for i, L in enumerate(lens)]
+    db, planted = make_db(0x5EED0004, int(500000 * scale), "aa", "lognormal", queries, 0.02)
+    return queries, db, planted
+
+
+def config_c5(scale=1.0):
+    """Served SW/NW mix: 1,000,000 lognormal protein sequences (cap 35,000)."""
+    db, _ = make_db(0x5EED0005, int(1000000 * scale), "aa", "lognormal")
+    return db
+
+
+def c5_requests(n, rate, seed=0x5EED0500):
+    """n requests of the served mix: arrival times of a Poisson stream of the
+    given rate (per second), 80 % SW / 20 % NW, query lengths from the mixture
+    70 % uniform [50, 400] and 30 % log-uniform [400, 5000]."""
+    rng = np.random.default_rng(seed)
+    arrivals = np.cumsum(rng.exponential(1.0 / rate, size=n))
+    kinds = np.where(rng.random(n) < 0.8, "sw", "nw")
+    short = rng.random(n) < 0.7
+    lens = np.where(short, rng.integers(50, 401, size=n), log_uniform_lengths(rng, n, 400, 5000))
+    queries = [make_query(seed + 1 + i, int(L)) for i, L in enumerate(lens)]
+    return [dict(t=float(arrivals[i]), kind=str(kinds[i]), query=queries[i]) for i in range(n)]
