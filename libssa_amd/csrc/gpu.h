@@ -87,6 +87,9 @@ hipError_t launch_init_results(hipStream_t st, PairResult* r, int64_t n, int32_t
 hipError_t launch_exact(int npairs, hipStream_t st, const ExactPair* pairs, const QueryDesc* queries,
                         const int32_t* qp32, const uint8_t* codes, int mode, int go, int ge, int32_t* scratch,
                         int64_t scratch_stride, int64_t* out);
+hipError_t launch_traceback_wave(int ntasks, hipStream_t st, const TraceTask* tasks, const uint8_t* codes,
+                                 const int64_t* mat, int go, int ge, uint8_t* dirs, int64_t* scratch, int32_t* ops,
+                                 TraceResult* out);
 hipError_t launch_traceback(int ntasks, hipStream_t st, const TraceTask* tasks, const uint8_t* codes,
                             const int64_t* mat, int go, int ge, uint8_t* dirs, int64_t* scratch, int32_t* ops,
                             TraceResult* out);

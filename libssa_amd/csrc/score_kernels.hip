@@ -76,6 +76,11 @@ __device__ __forceinline__ uint32_t shift_down(uint32_t top, uint32_t v, bool to
     }
 }
 
+template <bool B>
+struct Tag {
+    static constexpr bool value = B;
+};
+
 template <int N>
 struct Log2 {
     static constexpr int value = 1 + Log2<N / 2>::value;
@@ -289,73 +294,80 @@ __device__ __forceinline__ void run_pass(const ScoreParams& P, uint32_t gbase, c
         const bool ls_lo = (w.x & kResLast) != 0, ls_hi = (w.y & kResLast) != 0;
         const bool any_last = __any(ls_lo || ls_hi);
         const int z_lo = (int)((w.x >> kResZShift) & 3u), z_hi = (int)((w.y >> kResZShift) & 3u);
-#pragma unroll
-        for (int c = 0; c < 4; ++c) {
-            s16x2 f = as_v(tbf[c]);
-            s16x2 fR = adds(f, RR);
-            SSA_PIN();
-            s16x2 v = pair_row(pl, ph, 0);
-            SSA_PIN();
-            s16x2 h1 = adds(htop, v);  // row 0: diagonal H from the row above
-            SSA_PIN();
-            htop = as_v(tbh[c]);
-#pragma unroll
-            for (int r = 0; r < R; ++r) {
-                const bool more = r + 1 < R;
-                if (more) v = pair_row(pl, ph, r + 1);
-                SSA_PIN();
-                // the next column's profile rows, once this column's dwords are consumed
-                if (c < 3) {
-                    if ((R == 4 && r == 3) || (R >= 8 && r == 7)) {
-                        read_part<R, 0>(sp + colofs(w.x, c + 1), pl);
-                        read_part<R, 0>(sp + colofs(w.y, c + 1), ph);
-                    }
-                    if (R == 16 && r == 15) {
-                        read_part<R, 1>(sp + colofs(w.x, c + 1), pl);
-                        read_part<R, 1>(sp + colofs(w.y, c + 1), ph);
-                    }
-                }
-                s16x2 h = vmax(h1, f);
-                SSA_PIN();
-                const s16x2 eR = adds(El[r], RR);
-                SSA_PIN();
-                h = vmax(h, El[r]);
-                SSA_PIN();
-                if (more) h1 = adds(Hl[r], v);  // next row's diagonal term
-                SSA_PIN();
-                const s16x2 tt = adds(h, QR);
-                SSA_PIN();
-                if (MODE == 0) {
-                    S = vmax(S, h);
-                } else {
-                    hmn = vmin(hmn, h);
-                    SSA_PIN();
-                    hmx = vmax(hmx, h);
-                }
-                SSA_PIN();
-                f = vmax(fR, tt);
-                SSA_PIN();
-                El[r] = vmax(eR, tt);
-                SSA_PIN();
-                if (more) fR = adds(f, RR);
-                SSA_PIN();
-                Hl[r] = h;
+        // the block's four columns; CAP also records the NW score cell (the
+        // last query row at the pair's last column), a select per row on a
+        // wave-uniform condition, so the loop stays straight-line code
+        auto columns = [&](auto cap_tag) {
+            constexpr bool CAP = decltype(cap_tag)::value;
+            uint32_t cm_lo = 0u, cm_hi = 0u;
+            if (CAP && g == last_g) {
+                cm_lo = ls_lo ? 0x0000ffffu : 0u;
+                cm_hi = ls_hi ? 0xffff0000u : 0u;
             }
-            obh[c] = as_u(Hl[R - 1]);
-            obf[c] = as_u(f);
-            if (MODE == 1 && any_last) {
-                if (last_pass) {
-                    uint32_t cm =
-                        ((ls_lo && z_lo == c) ? 0x0000ffffu : 0u) | ((ls_hi && z_hi == c) ? 0xffff0000u : 0u);
-                    if (g != last_g) cm = 0u;
-                    s16x2 hq = Hl[0];
 #pragma unroll
-                    for (int r = 1; r < R; ++r)
-                        if (r == last_r) hq = Hl[r];
+            for (int c = 0; c < 4; ++c) {
+                s16x2 f = as_v(tbf[c]);
+                s16x2 fR = adds(f, RR);
+                SSA_PIN();
+                s16x2 v = pair_row(pl, ph, 0);
+                SSA_PIN();
+                s16x2 h1 = adds(htop, v);  // row 0: diagonal H from the row above
+                SSA_PIN();
+                htop = as_v(tbh[c]);
+                s16x2 hq = Z;
+#pragma unroll
+                for (int r = 0; r < R; ++r) {
+                    const bool more = r + 1 < R;
+                    if (more) v = pair_row(pl, ph, r + 1);
+                    SSA_PIN();
+                    // the next column's profile rows, once this column's dwords are consumed
+                    if (c < 3) {
+                        if ((R == 4 && r == 3) || (R >= 8 && r == 7)) {
+                            read_part<R, 0>(sp + colofs(w.x, c + 1), pl);
+                            read_part<R, 0>(sp + colofs(w.y, c + 1), ph);
+                        }
+                        if (R == 16 && r == 15) {
+                            read_part<R, 1>(sp + colofs(w.x, c + 1), pl);
+                            read_part<R, 1>(sp + colofs(w.y, c + 1), ph);
+                        }
+                    }
+                    s16x2 h = vmax(h1, f);
+                    SSA_PIN();
+                    const s16x2 eR = adds(El[r], RR);
+                    SSA_PIN();
+                    h = vmax(h, El[r]);
+                    SSA_PIN();
+                    if (more) h1 = adds(Hl[r], v);  // next row's diagonal term
+                    SSA_PIN();
+                    const s16x2 tt = adds(h, QR);
+                    SSA_PIN();
+                    if (MODE == 0) {
+                        S = vmax(S, h);
+                    } else {
+                        hmn = vmin(hmn, h);
+                        SSA_PIN();
+                        hmx = vmax(hmx, h);
+                    }
+                    SSA_PIN();
+                    f = vmax(fR, tt);
+                    SSA_PIN();
+                    El[r] = vmax(eR, tt);
+                    SSA_PIN();
+                    if (more) fR = adds(f, RR);
+                    SSA_PIN();
+                    Hl[r] = h;
+                    if (CAP && r == last_r) hq = h;
+                }
+                obh[c] = as_u(Hl[R - 1]);
+                obf[c] = as_u(f);
+                if (CAP) {
+                    const uint32_t cm = (z_lo == c ? cm_lo : 0u) | (z_hi == c ? cm_hi : 0u);
                     cap = sel(cm, hq, cap);
                 }
             }
-        }
+        };
+        if (MODE == 1 && last_pass && any_last) columns(Tag<true>{});
+        else columns(Tag<false>{});
         // overflow predicates at the end of the block: a packed test here (in
         // the cell loop's block, so the running reductions stay in the loop),
         // the exact per-half bookkeeping below only where a predicate fired

@@ -823,6 +823,9 @@ void gpu_traceback(Exec& gpu, const std::vector<SearchType>& types, std::vector<
     std::vector<uint8_t> codes;
     std::vector<TraceTask> tasks(al.size());
     int64_t dir_total = 0, hh_total = 0, ops_total = 0;
+    bool wave_ok = true;
+    int64_t smax = 0;
+    for (int i = 0; i < kDim * kDim; ++i) smax = std::max<int64_t>(smax, std::abs(matrix().s[i]));
     for (size_t k = 0; k < al.size(); ++k) {
         p_alignment a = al[k];
         TraceTask& t = tasks[k];
@@ -833,9 +836,17 @@ void gpu_traceback(Exec& gpu, const std::vector<SearchType>& types, std::vector<
         t.qlen = (int32_t)a->query.len;
         t.dlen = (int32_t)a->db_seq.len;
         t.dir_off = dir_total;
-        dir_total += (int64_t)t.qlen * t.dlen + 1;
+        dir_total += (((int64_t)t.qlen + 7) & ~(int64_t)7) * t.dlen + 8;
+        dir_total = (dir_total + 7) & ~(int64_t)7;  // 8-byte aligned per task (packed stores)
         t.hh_off = hh_total;
-        hh_total += 2 * (int64_t)std::max(t.qlen, t.dlen) + 2;
+        // int64 scratch of the one-thread kernel, or the int32 carries and
+        // last-row copies of the wave kernel, whichever is larger
+        hh_total += std::max<int64_t>(2 * (int64_t)std::max(t.qlen, t.dlen) + 2,
+                                      (3 * (int64_t)t.dlen + 4 * (int64_t)t.qlen) / 2 + 2);
+        // the wave kernel computes in int32: every value of the hit must fit
+        const int64_t bound = ((int64_t)t.qlen + t.dlen + 4) * (std::abs((int64_t)cf.gap_open) +
+                                                                 std::abs((int64_t)cf.gap_extend) + smax);
+        if (bound >= (int64_t)1 << 30) wave_ok = false;
         t.ops_off = ops_total;
         t.ops_cap = t.qlen + t.dlen + 1;
         ops_total += t.ops_cap;
@@ -852,8 +863,12 @@ void gpu_traceback(Exec& gpu, const std::vector<SearchType>& types, std::vector<
     gpu.upload(d_codes, codes.data(), codes.size());
     gpu.upload(d_tasks, tasks.data(), tasks.size() * sizeof(TraceTask));
     gpu.upload(d_mat, matrix().s, kDim * kDim * sizeof(int64_t));
-    SSA_HIP(launch_traceback((int)tasks.size(), gpu.stream(), d_tasks, d_codes, d_mat, cf.gap_open, cf.gap_extend,
-                             d_dirs, d_hh, d_ops, d_res));
+    if (wave_ok)
+        SSA_HIP(launch_traceback_wave((int)tasks.size(), gpu.stream(), d_tasks, d_codes, d_mat, cf.gap_open,
+                                      cf.gap_extend, d_dirs, d_hh, d_ops, d_res));
+    else
+        SSA_HIP(launch_traceback((int)tasks.size(), gpu.stream(), d_tasks, d_codes, d_mat, cf.gap_open,
+                                 cf.gap_extend, d_dirs, d_hh, d_ops, d_res));
     std::vector<TraceResult> res(tasks.size());
     std::vector<int32_t> ops((size_t)ops_total * 2);
     gpu.download(res.data(), d_res, res.size() * sizeof(TraceResult));
