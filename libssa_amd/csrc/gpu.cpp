@@ -24,6 +24,24 @@ void* DeviceArena::get(size_t need) {
     return ptr;
 }
 
+void* PinnedArena::get(size_t need) {
+    if (need == 0) need = 256;
+    if (need > bytes) {
+        if (ptr) SSA_HIP(hipHostFree(ptr));
+        size_t grow = bytes + bytes / 2;
+        bytes = need > grow ? need : grow;
+        bytes = (bytes + 4095) & ~(size_t)4095;
+        SSA_HIP(hipHostMalloc(&ptr, bytes, hipHostMallocDefault));
+    }
+    return ptr;
+}
+
+void PinnedArena::release() {
+    if (ptr) (void)hipHostFree(ptr);
+    ptr = nullptr;
+    bytes = 0;
+}
+
 void DeviceArena::release() {
     if (ptr) (void)hipFree(ptr);
     ptr = nullptr;
@@ -76,6 +94,8 @@ void Exec::release_all() {
     if (stream_) (void)hipStreamSynchronize(stream_);
     for (auto& kv : arenas_) kv.second.release();
     arenas_.clear();
+    for (auto& kv : pinned_) kv.second.release();
+    pinned_.clear();
 }
 
 GpuTimer::GpuTimer(hipStream_t st) : st_(st) {

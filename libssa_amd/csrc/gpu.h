@@ -17,6 +17,14 @@ namespace ssa {
 void hip_check(hipError_t e, const char* what);
 #define SSA_HIP(x) ::ssa::hip_check((x), #x)
 
+// A pinned host allocation that only grows; contents are not preserved on growth.
+struct PinnedArena {
+    void* ptr = nullptr;
+    size_t bytes = 0;
+    void* get(size_t need);
+    void release();
+};
+
 // A device allocation that only grows; contents are not preserved on growth.
 struct DeviceArena {
     void* ptr = nullptr;
@@ -37,6 +45,7 @@ public:
     hipStream_t stream() const { return stream_; }
     hipStream_t aux(size_t i);  // extra streams for graph branches (4, created on first use)
     void* arena(const std::string& name, size_t bytes) { return arenas_[name].get(bytes); }
+    void* pinned(const std::string& name, size_t bytes) { return pinned_[name].get(bytes); }
     void upload(void* dst, const void* src, size_t bytes);
     void download(void* dst, const void* src, size_t bytes);
     void sync();
@@ -46,6 +55,7 @@ private:
     hipStream_t stream_ = nullptr;
     hipStream_t aux_[4] = {nullptr, nullptr, nullptr, nullptr};
     std::unordered_map<std::string, DeviceArena> arenas_;
+    std::unordered_map<std::string, PinnedArena> pinned_;
 };
 
 class Gpu {

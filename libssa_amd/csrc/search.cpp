@@ -425,7 +425,8 @@ struct TierRun {
     const std::vector<uint32_t>* sorted_all = nullptr;  // ps is the whole DB in this length order
     Shape sh{0, 0};       // 0, 0 = chosen by the cost model
     std::string pfx;      // arena names of this run
-    std::vector<PairResult> out;
+    PairResult* out = nullptr;  // results in pinned host memory (PairSet::index order)
+    size_t nout = 0;
     uint64_t computed = 0;
 
     StreamLayout local;
@@ -452,8 +453,9 @@ void tier_prepare(Exec& ex, TierRun& R) {
     const int go = cf.gap_open, ge = cf.gap_extend;
     const PairSet& ps = *R.ps;
     const size_t n = ps.entries.size(), nq = R.qs.size();
-    R.out.assign(ps.size(), PairResult{-32768, kNoFlag, kNoFlag, 0});
+    R.nout = ps.size();
     if (R.empty()) return;
+    R.out = (PairResult*)ex.pinned(R.pfx + "results", R.nout * sizeof(PairResult));
 
     size_t qmax = 0;
     for (const QueryBuf& q : R.qs) qmax = std::max(qmax, q.len);
@@ -587,7 +589,7 @@ void tier_prepare(Exec& ex, TierRun& R) {
     auto* d_bnd = (uint4*)ex.arena(
         name("bnd"), (size_t)std::max<int64_t>((int64_t)R.grid * kWavesPerWG * bnd_per_wave, 1) * sizeof(uint4));
     auto* d_counter = (int32_t*)ex.arena(name("counter"), sizeof(int32_t));
-    R.d_out = (PairResult*)ex.arena(name("results"), R.out.size() * sizeof(PairResult));
+    R.d_out = (PairResult*)ex.arena(name("results"), R.nout * sizeof(PairResult));
     ex.upload(d_tasks, R.tasks.data(), R.tasks.size() * sizeof(ScoreTask));
     ex.upload(d_qd, R.qd.data(), R.qd.size() * sizeof(QueryDesc));
     ex.upload(d_prof, R.prof.data(), R.prof.size() * sizeof(int16_t));
@@ -611,13 +613,13 @@ void tier_prepare(Exec& ex, TierRun& R) {
 
 void tier_launch(hipStream_t st, TierRun& R) {
     if (R.empty()) return;
-    SSA_HIP(launch_init_results(st, R.d_out, (int64_t)R.out.size(), -32768));
+    SSA_HIP(launch_init_results(st, R.d_out, (int64_t)R.nout, -32768));
     SSA_HIP(launch_score(R.sh.lanes, R.sh.rows, R.type == kSW ? 0 : 1, st, R.K, R.P, R.grid));
 }
 
 void tier_finish(Exec& ex, TierRun& R) {
     if (R.empty()) return;
-    ex.download(R.out.data(), R.d_out, R.out.size() * sizeof(PairResult));
+    ex.download(R.out, R.d_out, R.nout * sizeof(PairResult));
 }
 
 // One tier run on its own, timed.
@@ -1251,9 +1253,21 @@ void run_batch(Exec& ex, const std::vector<SearchRequest>& reqs, std::vector<Sea
     std::vector<double> ms_re_job(jobs.size(), 0.0);
     std::vector<uint64_t> computed_job(jobs.size(), 0);
     const size_t nthreads = std::max<size_t>(1, std::min<size_t>(16, std::thread::hardware_concurrency()));
-    // host work per search, searches spread over threads
+    // host work per search: searches spread over threads, or, for fewer
+    // searches than threads, each search's entries spread over threads
+    const bool inner = jobs.size() < nthreads;
+    auto over_entries = [&](const std::function<void(uint32_t, uint32_t)>& fn) {
+        if (!inner || N < 65536) {
+            fn(0, N);
+            return;
+        }
+        std::vector<std::thread> th;
+        for (size_t t = 0; t < nthreads; ++t)
+            th.emplace_back(fn, (uint32_t)((uint64_t)N * t / nthreads), (uint32_t)((uint64_t)N * (t + 1) / nthreads));
+        for (auto& x : th) x.join();
+    };
     auto parallel_jobs = [&](const std::function<void(size_t)>& fn) {
-        const size_t nt = std::min(nthreads, jobs.size());
+        const size_t nt = inner ? 1 : std::min(nthreads, jobs.size());
         if (nt <= 1) {
             for (size_t j = 0; j < jobs.size(); ++j) fn(j);
             return;
@@ -1286,19 +1300,23 @@ void run_batch(Exec& ex, const std::vector<SearchRequest>& reqs, std::vector<Sea
             J.t8.resize(nq * N);
             if (J.sw_single) {
                 J.t16.resize(nq * N);
-                for (size_t q = 0; q < nq; ++q)
-                    for (uint32_t e = 0; e < N; ++e) {
-                        const PairResult& r = raw(q, e);
-                        J.t8[at(q, e)] = decode(type, kTier16, r, db.entries[e].len, true);
-                        J.t16[at(q, e)] = decode(type, kTier16, r, db.entries[e].len, false);
-                        if (!J.t8[at(q, e)].accept) J.in16[e] = 1;
-                    }
+                over_entries([&](uint32_t e0, uint32_t e1) {
+                    for (size_t q = 0; q < nq; ++q)
+                        for (uint32_t e = e0; e < e1; ++e) {
+                            const PairResult& r = raw(q, e);
+                            J.t8[at(q, e)] = decode(type, kTier16, r, db.entries[e].len, true);
+                            J.t16[at(q, e)] = decode(type, kTier16, r, db.entries[e].len, false);
+                            if (!J.t8[at(q, e)].accept) J.in16[e] = 1;
+                        }
+                });
             } else {
-                for (size_t q = 0; q < nq; ++q)
-                    for (uint32_t e = 0; e < N; ++e) {
-                        J.t8[at(q, e)] = decode(type, kTier8, raw(q, e), db.entries[e].len, false);
-                        if (!J.t8[at(q, e)].accept) J.in16[e] = 1;
-                    }
+                over_entries([&](uint32_t e0, uint32_t e1) {
+                    for (size_t q = 0; q < nq; ++q)
+                        for (uint32_t e = e0; e < e1; ++e) {
+                            J.t8[at(q, e)] = decode(type, kTier8, raw(q, e), db.entries[e].len, false);
+                            if (!J.t8[at(q, e)].accept) J.in16[e] = 1;
+                        }
+                });
                 PairSet sub;
                 sub.nq = nq;
                 for (uint32_t e = 0; e < N; ++e)
@@ -1314,11 +1332,13 @@ void run_batch(Exec& ex, const std::vector<SearchRequest>& reqs, std::vector<Sea
             }
         } else if (bw == BIT_WIDTH_16) {
             J.t16.resize(nq * N);
-            for (size_t q = 0; q < nq; ++q)
-                for (uint32_t e = 0; e < N; ++e) {
-                    J.t16[at(q, e)] = decode(type, kTier16, raw(q, e), db.entries[e].len, false);
-                    J.in16[e] = 1;
-                }
+            over_entries([&](uint32_t e0, uint32_t e1) {
+                for (size_t q = 0; q < nq; ++q)
+                    for (uint32_t e = e0; e < e1; ++e) {
+                        J.t16[at(q, e)] = decode(type, kTier16, raw(q, e), db.entries[e].len, false);
+                        J.in16[e] = 1;
+                    }
+            });
         }
     });
 
@@ -1413,8 +1433,9 @@ void run_batch(Exec& ex, const std::vector<SearchRequest>& reqs, std::vector<Sea
                 J.t16.resize(nq * N);
                 const int64_t gap = std::abs((int64_t)go) + std::abs((int64_t)ge);
                 std::fill(J.need64.begin(), J.need64.end(), 0);
+                over_entries([&](uint32_t e0, uint32_t e1) {
                 for (size_t q = 0; q < nq; ++q)
-                    for (uint32_t e = 0; e < N; ++e) {
+                    for (uint32_t e = e0; e < e1; ++e) {
                         TierOut o = decode(type, kTier16, raw(q, e), db.entries[e].len, false);
                         if (type == kNW) {
                             // exact when no value can reach either end of the int16 range
@@ -1427,11 +1448,14 @@ void run_batch(Exec& ex, const std::vector<SearchRequest>& reqs, std::vector<Sea
                         J.t16[at(q, e)] = o;
                         if (!o.accept) J.need64[e] = 1;
                     }
+                });
             }
         } else {
-            for (size_t q = 0; q < nq; ++q)
-                for (uint32_t e = 0; e < N; ++e)
-                    if (J.in16[e] && !J.t16[at(q, e)].accept) J.need64[e] = 1;
+            over_entries([&](uint32_t e0, uint32_t e1) {
+                for (size_t q = 0; q < nq; ++q)
+                    for (uint32_t e = e0; e < e1; ++e)
+                        if (J.in16[e] && !J.t16[at(q, e)].accept) J.need64[e] = 1;
+            });
         }
     });
     for (size_t j = 0; j < jobs.size(); ++j) {
