@@ -154,6 +154,11 @@ def bench_c4(args):
     path, tmp = setup_protein(db)
     hs = [ssa.init_sequence_fasta(ssa.READ_FROM_STRING, q.decode()) for q in qs]
     reqs = [(ssa.SEARCH_SW, h, 20, 8, ssa.COMPUTE_SCORE) for h in hs]
+    tuned, t_tune = [], 0.0
+    if not args.no_autotune:
+        ta = time.perf_counter()
+        tuned = ssa.autotune(ssa.SEARCH_SW, [30, 60, 120, 250, 500, 1000, 2000], copies=4)
+        t_tune = time.perf_counter() - ta
     ssa.batch_align(reqs)  # warm-up: builds the DB's streams for every lane width used
     t0 = time.perf_counter()
     plain = [ssa.sw_align(h, 20, 8, ssa.COMPUTE_SCORE) for h in hs]
@@ -175,6 +180,9 @@ def bench_c4(args):
                     "gpu_first_tier_ms": round(stats[0].ms_gpu_first, 3)},
         "one_query_per_launch": {"seconds": round(t1 - t0, 3), "gcups": round(cells / (t1 - t0) / 1e9, 2)},
         "speedup": round((t1 - t0) / (t2 - t1), 3), "results_equal": equal, "cells": cells,
+        "autotune": {"seconds": round(t_tune, 2),
+                     "shapes": [{"qlen": L, "lanes_per_stream": g, "rows_per_lane": r, "ms": round(ms, 3)}
+                                for L, g, r, ms in tuned]},
     }
 
 
@@ -263,7 +271,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-sample", type=float, default=1.0)
     ap.add_argument("--scale", type=float, default=1.0, help="c4/c5: DB size factor")
-    ap.add_argument("--rates", default="4,8,12,16", help="c5: arrival rates per second")
+    ap.add_argument("--rates", default="5,10,20,30", help="c5: arrival rates per second")
+    ap.add_argument("--no-autotune", action="store_true", help="c4: keep the cost model's shapes")
     ap.add_argument("--requests", type=int, default=60, help="c5: requests per rate")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="also run one device-to-device copy of MIB MiB (calibrates HBM byte counters)")

@@ -45,6 +45,24 @@ void ssa_amd_set_kernel_shape(int lanes_per_stream, int rows_per_lane);
  * Streams never get shorter than 16 blocks per lane of a group. */
 void ssa_amd_set_stream_waves(int waves);
 
+/* ---- autotuning ---- */
+
+/* Measures every score-kernel shape (lanes per stream pair x query rows per
+ * lane) on the loaded DB, for each of the n lengths with `copies` random
+ * queries of that length in one launch (1 = as single searches run; more =
+ * as batches run), with the current scoring and search type (0 = SW, 1 = NW).
+ * From then on a query's shape is the one with the least measured time at the
+ * tuned length nearest to its own (on a log scale), scaled by the number of
+ * passes the shape needs for the query's length.  lanes_out, rows_out, ms_out (NULL
+ * allowed) receive the chosen shape and its time per query for each length.
+ * Returns the number of lengths tuned, -1 for invalid arguments, -2 when
+ * scoring is not initialised.  Results do not depend on the shape. */
+int ssa_amd_autotune(int search_type, const size_t* lengths, size_t n, int copies, int* lanes_out, int* rows_out,
+                     double* ms_out);
+
+/* Forgets the measured shapes (the cost model chooses again). */
+void ssa_amd_autotune_reset(void);
+
 /* ---- batched searches ---- */
 
 /* One search of a batch: the arguments of sw_align (search_type 0) or

@@ -111,6 +111,9 @@ def _load():
         "ssa_amd_get_matrix": (None, [P]), "ssa_amd_device_arch": (cp, []),
         "ssa_amd_batch_align": (I, [_C.POINTER(_Request), S, _C.POINTER(_C.POINTER(_AlignmentList)),
                                     _C.POINTER(Stats)]),
+        "ssa_amd_autotune": (I, [I, _C.POINTER(_C.c_size_t), S, I, _C.POINTER(I), _C.POINTER(I),
+                                 _C.POINTER(_C.c_double)]),
+        "ssa_amd_autotune_reset": (None, []),
         "ssa_amd_server_start": (P, [I, S, S]),
         "ssa_amd_server_submit": (_C.c_int64, [P, _C.POINTER(_Request)]),
         "ssa_amd_server_wait": (_C.POINTER(_AlignmentList), [P, _C.c_int64, _C.POINTER(Stats),
@@ -273,3 +276,18 @@ class Server:
         if self.h:
             lib.ssa_amd_server_stop(self.h)
             self.h = None
+
+
+def autotune(search_type, lengths, copies=1):
+    """Measures the kernel shapes on the loaded DB (ssa_amd_autotune); returns
+    [(length, lanes_per_stream, rows_per_lane, ms per query)]."""
+    n = len(lengths)
+    L = (_C.c_size_t * n)(*lengths)
+    lanes, rows, ms = (_C.c_int * n)(), (_C.c_int * n)(), (_C.c_double * n)()
+    k = lib.ssa_amd_autotune(search_type, L, n, copies, lanes, rows, ms)
+    if k < 0:
+        raise ValueError("ssa_amd_autotune failed (code %d)" % k)
+    return [(lengths[i], lanes[i], rows[i], ms[i]) for i in range(k)]
+
+
+def autotune_reset(): lib.ssa_amd_autotune_reset()

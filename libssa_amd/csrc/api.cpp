@@ -180,6 +180,26 @@ int ssa_amd_batch_align(const ssa_amd_request_t* reqs, size_t n, p_alignment_lis
     return 0;
 }
 
+int ssa_amd_autotune(int search_type, const size_t* lengths, size_t n, int copies, int* lanes_out, int* rows_out,
+                     double* ms_out) {
+    if (search_type != 0 && search_type != 1) return -1;
+    if (n && !lengths) return -1;
+    if (!matrix().valid) return -2;
+    if (!matrix().constant && config().symtype == NUCLEOTIDE) return -2;
+    std::vector<size_t> L(lengths, lengths + n);
+    std::vector<AutotuneResult> res;
+    if (copies < 1) return -1;
+    const int k = autotune(search_type, L, copies, res);
+    for (size_t i = 0; i < res.size(); ++i) {
+        if (lanes_out) lanes_out[i] = res[i].lanes;
+        if (rows_out) rows_out[i] = res[i].rows;
+        if (ms_out) ms_out[i] = res[i].ms;
+    }
+    return k;
+}
+
+void ssa_amd_autotune_reset(void) { autotune_reset(); }
+
 void ssa_amd_get_matrix(int64_t* out) { memcpy(out, matrix().s, sizeof(matrix().s)); }
 
 const char* ssa_amd_device_arch(void) {

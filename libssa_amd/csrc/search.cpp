@@ -14,6 +14,7 @@
 #include <algorithm>
 #include <chrono>
 #include <climits>
+#include <cmath>
 #include <cstring>
 #include <numeric>
 #include <functional>
@@ -226,9 +227,59 @@ int& target_waves() {
 // on its own and shares its SIMD for part of that time (factor 2).  A
 // wave-step costs R rows times a per-row factor that charges the fixed
 // per-step work (DPP moves, addressing, pair bookkeeping) to short lanes.
-Shape choose_shape(size_t qlen_max, int64_t total_blocks, int64_t max_blocks) {
+// shape measurements (ssa_amd_autotune): per search type and tuned query
+// length, the time per query of every shape
+struct TunedShape {
+    int type;
+    size_t qlen;
+    Shape sh;
+    double ms;
+};
+
+std::vector<TunedShape>& tuned_shapes() {
+    static std::vector<TunedShape> t;
+    return t;
+}
+
+std::mutex& tuned_mutex() {
+    static std::mutex m;
+    return m;
+}
+
+Shape choose_shape(size_t qlen_max, int64_t total_blocks, int64_t max_blocks, int type = -1) {
     const Shape ov = shape_override();
     if (ov.lanes && ov.rows) return ov;
+    if (type >= 0) {
+        // measured: each shape's time at the nearest tuned length (log scale),
+        // scaled by the passes the shape needs for this length
+        std::lock_guard<std::mutex> lk(tuned_mutex());
+        size_t near = 0;
+        double bd = 1e300;
+        for (const TunedShape& t : tuned_shapes()) {
+            if (t.type != type) continue;
+            const double d = std::fabs(std::log((double)std::max<size_t>(qlen_max, 1)) - std::log((double)t.qlen));
+            if (d < bd) {
+                bd = d;
+                near = t.qlen;
+            }
+        }
+        if (near) {
+            Shape best{0, 0};
+            double bt = 1e300;
+            for (const TunedShape& t : tuned_shapes()) {
+                if (t.type != type || t.qlen != near) continue;
+                const size_t pr = (size_t)t.sh.lanes * t.sh.rows;
+                const double p_q = (double)std::max<size_t>(1, (qlen_max + pr - 1) / pr);
+                const double p_t = (double)std::max<size_t>(1, (t.qlen + pr - 1) / pr);
+                const double est = t.ms * p_q / p_t;
+                if (est < bt) {
+                    bt = est;
+                    best = t.sh;
+                }
+            }
+            if (best.lanes) return best;
+        }
+    }
     static const Shape kShapes[] = {{64, 16}, {32, 16}, {16, 16}, {8, 16}, {64, 8}, {32, 8},
                                     {16, 8},  {8, 8},   {64, 4},  {32, 4}, {16, 4}, {8, 4}};
     Shape best = kShapes[0];
@@ -466,7 +517,7 @@ void tier_prepare(Exec& ex, TierRun& R) {
             total_blocks += nb;
             max_blocks = std::max(max_blocks, nb);
         }
-        R.sh = choose_shape(qmax, total_blocks, max_blocks);
+        R.sh = choose_shape(qmax, total_blocks, max_blocks, R.type == kSW ? 0 : 1);
     }
     const Shape sh = R.sh;
     if (!score_shape_supported(sh.lanes, sh.rows)) fatal("libssa_amd: unsupported kernel shape");
@@ -1174,7 +1225,7 @@ void run_batch(Exec& ex, const std::vector<SearchRequest>& reqs, std::vector<Sea
         if (J.first_bits == 0 || N == 0 || nq == 0) continue;
         size_t qmax = 0;
         for (const QueryBuf& q : J.qs) qmax = std::max(qmax, q.len);
-        J.shape = choose_shape(qmax, total_blocks, max_blocks);
+        J.shape = choose_shape(qmax, total_blocks, max_blocks, J.rq.type == kSW ? 0 : 1);
         int gi = -1;
         for (size_t g = 0; g < groups.size(); ++g)
             if (groups[g].type == J.rq.type && groups[g].bits == J.first_bits &&
@@ -1374,7 +1425,7 @@ void run_batch(Exec& ex, const std::vector<SearchRequest>& reqs, std::vector<Sea
             if (!J.requeue_full) continue;
             size_t qmax = 0;
             for (const QueryBuf& q : J.qs) qmax = std::max(qmax, q.len);
-            const Shape sh = choose_shape(qmax, total_blocks, max_blocks);
+            const Shape sh = choose_shape(qmax, total_blocks, max_blocks, J.rq.type == kSW ? 0 : 1);
             int gi = -1;
             for (size_t g = 0; g < rgroups.size(); ++g)
                 if (rgroups[g].type == J.rq.type && rgroups[g].sh.lanes == sh.lanes && rgroups[g].sh.rows == sh.rows)
@@ -1521,6 +1572,86 @@ void run_batch(Exec& ex, const std::vector<SearchRequest>& reqs, std::vector<Sea
         results[j].list = J.list;
         results[j].stats = J.st;
     }
+}
+
+// Times every kernel shape on the whole loaded DB for `copies` random queries
+// of each length in one launch (the first 16-bit tier, a search's main
+// launch; copies > 1 measures the shape as a batch runs it) and keeps the
+// fastest per (type, length).
+int autotune(int type, const std::vector<size_t>& lengths, int copies, std::vector<AutotuneResult>& out) {
+    static const Shape kShapes[] = {{64, 16}, {32, 16}, {16, 16}, {8, 16}, {64, 8}, {32, 8},
+                                    {16, 8},  {8, 8},   {64, 4},  {32, 4}, {16, 4}, {8, 4}};
+    Exec& ex = Gpu::get().exec();
+    DeviceDB* dbp;
+    {
+        std::lock_guard<std::mutex> lk(db_mutex());
+        dbp = &device_db();
+    }
+    DeviceDB& db = *dbp;
+    const Config& cf = config();
+    const uint32_t N = (uint32_t)db.entries.size();
+    out.clear();
+    if (N == 0) return 0;
+    copies = std::max(1, copies);
+    PairSet all;
+    all.entries.resize(N);
+    std::iota(all.entries.begin(), all.entries.end(), 0u);
+    // query symbols of the current alphabet (codes of the standard residues)
+    std::vector<uint8_t> alphabet;
+    const char* syms = cf.symtype == NUCLEOTIDE ? "ACGT" : "ARNDCQEGHILKMFPSTWYV";
+    for (const char* c = syms; *c; ++c)
+        alphabet.push_back((uint8_t)(cf.symtype == NUCLEOTIDE ? map_nt(*c) : map_aa(*c)));
+    uint64_t x = 0x9E3779B97F4A7C15ull;
+    auto run = [&](const Shape& sh, const std::vector<std::vector<uint8_t>>& qs) {
+        TierRun R;
+        R.type = type == 0 ? kSW : kNW;
+        R.T = kTier16;
+        for (const auto& q : qs) R.qs.push_back(QueryBuf{q.data(), q.size(), 0, 0});
+        PairSet ps(all);
+        ps.nq = R.qs.size();
+        R.ps = &ps;
+        R.sorted_all = &db.by_length;
+        R.sh = sh;
+        R.pfx = "tune_";
+        double ms = 0;
+        gpu_score_tier(ex, R, &ms);
+        return ms;
+    };
+    // build every lane width's streams first, outside the timed runs
+    {
+        std::vector<std::vector<uint8_t>> tiny(1, std::vector<uint8_t>(8, alphabet[0]));
+        for (int lanes : {8, 16, 32, 64}) run(Shape{lanes, 4}, tiny);
+    }
+    for (size_t L : lengths) {
+        if (L == 0) continue;
+        std::vector<std::vector<uint8_t>> qs(copies, std::vector<uint8_t>(L));
+        for (auto& q : qs)
+            for (size_t i = 0; i < L; ++i) {
+                x ^= x << 13;
+                x ^= x >> 7;
+                x ^= x << 17;
+                q[i] = alphabet[x % alphabet.size()];
+            }
+        AutotuneResult best{L, 0, 0, 1e300};
+        std::vector<TunedShape> meas;
+        for (const Shape& sh : kShapes) {
+            const double ms = run(sh, qs) / copies;
+            meas.push_back(TunedShape{type, L, sh, ms});
+            if (ms < best.ms) best = AutotuneResult{L, sh.lanes, sh.rows, ms};
+        }
+        out.push_back(best);
+        std::lock_guard<std::mutex> lk(tuned_mutex());
+        auto& t = tuned_shapes();
+        t.erase(std::remove_if(t.begin(), t.end(), [&](const TunedShape& e) { return e.type == type && e.qlen == L; }),
+                t.end());
+        t.insert(t.end(), meas.begin(), meas.end());
+    }
+    return (int)out.size();
+}
+
+void autotune_reset() {
+    std::lock_guard<std::mutex> lk(tuned_mutex());
+    tuned_shapes().clear();
 }
 
 p_alignment_list run_search(SearchType type, p_query query, size_t hitcount, int bit_width, int align_type) {

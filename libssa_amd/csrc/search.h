@@ -49,6 +49,16 @@ struct SearchResult {
     ssa_amd_stats_t stats{};
 };
 
+struct AutotuneResult {
+    size_t qlen;
+    int lanes, rows;
+    double ms;
+};
+// shape measurements on the loaded DB; the fastest shape per (type, length)
+// becomes the choice for queries of the nearest tuned length
+int autotune(int type, const std::vector<size_t>& lengths, int copies, std::vector<AutotuneResult>& out);
+void autotune_reset();
+
 class Exec;
 // Runs the searches as one batch on the context's stream: every result is
 // what the search returns alone.  one_graph: the first-tier launches of all

@@ -151,3 +151,20 @@ def test_server_rejects_invalid_requests(protein_db):
             srv.wait(12345)
     finally:
         srv.stop()
+
+
+def test_autotune_keeps_results(protein_db):
+    rnd, db, qraws, qs = protein_db
+    reqs = make_requests(rnd, qs)
+    want = [key(alone(r)) for r in reqs]
+    try:
+        tuned = ssa.autotune(ssa.SEARCH_SW, [40, 300, 700], copies=2)
+        assert [t[0] for t in tuned] == [40, 300, 700]
+        for _, lanes, rows, ms in tuned:
+            assert lanes in (8, 16, 32, 64) and rows in (4, 8, 16) and ms > 0
+        ssa.autotune(ssa.SEARCH_NW, [100])
+        assert [key(alone(r)) for r in reqs] == want
+        lists, _ = ssa.batch_align(reqs)
+        assert [key(x) for x in lists] == want
+    finally:
+        ssa.autotune_reset()
