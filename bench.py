@@ -186,11 +186,13 @@ def bench_c4(args):
     }
 
 
-def serve_trace(ssa, reqs, handles, slots, max_batch):
+def serve_trace(ssa, reqs, handles, slots, max_batch, batch_cells=0.0):
     """Replays an arrival trace against a server; returns per-request
     (latency_ms, queue_ms, cells) and the wall time from first submit to last result."""
     import threading
     srv = ssa.Server(slots=slots, max_batch=max_batch)
+    if batch_cells:
+        srv.tune(batch_cells=batch_cells)
     tickets = [None] * len(reqs)
     t0 = time.perf_counter()
 
@@ -235,8 +237,8 @@ def bench_c5(args):
         handles = [ssa.init_sequence_fasta(ssa.READ_FROM_STRING, r["query"].decode()) for r in reqs]
         row = {"rate_per_s": rate, "requests": len(reqs)}
         results = {}
-        for name, slots, mb in (("one_request_per_launch", 1, 1), ("continuous_batching", 2, 16)):
-            out, wall, c = serve_trace(ssa, reqs, handles, slots, mb)
+        for name, slots, mb in (("one_request_per_launch", 1, 1), ("continuous_batching", args.slots, 16)):
+            out, wall, c = serve_trace(ssa, reqs, handles, slots, mb, args.batch_cells)
             lat = np.array([o[0] for o in out])
             cells = sum(o[2] for o in out)
             results[name] = [o[3] for o in out]
@@ -257,7 +259,7 @@ def bench_c5(args):
         "unit": "ms", "n_gpus": 1, "higher_is_better": False, "dtype": "int16", "data": "synthetic",
         "config": {"workload": "1M lognormal protein DB resident; Poisson arrivals; 80% SW / 20% NW, top 32 "
                                "with CIGARs, 8-bit start; |q| 70% U(50,400) + 30% log-U(400,5000)",
-                   "db_sequences": len(db)},
+                   "db_sequences": len(db), "slots": args.slots, "batch_cells": args.batch_cells or 2.5e11},
         "rates": rows,
     }
 
@@ -273,6 +275,8 @@ def main():
     ap.add_argument("--scale", type=float, default=1.0, help="c4/c5: DB size factor")
     ap.add_argument("--rates", default="5,10,20,30", help="c5: arrival rates per second")
     ap.add_argument("--no-autotune", action="store_true", help="c4: keep the cost model's shapes")
+    ap.add_argument("--slots", type=int, default=2, help="c5: server workers for continuous batching")
+    ap.add_argument("--batch-cells", type=float, default=0.0, help="c5: cell budget per batch (0 = default)")
     ap.add_argument("--requests", type=int, default=60, help="c5: requests per rate")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="also run one device-to-device copy of MIB MiB (calibrates HBM byte counters)")

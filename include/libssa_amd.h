@@ -107,6 +107,13 @@ typedef struct {
  * server runs.  Returns NULL if the arguments or the settings are invalid. */
 ssa_amd_server* ssa_amd_server_start(int slots, size_t max_batch, size_t device_bytes);
 
+/* Batch formation: a free worker takes the oldest waiting request first once
+ * it has waited more than aging_ms (default 750), then the cheapest waiting
+ * requests (estimated cells, shortest first) while the batch stays within
+ * batch_cells (default 2.5e11) and the other limits.  Values <= 0 keep the
+ * current setting. */
+void ssa_amd_server_tune(ssa_amd_server* server, double batch_cells, double aging_ms);
+
 /* Queues one search; returns its ticket (>= 0) or -1 for an invalid request.
  * The query must stay alive until the result has been taken and freed. */
 int64_t ssa_amd_server_submit(ssa_amd_server* server, const ssa_amd_request_t* request);

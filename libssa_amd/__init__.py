@@ -119,6 +119,7 @@ def _load():
         "ssa_amd_server_wait": (_C.POINTER(_AlignmentList), [P, _C.c_int64, _C.POINTER(Stats),
                                                              _C.POINTER(_C.c_double), _C.POINTER(_C.c_double)]),
         "ssa_amd_server_counters": (None, [P, _C.POINTER(ServerCounters)]),
+        "ssa_amd_server_tune": (None, [P, _C.c_double, _C.c_double]),
         "ssa_amd_server_stop": (None, [P]),
     }
     for name, (res, args) in sig.items():
@@ -266,6 +267,9 @@ class Server:
             return _convert(p), st, lat.value, q.value
         finally:
             lib.free_alignment(p)
+
+    def tune(self, batch_cells=0.0, aging_ms=0.0):
+        lib.ssa_amd_server_tune(self.h, batch_cells, aging_ms)
 
     def counters(self):
         c = ServerCounters()

@@ -2,12 +2,13 @@
 //
 // Requests (sw_align / nw_align arguments) are queued as they arrive.  Each of
 // `slots` workers owns a device context (stream and buffers); a free worker
-// takes the queued requests as one batch (run_batch: one captured graph for
-// the batch's first tiers), admitting requests in arrival order while the
-// batch's estimated device footprint stays within the worker's share of the
-// memory budget.  Batches of different workers run on the GPU at the same
-// time, so requests that arrive while others are on the GPU start as soon as
-// a worker and memory free up.  Every result is what the search returns alone
+// takes waiting requests as one batch (run_batch: one captured graph for the
+// batch's first tiers): first the oldest request if it has waited longer than
+// the aging limit, then the cheapest ones (estimated cells, shortest first)
+// while the batch stays within the cell budget, the batch size and the
+// worker's share of the device memory budget.  Batches of different workers
+// run on the GPU at the same time, so requests that arrive while others are
+// on the GPU start as soon as a worker and memory free up.  Every result is what the search returns alone
 // (run_batch's contract).
 #include <algorithm>
 #include <chrono>
@@ -38,6 +39,7 @@ struct Ticket {
     SearchRequest rq;
     SearchResult res;
     size_t footprint = 0;
+    double cells = 0;  // estimated GPU work
     double t_submit = 0, t_start = 0, t_done = 0;
     bool done = false;
 };
@@ -65,12 +67,19 @@ public:
             if (kv.second.res.list) free_alignment(kv.second.res.list);
     }
 
-    int64_t submit(const SearchRequest& rq, size_t footprint) {
+    void tune(double batch_cells, double aging_ms) {
+        std::lock_guard<std::mutex> lk(mu_);
+        if (batch_cells > 0) batch_cells_ = batch_cells;
+        if (aging_ms > 0) aging_ms_ = aging_ms;
+    }
+
+    int64_t submit(const SearchRequest& rq, size_t footprint, double cells) {
         std::lock_guard<std::mutex> lk(mu_);
         const int64_t id = next_id_++;
         Ticket& t = tickets_[id];
         t.rq = rq;
         t.footprint = footprint;
+        t.cells = cells;
         t.t_submit = now_ms();
         queue_.push_back(id);
         cv_work_.notify_one();
@@ -107,16 +116,26 @@ private:
                 std::unique_lock<std::mutex> lk(mu_);
                 cv_work_.wait(lk, [&] { return stopping_ || !queue_.empty(); });
                 if (queue_.empty()) return;  // stopping with nothing left
+                // the oldest request first once it has waited past the aging
+                // limit, then the cheapest ones
                 size_t bytes = 0;
-                while (!queue_.empty() && batch.size() < max_batch_) {
-                    const Ticket& t = tickets_[queue_.front()];
-                    // admit in arrival order while the footprint fits; a request
-                    // larger than the share runs alone
-                    if (!batch.empty() && bytes + t.footprint > slot_budget_) break;
+                double cells = 0;
+                const double now = now_ms();
+                std::vector<int64_t> order(queue_.begin(), queue_.end());
+                const Ticket& oldest = tickets_[order.front()];
+                const bool aged = now - oldest.t_submit > aging_ms_;
+                std::stable_sort(order.begin() + (aged ? 1 : 0), order.end(),
+                                 [&](int64_t a, int64_t b) { return tickets_[a].cells < tickets_[b].cells; });
+                for (int64_t id : order) {
+                    if (batch.size() >= max_batch_) break;
+                    const Ticket& t = tickets_[id];
+                    // a request larger than the budgets runs alone
+                    if (!batch.empty() && (bytes + t.footprint > slot_budget_ || cells + t.cells > batch_cells_)) break;
                     bytes += t.footprint;
-                    batch.push_back(queue_.front());
-                    queue_.pop_front();
+                    cells += t.cells;
+                    batch.push_back(id);
                 }
+                for (int64_t id : batch) queue_.erase(std::find(queue_.begin(), queue_.end(), id));
                 const double t0 = now_ms();
                 for (int64_t id : batch) tickets_[id].t_start = t0;
                 ++batches_;
@@ -152,10 +171,39 @@ private:
     std::vector<std::thread> workers_;
     size_t max_batch_;
     size_t budget_ = 0, slot_budget_ = 0;
+    double batch_cells_ = 2.5e11;  // GPU work per batch (about one average request, ~45 ms)
+    double aging_ms_ = 750.0;    // a request waiting longer goes first
     int64_t next_id_ = 0;
     bool stopping_ = false;
     size_t batches_ = 0, requests_ = 0, max_batch_seen_ = 0;
 };
+
+// GPU work of one request: cells of its first tier, twice for NW starting at
+// 8 bit (its 16-bit pass re-runs nearly every entry); residues = DB residues
+// as entries (both strands counted for nucleotide searches of both strands)
+double request_cells(const SearchRequest& rq, double residues) {
+    const Config& cf = config();
+    double qlen = 0;
+    if (cf.symtype == NUCLEOTIDE) {
+        for (int s = 0; s < 2; ++s)
+            if (((s + 1) & cf.strands) && rq.query->has_nt[s]) qlen += (double)rq.query->nt[s].size();
+    } else {
+        qlen = (double)rq.query->aa[0].size();
+    }
+    return qlen * residues * ((rq.type == kNW && rq.bit_width == BIT_WIDTH_8) ? 2.0 : 1.0);
+}
+
+double db_entry_residues() {
+    const Config& cf = config();
+    double res = 0;
+    const size_t n = ssa_db_get_sequence_count();
+    for (size_t i = 0; i < n; ++i) {
+        p_seqinfo s = ssa_db_get_sequence(i);
+        if (!s) break;
+        res += (double)s->seqlen;
+    }
+    return res * ((cf.symtype == NUCLEOTIDE && (cf.strands & 2)) ? 2 : 1);
+}
 
 // device bytes one request needs in a batch: pair results and pass boundaries
 // of its first tier, and the traceback matrices of its hits
@@ -188,6 +236,7 @@ extern "C" {
 
 struct ssa_amd_server {
     Server* s;
+    double residues;  // of the DB at start
 };
 
 ssa_amd_server* ssa_amd_server_start(int slots, size_t max_batch, size_t device_bytes) {
@@ -196,6 +245,7 @@ ssa_amd_server* ssa_amd_server_start(int slots, size_t max_batch, size_t device_
     const Config& c = config();
     if (!matrix().constant && c.symtype == NUCLEOTIDE) return nullptr;
     ssa_amd_server* h = new ssa_amd_server;
+    h->residues = db_entry_residues();
     h->s = new Server(slots, max_batch, device_bytes);
     return h;
 }
@@ -208,7 +258,7 @@ int64_t ssa_amd_server_submit(ssa_amd_server* h, const ssa_amd_request_t* r) {
     if (r->align_type != COMPUTE_SCORE && r->align_type != COMPUTE_ALIGNMENT) return -1;
     if (r->bit_width == BIT_WIDTH_8 && config().simd_mode < COMPUTE_ON_SSE41) return -1;
     const SearchRequest rq{r->search_type == 0 ? kSW : kNW, r->query, r->hitcount, r->bit_width, r->align_type};
-    return h->s->submit(rq, request_footprint(rq));
+    return h->s->submit(rq, request_footprint(rq), request_cells(rq, h->residues));
 }
 
 p_alignment_list ssa_amd_server_wait(ssa_amd_server* h, int64_t ticket, ssa_amd_stats_t* stats, double* latency_ms,
@@ -218,6 +268,10 @@ p_alignment_list ssa_amd_server_wait(ssa_amd_server* h, int64_t ticket, ssa_amd_
     if (!h->s->wait(ticket, &res, latency_ms, queue_ms)) return nullptr;
     if (stats) *stats = res.stats;
     return res.list;
+}
+
+void ssa_amd_server_tune(ssa_amd_server* h, double batch_cells, double aging_ms) {
+    if (h) h->s->tune(batch_cells, aging_ms);
 }
 
 void ssa_amd_server_counters(ssa_amd_server* h, ssa_amd_server_counters_t* out) {
