@@ -130,6 +130,16 @@ void ssa_amd_server_counters(ssa_amd_server* server, ssa_amd_server_counters_t* 
 /* Finishes every queued request, then stops the workers; results not taken are freed. */
 void ssa_amd_server_stop(ssa_amd_server* server);
 
+/* Writes the DB the plugin currently serves to a pre-packed file: the
+ * sequences, their codes mapped for the symbol type in force (nucleotide DBs
+ * with both strands' entries), the entry table and the entries in length
+ * order.  init_db on that file (built-in provider) then serves the same
+ * sequences and IDs and uploads the codes without mapping them again.  Packing
+ * a translated symbol type is refused.  Returns 0, or -1 after an error
+ * message.  No counterpart in the reference: it re-maps the DB on every search
+ * (src/db_adapter.c:47-110). */
+int ssa_amd_db_pack(const char* path);
+
 /* The current 32x32 score matrix as the search uses it ([db code * 32 + query code]). */
 void ssa_amd_get_matrix(int64_t* out);
 

@@ -51,6 +51,10 @@ class _AlignmentList(_C.Structure):
     _fields_ = [("alignments", _C.POINTER(_C.POINTER(_Alignment))), ("len", _C.c_size_t)]
 
 
+class _SeqInfo(_C.Structure):
+    _fields_ = [("ID", _C.c_size_t), ("seqlen", _C.c_size_t), ("seq", _C.POINTER(_C.c_char))]
+
+
 class _Request(_C.Structure):
     """ssa_amd_request_t (include/libssa_amd.h)."""
     _fields_ = [("search_type", _C.c_int), ("query", _C.c_void_p), ("hitcount", _C.c_size_t),
@@ -107,6 +111,7 @@ def _load():
         "sw_align": (_C.POINTER(_AlignmentList), [P, S, I, I]), "nw_align": (_C.POINTER(_AlignmentList), [P, S, I, I]),
         "free_alignment": (None, [_C.POINTER(_AlignmentList)]), "ssa_exit": (None, []),
         "ssa_db_init": (I, [cp]), "ssa_db_get_sequence_count": (S, []), "ssa_db_close": (None, []),
+        "ssa_db_get_sequence": (_C.POINTER(_SeqInfo), [S]), "ssa_amd_db_pack": (I, [cp]),
         "ssa_amd_last_stats": (None, [_C.POINTER(Stats)]), "ssa_amd_set_kernel_shape": (None, [I, I]), "ssa_amd_set_stream_waves": (None, [I]),
         "ssa_amd_get_matrix": (None, [P]), "ssa_amd_device_arch": (cp, []),
         "ssa_amd_batch_align": (I, [_C.POINTER(_Request), S, _C.POINTER(_C.POINTER(_AlignmentList)),
@@ -216,6 +221,22 @@ def get_matrix():
 
 def device_arch():
     return lib.ssa_amd_device_arch().decode()
+
+
+def db_sequence(index):
+    """(ID, residues as bytes) of the plugin DB's sequence, or None past the end."""
+    p = lib.ssa_db_get_sequence(index)
+    if not p:
+        return None
+    s = p.contents
+    return s.ID, _C.string_at(s.seq, s.seqlen) if s.seqlen else b""
+
+
+def pack_db(path):
+    """Writes the DB currently open to a pre-packed file (ssa_amd_db_pack),
+    mapped for the symbol type in force; init_db(path) loads it."""
+    if lib.ssa_amd_db_pack(_b(path)) != 0:
+        raise RuntimeError("cannot write packed database %s" % path)
 
 
 def batch_align(requests):

@@ -4,7 +4,9 @@
 // The reference links an external library (libsdb) for this; this provider
 // reads a FASTA file: records in file order get IDs 0, 1, ...; a record's
 // sequence is its lines joined without line terminators; NULL past the end.
-// Programs that define these four functions themselves replace this provider.
+// A pre-packed file (dbpack.h) is served from its memory map instead, with
+// its stored IDs.  Programs that define these four functions themselves
+// replace this provider.
 #include <cstdio>
 #include <cstring>
 #include <mutex>
@@ -12,6 +14,7 @@
 #include <vector>
 
 #include "common.h"
+#include "dbpack.h"
 #include "search.h"
 
 namespace {
@@ -21,6 +24,10 @@ struct FastaDB {
     std::vector<size_t> offset;   // record i occupies [offset[i], offset[i+1])
     std::vector<seqinfo_t> infos;
     bool open = false;
+    // a packed file: infos point into the map
+    void* map = nullptr;
+    size_t map_bytes = 0;
+    ssa::PackedView view;
 };
 
 FastaDB& db() {
@@ -71,7 +78,26 @@ int ssa_db_init(const char* db_name) {
     std::lock_guard<std::mutex> lk(g_mu);
     ssa::db_generation_bump();
     FastaDB& d = db();
+    ssa::packed_close(d.map, d.map_bytes);
     d = FastaDB();
+    if (db_name && ssa::is_packed_file(db_name)) {
+        const char* why = "";
+        if (!ssa::packed_open(db_name, &d.view, &d.map, &d.map_bytes, &why)) {
+            d = FastaDB();
+            d.open = true;
+            ssa::print_error("Invalid packed database file: %s (%s)", db_name, why);
+            return -1;
+        }
+        const ssa::PackedView& v = d.view;
+        d.infos.resize(v.h->seq_count);
+        for (size_t i = 0; i < d.infos.size(); ++i) {
+            d.infos[i].ID = v.seqs[i].id;
+            d.infos[i].seqlen = v.seqs[i].len;
+            d.infos[i].seq = (char*)v.ascii + v.seqs[i].ascii_off;
+        }
+        d.open = true;
+        return 0;
+    }
     if (!db_name || !load(db_name, d)) {
         d = FastaDB();
         d.open = true;
@@ -93,7 +119,22 @@ p_seqinfo ssa_db_get_sequence(size_t id) {
 void ssa_db_close(void) {
     std::lock_guard<std::mutex> lk(g_mu);
     ssa::db_generation_bump();
+    ssa::packed_close(db().map, db().map_bytes);
     db() = FastaDB();
 }
 
 }  // extern "C"
+
+namespace ssa {
+
+const PackedView* provider_packed_view() {
+    FastaDB& d = db();
+    return d.map ? &d.view : nullptr;
+}
+
+const seqinfo_t* provider_first_info() {
+    FastaDB& d = db();
+    return d.infos.empty() ? nullptr : &d.infos[0];
+}
+
+}  // namespace ssa

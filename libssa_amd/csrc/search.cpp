@@ -24,6 +24,7 @@
 #include <queue>
 #include <thread>
 
+#include "dbpack.h"
 #include "gpu.h"
 #include "kernels.h"
 
@@ -42,10 +43,10 @@ struct DeviceDB {
     size_t count = 0;
     std::vector<DbEntry> entries;
     std::vector<uint32_t> first_entry;  // per DB ID (count + 1 values)
-    std::vector<uint8_t> codes;         // all entries' codes, each padded to a multiple of 4
-    uint8_t* d_codes = nullptr;
+    uint8_t* d_codes = nullptr;         // all entries' codes, each padded to a multiple of 4
     size_t d_bytes = 0;
     std::vector<uint32_t> by_length;    // all entries, longest first (stable)
+    bool packed = false;                // loaded from a packed file
 };
 
 DeviceDB g_db;
@@ -62,7 +63,7 @@ void release_device_db() {
     g_db = DeviceDB();
 }
 
-static void map_db_codes(const char* s, size_t n, bool nucleotide, uint8_t* out) {
+static size_t map_db_codes(const char* s, size_t n, bool nucleotide, uint8_t* out) {
     size_t unknown = 0;
     for (size_t i = 0; i < n; ++i) {
         const int m = nucleotide ? map_nt((unsigned char)s[i]) : map_aa((unsigned char)s[i]);
@@ -73,39 +74,28 @@ static void map_db_codes(const char* s, size_t n, bool nucleotide, uint8_t* out)
             out[i] = (uint8_t)m;
         }
     }
+    return unknown;
+}
+
+static void warn_unknown(size_t unknown) {
     if (unknown) print_warning("%ld unknown symbols found and set to zero", (long)unknown);
 }
 
-// Reads the plugin DB once, maps it (src/db_adapter.c:47-110) and keeps the
-// codes resident on the GPU until the DB or the symbol settings change.
-static DeviceDB& device_db() {
-    const Config& cf = config();
+void map_provider_db(bool nuc, bool both, HostDb& d) {
+    d = HostDb();
     size_t count = ssa_db_get_sequence_count();
-    if (g_db.generation == g_generation && g_db.symtype == cf.symtype && g_db.strands == cf.strands &&
-        g_db.count == count && g_db.d_codes)
-        return g_db;
-    if (cf.symtype != NUCLEOTIDE && cf.symtype != AMINOACID)
-        fatal("libssa_amd: translated symbol types (TRANS_QUERY, TRANS_DB, TRANS_BOTH) are not supported");
-    release_device_db();
-    DeviceDB& d = g_db;
-    d.generation = g_generation;
-    d.symtype = cf.symtype;
-    d.strands = cf.strands;
-    d.count = count;
-    const bool nuc = cf.symtype == NUCLEOTIDE;
-    const bool both = nuc && (cf.strands & 2);
-    d.first_entry.assign(count + 1, 0);
     size_t total = 0;
     for (size_t id = 0; id < count; ++id) {
         p_seqinfo s = ssa_db_get_sequence(id);
         if (!s) {
-            d.count = count = id;
+            count = id;
             break;
         }
         const size_t padded = (s->seqlen + 3) & ~(size_t)3;
         total += padded * (both ? 2 : 1);
     }
-    d.first_entry.resize(count + 1);
+    d.count = count;
+    d.first_entry.assign(count + 1, 0);
     d.codes.assign(total + 4, 0);
     size_t off = 0;
     for (size_t id = 0; id < count; ++id) {
@@ -119,7 +109,8 @@ static DeviceDB& device_db() {
         e.strand = 0;
         e.frame = 0;
         e.code_off = (int64_t)off;
-        map_db_codes(s->seq, len, nuc, d.codes.data() + off);
+        const size_t unknown = map_db_codes(s->seq, len, nuc, d.codes.data() + off);
+        if (unknown) d.unknown.emplace_back(id, unknown);
         d.entries.push_back(e);
         off += padded;
         if (both) {
@@ -132,13 +123,85 @@ static DeviceDB& device_db() {
         }
     }
     d.first_entry[count] = (uint32_t)d.entries.size();
-    d.d_bytes = d.codes.size();
-    SSA_HIP(hipMalloc(&d.d_codes, d.d_bytes));
-    SSA_HIP(hipMemcpy(d.d_codes, d.codes.data(), d.d_bytes, hipMemcpyHostToDevice));
     d.by_length.resize(d.entries.size());
     std::iota(d.by_length.begin(), d.by_length.end(), 0u);
     std::stable_sort(d.by_length.begin(), d.by_length.end(),
                      [&](uint32_t a, uint32_t b) { return d.entries[a].len > d.entries[b].len; });
+}
+
+// Takes the entries of a packed file: all of them, or for a nucleotide search
+// of the forward strand only those of strand 0 (the packed file holds both).
+static void load_packed(const PackedView& pv, bool keep_reverse, DeviceDB& d) {
+    const PackHeader& h = *pv.h;
+    const size_t count = h.seq_count;
+    std::vector<uint32_t> renum(h.entry_count, UINT32_MAX);
+    d.entries.clear();
+    d.entries.reserve(keep_reverse ? h.entry_count : (h.entry_count + 1) / 2);
+    d.first_entry.assign(count + 1, 0);
+    size_t seq = 0;
+    for (uint64_t k = 0; k < h.entry_count; ++k) {
+        while (seq < count && pv.first[seq + 1] <= k) d.first_entry[++seq] = (uint32_t)d.entries.size();
+        const EntryRec& r = pv.entries[k];
+        if (r.strand == 1 && !keep_reverse) continue;
+        renum[k] = (uint32_t)d.entries.size();
+        DbEntry e;
+        e.id = r.id;
+        e.len = r.len;
+        e.strand = r.strand;
+        e.frame = r.frame;
+        e.code_off = r.code_off;
+        d.entries.push_back(e);
+    }
+    while (seq < count) d.first_entry[++seq] = (uint32_t)d.entries.size();
+    d.by_length.clear();
+    d.by_length.reserve(d.entries.size());
+    for (uint64_t k = 0; k < h.entry_count; ++k)
+        if (renum[pv.by_length[k]] != UINT32_MAX) d.by_length.push_back(renum[pv.by_length[k]]);
+    for (uint64_t u = 0; u < h.unknown_count; ++u) warn_unknown(pv.unknown[u].count);
+}
+
+// Reads the plugin DB once, maps it (src/db_adapter.c:47-110) and keeps the
+// codes resident on the GPU until the DB or the symbol settings change.  A
+// packed file opened by the built-in provider is uploaded as it is.
+static DeviceDB& device_db() {
+    const Config& cf = config();
+    size_t count = ssa_db_get_sequence_count();
+    if (g_db.generation == g_generation && g_db.symtype == cf.symtype && g_db.strands == cf.strands &&
+        g_db.count == count && g_db.d_codes)
+        return g_db;
+    if (cf.symtype != NUCLEOTIDE && cf.symtype != AMINOACID)
+        fatal("libssa_amd: translated symbol types (TRANS_QUERY, TRANS_DB, TRANS_BOTH) are not supported");
+    release_device_db();
+    DeviceDB& d = g_db;
+    d.generation = g_generation;
+    d.symtype = cf.symtype;
+    d.strands = cf.strands;
+    const bool nuc = cf.symtype == NUCLEOTIDE;
+    const bool both = nuc && (cf.strands & 2);
+    const PackedView* pv = provider_packed_view();
+    // the built-in provider's packed file, mapped for this symbol type, and
+    // not hidden by a caller's own provider
+    if (pv && pv->h->symtype == cf.symtype && pv->h->seq_count == count &&
+        (count == 0 || ssa_db_get_sequence(0) == provider_first_info())) {
+        d.count = count;
+        load_packed(*pv, both, d);
+        d.d_bytes = pv->h->code_bytes + 4;
+        SSA_HIP(hipMalloc(&d.d_codes, d.d_bytes));
+        SSA_HIP(hipMemset(d.d_codes + pv->h->code_bytes, 0, 4));
+        SSA_HIP(hipMemcpy(d.d_codes, pv->codes, pv->h->code_bytes, hipMemcpyHostToDevice));
+        d.packed = true;
+        return d;
+    }
+    HostDb hd;
+    map_provider_db(nuc, both, hd);
+    for (auto& u : hd.unknown) warn_unknown(u.second);
+    d.count = hd.count;
+    d.entries.swap(hd.entries);
+    d.first_entry.swap(hd.first_entry);
+    d.by_length.swap(hd.by_length);
+    d.d_bytes = hd.codes.size();
+    SSA_HIP(hipMalloc(&d.d_codes, d.d_bytes));
+    SSA_HIP(hipMemcpy(d.d_codes, hd.codes.data(), d.d_bytes, hipMemcpyHostToDevice));
     return d;
 }
 
@@ -860,7 +923,7 @@ namespace {
 std::vector<uint8_t> prepared_db_sequence(p_seqinfo s, int strand_field) {
     std::vector<uint8_t> v(s->seqlen);
     const bool nuc = config().symtype == NUCLEOTIDE;
-    map_db_codes(s->seq, s->seqlen, nuc, v.data());
+    warn_unknown(map_db_codes(s->seq, s->seqlen, nuc, v.data()));
     if (nuc && strand_field == 2) {
         std::vector<uint8_t> r(v.size());
         revcomp(v.data(), v.size(), r.data());

@@ -66,6 +66,19 @@ class Exec;
 void run_batch(Exec& ex, const std::vector<SearchRequest>& reqs, std::vector<SearchResult>& results, bool one_graph);
 void release_device_db();
 
+// The plugin DB mapped to search entries on the host (src/db_adapter.c:47-110,
+// 212-239): one entry per non-empty sequence (plus its reverse complement
+// when both_strands), codes padded to a multiple of 4 per entry.
+struct HostDb {
+    size_t count = 0;                    // sequences the provider returned
+    std::vector<DbEntry> entries;
+    std::vector<uint32_t> first_entry;   // per sequence index (count + 1 values)
+    std::vector<uint8_t> codes;
+    std::vector<uint32_t> by_length;     // entries longest first, ties in entry order
+    std::vector<std::pair<uint64_t, uint64_t>> unknown;  // (sequence index, symbols mapped to 0)
+};
+void map_provider_db(bool nucleotide, bool both_strands, HostDb& out);
+
 ssa_amd_stats_t& last_stats();
 void set_shape_override(int lanes, int rows);  // 0, 0 = chosen per query length
 void set_target_waves(int waves);             // 0 = default

@@ -168,3 +168,61 @@ def test_autotune_keeps_results(protein_db):
         assert [key(x) for x in lists] == want
     finally:
         ssa.autotune_reset()
+
+
+@pytest.mark.parametrize("symtype,strands", [(O.AMINOACID, 1), (O.NUCLEOTIDE, 3), (O.NUCLEOTIDE, 1)])
+def test_packed_db_searches_equal_fasta(tmp_path, symtype, strands):
+    # the same searches on the FASTA file and on its packed file (packed with
+    # both strands' entries; a forward-only search takes strand 0's only)
+    rnd = random.Random(31 + strands)
+    nuc = symtype == O.NUCLEOTIDE
+    alpha = "ACGT" if nuc else AA
+    base = "".join(rnd.choice(alpha) for _ in range(400))
+    db = rand_db(rnd, symtype, 300, 0, 380, homolog_of=base, frac=0.2)
+    matrix = ("const", (2, -4)) if nuc else ("builtin", "blosum62")
+    gaps = (-20, -2) if nuc else (-10, -1)
+    configure(symtype, 3 if nuc else 1, gaps, matrix)
+    fas = str(tmp_path / "db.fas")
+    write_fasta(fas, db)
+    ssa.init_db(fas)
+    packed = str(tmp_path / "db.ssapack")
+    ssa.pack_db(packed)
+    configure(symtype, strands, gaps, matrix)
+    qraws = [base[:300], base[50:120], "".join(rnd.choice(alpha) for _ in range(150))]
+    qs = [ssa.init_sequence_fasta(ssa.READ_FROM_STRING, s) for s in qraws]
+    reqs = [(ssa.SEARCH_SW, qs[0], 20, 8, ssa.COMPUTE_ALIGNMENT), (ssa.SEARCH_NW, qs[1], 10, 16, ssa.COMPUTE_ALIGNMENT),
+            (ssa.SEARCH_SW, qs[2], 15, 64, ssa.COMPUTE_SCORE), (ssa.SEARCH_NW, qs[0], 5, 8, ssa.COMPUTE_SCORE)]
+    try:
+        ssa.init_db(fas)
+        want = [key(alone(r)) for r in reqs]
+        ssa.init_db(packed)
+        assert [key(alone(r)) for r in reqs] == want
+        lists, _ = ssa.batch_align(reqs)
+        assert [key(x) for x in lists] == want
+    finally:
+        for q in qs:
+            ssa.free_sequence(q)
+    # and the FASTA path is the oracle-checked one
+    check(O.SW, 8, symtype, strands, qraws[0], db, 20, matrix, gaps, align=True)
+
+
+def test_packed_db_other_symbol_type_maps_from_residues(tmp_path):
+    # a protein-packed file searched as nucleotides: the codes are mapped
+    # again from the stored residues, as for a FASTA file
+    rnd = random.Random(77)
+    db = ["".join(rnd.choice("ACGT") for _ in range(rnd.randint(20, 200))) for _ in range(150)]
+    configure(O.AMINOACID, 1, (-10, -1), ("builtin", "blosum62"))
+    fas = str(tmp_path / "db.fas")
+    write_fasta(fas, db)
+    ssa.init_db(fas)
+    packed = str(tmp_path / "db.ssapack")
+    ssa.pack_db(packed)
+    configure(O.NUCLEOTIDE, 3, (-20, -2), ("const", (2, -4)))
+    q = ssa.init_sequence_fasta(ssa.READ_FROM_STRING, db[3][10:90])
+    try:
+        ssa.init_db(fas)
+        want = key(ssa.sw_align(q, 10, 16, ssa.COMPUTE_ALIGNMENT))
+        ssa.init_db(packed)
+        assert key(ssa.sw_align(q, 10, 16, ssa.COMPUTE_ALIGNMENT)) == want
+    finally:
+        ssa.free_sequence(q)
