@@ -110,7 +110,8 @@ bool packed_open(const char* path, PackedView* v, void** map, size_t* map_bytes,
         // every index and range the search follows
         for (uint64_t i = 0; i < h.seq_count && !bad; ++i) {
             const SeqRec& s = v->seqs[i];
-            if (s.ascii_off > h.ascii_bytes || s.len > h.ascii_bytes - s.ascii_off) bad = "sequence outside the residues";
+            if (s.ascii_off >= h.ascii_bytes || s.len >= h.ascii_bytes - s.ascii_off) bad = "sequence outside the residues";
+            else if (v->ascii[s.ascii_off + s.len] != 0) bad = "sequence not terminated";
             if (v->first[i] > v->first[i + 1]) bad = "entry table out of order";
         }
         if (!bad && (v->first[0] != 0 || v->first[h.seq_count] != h.entry_count)) bad = "entry table out of order";
@@ -167,7 +168,7 @@ int write_packed_db(const char* path) {
     for (size_t i = 0; i < d.count; ++i) {
         p_seqinfo s = ssa_db_get_sequence(i);
         seqs[i] = SeqRec{(uint64_t)s->ID, ascii_bytes, (uint64_t)s->seqlen};
-        ascii_bytes += s->seqlen;
+        ascii_bytes += s->seqlen + 1;  // NUL-terminated, like the FASTA provider's records
     }
     std::vector<EntryRec> ents(d.entries.size());
     for (size_t k = 0; k < ents.size(); ++k) {
@@ -210,7 +211,8 @@ int write_packed_db(const char* path) {
               write_padded(fp, at, h.off_seqs, seqs.data(), seqs.size() * sizeof(SeqRec));
     for (size_t i = 0; ok && i < d.count; ++i) {
         p_seqinfo s = ssa_db_get_sequence(i);
-        ok = write_padded(fp, at, i == 0 ? h.off_ascii : at, s->seq, s->seqlen);
+        ok = write_padded(fp, at, i == 0 ? h.off_ascii : at, s->seq, s->seqlen) &&
+             write_padded(fp, at, at, "", 1);
     }
     if (ok && d.count == 0) ok = write_padded(fp, at, h.off_ascii, nullptr, 0);
     ok = ok && write_padded(fp, at, h.off_entries, ents.data(), ents.size() * sizeof(EntryRec)) &&

@@ -9,7 +9,8 @@
 // Layout (little-endian; every section starts on a 64-byte boundary):
 //   PackHeader (128 bytes)
 //   SeqRec[seq_count]          provider ID, offset and length of the residues
-//   char   ascii[ascii_bytes]  the residues as the provider returned them
+//   char   ascii[ascii_bytes]  the residues as the provider returned them, each
+//                              followed by a NUL
 //   EntryRec[entry_count]      one per search entry, in DB order (strand 0
 //                              then strand 1 for nucleotide DBs)
 //   u32    first[seq_count+1]  first entry of each sequence (empty ones own none)

@@ -3,7 +3,8 @@
 //
 // The reference links an external library (libsdb) for this; this provider
 // reads a FASTA file: records in file order get IDs 0, 1, ...; a record's
-// sequence is its lines joined without line terminators; NULL past the end.
+// sequence is its lines joined without line terminators, as a NUL-terminated
+// string; NULL past the end.
 // A pre-packed file (dbpack.h) is served from its memory map instead, with
 // its stored IDs.  Programs that define these four functions themselves
 // replace this provider.
@@ -20,8 +21,8 @@
 namespace {
 
 struct FastaDB {
-    std::vector<char> residues;   // all sequences back to back
-    std::vector<size_t> offset;   // record i occupies [offset[i], offset[i+1])
+    std::vector<char> residues;   // all sequences back to back, each NUL-terminated
+    std::vector<size_t> offset;   // record i occupies [offset[i], offset[i+1] - 1)
     std::vector<seqinfo_t> infos;
     bool open = false;
     // a packed file: infos point into the map
@@ -47,6 +48,7 @@ bool load(const char* path, FastaDB& d) {
         for (size_t k = 0; k < n; ++k) {
             const char c = buf[k];
             if (at_line_start && c == '>') {
+                if (have_record) d.residues.push_back('\0');
                 d.offset.push_back(d.residues.size());
                 have_record = true;
                 in_header = true;
@@ -59,12 +61,13 @@ bool load(const char* path, FastaDB& d) {
         }
     }
     fclose(fp);
+    if (have_record) d.residues.push_back('\0');
     d.offset.push_back(d.residues.size());
     const size_t count = d.offset.size() - 1;
     d.infos.resize(count);
     for (size_t i = 0; i < count; ++i) {
         d.infos[i].ID = i;
-        d.infos[i].seqlen = d.offset[i + 1] - d.offset[i];
+        d.infos[i].seqlen = d.offset[i + 1] - d.offset[i] - 1;
         d.infos[i].seq = d.residues.data() + d.offset[i];
     }
     return true;
