@@ -37,6 +37,14 @@ typedef struct {
 /* Counters of the last search. */
 void ssa_amd_last_stats(ssa_amd_stats_t* out);
 
+/* Restores every setting made through libssa.h -- symbol type, strands,
+ * genetic codes, gap penalties, chunk size, SIMD mode, thread count, output
+ * mode -- to its initial value; the score matrix and the DB stay as they are.
+ * The reference's init_symbol_translation tests the symbol type in force
+ * before it changes it (src/libssa.c:146-151) and so exits once TRANS_DB or
+ * TRANS_BOTH is set; this is how a long-lived process leaves those types. */
+void ssa_amd_reset_settings(void);
+
 /* Shape of the score kernel: lanes per stream pair (8, 16, 32 or 64) and
  * query rows per lane (4, 8 or 16); 0, 0 chooses from the query length. */
 void ssa_amd_set_kernel_shape(int lanes_per_stream, int rows_per_lane);

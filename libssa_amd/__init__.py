@@ -112,7 +112,8 @@ def _load():
         "free_alignment": (None, [_C.POINTER(_AlignmentList)]), "ssa_exit": (None, []),
         "ssa_db_init": (I, [cp]), "ssa_db_get_sequence_count": (S, []), "ssa_db_close": (None, []),
         "ssa_db_get_sequence": (_C.POINTER(_SeqInfo), [S]), "ssa_amd_db_pack": (I, [cp]),
-        "ssa_amd_last_stats": (None, [_C.POINTER(Stats)]), "ssa_amd_set_kernel_shape": (None, [I, I]), "ssa_amd_set_stream_waves": (None, [I]),
+        "ssa_amd_last_stats": (None, [_C.POINTER(Stats)]), "ssa_amd_reset_settings": (None, []),
+        "ssa_amd_set_kernel_shape": (None, [I, I]), "ssa_amd_set_stream_waves": (None, [I]),
         "ssa_amd_get_matrix": (None, [P]), "ssa_amd_device_arch": (cp, []),
         "ssa_amd_batch_align": (I, [_C.POINTER(_Request), S, _C.POINTER(_C.POINTER(_AlignmentList)),
                                     _C.POINTER(Stats)]),
@@ -205,6 +206,12 @@ def last_stats():
     s = Stats()
     lib.ssa_amd_last_stats(_C.byref(s))
     return s
+
+
+def reset_settings():
+    """Every libssa.h setting back to its initial value (ssa_amd_reset_settings);
+    the matrix and the DB stay."""
+    lib.ssa_amd_reset_settings()
 
 
 def set_kernel_shape(lanes_per_stream, rows_per_lane): lib.ssa_amd_set_kernel_shape(lanes_per_stream, rows_per_lane)

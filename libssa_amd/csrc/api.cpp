@@ -139,6 +139,8 @@ void ssa_amd_last_stats(ssa_amd_stats_t* out) {
     if (out) *out = last_stats();
 }
 
+void ssa_amd_reset_settings(void) { config() = Config(); }
+
 void ssa_amd_set_kernel_shape(int lanes_per_stream, int rows_per_lane) {
     const bool autosel = lanes_per_stream == 0 && rows_per_lane == 0;
     const bool lanes_ok = lanes_per_stream == 8 || lanes_per_stream == 16 || lanes_per_stream == 32 ||

@@ -66,6 +66,17 @@ int map_nt(int c);
 int complement_nt(int code);
 void revcomp(const uint8_t* in, size_t len, uint8_t* out);
 bool gencode_valid(int code);
+// DB sequences are nucleotides for NUCLEOTIDE, TRANS_DB, TRANS_BOTH; queries
+// for NUCLEOTIDE, TRANS_QUERY, TRANS_BOTH (src/query.c:48-55, src/db_adapter.c:47-110)
+inline bool db_is_translated(int symtype) { return symtype == TRANS_DB || symtype == TRANS_BOTH; }
+inline bool query_is_translated(int symtype) { return symtype == TRANS_QUERY || symtype == TRANS_BOTH; }
+// the codon table of a genetic code: [256 a + 16 b + c] = amino-acid code of
+// the nt16 codon (a, b, c) (src/util/util_sequence.c:200-253)
+const int8_t* translation_table(int gencode);
+// us_translate_sequence (src/util/util_sequence.c:312-353): the protein of
+// strand 0 or, for any other strand value, of the reverse complement, from
+// frame 0..2.  A sequence shorter than its frame exits as the reference does.
+void translate(const int8_t* table, const uint8_t* dna, size_t len, int strand, int frame, std::vector<uint8_t>& out);
 
 }  // namespace ssa
 

@@ -156,13 +156,14 @@ int write_packed_db(const char* path) {
         print_error("Cannot write packed database: no file name");
         return -1;
     }
-    if (cf.symtype != NUCLEOTIDE && cf.symtype != AMINOACID) {
-        print_error("Cannot write packed database: translated symbol types are not packed");
+    if (db_is_translated(cf.symtype)) {
+        print_error("Cannot write packed database: translated DB sequences are not packed");
         return -1;
     }
+    // a translated query searches an amino-acid DB
     const bool nuc = cf.symtype == NUCLEOTIDE;
     HostDb d;
-    map_provider_db(nuc, nuc, d);
+    map_provider_db(nuc ? NUCLEOTIDE : AMINOACID, nuc ? BOTH_STRANDS : FORWARD_STRAND, 0, d);
     std::vector<SeqRec> seqs(d.count);
     uint64_t ascii_bytes = 0;
     for (size_t i = 0; i < d.count; ++i) {
@@ -183,7 +184,7 @@ int write_packed_db(const char* path) {
     memcpy(h.magic, kPackMagic, 8);
     h.version = kPackVersion;
     h.header_bytes = sizeof(PackHeader);
-    h.symtype = cf.symtype;
+    h.symtype = nuc ? NUCLEOTIDE : AMINOACID;
     h.strands = nuc ? 3 : 1;
     h.seq_count = d.count;
     h.entry_count = ents.size();

@@ -39,7 +39,7 @@ uint64_t g_generation = 1;
 
 struct DeviceDB {
     uint64_t generation = 0;
-    int symtype = -1, strands = -1;
+    int symtype = -1, strands = -1, gencode = -1;
     size_t count = 0;
     std::vector<DbEntry> entries;
     std::vector<uint32_t> first_entry;  // per DB ID (count + 1 values)
@@ -81,20 +81,50 @@ static void warn_unknown(size_t unknown) {
     if (unknown) print_warning("%ld unknown symbols found and set to zero", (long)unknown);
 }
 
-void map_provider_db(bool nuc, bool both, HostDb& d) {
-    d = HostDb();
-    size_t count = ssa_db_get_sequence_count();
-    size_t total = 0;
+// codes of an entry of len residues in the DB code buffer: a multiple of 4,
+// and 4 for an empty entry (a translated frame without a codon), which the
+// score kernels read as one block of padding
+static size_t entry_code_bytes(size_t len) { return len ? (len + 3) & ~(size_t)3 : 4; }
+
+// set_translated_sequences for TRANS_DB / TRANS_BOTH (src/db_adapter.c:72-104):
+// frames 0..2 of both strands, strand field 0 / 1; or of the one strand
+// selected, whose strand field is then the strand setting itself (1 or 2).
+static void map_translated_db(int strands, int gencode, size_t count, HostDb& d) {
+    const int8_t* table = translation_table(gencode);
+    const int per_seq = strands == BOTH_STRANDS ? 6 : 3;
+    std::vector<uint8_t> nt, prot;
+    d.first_entry.assign(count + 1, 0);
+    d.codes.clear();
     for (size_t id = 0; id < count; ++id) {
+        d.first_entry[id] = (uint32_t)d.entries.size();
         p_seqinfo s = ssa_db_get_sequence(id);
-        if (!s) {
-            count = id;
-            break;
+        if (s->seqlen == 0) continue;  // src/db_adapter.c:231-233
+        nt.resize(s->seqlen);
+        const size_t unknown = map_db_codes(s->seq, s->seqlen, true, nt.data());
+        if (unknown) d.unknown.emplace_back(id, unknown);
+        for (int k = 0; k < per_seq; ++k) {
+            const int strand = per_seq == 6 ? k / 3 : strands - 1, frame = k % 3;
+            translate(table, nt.data(), nt.size(), strand, frame, prot);
+            DbEntry e;
+            e.id = s->ID;
+            e.len = (uint32_t)prot.size();
+            e.strand = (uint8_t)(per_seq == 6 ? strand : strands);
+            e.frame = (uint8_t)frame;
+            e.code_off = (int64_t)d.codes.size();
+            d.codes.insert(d.codes.end(), prot.begin(), prot.end());
+            d.codes.resize(e.code_off + entry_code_bytes(prot.size()), 0);
+            d.entries.push_back(e);
         }
-        const size_t padded = (s->seqlen + 3) & ~(size_t)3;
-        total += padded * (both ? 2 : 1);
     }
-    d.count = count;
+    d.first_entry[count] = (uint32_t)d.entries.size();
+    d.codes.resize(d.codes.size() + 4, 0);
+}
+
+// one entry per sequence (amino acids), or the forward strand and, with
+// both, the reverse complement (nucleotides; src/db_adapter.c:47-70)
+static void map_plain_db(bool nuc, bool both, size_t count, HostDb& d) {
+    size_t total = 0;
+    for (size_t id = 0; id < count; ++id) total += ((ssa_db_get_sequence(id)->seqlen + 3) & ~(size_t)3) * (both ? 2 : 1);
     d.first_entry.assign(count + 1, 0);
     d.codes.assign(total + 4, 0);
     size_t off = 0;
@@ -123,6 +153,21 @@ void map_provider_db(bool nuc, bool both, HostDb& d) {
         }
     }
     d.first_entry[count] = (uint32_t)d.entries.size();
+}
+
+void map_provider_db(int symtype, int strands, int gencode, HostDb& d) {
+    d = HostDb();
+    size_t count = ssa_db_get_sequence_count();
+    for (size_t id = 0; id < count; ++id)
+        if (!ssa_db_get_sequence(id)) {
+            count = id;
+            break;
+        }
+    d.count = count;
+    if (db_is_translated(symtype))
+        map_translated_db(strands, gencode, count, d);
+    else
+        map_plain_db(symtype == NUCLEOTIDE, symtype == NUCLEOTIDE && (strands & 2), count, d);
     d.by_length.resize(d.entries.size());
     std::iota(d.by_length.begin(), d.by_length.end(), 0u);
     std::stable_sort(d.by_length.begin(), d.by_length.end(),
@@ -166,22 +211,27 @@ static void load_packed(const PackedView& pv, bool keep_reverse, DeviceDB& d) {
 static DeviceDB& device_db() {
     const Config& cf = config();
     size_t count = ssa_db_get_sequence_count();
-    if (g_db.generation == g_generation && g_db.symtype == cf.symtype && g_db.strands == cf.strands &&
-        g_db.count == count && g_db.d_codes)
+    // the DB entries depend on the symbol type of the DB sequences, the
+    // strands (nucleotide and translated DBs) and the DB genetic code
+    // (translated DBs); a translated query searches an amino-acid DB
+    const int db_type = cf.symtype == TRANS_QUERY ? AMINOACID : cf.symtype;
+    const int strands = db_type == AMINOACID ? 0 : cf.strands;
+    const int gencode = db_is_translated(db_type) ? cf.d_gencode : 0;
+    if (g_db.generation == g_generation && g_db.symtype == db_type && g_db.strands == strands &&
+        g_db.gencode == gencode && g_db.count == count && g_db.d_codes)
         return g_db;
-    if (cf.symtype != NUCLEOTIDE && cf.symtype != AMINOACID)
-        fatal("libssa_amd: translated symbol types (TRANS_QUERY, TRANS_DB, TRANS_BOTH) are not supported");
     release_device_db();
     DeviceDB& d = g_db;
     d.generation = g_generation;
-    d.symtype = cf.symtype;
-    d.strands = cf.strands;
-    const bool nuc = cf.symtype == NUCLEOTIDE;
+    d.symtype = db_type;
+    d.strands = strands;
+    d.gencode = gencode;
+    const bool nuc = db_type == NUCLEOTIDE;
     const bool both = nuc && (cf.strands & 2);
     const PackedView* pv = provider_packed_view();
     // the built-in provider's packed file, mapped for this symbol type, and
     // not hidden by a caller's own provider
-    if (pv && pv->h->symtype == cf.symtype && pv->h->seq_count == count &&
+    if (pv && pv->h->symtype == db_type && pv->h->seq_count == count &&
         (count == 0 || ssa_db_get_sequence(0) == provider_first_info())) {
         d.count = count;
         load_packed(*pv, both, d);
@@ -193,7 +243,7 @@ static DeviceDB& device_db() {
         return d;
     }
     HostDb hd;
-    map_provider_db(nuc, both, hd);
+    map_provider_db(db_type, cf.strands, cf.d_gencode, hd);
     for (auto& u : hd.unknown) warn_unknown(u.second);
     d.count = hd.count;
     d.entries.swap(hd.entries);
@@ -415,7 +465,7 @@ void build_layout(Exec& ex, StreamLayout& L, int lanes, const PairSet& ps, const
     std::vector<int32_t> nbk(n);
     int64_t total = 0;
     for (size_t k = 0; k < n; ++k) {
-        nbk[k] = (int32_t)((db.entries[ps.entries[k]].len + 3) / 4);
+        nbk[k] = entry_blocks(db.entries[ps.entries[k]].len);
         total += nbk[k];
     }
     const int64_t per_wave = std::max<int64_t>(1, ((int64_t)target_waves()));
@@ -479,8 +529,8 @@ void build_layout(Exec& ex, StreamLayout& L, int lanes, const PairSet& ps, const
                 PackEntry p;
                 p.code_off = de.code_off;
                 p.dst = (L.wave_res_off[w] + pos * ng + k) * 2 + h;
-                p.nblocks = (int32_t)((de.len + 3) / 4);
-                p.last_col = (int32_t)((de.len - 1) % 4);
+                p.nblocks = entry_blocks(de.len);
+                p.last_col = (int32_t)((de.len + 3) % 4);  // 3 for an empty entry, as (len + 3) % 4 in the reference
                 p.stride = 2 * ng;
                 p.pad = 0;
                 pe.push_back(p);
@@ -576,7 +626,7 @@ void tier_prepare(Exec& ex, TierRun& R) {
     if (R.sh.lanes == 0) {
         int64_t total_blocks = 0, max_blocks = 0;
         for (uint32_t e : ps.entries) {
-            const int64_t nb = (db.entries[e].len + 3) / 4;
+            const int64_t nb = entry_blocks(db.entries[e].len);
             total_blocks += nb;
             max_blocks = std::max(max_blocks, nb);
         }
@@ -806,7 +856,7 @@ void gpu_exact(Exec& gpu, SearchType type, const std::vector<QueryBuf>& qs,
 
 // Decodes one tier result into (duration, accept, score).
 TierOut decode(SearchType type, const Tier& T, const PairResult& r, uint32_t len, bool sw16_as_8) {
-    const int32_t nb = (int32_t)((len + 3) / 4);
+    const int32_t nb = entry_blocks(len);
     TierOut o;
     int32_t flag;
     if (type == kSW) {
@@ -918,13 +968,23 @@ void lane_schedule(int W, size_t n, const Dur& dur, const Emit& emit) {
 namespace {
 
 // the sequence the reference attaches to a hit (us_prepare_sequence,
-// src/util/util_sequence.c:384-417): mapped forward strand unless strand == 2,
-// which the adapter never produces; unknown symbols become 0
-std::vector<uint8_t> prepared_db_sequence(p_seqinfo s, int strand_field) {
+// src/util/util_sequence.c:384-417), unknown symbols mapped to 0: for a
+// nucleotide DB the forward strand unless strand == 2, which the adapter never
+// produces; for a translated DB the frame of the strand the field names (0
+// forward, otherwise the reverse complement), so a one-strand forward search,
+// whose entries carry strand 1, gets the reverse strand's frame here.  (The
+// reference translates into an uninitialised buffer there; this is the
+// result when that buffer is a fresh allocation.)
+std::vector<uint8_t> prepared_db_sequence(p_seqinfo s, int strand_field, int frame_field) {
     std::vector<uint8_t> v(s->seqlen);
-    const bool nuc = config().symtype == NUCLEOTIDE;
+    const int st = config().symtype;
+    const bool nuc = st == NUCLEOTIDE || db_is_translated(st);
     warn_unknown(map_db_codes(s->seq, s->seqlen, nuc, v.data()));
-    if (nuc && strand_field == 2) {
+    if (db_is_translated(st)) {
+        std::vector<uint8_t> p;
+        translate(translation_table(config().d_gencode), v.data(), v.size(), strand_field, frame_field, p);
+        v.swap(p);
+    } else if (nuc && strand_field == 2) {
         std::vector<uint8_t> r(v.size());
         revcomp(v.data(), v.size(), r.data());
         v.swap(r);
@@ -1086,10 +1146,13 @@ void job_queries(Job& J) {
         for (int s = 0; s < 2; ++s)
             if (((s + 1) & cf.strands) && query->has_nt[s])
                 J.qs.push_back(QueryBuf{query->nt[s].data(), query->nt[s].size(), s, 0});
-    } else if (cf.symtype == AMINOACID) {
-        J.qs.push_back(QueryBuf{query->aa[0].data(), query->aa[0].size(), 0, 0});
+    } else if (query_is_translated(cf.symtype)) {
+        for (int s = 0; s < 2; ++s)
+            if ((s + 1) & cf.strands)
+                for (int f = 0; f < 3; ++f)
+                    J.qs.push_back(QueryBuf{query->aa[3 * s + f].data(), query->aa[3 * s + f].size(), s, f});
     } else {
-        fatal("libssa_amd: translated symbol types (TRANS_QUERY, TRANS_DB, TRANS_BOTH) are not supported");
+        J.qs.push_back(QueryBuf{query->aa[0].data(), query->aa[0].size(), 0, 0});
     }
     for (const QueryBuf& q : J.qs)
         if (q.len == 0 && J.rq.type == kNW) fatal("libssa_amd: NW search with an empty query");
@@ -1189,7 +1252,7 @@ void job_list(Job& J, std::vector<p_alignment>& al_out, std::vector<SearchType>&
         const Elem& h = hits[k];
         p_seqinfo info = ssa_db_get_sequence(h.db_id);
         if (!info) fatal("Could not get sequence from DB: %ld", (long)h.db_id);
-        std::vector<uint8_t> dseq = prepared_db_sequence(info, h.db_strand);
+        std::vector<uint8_t> dseq = prepared_db_sequence(info, h.db_strand, h.db_frame);
         p_alignment a = (p_alignment)calloc(1, sizeof(alignment_t));
         a->db_seq.seq = (char*)malloc(dseq.size() + 1);
         memcpy(a->db_seq.seq, dseq.data(), dseq.size());
@@ -1261,7 +1324,7 @@ void run_batch(Exec& ex, const std::vector<SearchRequest>& reqs, std::vector<Sea
     int64_t total_blocks = 0, max_blocks = 0;
     uint64_t total_residues = 0;
     for (uint32_t e = 0; e < N; ++e) {
-        const int64_t nb = (db.entries[e].len + 3) / 4;
+        const int64_t nb = entry_blocks(db.entries[e].len);
         total_blocks += nb;
         max_blocks = std::max(max_blocks, nb);
         total_residues += db.entries[e].len;
@@ -1551,6 +1614,8 @@ void run_batch(Exec& ex, const std::vector<SearchRequest>& reqs, std::vector<Sea
                 for (size_t q = 0; q < nq; ++q)
                     for (uint32_t e = e0; e < e1; ++e) {
                         TierOut o = decode(type, kTier16, raw(q, e), db.entries[e].len, false);
+                        // an empty entry's lane scores a padding block; 64 bit scores the empty sequence
+                        if (db.entries[e].len == 0) o.accept = false;
                         if (type == kNW) {
                             // exact when no value can reach either end of the int16 range
                             const int64_t ml = (int64_t)std::min<size_t>(J.qs[q].len, db.entries[e].len);

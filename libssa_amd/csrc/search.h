@@ -33,6 +33,11 @@ struct DbEntry {
 // bumped whenever the DB plugin is (re)initialised or closed
 void db_generation_bump();
 
+// 4-column blocks a score kernel spends on an entry: an empty entry (a
+// translated frame without a codon) takes one block of padding, as a lane of
+// the reference's kernels does (src/algo/8/search_8_util.h:60-74)
+inline int32_t entry_blocks(uint32_t len) { return len ? (int32_t)((len + 3) / 4) : 1; }
+
 p_alignment_list run_search(SearchType type, p_query query, size_t hitcount, int bit_width, int align_type);
 
 // one sw_align / nw_align call of a batch
@@ -67,8 +72,10 @@ void run_batch(Exec& ex, const std::vector<SearchRequest>& reqs, std::vector<Sea
 void release_device_db();
 
 // The plugin DB mapped to search entries on the host (src/db_adapter.c:47-110,
-// 212-239): one entry per non-empty sequence (plus its reverse complement
-// when both_strands), codes padded to a multiple of 4 per entry.
+// 212-239): per non-empty sequence one entry (amino acids), one or two
+// (nucleotides: forward, and the reverse complement when strands has bit 2),
+// or three or six translated frames (TRANS_DB / TRANS_BOTH, with the genetic
+// code); codes padded to a multiple of 4 per entry, 4 bytes for an empty frame.
 struct HostDb {
     size_t count = 0;                    // sequences the provider returned
     std::vector<DbEntry> entries;
@@ -77,7 +84,7 @@ struct HostDb {
     std::vector<uint32_t> by_length;     // entries longest first, ties in entry order
     std::vector<std::pair<uint64_t, uint64_t>> unknown;  // (sequence index, symbols mapped to 0)
 };
-void map_provider_db(bool nucleotide, bool both_strands, HostDb& out);
+void map_provider_db(int symtype, int strands, int gencode, HostDb& out);
 
 ssa_amd_stats_t& last_stats();
 void set_shape_override(int lanes, int rows);  // 0, 0 = chosen per query length

@@ -1,12 +1,14 @@
-// sequence.cpp -- symbol codes, reverse complement, query reading.
+// sequence.cpp -- symbol codes, reverse complement, translation, query reading.
 //
 // Codes follow the reference's tables (src/util/util_sequence.c:40-88):
 // amino acids are numbered by their position in "-ABCDEFGHIKLMNPQRSTVWXYZU*OJ"
 // ('-' itself is not an amino-acid symbol, '*' is); nucleotides by their
 // position in "-ACMGRSVTWYHKDBN" with U read as T, which makes a nucleotide
 // code the IUPAC bit set A=1, C=2, G=4, T=8.
+#include <algorithm>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 
 #include "common.h"
 
@@ -41,10 +43,79 @@ void revcomp(const uint8_t* in, size_t len, uint8_t* out) {
     for (size_t i = 0; i < len; ++i) out[i] = (uint8_t)complement_nt(in[len - 1 - i]);
 }
 
-// genetic codes 1..23 that exist (src/util/util_sequence.c:93-118)
-bool gencode_valid(int code) {
-    if (code < 1 || code > 23) return false;
-    return !(code == 7 || code == 8 || (code >= 17 && code <= 20));
+// NCBI's genetic codes 1..23 as published (the ones the reference names,
+// src/util/util_sequence.c:95-123): the 64 amino acids of the codons in TCAG
+// order, first base slowest; nullptr where NCBI has no code of that number
+static const char* const kGeneticCodes[23] = {
+    "FFLLSSSSYY**CC*WLLLLPPPPHHQQRRRRIIIMTTTTNNKKSSRRVVVVAAAADDEEGGGG",  // 1 standard
+    "FFLLSSSSYY**CCWWLLLLPPPPHHQQRRRRIIMMTTTTNNKKSS**VVVVAAAADDEEGGGG",  // 2 vertebrate mitochondrial
+    "FFLLSSSSYY**CCWWTTTTPPPPHHQQRRRRIIMMTTTTNNKKSSRRVVVVAAAADDEEGGGG",  // 3 yeast mitochondrial
+    "FFLLSSSSYY**CCWWLLLLPPPPHHQQRRRRIIIMTTTTNNKKSSRRVVVVAAAADDEEGGGG",  // 4 mold, protozoan, coelenterate mt.
+    "FFLLSSSSYY**CCWWLLLLPPPPHHQQRRRRIIMMTTTTNNKKSSSSVVVVAAAADDEEGGGG",  // 5 invertebrate mitochondrial
+    "FFLLSSSSYYQQCC*WLLLLPPPPHHQQRRRRIIIMTTTTNNKKSSRRVVVVAAAADDEEGGGG",  // 6 ciliate, dasycladacean, hexamita
+    nullptr, nullptr,
+    "FFLLSSSSYY**CCWWLLLLPPPPHHQQRRRRIIIMTTTTNNNKSSSSVVVVAAAADDEEGGGG",  // 9 echinoderm, flatworm mt.
+    "FFLLSSSSYY**CCCWLLLLPPPPHHQQRRRRIIIMTTTTNNKKSSRRVVVVAAAADDEEGGGG",  // 10 euplotid nuclear
+    "FFLLSSSSYY**CC*WLLLLPPPPHHQQRRRRIIIMTTTTNNKKSSRRVVVVAAAADDEEGGGG",  // 11 bacterial, archaeal, plastid
+    "FFLLSSSSYY**CC*WLLLSPPPPHHQQRRRRIIIMTTTTNNKKSSRRVVVVAAAADDEEGGGG",  // 12 alternative yeast nuclear
+    "FFLLSSSSYY**CCWWLLLLPPPPHHQQRRRRIIMMTTTTNNKKSSGGVVVVAAAADDEEGGGG",  // 13 ascidian mitochondrial
+    "FFLLSSSSYYY*CCWWLLLLPPPPHHQQRRRRIIIMTTTTNNNKSSSSVVVVAAAADDEEGGGG",  // 14 alternative flatworm mt.
+    "FFLLSSSSYY*QCC*WLLLLPPPPHHQQRRRRIIIMTTTTNNKKSSRRVVVVAAAADDEEGGGG",  // 15 blepharisma nuclear
+    "FFLLSSSSYY*LCC*WLLLLPPPPHHQQRRRRIIIMTTTTNNKKSSRRVVVVAAAADDEEGGGG",  // 16 chlorophycean mt.
+    nullptr, nullptr, nullptr, nullptr,
+    "FFLLSSSSYY**CCWWLLLLPPPPHHQQRRRRIIMMTTTTNNNKSSSSVVVVAAAADDEEGGGG",  // 21 trematode mitochondrial
+    "FFLLSS*SYY*LCC*WLLLLPPPPHHQQRRRRIIIMTTTTNNKKSSRRVVVVAAAADDEEGGGG",  // 22 scenedesmus obliquus mt.
+    "FF*LSSSSYY**CC*WLLLLPPPPHHQQRRRRIIIMTTTTNNKKSSRRVVVVAAAADDEEGGGG",  // 23 thraustochytrium mt.
+};
+
+bool gencode_valid(int code) { return code >= 1 && code <= 23 && kGeneticCodes[code - 1] != nullptr; }
+
+// An nt16 code is a set of bases (bit 0 A, 1 C, 2 G, 3 T).  A codon of base
+// sets translates to the amino acid that every codon in it gives; a mix of
+// only D and N is B, of only E and Q is Z; anything else, or no base, is X.
+const int8_t* translation_table(int gencode) {
+    static std::vector<int8_t> tables[23];
+    static std::mutex mu;
+    std::lock_guard<std::mutex> lk(mu);
+    std::vector<int8_t>& t = tables[gencode - 1];
+    if (!t.empty()) return t.data();
+    const char* code = kGeneticCodes[gencode - 1];
+    static const int kTcag[4] = {2, 1, 3, 0};  // base bit (A, C, G, T) -> TCAG position
+    t.resize(4096);
+    for (int a = 0; a < 16; ++a)
+        for (int b = 0; b < 16; ++b)
+            for (int c = 0; c < 16; ++c) {
+                std::string set;
+                for (int i = 0; i < 4; ++i)
+                    for (int j = 0; j < 4; ++j)
+                        for (int k = 0; k < 4; ++k)
+                            if ((a >> i & 1) && (b >> j & 1) && (c >> k & 1)) {
+                                const char x = code[16 * kTcag[i] + 4 * kTcag[j] + kTcag[k]];
+                                if (set.find(x) == std::string::npos) set.push_back(x);
+                            }
+                std::sort(set.begin(), set.end());
+                char aa = 'X';
+                if (set.size() == 1) aa = set[0];
+                else if (set == "DN") aa = 'B';
+                else if (set == "EQ") aa = 'Z';
+                t[256 * a + 16 * b + c] = (int8_t)map_aa(aa);
+            }
+    return t.data();
+}
+
+void translate(const int8_t* table, const uint8_t* dna, size_t len, int strand, int frame, std::vector<uint8_t>& out) {
+    if (len < (size_t)frame)  // the reference's (len - frame) / 3 wraps and the allocation fails
+        fatal("Unable to re-allocate enough memory: %ld.", (long)(((size_t)len - (size_t)frame) / 3 + 1));
+    const size_t plen = (len - (size_t)frame) / 3;
+    out.resize(plen);
+    for (size_t p = 0; p < plen; ++p) {
+        int c[3];
+        for (int k = 0; k < 3; ++k) {
+            const size_t i = (size_t)frame + 3 * p + (size_t)k;  // position on the strand read
+            c[k] = strand == 0 ? dna[i] : complement_nt(dna[len - 1 - i]);
+        }
+        out[p] = (uint8_t)table[256 * c[0] + 16 * c[1] + c[2]];
+    }
 }
 
 // map_sequence (src/query.c:102-130): unknown symbols are dropped; anything but
@@ -64,27 +135,34 @@ static std::vector<uint8_t> map_query_text(const char* s, size_t len, bool nucle
     return out;
 }
 
-// fill_and_map_query (src/query.c:132-163) for the symbol types the search
-// supports; the translated types are refused when a search starts.
+// fill_and_map_query (src/query.c:132-163): a nucleotide query keeps its
+// strands; a translated one becomes the three frames of each strand selected,
+// with the query genetic code in force when it is read.
 static p_query make_query(const char* text, size_t len) {
     p_query q = new _query();
-    const int st = config().symtype;
+    const Config& cf = config();
+    const int st = cf.symtype;
     const bool nuc = !(st == AMINOACID || st == TRANS_DB);
     std::vector<uint8_t> mapped = map_query_text(text, len, nuc);
     if (st == NUCLEOTIDE) {
         q->nt[0] = mapped;
         q->has_nt[0] = true;
-        if (config().strands & 2) {
+        if (cf.strands & 2) {
             q->nt[1].resize(mapped.size());
             revcomp(mapped.data(), mapped.size(), q->nt[1].data());
             q->has_nt[1] = true;
         }
-    } else if (st == AMINOACID || st == TRANS_DB) {
+    } else if (query_is_translated(st)) {
+        const int8_t* table = translation_table(cf.q_gencode);
+        for (int s = 0; s < 2; ++s)
+            if ((s + 1) & cf.strands)
+                for (int f = 0; f < 3; ++f) {
+                    translate(table, mapped.data(), mapped.size(), s, f, q->aa[3 * s + f]);
+                    q->has_aa[3 * s + f] = true;
+                }
+    } else {
         q->aa[0] = mapped;
         q->has_aa[0] = true;
-    } else {
-        q->nt[0] = mapped;  // kept untranslated
-        q->has_nt[0] = true;
     }
     return q;
 }

@@ -178,23 +178,44 @@ private:
     size_t batches_ = 0, requests_ = 0, max_batch_seen_ = 0;
 };
 
-// GPU work of one request: cells of its first tier, twice for NW starting at
-// 8 bit (its 16-bit pass re-runs nearly every entry); residues = DB residues
-// as entries (both strands counted for nucleotide searches of both strands)
-double request_cells(const SearchRequest& rq, double residues) {
+// lengths of the query buffers a search makes of the query
+// (src/algo/searcher.c:42-90): strands, six or three frames, or one
+std::vector<size_t> query_lengths(p_query q) {
     const Config& cf = config();
-    double qlen = 0;
+    std::vector<size_t> v;
     if (cf.symtype == NUCLEOTIDE) {
         for (int s = 0; s < 2; ++s)
-            if (((s + 1) & cf.strands) && rq.query->has_nt[s]) qlen += (double)rq.query->nt[s].size();
+            if (((s + 1) & cf.strands) && q->has_nt[s]) v.push_back(q->nt[s].size());
+    } else if (query_is_translated(cf.symtype)) {
+        for (int s = 0; s < 2; ++s)
+            if ((s + 1) & cf.strands)
+                for (int f = 0; f < 3; ++f) v.push_back(q->aa[3 * s + f].size());
     } else {
-        qlen = (double)rq.query->aa[0].size();
+        v.push_back(q->aa[0].size());
     }
+    return v;
+}
+
+// search entries per DB sequence, and their residues per DB residue
+// (src/db_adapter.c:124-138): 2 and 2 for nucleotides of both strands, 6 or 3
+// frames of a third of the length for translated DBs, else 1 and 1
+std::pair<double, double> entry_factors() {
+    const Config& cf = config();
+    if (cf.symtype == NUCLEOTIDE && (cf.strands & 2)) return {2.0, 2.0};
+    if (db_is_translated(cf.symtype)) return cf.strands == BOTH_STRANDS ? std::make_pair(6.0, 2.0) : std::make_pair(3.0, 1.0);
+    return {1.0, 1.0};
+}
+
+// GPU work of one request: cells of its first tier, twice for NW starting at
+// 8 bit (its 16-bit pass re-runs nearly every entry); residues = residues of
+// the DB's search entries
+double request_cells(const SearchRequest& rq, double residues) {
+    double qlen = 0;
+    for (size_t l : query_lengths(rq.query)) qlen += (double)l;
     return qlen * residues * ((rq.type == kNW && rq.bit_width == BIT_WIDTH_8) ? 2.0 : 1.0);
 }
 
 double db_entry_residues() {
-    const Config& cf = config();
     double res = 0;
     const size_t n = ssa_db_get_sequence_count();
     for (size_t i = 0; i < n; ++i) {
@@ -202,26 +223,18 @@ double db_entry_residues() {
         if (!s) break;
         res += (double)s->seqlen;
     }
-    return res * ((cf.symtype == NUCLEOTIDE && (cf.strands & 2)) ? 2 : 1);
+    return res * entry_factors().second;
 }
 
 // device bytes one request needs in a batch: pair results and pass boundaries
 // of its first tier, and the traceback matrices of its hits
 size_t request_footprint(const SearchRequest& rq) {
-    const Config& cf = config();
     const size_t n = ssa_db_get_sequence_count();
-    const size_t entries = n * ((cf.symtype == NUCLEOTIDE && (cf.strands & 2)) ? 2 : 1);
-    size_t qlen = 0, nq = 1;
-    if (cf.symtype == NUCLEOTIDE) {
-        nq = 0;
-        for (int s = 0; s < 2; ++s)
-            if (((s + 1) & cf.strands) && rq.query->has_nt[s]) {
-                ++nq;
-                qlen = std::max(qlen, rq.query->nt[s].size());
-            }
-    } else {
-        qlen = rq.query->aa[0].size();
-    }
+    const size_t entries = (size_t)((double)n * entry_factors().first);
+    const std::vector<size_t> ql = query_lengths(rq.query);
+    const size_t nq = ql.size();
+    size_t qlen = 0;
+    for (size_t l : ql) qlen = std::max(qlen, l);
     const size_t mean_len = 400;  // a planning figure, not a limit
     size_t bytes = nq * entries * sizeof(PairResult) + nq * entries * (mean_len / 4) * 16;
     if (rq.align_type != COMPUTE_SCORE) bytes += rq.hitcount * qlen * mean_len * 4;

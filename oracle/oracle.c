@@ -15,7 +15,11 @@
  *                       overflow counts of tests/algo/8/test_8_simd_*.c.
  *   orc_heap_*          pinned: tests/util/test_minheap.c:33-90, oracle/_ref heap.
  *   orc_search          pinned: tests/algo/test_searcher.c, tests/test_bigger_databases.c:70-92,
- *                       tests/algo/64/test_search_64.c, oracle/_ref driver.
+ *                       tests/algo/64/test_search_64.c, oracle/_ref driver; translated
+ *                       queries by the TRANS_QUERY goldens of tests/algo/test_searcher.c:248-330.
+ *   orc_translate*      pinned: every table entry of every genetic code and random
+ *                       sequences against oracle/_ref (the reference's util_sequence.c);
+ *                       textbook codons and IUPAC ambiguity cases.
  *   orc_region/cigar    pinned: tests/algo/test_cigar.c, test_align.c, test_aligner.c,
  *                       tests/test_libssa.c:48-93 CIGAR strings, oracle/_ref.
  */
@@ -121,10 +125,18 @@ int orc_complement(int code) {
     return ((code & 1) << 3) | ((code & 2) << 1) | ((code & 4) >> 1) | ((code & 8) >> 3);
 }
 
+/* DB sequences are nucleotides for NUCLEOTIDE, TRANS_DB and TRANS_BOTH
+ * (src/db_adapter.c:47-110); queries for NUCLEOTIDE, TRANS_QUERY and
+ * TRANS_BOTH (src/query.c:48-55). */
+static int db_is_nt(int symtype) {
+    return symtype == ORC_NUCLEOTIDE || symtype == ORC_TRANS_DB || symtype == ORC_TRANS_BOTH;
+}
+static int query_is_nt(int symtype) { return symtype != ORC_AMINOACID && symtype != ORC_TRANS_DB; }
+
 /* src/util/util_sequence.c:298-318 */
 void orc_map_db(int symtype, const char* raw, size_t len, uint8_t* out) {
     for (size_t i = 0; i < len; i++) {
-        int v = symtype == ORC_NUCLEOTIDE ? orc_map_nt((unsigned char)raw[i]) : orc_map_aa((unsigned char)raw[i]);
+        int v = db_is_nt(symtype) ? orc_map_nt((unsigned char)raw[i]) : orc_map_aa((unsigned char)raw[i]);
         out[i] = (uint8_t)(v < 0 ? 0 : v);
     }
 }
@@ -133,7 +145,7 @@ void orc_map_db(int symtype, const char* raw, size_t len, uint8_t* out) {
 size_t orc_map_query(int symtype, const char* raw, size_t len, uint8_t* out) {
     size_t k = 0;
     for (size_t i = 0; i < len; i++) {
-        int v = symtype == ORC_NUCLEOTIDE ? orc_map_nt((unsigned char)raw[i]) : orc_map_aa((unsigned char)raw[i]);
+        int v = query_is_nt(symtype) ? orc_map_nt((unsigned char)raw[i]) : orc_map_aa((unsigned char)raw[i]);
         if (v >= 0) out[k++] = (uint8_t)v;
     }
     return k;
@@ -142,6 +154,87 @@ size_t orc_map_query(int symtype, const char* raw, size_t len, uint8_t* out) {
 /* src/util/util_sequence.c:427-437 */
 void orc_revcomp(const uint8_t* in, size_t len, uint8_t* out) {
     for (size_t i = 0; i < len; i++) out[i] = (uint8_t)orc_complement(in[len - 1 - i]);
+}
+
+/* ------------------------------------------------------------------------ */
+/* translation                                                               */
+/* ------------------------------------------------------------------------ */
+
+/* The NCBI genetic codes the reference names (src/util/util_sequence.c:95-123),
+ * each as the 64 amino acids of the codons in TCAG order with the first base
+ * slowest (TTT TTC TTA TTG TCT ... GGG), from NCBI's published translation
+ * tables.  Index = code number - 1; NULL where NCBI has no such code. */
+static const char* const kGencodes[23] = {
+    /*  1 */ "FFLLSSSSYY**CC*WLLLLPPPPHHQQRRRRIIIMTTTTNNKKSSRRVVVVAAAADDEEGGGG",
+    /*  2 */ "FFLLSSSSYY**CCWWLLLLPPPPHHQQRRRRIIMMTTTTNNKKSS**VVVVAAAADDEEGGGG",
+    /*  3 */ "FFLLSSSSYY**CCWWTTTTPPPPHHQQRRRRIIMMTTTTNNKKSSRRVVVVAAAADDEEGGGG",
+    /*  4 */ "FFLLSSSSYY**CCWWLLLLPPPPHHQQRRRRIIIMTTTTNNKKSSRRVVVVAAAADDEEGGGG",
+    /*  5 */ "FFLLSSSSYY**CCWWLLLLPPPPHHQQRRRRIIMMTTTTNNKKSSSSVVVVAAAADDEEGGGG",
+    /*  6 */ "FFLLSSSSYYQQCC*WLLLLPPPPHHQQRRRRIIIMTTTTNNKKSSRRVVVVAAAADDEEGGGG",
+    /*  7 */ NULL,
+    /*  8 */ NULL,
+    /*  9 */ "FFLLSSSSYY**CCWWLLLLPPPPHHQQRRRRIIIMTTTTNNNKSSSSVVVVAAAADDEEGGGG",
+    /* 10 */ "FFLLSSSSYY**CCCWLLLLPPPPHHQQRRRRIIIMTTTTNNKKSSRRVVVVAAAADDEEGGGG",
+    /* 11 */ "FFLLSSSSYY**CC*WLLLLPPPPHHQQRRRRIIIMTTTTNNKKSSRRVVVVAAAADDEEGGGG",
+    /* 12 */ "FFLLSSSSYY**CC*WLLLSPPPPHHQQRRRRIIIMTTTTNNKKSSRRVVVVAAAADDEEGGGG",
+    /* 13 */ "FFLLSSSSYY**CCWWLLLLPPPPHHQQRRRRIIMMTTTTNNKKSSGGVVVVAAAADDEEGGGG",
+    /* 14 */ "FFLLSSSSYYY*CCWWLLLLPPPPHHQQRRRRIIIMTTTTNNNKSSSSVVVVAAAADDEEGGGG",
+    /* 15 */ "FFLLSSSSYY*QCC*WLLLLPPPPHHQQRRRRIIIMTTTTNNKKSSRRVVVVAAAADDEEGGGG",
+    /* 16 */ "FFLLSSSSYY*LCC*WLLLLPPPPHHQQRRRRIIIMTTTTNNKKSSRRVVVVAAAADDEEGGGG",
+    /* 17 */ NULL,
+    /* 18 */ NULL,
+    /* 19 */ NULL,
+    /* 20 */ NULL,
+    /* 21 */ "FFLLSSSSYY**CCWWLLLLPPPPHHQQRRRRIIMMTTTTNNNKSSSSVVVVAAAADDEEGGGG",
+    /* 22 */ "FFLLSS*SYY*LCC*WLLLLPPPPHHQQRRRRIIIMTTTTNNKKSSRRVVVVAAAADDEEGGGG",
+    /* 23 */ "FF*LSSSSYY**CC*WLLLLPPPPHHQQRRRRIIIMTTTTNNKKSSRRVVVVAAAADDEEGGGG",
+};
+
+int orc_gencode_valid(int gencode) { return gencode >= 1 && gencode <= 23 && kGencodes[gencode - 1] != NULL; }
+
+/* src/util/util_sequence.c:200-253.  The nt16 bit of a base: A=1, C=2, G=4,
+ * T=8; its TCAG position: T=0, C=1, A=2, G=3. */
+void orc_translate_table(int gencode, int8_t* table) {
+    static const int tcag[4] = {2, 1, 3, 0}; /* bit index 0..3 (A, C, G, T) -> TCAG position */
+    const char* code = kGencodes[gencode - 1];
+    for (int a = 0; a < 16; a++)
+        for (int b = 0; b < 16; b++)
+            for (int c = 0; c < 16; c++) {
+                /* the set of amino acids over every codon the three base sets allow */
+                int seen[128] = {0};
+                int nseen = 0;
+                for (int i = 0; i < 4; i++)
+                    for (int j = 0; j < 4; j++)
+                        for (int k = 0; k < 4; k++)
+                            if ((a >> i & 1) && (b >> j & 1) && (c >> k & 1)) {
+                                int x = (unsigned char)code[16 * tcag[i] + 4 * tcag[j] + tcag[k]];
+                                if (!seen[x]) { seen[x] = 1; nseen++; }
+                            }
+                char aa = 'X';
+                if (nseen == 1) {
+                    for (int x = 0; x < 128; x++) if (seen[x]) aa = (char)x;
+                } else if (nseen == 2 && seen['D'] && seen['N']) {
+                    aa = 'B';
+                } else if (nseen == 2 && seen['E'] && seen['Q']) {
+                    aa = 'Z';
+                }
+                table[256 * a + 16 * b + c] = (int8_t)orc_map_aa(aa);
+            }
+}
+
+/* src/util/util_sequence.c:312-353 */
+size_t orc_translate(const int8_t* table, const uint8_t* dna, size_t len, int strand, int frame, uint8_t* out) {
+    if (len < (size_t)frame) return (size_t)-1;
+    size_t plen = (len - (size_t)frame) / 3;
+    for (size_t p = 0; p < plen; p++) {
+        int c[3];
+        for (int k = 0; k < 3; k++) {
+            size_t i = (size_t)frame + 3 * p + (size_t)k; /* position on the strand read */
+            c[k] = strand == 0 ? dna[i] : orc_complement(dna[len - 1 - i]);
+        }
+        out[p] = (uint8_t)table[256 * c[0] + 16 * c[1] + c[2]];
+    }
+    return plen;
 }
 
 /* ------------------------------------------------------------------------ */
@@ -581,7 +674,11 @@ size_t orc_search(const orc_search_params* p, orc_elem* out, orc_stats* st) {
     memset(st, 0, sizeof *st);
 
     /* DB entries as the adapter makes them (src/db_adapter.c:47-110, 212-239) */
+    const int trans_db = p->symtype == ORC_TRANS_DB || p->symtype == ORC_TRANS_BOTH;
     int per_seq = (p->symtype == ORC_NUCLEOTIDE && (p->strands & 2)) ? 2 : 1;
+    if (trans_db) per_seq = p->strands == 3 ? 6 : 3;
+    int8_t dtab[4096];
+    if (trans_db) orc_translate_table(p->d_gencode, dtab);
     entry_t* entries = (entry_t*)calloc(p->db_count * per_seq + 1, sizeof(entry_t));
     size_t* first_entry = (size_t*)calloc(p->db_count + 1, sizeof(size_t));
     size_t ne = 0;
@@ -589,6 +686,32 @@ size_t orc_search(const orc_search_params* p, orc_elem* out, orc_stats* st) {
         first_entry[id] = ne;
         size_t len = p->db_len[id];
         if (len == 0) continue;
+        if (trans_db) {
+            /* set_translated_sequences (src/db_adapter.c:72-104): frames 0..2 of
+             * both strands (strand field 0/1), or of the one strand selected,
+             * whose strand field is then the strand setting itself (1 or 2) */
+            uint8_t* nt = (uint8_t*)malloc(len + 1);
+            orc_map_db(p->symtype, p->db_raw[id], len, nt);
+            int bad = 0;
+            for (int k = 0; k < per_seq; k++) {
+                int s = per_seq == 6 ? k / 3 : p->strands - 1, f = k % 3;
+                entry_t* e = &entries[ne++];
+                e->id = id;
+                e->strand = (uint8_t)(per_seq == 6 ? s : p->strands);
+                e->frame = (uint8_t)f;
+                e->seq = (uint8_t*)malloc(len / 3 + 1);
+                e->len = orc_translate(dtab, nt, len, s, f, e->seq);
+                if (e->len == (size_t)-1) { e->len = 0; bad = 1; }
+            }
+            free(nt);
+            if (bad) {
+                for (size_t i = 0; i < ne; i++) free(entries[i].seq);
+                free(entries);
+                free(first_entry);
+                return (size_t)-1;
+            }
+            continue;
+        }
         entry_t* e = &entries[ne++];
         e->id = id;
         e->len = len;

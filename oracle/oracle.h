@@ -31,6 +31,9 @@ extern "C" {
 /* symbol types (src/libssa.h:44-48) */
 #define ORC_NUCLEOTIDE 0
 #define ORC_AMINOACID 1
+#define ORC_TRANS_QUERY 2
+#define ORC_TRANS_DB 3
+#define ORC_TRANS_BOTH 4
 
 typedef struct {
     int64_t s[ORC_DIM * ORC_DIM]; /* [db_code * 32 + query_code], as SCORE_MATRIX_64 */
@@ -51,6 +54,22 @@ void orc_map_db(int symtype, const char* raw, size_t len, uint8_t* out);
 /* Query mapping: unknown symbols removed (src/query.c:102-130). Returns new length. */
 size_t orc_map_query(int symtype, const char* raw, size_t len, uint8_t* out);
 void orc_revcomp(const uint8_t* in, size_t len, uint8_t* out);
+
+/* ---- translation (src/util/util_sequence.c:190-353) ---- */
+/* 1 if the NCBI genetic code number is one the reference names
+ * (src/util/util_sequence.c:95-123, checked by src/libssa.c:139-145). */
+int orc_gencode_valid(int gencode);
+/* table[256*a + 16*b + c] = amino-acid code of the nt16 codon (a, b, c); each
+ * nt16 code is an IUPAC base set, and a codon of several bases translates to
+ * the one amino acid all its codons give, B for a mix of D and N, Z for a mix
+ * of E and Q, X otherwise (init_translate_table, :200-253). */
+void orc_translate_table(int gencode, int8_t* table4096);
+/* us_translate_sequence (:312-353): the protein of strand 0 (forward) or, for
+ * any other strand value, the reverse complement, read from frame 0..2.
+ * Writes (len - frame) / 3 codes and returns that length; returns (size_t)-1
+ * when len < frame (the reference then fails to allocate and exits). */
+size_t orc_translate(const int8_t* table, const uint8_t* dna, size_t len, int strand, int frame,
+                     uint8_t* out);
 
 /* ---- 64-bit tier (src/algo/64/smith_waterman_63.c:32-98, needleman_wunsch_64.c:32-98) ---- */
 int64_t orc_full_sw(const uint8_t* d, size_t dlen, const uint8_t* q, size_t qlen,
@@ -114,6 +133,8 @@ typedef struct {
     size_t db_count;
     const char* const* db_raw;
     const size_t* db_len;
+    /* TRANS_DB / TRANS_BOTH: genetic code of the DB sequences */
+    int d_gencode;
 } orc_search_params;
 
 typedef struct {
@@ -122,7 +143,8 @@ typedef struct {
 } orc_stats;
 
 /* Returns the number of hits written to out (<= hit_count), sorted as the
- * reference's final list. */
+ * reference's final list, or (size_t)-1 when a DB sequence is shorter than a
+ * frame it is translated in (the reference exits there). */
 size_t orc_search(const orc_search_params* p, orc_elem* out, orc_stats* stats);
 
 /* ---- region, directions, CIGAR (src/algo/align.c:39-150, src/algo/cigar.c:48-346) ---- */

@@ -85,6 +85,18 @@ void ref_revcomp(const char* in, size_t len, char* out) {
     us_revcompl(a, b);
 }
 
+/* ---- translation: the reference's tables and us_translate_sequence ---- */
+void ref_translate(int db, int q_gencode, int d_gencode, const char* dna, size_t len, int strand, int frame,
+                   char* out, size_t* out_len) {
+    us_init_translation(q_gencode, d_gencode);
+    sequence_t d = { (char*)dna, len };
+    sequence_t prot = { NULL, 0 };
+    us_translate_sequence(db, d, strand, frame, &prot);
+    memcpy(out, prot.seq, prot.len);
+    *out_len = prot.len;
+    free(prot.seq);
+}
+
 /* ---- 64-bit ---- */
 int64_t ref_full(int type, const char* d, size_t dlen, const char* q, size_t qlen) {
     sequence_t ds = { (char*)d, dlen };

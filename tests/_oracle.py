@@ -13,7 +13,7 @@ ORACLE_SO = os.path.join(ROOT, "oracle", "liboracle.so")
 REF_SO = os.path.join(ROOT, "oracle", "_ref", "libssaref.so")
 
 SW, NW = 0, 1
-NUCLEOTIDE, AMINOACID = 0, 1
+NUCLEOTIDE, AMINOACID, TRANS_QUERY, TRANS_DB, TRANS_BOTH = 0, 1, 2, 3, 4
 
 
 def _ensure_oracle():
@@ -49,7 +49,7 @@ class Params(C.Structure):
                 ("matrix", C.POINTER(Matrix)), ("q_count", C.c_int),
                 ("q_seq", C.c_void_p * 6), ("q_len", C.c_size_t * 6),
                 ("db_count", C.c_size_t), ("db_raw", C.POINTER(C.c_char_p)),
-                ("db_len", C.POINTER(C.c_size_t))]
+                ("db_len", C.POINTER(C.c_size_t)), ("d_gencode", C.c_int)]
 
 
 class Stats(C.Structure):
@@ -77,6 +77,9 @@ _sig(lib, "orc_map_nt", _I, _I)
 _sig(lib, "orc_map_db", None, _I, C.c_char_p, _S, _P)
 _sig(lib, "orc_map_query", _S, _I, C.c_char_p, _S, _P)
 _sig(lib, "orc_revcomp", None, C.c_char_p, _S, _P)
+_sig(lib, "orc_gencode_valid", _I, _I)
+_sig(lib, "orc_translate_table", None, _I, _P)
+_sig(lib, "orc_translate", _S, _P, C.c_char_p, _S, _I, _I, _P)
 _sig(lib, "orc_full_sw", C.c_int64, C.c_char_p, _S, C.c_char_p, _S, _P, _I, _I)
 _sig(lib, "orc_full_nw", C.c_int64, C.c_char_p, _S, C.c_char_p, _S, _P, _I, _I)
 _sig(lib, "orc_lane_kernel", _S, _I, _I, _I, _P, _P, _S, C.c_char_p, _S, _P, _I, _I, _P, _P)
@@ -147,12 +150,31 @@ def revcomp(codes):
     return bytes(out[:len(codes)])
 
 
-def query_buffers(symtype, strands, raw):
+def translate_table(gencode):
+    t = (C.c_int8 * 4096)()
+    lib.orc_translate_table(gencode, t)
+    return t
+
+
+def translate(table, dna_codes, strand, frame):
+    """Protein codes of one strand and frame of mapped nucleotide codes, or None
+    where the sequence is shorter than the frame (the reference exits there)."""
+    out = (C.c_uint8 * (len(dna_codes) // 3 + 1))()
+    n = lib.orc_translate(table, dna_codes, len(dna_codes), strand, frame, out)
+    if n == 2 ** 64 - 1:
+        return None
+    return bytes(out[:n])
+
+
+def query_buffers(symtype, strands, raw, q_gencode=1):
     """The query buffers s_create_searchdata builds (src/algo/searcher.c:42-90)."""
     q = map_query(symtype, raw)
     if symtype == NUCLEOTIDE:
         nt = [q, revcomp(q)]
         return [nt[s] for s in range(2) if (s + 1) & strands]
+    if symtype in (TRANS_QUERY, TRANS_BOTH):
+        t = translate_table(q_gencode)
+        return [translate(t, q, s, f) for s in range(2) if (s + 1) & strands for f in range(3)]
     return [q]
 
 
@@ -184,9 +206,11 @@ def heap_run(k, elems):
     return [(store[i].score, store[i].db_id, store[i].query_id) for i in range(h.count)]
 
 
-def search(type_, bit_width, symtype, strands, queries, db, k, m, go, ge, chunk=1000, lanes8=32, lanes16=16):
+def search(type_, bit_width, symtype, strands, queries, db, k, m, go, ge, chunk=1000, lanes8=32, lanes16=16,
+           d_gencode=1, frames=False):
     """queries: mapped query buffers; db: list of raw DB strings in ID order.
-    Returns ([(score, db_id, strand, query_id)], stats)."""
+    Returns ([(score, db_id, strand, query_id)], stats), or with frames=True
+    ([(score, db_id, strand, query_id, frame)], stats)."""
     p = Params()
     p.type, p.bit_width, p.symtype, p.strands = type_, bit_width, symtype, strands
     p.lanes8, p.lanes16, p.chunk_size, p.hit_count = lanes8, lanes16, chunk, k
@@ -205,7 +229,13 @@ def search(type_, bit_width, symtype, strands, queries, db, k, m, go, ge, chunk=
     p.db_len = (C.c_size_t * max(len(raw), 1))(*[len(s) for s in raw])
     out = (Elem * max(k, 1))()
     st = Stats()
+    p.d_gencode = d_gencode
     n = lib.orc_search(C.byref(p), out, C.byref(st))
+    if n == 2 ** 64 - 1:
+        raise ValueError("a DB sequence is shorter than a frame it is translated in")
+    if frames:
+        return [(out[i].score, out[i].db_id, out[i].db_strand, out[i].query_id, out[i].db_frame)
+                for i in range(n)], st
     hits = [(out[i].score, out[i].db_id, out[i].db_strand, out[i].query_id) for i in range(n)]
     return hits, st
 
@@ -242,7 +272,15 @@ def ref_lib():
     _sig(r, "ref_lane_kernel", _S, _I, _I, _I, _P, _P, _S, C.c_char_p, _S, _P, _P, _P, _P)
     _sig(r, "ref_heap_run", _S, _S, _P, _P, _P, _S, _P, _P, _P)
     _sig(r, "ref_align", _I, _I, C.c_char_p, _S, C.c_char_p, _S, _P, _P, _S)
+    _sig(r, "ref_translate", None, _I, _I, _I, C.c_char_p, _S, _I, _I, _P, _P)
     return r
+
+
+def ref_translate(r, db, q_gencode, d_gencode, dna_codes, strand, frame):
+    out = (C.c_char * (len(dna_codes) // 3 + 1))()
+    n = C.c_size_t()
+    r.ref_translate(db, q_gencode, d_gencode, dna_codes, len(dna_codes), strand, frame, out, C.byref(n))
+    return bytes(out[:n.value])
 
 
 def ref_lane_kernel(r, type_, width, avx2, seqs, q):

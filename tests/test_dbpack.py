@@ -70,6 +70,7 @@ def unknown_count(symtype, s):
 
 
 def setup(symtype, strands, tmp_path, seqs):
+    ssa.reset_settings()
     ssa.set_output_mode(ssa.OUTPUT_ERROR)
     ssa.init_symbol_translation(symtype, strands, 1, 1)
     fas = str(tmp_path / "db.fas")
@@ -185,12 +186,25 @@ def test_corrupt_packed_files_are_rejected(tmp_path):
     ssa.set_output_mode(ssa.OUTPUT_ERROR)
 
 
-def test_pack_refuses_translated_types(tmp_path):
-    setup(O.AMINOACID, 1, tmp_path, ["ACDE", "KLMN"])
+@pytest.mark.parametrize("symtype", [O.TRANS_DB, O.TRANS_BOTH])
+def test_pack_refuses_translated_db(tmp_path, symtype):
+    setup(O.AMINOACID, 1, tmp_path, ["ACGTAC", "GGGTTT"])
     ssa.set_output_mode(ssa.OUTPUT_SILENT)
-    ssa.init_symbol_translation(ssa.TRANS_QUERY, 3, 1, 1)
+    ssa.init_symbol_translation(symtype, 3, 1, 1)
     with pytest.raises(RuntimeError):
         ssa.pack_db(str(tmp_path / "t.ssapack"))
-    ssa.init_symbol_translation(ssa.AMINOACID, 1, 1, 1)
+    ssa.reset_settings()  # init_symbol_translation exits once TRANS_DB / TRANS_BOTH is set
     ssa.set_output_mode(ssa.OUTPUT_ERROR)
     assert not os.path.exists(str(tmp_path / "t.ssapack"))
+
+
+def test_pack_translated_query_db_is_amino_acid(tmp_path):
+    # a translated query searches the amino-acid DB: its pack is that of AMINOACID
+    seqs = ["ACDEKLMN", "MKV*X", "WWYY"]
+    setup(O.AMINOACID, 1, tmp_path, seqs)
+    ssa.pack_db(str(tmp_path / "a.ssapack"))
+    ssa.init_symbol_translation(ssa.TRANS_QUERY, 3, 1, 1)
+    ssa.pack_db(str(tmp_path / "t.ssapack"))
+    ssa.init_symbol_translation(ssa.AMINOACID, 1, 1, 1)
+    assert open(str(tmp_path / "t.ssapack"), "rb").read() == open(str(tmp_path / "a.ssapack"), "rb").read()
+    assert parse(str(tmp_path / "t.ssapack"))[1]["symtype"] == O.AMINOACID

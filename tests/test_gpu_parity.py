@@ -30,8 +30,8 @@ def write_fasta(path, seqs):
 
 
 def configure(symtype, strands, gaps, matrix, isa="avx2", chunk=1000):
+    ssa.reset_settings()  # init_symbol_translation tests the type in force (src/libssa.c:146-151)
     ssa.set_output_mode(ssa.OUTPUT_ERROR)
-    ssa.init_symbol_translation(ssa.AMINOACID, ssa.FORWARD_STRAND, 1, 1)  # reset to a checkable state first
     ssa.init_symbol_translation(symtype, strands, 1, 1)
     ssa.set_simd_compute_mode(LANES[isa][2])
     ssa.set_chunk_size(chunk)

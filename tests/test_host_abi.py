@@ -101,6 +101,7 @@ def test_fatal_paths_exit_with_status_1():
 
 def test_query_reading():
     # tests/test_query.c:210-235: header and mapped length; unknown symbols dropped
+    ssa.reset_settings()
     ssa.init_symbol_translation(ssa.AMINOACID, ssa.FORWARD_STRAND, 1, 1)
     q = ssa.init_sequence_fasta(ssa.READ_FROM_FILE, O.data("NP_009305.1.fas"))
     assert q
