@@ -3,6 +3,7 @@
 
 #include <cstddef>
 #include <cstdint>
+#include <functional>
 #include <string>
 #include <vector>
 
@@ -40,6 +41,14 @@ struct Config {
     int q_gencode = 1, d_gencode = 1;
 };
 Config& config();
+
+// Runs fn(i) for every i in [0, n) on the library's worker threads and the
+// calling thread, and returns when all have run.  The workers are started
+// once and shared by concurrent callers; the caller works through its own
+// items too, so a call always finishes even when every worker is busy.
+void parallel_for(size_t n, const std::function<void(size_t)>& fn);
+// threads a parallel_for spreads over (workers + caller, at most 16)
+size_t parallel_width();
 
 // lanes of the reference's 8-bit and 16-bit kernels for the configured ISA
 // (src/algo/8/search_8_util.h:26-35, src/algo/16/search_16_util.h:27-37)

@@ -89,7 +89,7 @@ private:
 };
 
 // ---- kernel launchers (score_kernels.hip, traceback_kernels.hip, pack_kernels.hip) ----
-bool score_shape_supported(int lanes, int rows);
+bool score_shape_supported(int lanes, int rows, int mode);
 int score_resident_workgroups(int lanes, int rows, int mode);  // persistent grid size of a shape
 hipError_t launch_score(int lanes, int rows, int mode, hipStream_t st, const ScoreLaunch& L, const ScoreParams& P,
                         int max_wg);

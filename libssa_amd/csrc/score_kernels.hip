@@ -91,26 +91,27 @@ struct Log2<1> {
 };
 
 // Part q of the profile scores of one lane for one DB code: rows 8q .. 8q+7
-// (R >= 8) or rows 0 .. 3 (R = 4), two rows per dword.
-template <int R, int Q>
-__device__ __forceinline__ void read_part(const char* p, uint32_t* d) {
+// (R >= 8) or rows 0 .. 3 (R = 4), two rows per dword.  q is a constant once
+// the callers' loops are unrolled.
+template <int R>
+__device__ __forceinline__ void read_part(const char* p, uint32_t* d, int q) {
     if constexpr (R == 4) {
         const uint2 v = *reinterpret_cast<const uint2*>(p);
         d[0] = v.x;
         d[1] = v.y;
     } else {
-        const uint4 v = *reinterpret_cast<const uint4*>(p + 16 * Q);
-        d[4 * Q + 0] = v.x;
-        d[4 * Q + 1] = v.y;
-        d[4 * Q + 2] = v.z;
-        d[4 * Q + 3] = v.w;
+        const uint4 v = *reinterpret_cast<const uint4*>(p + 16 * q);
+        d[4 * q + 0] = v.x;
+        d[4 * q + 1] = v.y;
+        d[4 * q + 2] = v.z;
+        d[4 * q + 3] = v.w;
     }
 }
 
 template <int R>
 __device__ __forceinline__ void read_rows(const char* p, uint32_t* d) {
-    read_part<R, 0>(p, d);
-    if constexpr (R == 16) read_part<R, 1>(p, d);
+#pragma unroll
+    for (int q = 0; q < (R >= 8 ? R / 8 : 1); ++q) read_part<R>(p, d, q);
 }
 
 // the packed (lo, hi) score pair of row r from the two halves' profile dwords
@@ -118,11 +119,6 @@ __device__ __forceinline__ s16x2 pair_row(const uint32_t* lo, const uint32_t* hi
     return as_v(__builtin_amdgcn_perm(hi[r / 2], lo[r / 2], (r & 1) ? 0x07060302u : 0x05040100u));
 }
 
-// Pins the instruction order of the cell loop.  gfx950 stalls a packed VALU
-// op that reads the result of the packed op issued just before it (the
-// compiler pads such pairs with s_nop); the cell loop below is written so
-// that every dependent op has an independent one in front of it.
-#define SSA_PIN() __builtin_amdgcn_sched_barrier(0)
 
 // SW: the group's bottom lane adds the maximum over the group's lanes (handed
 // down with the pair's last block); NW: the lane holding the last query row
@@ -304,57 +300,44 @@ __device__ __forceinline__ void run_pass(const ScoreParams& P, uint32_t gbase, c
                 cm_lo = ls_lo ? 0x0000ffffu : 0u;
                 cm_hi = ls_hi ? 0xffff0000u : 0u;
             }
+            // The cell statements are written so that every dependent packed
+            // op has an independent one in front of it (gfx950 needs a wait
+            // state between a packed op and a packed op that reads its
+            // result).  The order is not pinned with scheduling barriers:
+            // instruction selection already reorders pure ops, and left free
+            // the post-RA scheduler removes most of the s_nop padding (what
+            // it leaves is hidden by the other waves of the SIMD).
 #pragma unroll
             for (int c = 0; c < 4; ++c) {
                 s16x2 f = as_v(tbf[c]);
                 s16x2 fR = adds(f, RR);
-                SSA_PIN();
                 s16x2 v = pair_row(pl, ph, 0);
-                SSA_PIN();
                 s16x2 h1 = adds(htop, v);  // row 0: diagonal H from the row above
-                SSA_PIN();
                 htop = as_v(tbh[c]);
                 s16x2 hq = Z;
 #pragma unroll
                 for (int r = 0; r < R; ++r) {
                     const bool more = r + 1 < R;
                     if (more) v = pair_row(pl, ph, r + 1);
-                    SSA_PIN();
                     // the next column's profile rows, once this column's dwords are consumed
-                    if (c < 3) {
-                        if ((R == 4 && r == 3) || (R >= 8 && r == 7)) {
-                            read_part<R, 0>(sp + colofs(w.x, c + 1), pl);
-                            read_part<R, 0>(sp + colofs(w.y, c + 1), ph);
-                        }
-                        if (R == 16 && r == 15) {
-                            read_part<R, 1>(sp + colofs(w.x, c + 1), pl);
-                            read_part<R, 1>(sp + colofs(w.y, c + 1), ph);
-                        }
+                    if (c < 3 && ((R == 4 && r == 3) || (R >= 8 && r % 8 == 7))) {
+                        read_part<R>(sp + colofs(w.x, c + 1), pl, r / 8);
+                        read_part<R>(sp + colofs(w.y, c + 1), ph, r / 8);
                     }
                     s16x2 h = vmax(h1, f);
-                    SSA_PIN();
                     const s16x2 eR = adds(El[r], RR);
-                    SSA_PIN();
                     h = vmax(h, El[r]);
-                    SSA_PIN();
                     if (more) h1 = adds(Hl[r], v);  // next row's diagonal term
-                    SSA_PIN();
                     const s16x2 tt = adds(h, QR);
-                    SSA_PIN();
                     if (MODE == 0) {
                         S = vmax(S, h);
                     } else {
                         hmn = vmin(hmn, h);
-                        SSA_PIN();
                         hmx = vmax(hmx, h);
                     }
-                    SSA_PIN();
                     f = vmax(fR, tt);
-                    SSA_PIN();
                     El[r] = vmax(eR, tt);
-                    SSA_PIN();
                     if (more) fR = adds(f, RR);
-                    SSA_PIN();
                     Hl[r] = h;
                     if (CAP && r == last_r) hq = h;
                 }
@@ -645,16 +628,32 @@ static int resident_wg_t() {
     return cap;
 }
 
-bool score_shape_supported(int lanes, int rows) {
-    return (lanes == 8 || lanes == 16 || lanes == 32 || lanes == 64) && (rows == 4 || rows == 8 || rows == 16);
+// 32 rows per lane: SW only (NW holds more state per row and would run one
+// wave per SIMD), and not at 64 lanes per group (128 KiB of profile)
+bool score_shape_supported(int lanes, int rows, int mode) {
+    return (lanes == 8 || lanes == 16 || lanes == 32 || lanes == 64) &&
+           (rows == 4 || rows == 8 || rows == 16 || (rows == 32 && lanes <= 32 && mode == 0));
 }
 
-#define SSA_SHAPES(X) \
-    X(8, 4) X(8, 8) X(8, 16) X(16, 4) X(16, 8) X(16, 16) X(32, 4) X(32, 8) X(32, 16) X(64, 4) X(64, 8) X(64, 16)
+template <int R, int G, int MODE>
+static int resident_wg_shape() {
+    if constexpr (R == 32 && MODE == 1) return 0;
+    else return resident_wg_t<R, G, MODE>();
+}
+
+template <int R, int G, int MODE>
+static hipError_t launch_shape(hipStream_t st, const ScoreLaunch& L, const ScoreParams& P, int max_wg) {
+    if constexpr (R == 32 && MODE == 1) return hipErrorInvalidValue;
+    else return launch_score_t<R, G, MODE>(st, L, P, max_wg);
+}
+
+#define SSA_SHAPES(X)                                                                                        \
+    X(8, 4) X(8, 8) X(8, 16) X(8, 32) X(16, 4) X(16, 8) X(16, 16) X(16, 32) X(32, 4) X(32, 8) X(32, 16) X(32, 32) \
+    X(64, 4) X(64, 8) X(64, 16)
 
 int score_resident_workgroups(int lanes, int rows, int mode) {
 #define SSA_CASE(GG, RR) \
-    if (lanes == GG && rows == RR) return mode == 0 ? resident_wg_t<RR, GG, 0>() : resident_wg_t<RR, GG, 1>();
+    if (lanes == GG && rows == RR) return mode == 0 ? resident_wg_shape<RR, GG, 0>() : resident_wg_shape<RR, GG, 1>();
     SSA_SHAPES(SSA_CASE)
 #undef SSA_CASE
     return 0;
@@ -665,7 +664,7 @@ hipError_t launch_score(int lanes, int rows, int mode, hipStream_t st, const Sco
     if (L.ntasks <= 0) return hipSuccess;
 #define SSA_CASE(GG, RR)                                                                               \
     if (lanes == GG && rows == RR)                                                                   \
-        return mode == 0 ? launch_score_t<RR, GG, 0>(st, L, P, max_wg) : launch_score_t<RR, GG, 1>(st, L, P, max_wg);
+        return mode == 0 ? launch_shape<RR, GG, 0>(st, L, P, max_wg) : launch_shape<RR, GG, 1>(st, L, P, max_wg);
     SSA_SHAPES(SSA_CASE)
 #undef SSA_CASE
     return hipErrorInvalidValue;

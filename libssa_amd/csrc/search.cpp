@@ -359,9 +359,20 @@ std::mutex& tuned_mutex() {
     return m;
 }
 
-Shape choose_shape(size_t qlen_max, int64_t total_blocks, int64_t max_blocks, int type = -1) {
+// every shape of the score kernel for a search type (0 SW, 1 NW)
+const std::vector<Shape>& kernel_shapes(int type) {
+    static const std::vector<Shape> sw = {{32, 32}, {16, 32}, {8, 32}, {64, 16}, {32, 16}, {16, 16}, {8, 16},
+                                          {64, 8},  {32, 8},  {16, 8}, {8, 8},   {64, 4},  {32, 4},  {16, 4},
+                                          {8, 4}};
+    static const std::vector<Shape> nw = {{64, 16}, {32, 16}, {16, 16}, {8, 16}, {64, 8}, {32, 8},
+                                          {16, 8},  {8, 8},   {64, 4},  {32, 4}, {16, 4}, {8, 4}};
+    return type == 0 ? sw : nw;
+}
+
+Shape choose_shape(size_t qlen_max, int64_t total_blocks, int64_t max_blocks, int type) {
     const Shape ov = shape_override();
-    if (ov.lanes && ov.rows) return ov;
+    // a forced shape of 32 rows per lane runs NW with 16 (the NW kernel has no 32-row shape)
+    if (ov.lanes && ov.rows) return (type == 1 && ov.rows == 32) ? Shape{ov.lanes, 16} : ov;
     if (type >= 0) {
         // measured: each shape's time at the nearest tuned length (log scale),
         // scaled by the passes the shape needs for this length
@@ -393,18 +404,17 @@ Shape choose_shape(size_t qlen_max, int64_t total_blocks, int64_t max_blocks, in
             if (best.lanes) return best;
         }
     }
-    static const Shape kShapes[] = {{64, 16}, {32, 16}, {16, 16}, {8, 16}, {64, 8}, {32, 8},
-                                    {16, 8},  {8, 8},   {64, 4},  {32, 4}, {16, 4}, {8, 4}};
-    Shape best = kShapes[0];
+    Shape best = kernel_shapes(type == 0 ? 0 : 1)[0];
     double best_cost = 1e300;
-    for (const Shape& sh : kShapes) {
+    for (const Shape& sh : kernel_shapes(type == 0 ? 0 : 1)) {
         const int64_t pr = (int64_t)sh.lanes * sh.rows;
         const int64_t passes = std::max<int64_t>(1, ((int64_t)qlen_max + pr - 1) / pr);
-        // measured per-row costs: short lanes pay the fixed per-step work; G = 8
-        // and 32 fix up their top lanes after the DPP shift; G = 64 at R = 16
-        // stages 64 KiB of profile per workgroup (two workgroups per CU)
-        double per_row = sh.rows == 4 ? 1.32 : (sh.rows == 8 ? 1.10 : 1.0);
-        if (sh.lanes == 8 || sh.lanes == 32) per_row *= 1.005;
+        // measured per-row costs: short lanes pay the fixed per-step work (32
+        // rows run at two waves per SIMD, 16 at three); G = 8 and 32 fix up
+        // their top lanes after the DPP shift; G = 64 at R = 16 stages 64 KiB
+        // of profile per workgroup (two workgroups per CU)
+        double per_row = sh.rows == 4 ? 1.32 : (sh.rows == 8 ? 1.10 : (sh.rows == 16 ? 1.0 : 0.975));
+        if (sh.lanes == 8 || sh.lanes == 32) per_row *= 1.04;
         if (sh.lanes == 64 && sh.rows == 16) per_row *= 1.035;
         const double waves = std::max<double>(target_waves(), 1.0);
         const double work =
@@ -633,7 +643,7 @@ void tier_prepare(Exec& ex, TierRun& R) {
         R.sh = choose_shape(qmax, total_blocks, max_blocks, R.type == kSW ? 0 : 1);
     }
     const Shape sh = R.sh;
-    if (!score_shape_supported(sh.lanes, sh.rows)) fatal("libssa_amd: unsupported kernel shape");
+    if (!score_shape_supported(sh.lanes, sh.rows, R.type == kSW ? 0 : 1)) fatal("libssa_amd: unsupported kernel shape");
     const int PR = sh.lanes * sh.rows, ng = 64 / sh.lanes, spw = 2 * ng;
 
     // ---- profiles [pass][code][row], left boundaries, per-query descriptors ----
@@ -1218,14 +1228,8 @@ void job_replay(Job& J, const std::vector<std::pair<uint32_t, uint32_t>>& chunks
             tier64(items);
         }
     };
-    const size_t nthreads = std::max<size_t>(1, std::min<size_t>(16, std::thread::hardware_concurrency()));
-    if (threads && chunks.size() > 8 && nthreads > 1) {
-        std::vector<std::thread> th;
-        for (size_t t = 0; t < nthreads; ++t)
-            th.emplace_back([&, t] {
-                for (size_t c = t; c < chunks.size(); c += nthreads) do_chunk(c);
-            });
-        for (auto& x : th) x.join();
+    if (threads && chunks.size() > 8) {
+        parallel_for(chunks.size(), do_chunk);
     } else {
         for (size_t c = 0; c < chunks.size(); ++c) do_chunk(c);
     }
@@ -1429,7 +1433,7 @@ void run_batch(Exec& ex, const std::vector<SearchRequest>& reqs, std::vector<Sea
     // ---- decode the first tiers; re-queued and exact tiers per search ----
     std::vector<double> ms_re_job(jobs.size(), 0.0);
     std::vector<uint64_t> computed_job(jobs.size(), 0);
-    const size_t nthreads = std::max<size_t>(1, std::min<size_t>(16, std::thread::hardware_concurrency()));
+    const size_t nthreads = parallel_width();
     // host work per search: searches spread over threads, or, for fewer
     // searches than threads, each search's entries spread over threads
     const bool inner = jobs.size() < nthreads;
@@ -1438,23 +1442,16 @@ void run_batch(Exec& ex, const std::vector<SearchRequest>& reqs, std::vector<Sea
             fn(0, N);
             return;
         }
-        std::vector<std::thread> th;
-        for (size_t t = 0; t < nthreads; ++t)
-            th.emplace_back(fn, (uint32_t)((uint64_t)N * t / nthreads), (uint32_t)((uint64_t)N * (t + 1) / nthreads));
-        for (auto& x : th) x.join();
+        parallel_for(nthreads, [&](size_t t) {
+            fn((uint32_t)((uint64_t)N * t / nthreads), (uint32_t)((uint64_t)N * (t + 1) / nthreads));
+        });
     };
     auto parallel_jobs = [&](const std::function<void(size_t)>& fn) {
-        const size_t nt = inner ? 1 : std::min(nthreads, jobs.size());
-        if (nt <= 1) {
+        if (inner) {
             for (size_t j = 0; j < jobs.size(); ++j) fn(j);
             return;
         }
-        std::vector<std::thread> th;
-        for (size_t t = 0; t < nt; ++t)
-            th.emplace_back([&, t] {
-                for (size_t j = t; j < jobs.size(); j += nt) fn(j);
-            });
-        for (auto& x : th) x.join();
+        parallel_for(jobs.size(), fn);
     };
     parallel_jobs([&](size_t j) {
         Job& J = jobs[j];
@@ -1667,13 +1664,7 @@ void run_batch(Exec& ex, const std::vector<SearchRequest>& reqs, std::vector<Sea
 
     // ---- replay: searches in parallel, or chunks in parallel for few searches ----
     if (jobs.size() >= 4 && nthreads > 1) {
-        std::vector<std::thread> th;
-        const size_t nt = std::min(nthreads, jobs.size());
-        for (size_t t = 0; t < nt; ++t)
-            th.emplace_back([&, t] {
-                for (size_t j = t; j < jobs.size(); j += nt) job_replay(jobs[j], chunks, N, false);
-            });
-        for (auto& x : th) x.join();
+        parallel_for(jobs.size(), [&](size_t j) { job_replay(jobs[j], chunks, N, false); });
     } else {
         for (Job& J : jobs) job_replay(J, chunks, N, true);
     }
@@ -1707,8 +1698,7 @@ void run_batch(Exec& ex, const std::vector<SearchRequest>& reqs, std::vector<Sea
 // launch; copies > 1 measures the shape as a batch runs it) and keeps the
 // fastest per (type, length).
 int autotune(int type, const std::vector<size_t>& lengths, int copies, std::vector<AutotuneResult>& out) {
-    static const Shape kShapes[] = {{64, 16}, {32, 16}, {16, 16}, {8, 16}, {64, 8}, {32, 8},
-                                    {16, 8},  {8, 8},   {64, 4},  {32, 4}, {16, 4}, {8, 4}};
+    const std::vector<Shape>& kShapes = kernel_shapes(type);
     Exec& ex = Gpu::get().exec();
     DeviceDB* dbp;
     {

@@ -1,11 +1,107 @@
-// util.cpp -- messages, fatal errors and process-wide configuration.
+// util.cpp -- messages, fatal errors, process-wide configuration and the
+// host worker pool.
+#include <atomic>
+#include <condition_variable>
 #include <cstdarg>
 #include <cstdio>
 #include <cstdlib>
+#include <deque>
+#include <memory>
+#include <mutex>
+#include <thread>
 
 #include "common.h"
 
 namespace ssa {
+
+namespace {
+
+// One parallel_for call: items are claimed by index; the caller waits for done.
+struct WorkBatch {
+    const std::function<void(size_t)>* fn;
+    size_t n;
+    std::atomic<size_t> next{0};
+    std::atomic<size_t> done{0};
+    std::mutex mu;
+    std::condition_variable cv;
+};
+
+class WorkerPool {
+public:
+    static WorkerPool& get() {
+        static WorkerPool* p = new WorkerPool();  // never destroyed: workers may outlive static teardown
+        return *p;
+    }
+    size_t width() const { return threads_.size() + 1; }
+
+    void run(size_t n, const std::function<void(size_t)>& fn) {
+        if (n == 0) return;
+        if (n == 1 || threads_.empty()) {
+            for (size_t i = 0; i < n; ++i) fn(i);
+            return;
+        }
+        auto b = std::make_shared<WorkBatch>();
+        b->fn = &fn;
+        b->n = n;
+        {
+            std::lock_guard<std::mutex> lk(mu_);
+            queue_.push_back(b);
+        }
+        cv_.notify_all();
+        work(*b);  // the caller takes items too
+        std::unique_lock<std::mutex> lk(b->mu);
+        b->cv.wait(lk, [&] { return b->done.load() == b->n; });
+    }
+
+private:
+    WorkerPool() {
+        const size_t hw = std::thread::hardware_concurrency();
+        const size_t n = std::min<size_t>(16, hw ? hw : 1);
+        for (size_t i = 0; i + 1 < n; ++i) threads_.emplace_back([this] { loop(); });
+        for (std::thread& t : threads_) t.detach();
+    }
+
+    // claims and runs items of b until none are left
+    static void work(WorkBatch& b) {
+        for (;;) {
+            const size_t i = b.next.fetch_add(1);
+            if (i >= b.n) return;
+            (*b.fn)(i);
+            if (b.done.fetch_add(1) + 1 == b.n) {
+                std::lock_guard<std::mutex> lk(b.mu);
+                b.cv.notify_all();
+            }
+        }
+    }
+
+    void loop() {
+        for (;;) {
+            std::shared_ptr<WorkBatch> b;
+            {
+                std::unique_lock<std::mutex> lk(mu_);
+                cv_.wait(lk, [&] { return !queue_.empty(); });
+                b = queue_.front();
+                // a batch whose items are all claimed leaves the queue
+                if (b->next.load() >= b->n) {
+                    queue_.pop_front();
+                    continue;
+                }
+            }
+            work(*b);
+        }
+    }
+
+    std::vector<std::thread> threads_;
+    std::mutex mu_;
+    std::condition_variable cv_;
+    std::deque<std::shared_ptr<WorkBatch>> queue_;
+};
+
+}  // namespace
+
+void parallel_for(size_t n, const std::function<void(size_t)>& fn) { WorkerPool::get().run(n, fn); }
+
+size_t parallel_width() { return WorkerPool::get().width(); }
 
 Config& config() {
     static Config c;

@@ -218,7 +218,8 @@ def test_random_parity(seed):
     check(t, bw, symtype, strands, qraw, db, k, matrix, gaps, isa=isa, chunk=chunk, align=(seed % 4 == 1))
 
 
-@pytest.mark.parametrize("shape", [(8, 4), (8, 16), (16, 8), (32, 4), (32, 16), (64, 4), (64, 8), (64, 16)])
+@pytest.mark.parametrize("shape", [(8, 4), (8, 16), (16, 8), (32, 4), (32, 16), (64, 4), (64, 8), (64, 16),
+                                   (8, 32), (16, 32), (32, 32)])
 def test_kernel_shapes_do_not_change_results(shape):
     rnd = random.Random(77)
     qraw = "".join(rnd.choice("ARNDCQEGHILKMFPSTWYV") for _ in range(131))
@@ -227,6 +228,22 @@ def test_kernel_shapes_do_not_change_results(shape):
     try:
         for t, bw in [(O.SW, 8), (O.NW, 16), (O.NW, 8)]:
             check(t, bw, O.AMINOACID, 1, qraw, db, 20, ("builtin", "blosum62"), (-10, -1))
+    finally:
+        ssa.set_kernel_shape(0, 0)
+
+
+@pytest.mark.parametrize("shape", [(8, 32), (16, 32), (32, 32)])
+def test_32_row_shapes_over_passes(shape):
+    # 32 rows per lane (SW only; NW runs 16): several passes, pair boundaries
+    # handed between them, 8-bit saturation and the 16-bit tier
+    rnd = random.Random(91)
+    qraw = "".join(rnd.choice("ARNDCQEGHILKMFPSTWYV") for _ in range(1100))
+    db = rand_db(rnd, O.AMINOACID, 300, 1, 900, homolog_of=qraw, frac=0.15)
+    ssa.set_kernel_shape(*shape)
+    try:
+        for t, bw in [(O.SW, 8), (O.SW, 16), (O.NW, 16)]:
+            check(t, bw, O.AMINOACID, 1, qraw, db, 30, ("builtin", "blosum62"), (-10, -1))
+        check(O.SW, 8, O.AMINOACID, 1, qraw, db, 30, ("const", (5, -4)), (-3, -1))
     finally:
         ssa.set_kernel_shape(0, 0)
 
