@@ -483,70 +483,76 @@ void build_layout(Exec& ex, StreamLayout& L, int lanes, const PairSet& ps, const
     int64_t ns = std::max<int64_t>(1, (total + B - 1) / B);
     const int64_t unit = (int64_t)spw * kWavesPerWG;
     ns = (ns + unit - 1) / unit * unit;
+    const int nw = (int)(ns / spw);
 
-    // LPT: longest entry to the least loaded stream
-    std::vector<int64_t> load(ns, 0);
-    std::vector<int32_t> stream_of(n);
+    // Lockstep groups: the entries in length order (longest first) are cut
+    // into groups of spw; a wave runs its groups one after the other, stream
+    // s of the wave taking entry s of every group, each entry padded to the
+    // group's longest.  All pairs of a group start at the same block, so the
+    // lanes of a wave reset their state at a pair start in the same steps
+    // (one run of G steps per group) instead of whenever any of the wave's
+    // streams starts a pair; consecutive entries in length order differ by
+    // little, so the padding is small.  Groups go longest first to the least
+    // loaded wave (LPT).
+    const size_t ngroups = (n + spw - 1) / spw;
+    std::vector<int64_t> wload(nw, 0);
+    std::vector<std::vector<uint32_t>> wgroups(nw);
     {
         typedef std::pair<int64_t, int32_t> Item;
         std::priority_queue<Item, std::vector<Item>, std::greater<Item>> pq;
-        for (int64_t s = 0; s < ns; ++s) pq.push(Item(0, (int32_t)s));
-        for (uint32_t pos : order) {
+        for (int w = 0; w < nw; ++w) pq.push(Item(0, w));
+        for (size_t gi = 0; gi < ngroups; ++gi) {
             Item it = pq.top();
             pq.pop();
-            stream_of[pos] = it.second;
-            it.first += nbk[pos];
-            load[it.second] = it.first;
+            wgroups[it.second].push_back((uint32_t)gi);
+            it.first += nbk[order[gi * spw]];  // the group's first entry is its longest
+            wload[it.second] = it.first;
             pq.push(it);
         }
     }
-    std::vector<int32_t> sorder(ns);
-    std::iota(sorder.begin(), sorder.end(), 0);
-    std::stable_sort(sorder.begin(), sorder.end(), [&](int32_t a, int32_t b) { return load[a] > load[b]; });
-    std::vector<int32_t> rank(ns);
-    for (int64_t i = 0; i < ns; ++i) rank[sorder[i]] = (int32_t)i;
-    // CSR of entries per stream slot (slot index = rank), entries in LPT order
-    L.slot_first.assign(ns + 1, 0);
-    for (uint32_t pos : order) L.slot_first[rank[stream_of[pos]] + 1]++;
-    for (int64_t i = 0; i < ns; ++i) L.slot_first[i + 1] += L.slot_first[i];
-    L.slot_entries.assign(n, 0);
-    {
-        std::vector<int32_t> fill(L.slot_first.begin(), L.slot_first.end() - 1);
-        for (uint32_t pos : order) L.slot_entries[fill[rank[stream_of[pos]]]++] = pos;
-    }
-    L.nwaves = (int)(ns / spw);
-    L.wave_nblocks.assign(L.nwaves, 0);
-    L.wave_res_off.assign(L.nwaves, 0);
+    // waves longest first, so that the four waves of a workgroup task are alike
+    std::vector<int32_t> word(nw);
+    std::iota(word.begin(), word.end(), 0);
+    std::stable_sort(word.begin(), word.end(), [&](int32_t a, int32_t b) { return wload[a] > wload[b]; });
+    L.nwaves = nw;
+    L.wave_nblocks.assign(nw, 0);
+    L.wave_res_off.assign(nw, 0);
+    L.slot_first.assign((size_t)nw * spw + 1, 0);
+    L.slot_entries.clear();
+    L.slot_entries.reserve(n);
+    std::vector<PackEntry> pe;
+    pe.reserve(n);
     int64_t off = 0;
-    for (int w = 0; w < L.nwaves; ++w) {
-        L.wave_nblocks[w] = (int32_t)load[sorder[(int64_t)w * spw]];
+    for (int w = 0; w < nw; ++w) {
+        const std::vector<uint32_t>& gs = wgroups[word[w]];
+        L.wave_nblocks[w] = (int32_t)wload[word[w]];
         L.wave_res_off[w] = off;
+        for (int sl = 0; sl < spw; ++sl) {
+            const int k = sl / 2, h = sl % 2;
+            int64_t pos = 0;
+            for (uint32_t gi : gs) {
+                const size_t i = gi * spw + sl;
+                if (i < n) {
+                    const uint32_t at = order[i];
+                    L.slot_entries.push_back(at);
+                    const DbEntry& de = db.entries[ps.entries[at]];
+                    PackEntry p;
+                    p.code_off = de.code_off;
+                    p.dst = (off + pos * ng + k) * 2 + h;
+                    p.nblocks = entry_blocks(de.len);
+                    p.last_col = (int32_t)((de.len + 3) % 4);  // 3 for an empty entry, as (len + 3) % 4 in the reference
+                    p.stride = 2 * ng;
+                    p.pad = 0;
+                    pe.push_back(p);
+                }
+                pos += nbk[order[gi * spw]];
+            }
+            L.slot_first[(size_t)w * spw + sl + 1] = (int32_t)L.slot_entries.size();
+        }
         off += (int64_t)L.wave_nblocks[w] * ng;
     }
     L.res_words = off;
 
-    // residue words on the device
-    std::vector<PackEntry> pe;
-    pe.reserve(n);
-    for (int w = 0; w < L.nwaves; ++w)
-        for (int sl = 0; sl < spw; ++sl) {
-            const int k = sl / 2, h = sl % 2;
-            const int64_t slot = (int64_t)w * spw + sl;
-            int64_t pos = 0;
-            for (int32_t j = L.slot_first[slot]; j < L.slot_first[slot + 1]; ++j) {
-                const uint32_t e = ps.entries[L.slot_entries[j]];
-                const DbEntry& de = db.entries[e];
-                PackEntry p;
-                p.code_off = de.code_off;
-                p.dst = (L.wave_res_off[w] + pos * ng + k) * 2 + h;
-                p.nblocks = entry_blocks(de.len);
-                p.last_col = (int32_t)((de.len + 3) % 4);  // 3 for an empty entry, as (len + 3) % 4 in the reference
-                p.stride = 2 * ng;
-                p.pad = 0;
-                pe.push_back(p);
-                pos += p.nblocks;
-            }
-        }
     L.release();
     std::vector<ScoreWave> wt(L.nwaves);
     for (int w = 0; w < L.nwaves; ++w) wt[w] = ScoreWave{L.wave_nblocks[w], w * spw, L.wave_res_off[w]};
