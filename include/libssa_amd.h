@@ -46,8 +46,9 @@ void ssa_amd_last_stats(ssa_amd_stats_t* out);
 void ssa_amd_reset_settings(void);
 
 /* Shape of the score kernel: lanes per stream pair (8, 16, 32 or 64) and
- * query rows per lane (4, 8 or 16, or 32 with at most 32 lanes; NW searches
- * run a 32-row shape with 16 rows); 0, 0 chooses from the query length. */
+ * query rows per lane (4, 8, 16 or 32), at most 512 rows per pass (lanes x
+ * rows); NW searches run a 32-row shape with 16 rows; 0, 0 chooses from the
+ * query length. */
 void ssa_amd_set_kernel_shape(int lanes_per_stream, int rows_per_lane);
 
 /* Number of waves the DB's pair streams are cut into (0 = the default, 10240).

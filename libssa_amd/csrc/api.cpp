@@ -145,11 +145,11 @@ void ssa_amd_set_kernel_shape(int lanes_per_stream, int rows_per_lane) {
     const bool autosel = lanes_per_stream == 0 && rows_per_lane == 0;
     const bool lanes_ok = lanes_per_stream == 8 || lanes_per_stream == 16 || lanes_per_stream == 32 ||
                           lanes_per_stream == 64;
-    const bool rows_ok = rows_per_lane == 4 || rows_per_lane == 8 || rows_per_lane == 16 ||
-                         (rows_per_lane == 32 && lanes_per_stream <= 32);
+    const bool rows_ok = (rows_per_lane == 4 || rows_per_lane == 8 || rows_per_lane == 16 || rows_per_lane == 32) &&
+                         lanes_per_stream * rows_per_lane <= 512;
     if (!autosel && !(lanes_ok && rows_ok))
-        fatal("libssa_amd: kernel shape must be 0, 0 or lanes in {8, 16, 32, 64} and rows in {4, 8, 16} "
-              "(or 32 with at most 32 lanes) (got %d, %d)",
+        fatal("libssa_amd: kernel shape must be 0, 0 or lanes in {8, 16, 32, 64} and rows in {4, 8, 16, 32} "
+              "with at most 512 rows per pass (got %d, %d)",
               lanes_per_stream, rows_per_lane);
     set_shape_override(lanes_per_stream, rows_per_lane);
 }

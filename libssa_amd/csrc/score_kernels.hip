@@ -134,22 +134,27 @@ __device__ __forceinline__ void finish_pair(PairResult* results, int pair, bool 
     if (MODE == 0 && fb != kNoFlag) atomicMin(&results[pair].flag_b, fb);
 }
 
-// One pass of a wave over its streams: nblocks + G - 1 steps of one block per
-// lane, rounded up to an even count (the extra step is idle on every lane) so
-// that the loop runs two steps per iteration and the one-step-ahead loads
-// alternate between two register sets.
-template <int R, int G, int MODE, bool FIRST>
-__device__ __forceinline__ void run_pass(const ScoreParams& P, uint32_t gbase, const char* sp,
-                                         const uint32_t* slbh, const uint32_t* slbe,
+// All passes of a wave over its streams, as one pipeline: lane g handles
+// block t - g of the passes laid end to end (nbp blocks per pass), so lane 0
+// starts pass p + 1 while the lanes below still finish pass p, and a task
+// takes npass * nbp + G - 1 steps instead of npass * (nbp + G - 1).  Lane 0
+// of pass p + 1 reads the bottom row that lane G - 1 wrote for the same block
+// in pass p, nbp - G + 1 steps earlier (nbp > G).  Each lane keeps its own
+// pass: the staged profile and NW left border are double-buffered, and
+// stage(p) loads pass p's into buffer p & 1 once every lane has left pass
+// p - 2.  Steps run two per iteration (an even count; an extra step is idle)
+// so that the one-step-ahead loads alternate between two register sets.
+template <int R, int G, int MODE, bool MULTI, class Stage>
+__device__ __forceinline__ void run_task(const ScoreParams& P, uint32_t gbase0, const char* sp,
+                                         const uint32_t* slbh0, const uint32_t* slbe0,
                                          const uint2* __restrict__ wres, uint4* __restrict__ wbnd,
                                          const int32_t* __restrict__ slot_entries, int pair_off,
-                                         PairResult* __restrict__ results,
-                                         int sb_lo, int sb_hi, int k, int g, int nb, bool last_pass, int last_g,
-                                         int last_r) {
+                                         PairResult* __restrict__ results, int sb_lo, int sb_hi, int k, int g,
+                                         int nbw, int nbp, int npass, int last_g, int last_r, const Stage& stage) {
     constexpr int NG = 64 / G;
-    constexpr int LPR = Log2<G * R>::value;
-    // byte offset of the lane's rows for column c's DB code in the staged profile
-    auto colofs = [gbase](uint32_t word, int c) { return (((word >> (8 * c)) & 0x3eu) << LPR) + gbase; };
+    constexpr int PR = G * R;
+    constexpr int LPR = Log2<PR>::value;
+    constexpr uint32_t kProfBytes = 32 * PR * 2;  // one staged profile buffer
     const s16x2 QR = as_v(P.qr), RR = as_v(P.r), Z = as_v(P.zero);
     // per-half thresholds of the overflow predicates; a half whose predicate
     // cannot fire (flag taken, or no pair running) holds the neutral value
@@ -159,10 +164,13 @@ __device__ __forceinline__ void run_pass(const ScoreParams& P, uint32_t gbase, c
     const uint32_t TB_off = 0x7fff7fffu;
 
     const bool top_lane = g == 0;
-    const bool top_in = top_lane && !FIRST;         // top boundary from the previous pass
-    const bool bot_out = g == G - 1 && !last_pass;  // bottom row to the next pass
-    const bool score_lane = MODE == 0 ? g == G - 1 : (last_pass && g == last_g);
-    const int nsteps = nb > 0 ? ((nb + G) & ~1) : 0;
+    // the lane's pass and block within it (negative before the lane's first block)
+    int pass = 0, b = -g;
+    uint32_t gbase = gbase0;  // the lane's rows in the profile buffer of its pass
+    const uint32_t* slbh = slbh0;
+    const uint32_t* slbe = slbe0;
+    // byte offset of the lane's rows for column c's DB code in the staged profile
+    auto colofs = [&gbase](uint32_t word, int c) { return (((word >> (8 * c)) & 0x3eu) << LPR) + gbase; };
 
     s16x2 Hl[R], El[R];
 #pragma unroll
@@ -185,23 +193,27 @@ __device__ __forceinline__ void run_pass(const ScoreParams& P, uint32_t gbase, c
     int kp_lo = 0, kp_hi = 0, pair_lo = -1, pair_hi = -1, blk_lo = 0, blk_hi = 0;
     int fa_lo = kNoFlag, fa_hi = kNoFlag, fb_lo = kNoFlag, fb_hi = kNoFlag;
 
-    const uint2* wp = wres + (int64_t)(-g) * NG + k;  // residue word of block t - g
-    const uint4* tp = wbnd + (int64_t)k * 2;          // top lane: boundary of block t
-
-    auto step = [&](int t, const uint2 w, const uint4 t0, const uint4 t1, uint2& wn, uint4& tn0, uint4& tn1) {
+    // FIRST: the top lane is in the first pass and takes the matrix border
+    auto step = [&](auto first_tag, const uint2 w, const uint4 t0, const uint4 t1, uint2& wn, uint4& tn0,
+                    uint4& tn1) {
+        constexpr bool FIRST = decltype(first_tag)::value;
         // profile dwords of the current column (lo, hi halves); column 0's now
         uint32_t pl[R / 2], ph[R / 2];
         read_rows<R>(sp + colofs(w.x, 0), pl);
         read_rows<R>(sp + colofs(w.y, 0), ph);
-        const int b = t - g;
-        const bool valid = (unsigned)b < (unsigned)nb;
-        wp += NG;
-        tp += 2 * NG;
-        if ((unsigned)(b + 1) < (unsigned)nb) {
-            wn = wp[0];
-            if (top_in) {
-                tn0 = tp[0];
-                tn1 = tp[1];
+        const bool lane_last = pass == npass - 1;
+        const bool valid = pass < npass && (unsigned)b < (unsigned)nbw;
+        const bool bot_out = g == G - 1 && pass < npass - 1;  // bottom row to the next pass
+        const bool score_lane = MODE == 0 ? g == G - 1 : (lane_last && g == last_g);
+        // the lane's next block, in this pass or (MULTI) at the start of the next
+        const bool wrap = MULTI && b + 1 == nbp;
+        const int b_n = wrap ? 0 : b + 1, pass_n = wrap ? pass + 1 : pass;
+        if ((!MULTI || pass_n < npass) && (unsigned)b_n < (unsigned)nbw) {
+            const uint32_t o = (uint32_t)(b_n * NG + k);
+            wn = wres[o];
+            if (MULTI && top_lane && pass_n > 0) {
+                tn0 = wbnd[2 * o];
+                tn1 = wbnd[2 * o + 1];
             }
         } else {
             wn = make_uint2(0, 0);
@@ -296,7 +308,7 @@ __device__ __forceinline__ void run_pass(const ScoreParams& P, uint32_t gbase, c
         auto columns = [&](auto cap_tag) {
             constexpr bool CAP = decltype(cap_tag)::value;
             uint32_t cm_lo = 0u, cm_hi = 0u;
-            if (CAP && g == last_g) {
+            if (CAP && g == last_g && lane_last) {
                 cm_lo = ls_lo ? 0x0000ffffu : 0u;
                 cm_hi = ls_hi ? 0xffff0000u : 0u;
             }
@@ -349,7 +361,7 @@ __device__ __forceinline__ void run_pass(const ScoreParams& P, uint32_t gbase, c
                 }
             }
         };
-        if (MODE == 1 && last_pass && any_last) columns(Tag<true>{});
+        if (MODE == 1 && __any(lane_last && (ls_lo || ls_hi))) columns(Tag<true>{});
         else columns(Tag<false>{});
         // overflow predicates at the end of the block: a packed test here (in
         // the cell loop's block, so the running reductions stay in the loop),
@@ -384,8 +396,8 @@ __device__ __forceinline__ void run_pass(const ScoreParams& P, uint32_t gbase, c
             TA = (offa & TA_off) | (~offa & TA);
             TB = (offb & TB_off) | (~offb & TB);
         }
-        if (bot_out && valid) {
-            const int64_t o = ((int64_t)b * NG + k) * 2;
+        if (MULTI && bot_out && valid) {
+            const uint32_t o = (uint32_t)(b * NG + k) * 2;
             wbnd[o] = make_uint4(obh[0], obf[0], obh[1], obf[1]);
             wbnd[o + 1] = make_uint4(obh[2], obf[2], obh[3], obf[3]);
         }
@@ -422,22 +434,41 @@ __device__ __forceinline__ void run_pass(const ScoreParams& P, uint32_t gbase, c
         }
         blk_lo++;
         blk_hi++;
+        b = b_n;
+        if (MULTI && __any(wrap)) {  // lanes entering their next pass: its entry list and staged buffers
+            if (wrap) {
+                pass = pass_n;
+                kp_lo = 0;
+                kp_hi = 0;
+                gbase ^= kProfBytes;
+                const bool odd = pass & 1;
+                slbh = odd ? slbh0 + (PR + 1) : slbh0;
+                slbe = odd ? slbe0 + PR : slbe0;
+            }
+        }
     };
 
     // block -g for step 0: only the top lanes start with a real block
     uint2 wa = make_uint2(0, 0), wb = make_uint2(0, 0);
     const uint4 zero4 = make_uint4(0, 0, 0, 0);
     uint4 ta0 = zero4, ta1 = zero4, tb0 = zero4, tb1 = zero4;
-    if (nb > 0 && g == 0) {
-        wa = wp[0];
-        if (top_in) {
-            ta0 = tp[0];
-            ta1 = tp[1];
-        }
+    if (nbw > 0 && g == 0) wa = wres[k];
+    // steps with the top lane in the first pass (nbp is even)
+    for (int t = 0; t < nbp; t += 2) {
+        step(Tag<true>{}, wa, ta0, ta1, wb, tb0, tb1);
+        step(Tag<true>{}, wb, tb0, tb1, wa, ta0, ta1);
     }
-    for (int t = 0; t < nsteps; t += 2) {
-        step(t, wa, ta0, ta1, wb, tb0, tb1);
-        step(t + 1, wb, tb0, tb1, wa, ta0, ta1);
+    // the rest: pass p + 2 is staged once lane G - 1 has left pass p
+    const int nsteps = (npass * nbp + G) & ~1;
+    int next_stage = ((nbp + G - 1) + 1) & ~1, staged = 2;
+    for (int t = nbp; t < nsteps; t += 2) {
+        if (MULTI && t == next_stage && staged < npass) {  // the same for the whole workgroup
+            stage(staged);
+            ++staged;
+            next_stage += nbp;
+        }
+        step(Tag<false>{}, wa, ta0, ta1, wb, tb0, tb1);
+        step(Tag<false>{}, wb, tb0, tb1, wa, ta0, ta1);
     }
 }
 
@@ -445,9 +476,10 @@ template <int R, int G, int MODE>
 __global__ __launch_bounds__(64 * kWavesPerWG) void score_kernel(const ScoreLaunch L, const ScoreParams P) {
     constexpr int PR = G * R;  // rows per pass
     static_assert((1 << Log2<PR>::value) == PR, "rows per pass must be a power of two");
-    __shared__ __align__(16) int16_t sprof[32 * PR];
-    __shared__ uint32_t slbh[MODE == 1 ? PR + 1 : 1];  // NW left boundary H of rows base-1 .. base+PR-1
-    __shared__ uint32_t slbe[MODE == 1 ? PR : 1];      // NW left boundary E of rows base .. base+PR-1
+    static_assert(PR <= 512, "two staged profiles of 32 codes x PR rows must fit the LDS budget");
+    __shared__ __align__(16) int16_t sprof[2 * 32 * PR];      // profiles of passes p (even, odd)
+    __shared__ uint32_t slbh[MODE == 1 ? 2 * (PR + 1) : 1];  // NW left boundary H of rows base-1 .. base+PR-1
+    __shared__ uint32_t slbe[MODE == 1 ? 2 * PR : 1];        // NW left boundary E of rows base .. base+PR-1
     __shared__ int s_task;
 
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
@@ -457,7 +489,7 @@ __global__ __launch_bounds__(64 * kWavesPerWG) void score_kernel(const ScoreLaun
     uint4* __restrict__ wbnd = L.bnd + ((int64_t)blockIdx.x * kWavesPerWG + wv) * L.bnd_per_wave;
 
     for (;;) {
-        __syncthreads();  // the previous task is done with s_task and the staged profile
+        __syncthreads();  // the previous task is done with s_task and the staged profiles
         if (threadIdx.x == 0) s_task = atomicAdd(L.counter, 1);
         __syncthreads();
         const int task = s_task;
@@ -466,38 +498,51 @@ __global__ __launch_bounds__(64 * kWavesPerWG) void score_kernel(const ScoreLaun
         const ScoreWave Wv = L.waves[T.wave0 + wv];
         const QueryDesc Q = L.queries[T.query];
         const int npass = (Q.qlen + PR - 1) / PR;
-        const int nb = Wv.nblocks;
+        // the workgroup's waves share the pass length: the longest of them,
+        // at least G + 2 blocks (so that a pass's bottom rows are written
+        // before the next pass reads them) and even
+        int nbmax = 0;
+#pragma unroll
+        for (int i = 0; i < kWavesPerWG; ++i) nbmax = max(nbmax, L.waves[T.wave0 + i].nblocks);
+        if (nbmax == 0) continue;
+        const int nbp = npass > 1 ? (max(nbmax, G + 2) + 1) & ~1 : (nbmax + 1) & ~1;
         const uint2* __restrict__ wres = L.res + Wv.res_off;
         const int sb_lo = L.slot_first[Wv.slot_base + 2 * k];
         const int sb_hi = L.slot_first[Wv.slot_base + 2 * k + 1];
         const int last_lr = Q.qlen - 1 - (npass - 1) * PR;  // last real row within the last pass
         const int last_g = last_lr / R, last_r = last_lr % R;
 
-        for (int pass = 0; pass < npass; ++pass) {
-            __syncthreads();
-            {
-                const uint4* src = reinterpret_cast<const uint4*>(L.qp + Q.qp_off + (int64_t)pass * 32 * PR);
-                uint4* dst = reinterpret_cast<uint4*>(sprof);
-                for (int i = threadIdx.x; i < 32 * PR / 8; i += 64 * kWavesPerWG) dst[i] = src[i];
-                if (MODE == 1) {
-                    const int base = pass * PR;
-                    for (int i = threadIdx.x; i < PR; i += 64 * kWavesPerWG) {
-                        slbh[1 + i] = L.lb[Q.lb_off + 2 * (base + i)];
-                        slbe[i] = L.lb[Q.lb_off + 2 * (base + i) + 1];
-                    }
-                    if (threadIdx.x == 0) slbh[0] = base > 0 ? L.lb[Q.lb_off + 2 * (base - 1)] : 0u;
+        // pass p's profile (and NW left border) into buffer p & 1
+        auto load_pass = [&](int pass) {
+            const int buf = pass & 1;
+            const uint4* src = reinterpret_cast<const uint4*>(L.qp + Q.qp_off + (int64_t)pass * 32 * PR);
+            uint4* dst = reinterpret_cast<uint4*>(sprof + buf * 32 * PR);
+            for (int i = threadIdx.x; i < 32 * PR / 8; i += 64 * kWavesPerWG) dst[i] = src[i];
+            if (MODE == 1) {
+                const int base = pass * PR;
+                uint32_t* bh = slbh + buf * (PR + 1);
+                uint32_t* be = slbe + buf * PR;
+                for (int i = threadIdx.x; i < PR; i += 64 * kWavesPerWG) {
+                    bh[1 + i] = L.lb[Q.lb_off + 2 * (base + i)];
+                    be[i] = L.lb[Q.lb_off + 2 * (base + i) + 1];
                 }
+                if (threadIdx.x == 0) bh[0] = base > 0 ? L.lb[Q.lb_off + 2 * (base - 1)] : 0u;
             }
+        };
+        load_pass(0);
+        if (npass > 1) load_pass(1);
+        __syncthreads();
+        auto stage = [&](int pass) {
+            __syncthreads();  // every lane of the workgroup has left pass - 2
+            load_pass(pass);
             __syncthreads();
-            const bool last_pass = pass == npass - 1;
-            if (pass == 0) {
-                run_pass<R, G, MODE, true>(P, gbase, sp, slbh, slbe, wres, wbnd, L.slot_entries, Q.pair_off,
-                                           L.results, sb_lo, sb_hi, k, g, nb, last_pass, last_g, last_r);
-            } else {
-                run_pass<R, G, MODE, false>(P, gbase, sp, slbh, slbe, wres, wbnd, L.slot_entries, Q.pair_off,
-                                            L.results, sb_lo, sb_hi, k, g, nb, last_pass, last_g, last_r);
-            }
-        }
+        };
+        if (npass > 1)
+            run_task<R, G, MODE, true>(P, gbase, sp, slbh, slbe, wres, wbnd, L.slot_entries, Q.pair_off, L.results,
+                                       sb_lo, sb_hi, k, g, Wv.nblocks, nbp, npass, last_g, last_r, stage);
+        else
+            run_task<R, G, MODE, false>(P, gbase, sp, slbh, slbe, wres, wbnd, L.slot_entries, Q.pair_off, L.results,
+                                        sb_lo, sb_hi, k, g, Wv.nblocks, nbp, npass, last_g, last_r, stage);
     }
 }
 
@@ -628,11 +673,11 @@ static int resident_wg_t() {
     return cap;
 }
 
-// 32 rows per lane: SW only (NW holds more state per row and would run one
-// wave per SIMD), and not at 64 lanes per group (128 KiB of profile)
+// Passes of at most 512 rows (two staged profiles of 32 KiB); 32 rows per
+// lane for SW only (NW holds more state per row and would run one wave per SIMD)
 bool score_shape_supported(int lanes, int rows, int mode) {
     return (lanes == 8 || lanes == 16 || lanes == 32 || lanes == 64) &&
-           (rows == 4 || rows == 8 || rows == 16 || (rows == 32 && lanes <= 32 && mode == 0));
+           (rows == 4 || rows == 8 || rows == 16 || (rows == 32 && mode == 0)) && lanes * rows <= 512;
 }
 
 template <int R, int G, int MODE>
@@ -647,9 +692,9 @@ static hipError_t launch_shape(hipStream_t st, const ScoreLaunch& L, const Score
     else return launch_score_t<R, G, MODE>(st, L, P, max_wg);
 }
 
-#define SSA_SHAPES(X)                                                                                        \
-    X(8, 4) X(8, 8) X(8, 16) X(8, 32) X(16, 4) X(16, 8) X(16, 16) X(16, 32) X(32, 4) X(32, 8) X(32, 16) X(32, 32) \
-    X(64, 4) X(64, 8) X(64, 16)
+#define SSA_SHAPES(X)                                                                                   \
+    X(8, 4) X(8, 8) X(8, 16) X(8, 32) X(16, 4) X(16, 8) X(16, 16) X(16, 32) X(32, 4) X(32, 8) X(32, 16) \
+    X(64, 4) X(64, 8)
 
 int score_resident_workgroups(int lanes, int rows, int mode) {
 #define SSA_CASE(GG, RR) \

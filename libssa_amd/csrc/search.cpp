@@ -361,11 +361,10 @@ std::mutex& tuned_mutex() {
 
 // every shape of the score kernel for a search type (0 SW, 1 NW)
 const std::vector<Shape>& kernel_shapes(int type) {
-    static const std::vector<Shape> sw = {{32, 32}, {16, 32}, {8, 32}, {64, 16}, {32, 16}, {16, 16}, {8, 16},
-                                          {64, 8},  {32, 8},  {16, 8}, {8, 8},   {64, 4},  {32, 4},  {16, 4},
-                                          {8, 4}};
-    static const std::vector<Shape> nw = {{64, 16}, {32, 16}, {16, 16}, {8, 16}, {64, 8}, {32, 8},
-                                          {16, 8},  {8, 8},   {64, 4},  {32, 4}, {16, 4}, {8, 4}};
+    static const std::vector<Shape> sw = {{16, 32}, {8, 32}, {32, 16}, {16, 16}, {8, 16}, {64, 8}, {32, 8},
+                                          {16, 8},  {8, 8},  {64, 4},  {32, 4},  {16, 4}, {8, 4}};
+    static const std::vector<Shape> nw = {{32, 16}, {16, 16}, {8, 16}, {64, 8}, {32, 8}, {16, 8},
+                                          {8, 8},   {64, 4},  {32, 4}, {16, 4}, {8, 4}};
     return type == 0 ? sw : nw;
 }
 
@@ -410,16 +409,18 @@ Shape choose_shape(size_t qlen_max, int64_t total_blocks, int64_t max_blocks, in
         const int64_t pr = (int64_t)sh.lanes * sh.rows;
         const int64_t passes = std::max<int64_t>(1, ((int64_t)qlen_max + pr - 1) / pr);
         // measured per-row costs: short lanes pay the fixed per-step work (32
-        // rows run at two waves per SIMD, 16 at three); G = 8 and 32 fix up
-        // their top lanes after the DPP shift; G = 64 at R = 16 stages 64 KiB
-        // of profile per workgroup (two workgroups per CU)
-        double per_row = sh.rows == 4 ? 1.32 : (sh.rows == 8 ? 1.10 : (sh.rows == 16 ? 1.0 : 0.975));
+        // rows run at two waves per SIMD, 16 at up to three); G = 8 and 32 fix
+        // up their top lanes after the DPP shift
+        double per_row = sh.rows == 4 ? 1.32 : (sh.rows == 8 ? 1.10 : (sh.rows == 16 ? 1.0 : 0.955));
         if (sh.lanes == 8 || sh.lanes == 32) per_row *= 1.04;
-        if (sh.lanes == 64 && sh.rows == 16) per_row *= 1.035;
+        // 512-row passes double-buffer 64 KiB of profile: two workgroups per
+        // CU, which costs the 16-row shape its third wave per SIMD
+        if (sh.lanes * sh.rows == 512 && sh.rows == 16) per_row *= 1.06;
         const double waves = std::max<double>(target_waves(), 1.0);
+        // the passes of a task run as one pipeline: G - 1 fill steps per task
         const double work =
-            (double)passes * ((double)total_blocks * sh.lanes / 128.0 + waves * (sh.lanes - 1)) / 1024.0;
-        const double tail = 2.0 * (double)passes * (double)(max_blocks + sh.lanes - 1);
+            ((double)passes * (double)total_blocks * sh.lanes / 128.0 + waves * (sh.lanes - 1)) / 1024.0;
+        const double tail = 2.0 * ((double)passes * (double)max_blocks + sh.lanes - 1);
         const double cost = std::max(work, tail) * sh.rows * per_row;
         if (cost < best_cost) {
             best = sh;
@@ -742,7 +743,7 @@ void tier_prepare(Exec& ex, TierRun& R) {
         multipass |= npass[qi] > 1;
         for (int t = 0; t < nt_q; ++t) {
             R.tasks.push_back(ScoreTask{(int32_t)qi, t * kWavesPerWG});
-            cost.push_back((int64_t)(L->wave_nblocks[t * kWavesPerWG] + sh.lanes) * npass[qi]);
+            cost.push_back((int64_t)L->wave_nblocks[t * kWavesPerWG] * npass[qi] + sh.lanes);
         }
         for (int w = 0; w < nw; ++w) R.computed += (uint64_t)L->wave_nblocks[w] * 4 * spw * (uint64_t)npass[qi] * PR;
     }
@@ -758,7 +759,9 @@ void tier_prepare(Exec& ex, TierRun& R) {
     R.grid = std::max(1, std::min<int>((int)R.tasks.size(), cap > 0 ? cap : (int)R.tasks.size()));
     int32_t max_nb = 0;
     for (int w = 0; w < nw; ++w) max_nb = std::max(max_nb, L->wave_nblocks[w]);
-    const int64_t bnd_per_wave = multipass ? (int64_t)max_nb * ng * 2 : 0;
+    // the kernel's pass length: the longest wave of a task, at least G + 2, even
+    const int64_t max_nbp = (std::max<int64_t>(max_nb, sh.lanes + 2) + 1) & ~(int64_t)1;
+    const int64_t bnd_per_wave = multipass ? max_nbp * ng * 2 : 0;
 
     // ---- device buffers ----
     auto name = [&](const char* s) { return R.pfx + s; };
