@@ -908,6 +908,9 @@ struct Elem {
 class MinHeap {
 public:
     explicit MinHeap(size_t k) : alloc_(k) { a_.reserve(k); }
+    // true when add(e) would leave the heap as it is: full, and the root's
+    // score is not below e's
+    bool rejects(int64_t score) const { return a_.size() >= alloc_ && (a_.empty() || !(a_[0].score < score)); }
     void add(const Elem& e) {
         if (a_.size() < alloc_) {
             size_t i = a_.size();
@@ -952,30 +955,43 @@ private:
 // takes an item in iteration t retires it in iteration t + dur; all lanes
 // retiring in one iteration are served in lane order and immediately take the
 // next item (src/algo/simd/search_simd_sw.c:338-428).  Calls emit(k) for the
-// items (positions into the list) in retirement order.
+// items (positions into the list) in retirement order.  The lanes sit in a
+// binary min-heap of keys (retire iteration << 6 | lane), so one integer
+// compare orders them by iteration, then lane (W <= 64).
 template <class Dur, class Emit>
 void lane_schedule(int W, size_t n, const Dur& dur, const Emit& emit) {
-    struct L {
-        int64_t t;
-        int lane;
-        size_t item;
-    };
-    auto later = [](const L& a, const L& b) { return a.t != b.t ? a.t > b.t : a.lane > b.lane; };
-    std::vector<L> h;
-    h.reserve(W);
+    uint64_t key[64];
+    size_t item[64];
+    int m = 0;
     size_t next = 0;
-    for (int c = 0; c < W && next < n; ++c, ++next) h.push_back(L{dur(next), c, next});
-    std::make_heap(h.begin(), h.end(), later);
-    while (!h.empty()) {
-        std::pop_heap(h.begin(), h.end(), later);
-        L x = h.back();
-        h.pop_back();
-        emit(x.item);
-        if (next < n) {
-            h.push_back(L{x.t + dur(next), x.lane, next});
-            ++next;
-            std::push_heap(h.begin(), h.end(), later);
+    for (int c = 0; c < W && next < n; ++c, ++next) {
+        key[m++] = ((uint64_t)dur(next) << 6) | (uint64_t)c;
+        item[c] = next;
+    }
+    auto sift_down = [&](int i) {
+        const uint64_t k = key[i];
+        for (;;) {
+            int c = 2 * i + 1;
+            if (c >= m) break;
+            if (c + 1 < m && key[c + 1] < key[c]) ++c;
+            if (key[c] >= k) break;
+            key[i] = key[c];
+            i = c;
         }
+        key[i] = k;
+    };
+    for (int i = m / 2 - 1; i >= 0; --i) sift_down(i);
+    while (m > 0) {
+        const uint64_t k = key[0];
+        const int lane = (int)(k & 63);
+        emit(item[lane]);
+        if (next < n) {
+            key[0] = (((k >> 6) + (uint64_t)dur(next)) << 6) | (uint64_t)lane;
+            item[lane] = next++;
+        } else {
+            key[0] = key[--m];
+        }
+        if (m > 0) sift_down(0);
     }
 }
 
@@ -1197,6 +1213,7 @@ void job_replay(Job& J, const std::vector<std::pair<uint32_t, uint32_t>>& chunks
     auto do_chunk = [&](size_t c) {
         const uint32_t e0 = chunks[c].first, e1 = chunks[c].second;
         std::vector<Ins>& out = per_chunk[c];
+        out.reserve((size_t)(e1 - e0) * nq);
         std::vector<uint32_t> items(e1 - e0), ovf, ovf2;
         std::iota(items.begin(), items.end(), e0);
         auto tier64 = [&](const std::vector<uint32_t>& list) {
@@ -1245,6 +1262,7 @@ void job_replay(Job& J, const std::vector<std::pair<uint32_t, uint32_t>>& chunks
     MinHeap heap(J.rq.hitcount);
     for (size_t c = 0; c < chunks.size(); ++c) {
         for (const Ins& x : per_chunk[c]) {
+            if (heap.rejects(x.score)) continue;
             const DbEntry& e = db.entries[x.e];
             heap.add(Elem{e.id, e.frame, e.strand, x.q, x.score});
         }
