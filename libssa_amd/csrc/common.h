@@ -3,6 +3,7 @@
 
 #include <cstddef>
 #include <cstdint>
+#include <cstring>
 #include <functional>
 #include <string>
 #include <vector>
@@ -49,6 +50,58 @@ Config& config();
 void parallel_for(size_t n, const std::function<void(size_t)>& fn);
 // threads a parallel_for spreads over (workers + caller, at most 16)
 size_t parallel_width();
+
+// Large host blocks for per-search arrays that are written before they are
+// read.  Released blocks stay mapped in a process-wide cache (up to 4 GiB) and
+// serve later requests of up to their size, so repeated searches neither
+// fault in nor zero fresh pages.
+void* host_block_acquire(size_t bytes, size_t* capacity);
+void host_block_release(void* p, size_t capacity);
+
+// An array of trivially copyable T over a cached host block; elements start
+// unset.
+template <class T>
+class HostArray {
+public:
+    HostArray() = default;
+    HostArray(const HostArray&) = delete;
+    HostArray& operator=(const HostArray&) = delete;
+    HostArray(HostArray&& o) noexcept : p_(o.p_), n_(o.n_), cap_(o.cap_) { o.p_ = nullptr, o.n_ = o.cap_ = 0; }
+    HostArray& operator=(HostArray&& o) noexcept {
+        if (this != &o) {
+            release();
+            p_ = o.p_, n_ = o.n_, cap_ = o.cap_;
+            o.p_ = nullptr, o.n_ = o.cap_ = 0;
+        }
+        return *this;
+    }
+    ~HostArray() { release(); }
+    // n unset elements (previous contents are not kept)
+    void reset(size_t n) {
+        if (n * sizeof(T) > cap_) {
+            release();
+            p_ = (T*)host_block_acquire(n * sizeof(T), &cap_);
+        }
+        n_ = n;
+    }
+    void fill_zero() {
+        if (n_) memset((void*)p_, 0, n_ * sizeof(T));
+    }
+    T& operator[](size_t i) { return p_[i]; }
+    const T& operator[](size_t i) const { return p_[i]; }
+    T* data() { return p_; }
+    const T* data() const { return p_; }
+    size_t size() const { return n_; }
+    bool empty() const { return n_ == 0; }
+
+private:
+    void release() {
+        if (p_) host_block_release(p_, cap_);
+        p_ = nullptr, n_ = cap_ = 0;
+    }
+    T* p_ = nullptr;
+    size_t n_ = 0, cap_ = 0;
+};
 
 // lanes of the reference's 8-bit and 16-bit kernels for the configured ISA
 // (src/algo/8/search_8_util.h:26-35, src/algo/16/search_16_util.h:27-37)

@@ -300,11 +300,14 @@ MatrixInfo matrix_info() {
     return mi;
 }
 
-// result of one pair at one tier, decoded
+// result of one pair at one tier, decoded: d > 0 is a heap insert of score
+// after d blocks in a reference lane, d < 0 an overflow after -d blocks
+// (blocks >= 1 either way)
 struct TierOut {
-    int32_t dur;     // blocks the pair occupies a reference lane
-    bool accept;     // heap insert (true) or overflow (false)
-    int64_t score;   // valid when accept
+    int32_t d;
+    int32_t score;
+    bool accept() const { return d > 0; }
+    int32_t dur() const { return d > 0 ? d : -d; }
 };
 
 }  // namespace
@@ -876,19 +879,19 @@ void gpu_exact(Exec& gpu, SearchType type, const std::vector<QueryBuf>& qs,
 // Decodes one tier result into (duration, accept, score).
 TierOut decode(SearchType type, const Tier& T, const PairResult& r, uint32_t len, bool sw16_as_8) {
     const int32_t nb = entry_blocks(len);
-    TierOut o;
-    int32_t flag;
+    int32_t flag, score;
+    bool accept;
     if (type == kSW) {
         flag = sw16_as_8 ? r.flag_a : (T.bits == 16 ? r.flag_b : r.flag_a);
-        o.score = T.decode(r.score) - T.imin();
-        o.accept = flag == kNoFlag;
+        score = (int32_t)(T.decode(r.score) - T.imin());
+        accept = flag == kNoFlag;
     } else {
         flag = r.flag_a;
-        o.score = T.decode(r.score);
-        o.accept = flag == kNoFlag && o.score > T.imin() && o.score < T.imax();
+        score = (int32_t)T.decode(r.score);
+        accept = flag == kNoFlag && score > T.imin() && score < T.imax();
     }
-    o.dur = flag == kNoFlag ? nb : std::min(nb, flag + 1);
-    return o;
+    const int32_t dur = flag == kNoFlag ? nb : std::min(nb, flag + 1);
+    return TierOut{accept ? dur : -dur, score};
 }
 
 }  // namespace
@@ -1149,9 +1152,12 @@ struct Job {
     std::vector<QueryBuf> qs;
     bool sw_single = false;  // SW 8-bit whose 8-bit results come from one int16 pass
     int first_bits = 0;      // first tier: 8, 16, or 0 (exact tier only)
-    std::vector<TierOut> t8, t16;
-    std::vector<int64_t> s64;
-    std::vector<uint8_t> in16, need64;
+    HostArray<TierOut> t8, t16;      // per pair (q * N + e)
+    HostArray<uint8_t> in16, need64;  // per entry: reaches the 16-bit / exact tier
+    // exact scores: the entries that reach the exact tier, ascending, and
+    // their scores, one per query each
+    std::vector<uint32_t> x_entries;
+    std::vector<int64_t> x64;
     ssa_amd_stats_t st;
     Shape shape{0, 0};
     size_t q_off = 0;  // first query of this job in its first-tier group
@@ -1193,82 +1199,214 @@ void job_queries(Job& J) {
         if (q.len == 0 && J.rq.type == kNW) fatal("libssa_amd: NW search with an empty query");
 }
 
+// The score an exact-tier insert of (q, e) carries: the exact tier's where
+// the entry reached it, else the 16-bit tier's (a 64-bit search whose 16-bit
+// result is provably exact).
+int64_t score64(const Job& J, size_t q, uint32_t e, uint32_t N) {
+    if (J.need64[e]) {
+        const size_t k = std::lower_bound(J.x_entries.begin(), J.x_entries.end(), e) - J.x_entries.begin();
+        return J.x64[k * J.qs.size() + q];
+    }
+    return J.t16[q * N + e].score;
+}
+
 // The host replay of one search: the reference's chunk loop, lane schedules,
 // cascade and heap (src/algo/8/search_8.c:94-146, src/algo/16/search_16.c:92-134,
 // src/algo/64/search_64.c:44-79, src/algo/manager.c:120-185).
+//
+// The heap's scores are always the k largest inserted so far, so when chunk c
+// starts with a full heap, its root is at least the k-th largest of any k
+// inserts of the chunks before, and an insert of chunk c at or below that is
+// rejected wherever the lane schedule puts it.  The replay therefore takes a
+// threshold per chunk from the earlier chunks' largest inserts (which do not
+// depend on order), runs the lane schedules only for chunks with two or more
+// inserts above it, and feeds the heap only those inserts, in the reference's
+// order.
 void job_replay(Job& J, const std::vector<std::pair<uint32_t, uint32_t>>& chunks, uint32_t N, bool threads) {
     DeviceDB& db = g_db;
     const int64_t t_replay = now_us();
     const int W8 = lanes8(), W16 = lanes16();
     const size_t nq = J.qs.size();
     const int bit_width = J.rq.bit_width;
+    const size_t k = J.rq.hitcount;
+    const size_t nc = chunks.size();
     auto at = [N](size_t q, size_t e) { return q * N + e; };
+    auto s64 = [&](size_t q, uint32_t e) { return score64(J, q, e, N); };
+    auto over_chunks = [&](size_t n, const std::function<void(size_t)>& fn) {
+        if (threads && n > 8) {
+            parallel_for(n, fn);
+        } else {
+            for (size_t i = 0; i < n; ++i) fn(i);
+        }
+    };
     struct Ins {
         uint32_t e;
         uint8_t q;
         int64_t score;
     };
-    std::vector<std::vector<Ins>> per_chunk(chunks.size());
-    std::vector<size_t> ov8(chunks.size(), 0), ov16(chunks.size(), 0);
-    auto do_chunk = [&](size_t c) {
-        const uint32_t e0 = chunks[c].first, e1 = chunks[c].second;
+
+    // Every insert of chunk c in no particular order, as f(q, e, score,
+    // copies): an entry that overflows for m8 queries at 8 bit stands m8 times
+    // in the chunk's 16-bit list, which every query scans; each 16-bit
+    // overflow met in that scan adds the entry to the exact list once more,
+    // which every query scans again.  Also counts the overflows.
+    auto inserts = [&](size_t c, size_t& ov8, size_t& ov16, auto&& f) {
+        for (uint32_t e = chunks[c].first; e < chunks[c].second; ++e) {
+            if (bit_width == BIT_WIDTH_64) {
+                for (size_t q = 0; q < nq; ++q) f(q, e, s64(q, e), (size_t)1);
+                continue;
+            }
+            size_t m8 = 1;
+            if (bit_width == BIT_WIDTH_8) {
+                m8 = 0;
+                for (size_t q = 0; q < nq; ++q) {
+                    const TierOut o = J.t8[at(q, e)];
+                    if (o.accept()) f(q, e, (int64_t)o.score, (size_t)1);
+                    else ++m8;
+                }
+                ov8 += m8;
+                if (m8 == 0) continue;
+            }
+            size_t m16 = 0;
+            for (size_t q = 0; q < nq; ++q) {
+                const TierOut o = J.t16[at(q, e)];
+                if (o.accept()) f(q, e, (int64_t)o.score, m8);
+                else ++m16;
+            }
+            ov16 += m8 * m16;
+            if (m16)
+                for (size_t q = 0; q < nq; ++q) f(q, e, s64(q, e), m8 * m16);
+        }
+    };
+
+    // each chunk's m largest inserts (descending) and its overflow counts
+    constexpr size_t kTop = 16;
+    const size_t m = std::min<size_t>(k + 1, kTop);
+    std::vector<int64_t> top(nc * kTop);
+    std::vector<uint8_t> ntop(nc, 0);
+    std::vector<size_t> ov8(nc, 0), ov16(nc, 0);
+    over_chunks(nc, [&](size_t c) {
+        int64_t tv[kTop];
+        size_t tn = 0, n8 = 0, n16 = 0;
+        inserts(c, n8, n16, [&](size_t, uint32_t, int64_t score, size_t copies) {
+            for (size_t r = 0; r < copies; ++r) {
+                if (tn == m && !(score > tv[m - 1])) break;
+                size_t i = tn < m ? tn++ : m - 1;
+                while (i > 0 && tv[i - 1] < score) {
+                    tv[i] = tv[i - 1];
+                    --i;
+                }
+                tv[i] = score;
+            }
+        });
+        std::copy(tv, tv + tn, top.begin() + c * kTop);
+        ntop[c] = (uint8_t)tn;
+        ov8[c] = n8;
+        ov16[c] = n16;
+    });
+    for (size_t c = 0; c < nc; ++c) {
+        J.st.overflow_8_bit_count += ov8[c];
+        J.st.overflow_16_bit_count += ov16[c];
+    }
+    if (k == 0) {  // a heap of no elements keeps nothing
+        J.st.ms_replay = (now_us() - t_replay) / 1000.0;
+        return;
+    }
+
+    // Thresholds: the k-th largest of the chunks' m largest inserts before
+    // each chunk, a lower bound of the heap's root there (exact for m = k+1).
+    // above[c] counts the chunk's inserts over it, exactly while below m.
+    std::vector<uint8_t> full(nc, 0);
+    std::vector<int64_t> thr(nc, 0);
+    std::vector<size_t> above(nc, 0);
+    {
+        std::priority_queue<int64_t, std::vector<int64_t>, std::greater<int64_t>> best;
+        for (size_t c = 0; c < nc; ++c) {
+            const int64_t* tv = &top[c * kTop];
+            full[c] = best.size() >= k;
+            if (full[c]) {
+                thr[c] = best.top();
+                for (size_t i = 0; i < ntop[c]; ++i) above[c] += tv[i] > thr[c];
+            }
+            for (size_t i = 0; i < ntop[c]; ++i) {
+                if (best.size() < k) {
+                    best.push(tv[i]);
+                } else if (best.top() < tv[i]) {
+                    best.pop();
+                    best.push(tv[i]);
+                } else {
+                    break;  // descending: the rest are no larger
+                }
+            }
+        }
+    }
+
+    // the inserts that can change the heap, chunk by chunk in the reference's order
+    std::vector<std::vector<Ins>> per_chunk(nc);
+    std::vector<uint32_t> work;
+    for (size_t c = 0; c < nc; ++c)
+        if (!full[c] || above[c] > 0) work.push_back((uint32_t)c);
+    over_chunks(work.size(), [&](size_t w) {
+        const size_t c = work[w];
         std::vector<Ins>& out = per_chunk[c];
-        out.reserve((size_t)(e1 - e0) * nq);
+        const bool filter = full[c];
+        const int64_t t = thr[c];
+        auto keep = [&](int64_t score) { return !filter || score > t; };
+        if (filter && above[c] == 1) {
+            // one insert above the threshold (m >= 2 here, so the count is
+            // exact): its place in the order does not matter
+            size_t d8 = 0, d16 = 0;
+            inserts(c, d8, d16, [&](size_t q, uint32_t e, int64_t score, size_t) {
+                if (score > t) out.push_back(Ins{e, (uint8_t)q, score});
+            });
+            return;
+        }
+        const uint32_t e0 = chunks[c].first, e1 = chunks[c].second;
         std::vector<uint32_t> items(e1 - e0), ovf, ovf2;
         std::iota(items.begin(), items.end(), e0);
         auto tier64 = [&](const std::vector<uint32_t>& list) {
             for (size_t q = 0; q < nq; ++q)
-                for (uint32_t e : list) out.push_back(Ins{e, (uint8_t)q, J.s64[at(q, e)]});
+                for (uint32_t e : list) {
+                    const int64_t score = s64(q, e);
+                    if (keep(score)) out.push_back(Ins{e, (uint8_t)q, score});
+                }
         };
         auto tier16 = [&](const std::vector<uint32_t>& list) {
             ovf2.clear();
             for (size_t q = 0; q < nq; ++q)
                 lane_schedule(
-                    W16, list.size(), [&](size_t k) { return (int64_t)J.t16[at(q, list[k])].dur; },
-                    [&](size_t k) {
-                        const TierOut& o = J.t16[at(q, list[k])];
-                        if (o.accept) out.push_back(Ins{list[k], (uint8_t)q, o.score});
-                        else ovf2.push_back(list[k]);
+                    W16, list.size(), [&](size_t i) { return (int64_t)J.t16[at(q, list[i])].dur(); },
+                    [&](size_t i) {
+                        const TierOut o = J.t16[at(q, list[i])];
+                        if (!o.accept()) ovf2.push_back(list[i]);
+                        else if (keep(o.score)) out.push_back(Ins{list[i], (uint8_t)q, o.score});
                     });
-            if (!ovf2.empty()) {
-                ov16[c] += ovf2.size();
-                tier64(ovf2);
-            }
+            if (!ovf2.empty()) tier64(ovf2);
         };
         if (bit_width == BIT_WIDTH_8) {
             for (size_t q = 0; q < nq; ++q)
                 lane_schedule(
-                    W8, items.size(), [&](size_t k) { return (int64_t)J.t8[at(q, items[k])].dur; },
-                    [&](size_t k) {
-                        const TierOut& o = J.t8[at(q, items[k])];
-                        if (o.accept) out.push_back(Ins{items[k], (uint8_t)q, o.score});
-                        else ovf.push_back(items[k]);
+                    W8, items.size(), [&](size_t i) { return (int64_t)J.t8[at(q, items[i])].dur(); },
+                    [&](size_t i) {
+                        const TierOut o = J.t8[at(q, items[i])];
+                        if (!o.accept()) ovf.push_back(items[i]);
+                        else if (keep(o.score)) out.push_back(Ins{items[i], (uint8_t)q, o.score});
                     });
-            if (!ovf.empty()) {
-                ov8[c] += ovf.size();
-                tier16(ovf);
-            }
+            if (!ovf.empty()) tier16(ovf);
         } else if (bit_width == BIT_WIDTH_16) {
             tier16(items);
         } else {
             tier64(items);
         }
-    };
-    if (threads && chunks.size() > 8) {
-        parallel_for(chunks.size(), do_chunk);
-    } else {
-        for (size_t c = 0; c < chunks.size(); ++c) do_chunk(c);
-    }
-    MinHeap heap(J.rq.hitcount);
-    for (size_t c = 0; c < chunks.size(); ++c) {
+    });
+
+    MinHeap heap(k);
+    for (size_t c = 0; c < nc; ++c)
         for (const Ins& x : per_chunk[c]) {
             if (heap.rejects(x.score)) continue;
             const DbEntry& e = db.entries[x.e];
             heap.add(Elem{e.id, e.frame, e.strand, x.q, x.score});
         }
-        J.st.overflow_8_bit_count += ov8[c];
-        J.st.overflow_16_bit_count += ov16[c];
-    }
     J.hits = heap.sorted();
     J.st.ms_replay = (now_us() - t_replay) / 1000.0;
 }
@@ -1372,9 +1510,10 @@ void run_batch(Exec& ex, const std::vector<SearchRequest>& reqs, std::vector<Sea
         J.st.seq_count = N;
         for (const QueryBuf& q : J.qs) J.st.cells += (uint64_t)q.len * total_residues;
         const size_t nq = J.qs.size();
-        J.in16.assign(N, 0);
-        J.need64.assign(N, 0);
-        J.s64.assign(nq * N, 0);
+        J.in16.reset(N);
+        J.in16.fill_zero();
+        J.need64.reset(N);
+        J.need64.fill_zero();
         J.sw_single = J.rq.type == kSW && mi.fits8 && (go + ge) >= -128;
         if (J.rq.bit_width == BIT_WIDTH_8) J.first_bits = J.sw_single ? 16 : 8;
         else if (J.rq.bit_width == BIT_WIDTH_16) J.first_bits = 16;
@@ -1498,46 +1637,58 @@ void run_batch(Exec& ex, const std::vector<SearchRequest>& reqs, std::vector<Sea
         };
         const int bw = J.rq.bit_width;
         if (bw == BIT_WIDTH_8) {
-            J.t8.resize(nq * N);
+            J.t8.reset(nq * N);
             if (J.sw_single) {
-                J.t16.resize(nq * N);
+                J.t16.reset(nq * N);
                 over_entries([&](uint32_t e0, uint32_t e1) {
                     for (size_t q = 0; q < nq; ++q)
                         for (uint32_t e = e0; e < e1; ++e) {
                             const PairResult& r = raw(q, e);
-                            J.t8[at(q, e)] = decode(type, kTier16, r, db.entries[e].len, true);
-                            J.t16[at(q, e)] = decode(type, kTier16, r, db.entries[e].len, false);
-                            if (!J.t8[at(q, e)].accept) J.in16[e] = 1;
+                            const TierOut o8 = decode(type, kTier16, r, db.entries[e].len, true);
+                            const TierOut o16 = decode(type, kTier16, r, db.entries[e].len, false);
+                            J.t8[at(q, e)] = o8;
+                            J.t16[at(q, e)] = o16;
+                            if (!o8.accept()) J.in16[e] = 1;
                         }
+                    // an entry reaches the exact tier when some query overflows
+                    // both tiers
+                    for (size_t q = 0; q < nq; ++q)
+                        for (uint32_t e = e0; e < e1; ++e)
+                            if (J.in16[e] && !J.t16[at(q, e)].accept()) J.need64[e] = 1;
                 });
             } else {
+                size_t n16 = 0;
+                std::mutex mu;
                 over_entries([&](uint32_t e0, uint32_t e1) {
                     for (size_t q = 0; q < nq; ++q)
                         for (uint32_t e = e0; e < e1; ++e) {
-                            J.t8[at(q, e)] = decode(type, kTier8, raw(q, e), db.entries[e].len, false);
-                            if (!J.t8[at(q, e)].accept) J.in16[e] = 1;
+                            const TierOut o8 = decode(type, kTier8, raw(q, e), db.entries[e].len, false);
+                            J.t8[at(q, e)] = o8;
+                            if (!o8.accept()) J.in16[e] = 1;
                         }
+                    size_t n = 0;
+                    for (uint32_t e = e0; e < e1; ++e) n += J.in16[e];
+                    std::lock_guard<std::mutex> lk(mu);
+                    n16 += n;
                 });
-                PairSet sub;
-                sub.nq = nq;
-                for (uint32_t e = 0; e < N; ++e)
-                    if (J.in16[e]) sub.entries.push_back(e);
-                J.t16.resize(nq * N);
-                if (sub.entries.size() * 2 >= N && N > 0) {
+                J.t16.reset(nq * N);
+                if (n16 * 2 >= N && N > 0) {
                     // most entries overflowed: re-run the whole DB on its cached
                     // streams together with the batch's other such searches
                     J.requeue_full = true;
-                } else if (!sub.entries.empty()) {
+                } else if (n16 > 0) {
                     J.requeue_sub = true;
                 }
             }
         } else if (bw == BIT_WIDTH_16) {
-            J.t16.resize(nq * N);
+            J.t16.reset(nq * N);
+            if (N) memset(J.in16.data(), 1, N);
             over_entries([&](uint32_t e0, uint32_t e1) {
                 for (size_t q = 0; q < nq; ++q)
                     for (uint32_t e = e0; e < e1; ++e) {
-                        J.t16[at(q, e)] = decode(type, kTier16, raw(q, e), db.entries[e].len, false);
-                        J.in16[e] = 1;
+                        const TierOut o = decode(type, kTier16, raw(q, e), db.entries[e].len, false);
+                        J.t16[at(q, e)] = o;
+                        if (!o.accept()) J.need64[e] = 1;
                     }
             });
         }
@@ -1563,7 +1714,9 @@ void run_batch(Exec& ex, const std::vector<SearchRequest>& reqs, std::vector<Sea
         for (size_t q = 0; q < nq; ++q)
             for (size_t k = 0; k < sub.entries.size(); ++k) {
                 const uint32_t e = sub.entries[k];
-                J.t16[at(q, e)] = decode(J.rq.type, kTier16, R2.out[sub.index(q, k)], db.entries[e].len, false);
+                const TierOut o = decode(J.rq.type, kTier16, R2.out[sub.index(q, k)], db.entries[e].len, false);
+                J.t16[at(q, e)] = o;
+                if (!o.accept()) J.need64[e] = 1;
             }
     }
 
@@ -1610,9 +1763,12 @@ void run_batch(Exec& ex, const std::vector<SearchRequest>& reqs, std::vector<Sea
                 computed_job[j] += R.computed * nq / std::max<size_t>(R.qs.size(), 1);
                 for (size_t q = 0; q < nq; ++q)
                     for (uint32_t e = 0; e < N; ++e)
-                        if (J.in16[e])
-                            J.t16[at(q, e)] =
+                        if (J.in16[e]) {
+                            const TierOut o =
                                 decode(J.rq.type, kTier16, R.out[ps.index(J.rq_off + q, e)], db.entries[e].len, false);
+                            J.t16[at(q, e)] = o;
+                            if (!o.accept()) J.need64[e] = 1;
+                        }
             }
         }
     }
@@ -1626,60 +1782,62 @@ void run_batch(Exec& ex, const std::vector<SearchRequest>& reqs, std::vector<Sea
             const TierRun& R = *runs[J.group];
             return R.out[R.ps->index(J.q_off + q, e)];
         };
-        // entries that reach the exact tier
-        if (bw == BIT_WIDTH_64) {
-            // scores only: the 16-bit pass where it is provably exact
-            std::fill(J.need64.begin(), J.need64.end(), 1);
-            if (J.group >= 0) {
-                J.t16.resize(nq * N);
-                const int64_t gap = std::abs((int64_t)go) + std::abs((int64_t)ge);
-                std::fill(J.need64.begin(), J.need64.end(), 0);
-                over_entries([&](uint32_t e0, uint32_t e1) {
-                for (size_t q = 0; q < nq; ++q)
-                    for (uint32_t e = e0; e < e1; ++e) {
-                        TierOut o = decode(type, kTier16, raw(q, e), db.entries[e].len, false);
-                        // an empty entry's lane scores a padding block; 64 bit scores the empty sequence
-                        if (db.entries[e].len == 0) o.accept = false;
-                        if (type == kNW) {
-                            // exact when no value can reach either end of the int16 range
-                            const int64_t ml = (int64_t)std::min<size_t>(J.qs[q].len, db.entries[e].len);
-                            const int64_t span = 3 * gap + gap * (int64_t)(J.qs[q].len + db.entries[e].len + 4);
-                            const int64_t up = std::max<int64_t>(mi.maxv, 0) * ml;
-                            const int64_t lowest = -span - std::max<int64_t>(-mi.minv, 0) * ml;
-                            if (!(lowest > -32768 + gap + 1 && up < 32767)) o.accept = false;
-                        }
-                        J.t16[at(q, e)] = o;
-                        if (!o.accept) J.need64[e] = 1;
-                    }
-                });
-            }
-        } else {
-            over_entries([&](uint32_t e0, uint32_t e1) {
-                for (size_t q = 0; q < nq; ++q)
-                    for (uint32_t e = e0; e < e1; ++e)
-                        if (J.in16[e] && !J.t16[at(q, e)].accept) J.need64[e] = 1;
-            });
+        // a 64-bit search scores with the 16-bit pass where it is provably
+        // exact; the other entries reach the exact tier
+        if (bw != BIT_WIDTH_64) return;
+        if (J.group < 0) {
+            if (N) memset(J.need64.data(), 1, N);
+            return;
         }
+        J.t16.reset(nq * N);
+        const int64_t gap = std::abs((int64_t)go) + std::abs((int64_t)ge);
+        over_entries([&](uint32_t e0, uint32_t e1) {
+            for (size_t q = 0; q < nq; ++q)
+                for (uint32_t e = e0; e < e1; ++e) {
+                    TierOut o = decode(type, kTier16, raw(q, e), db.entries[e].len, false);
+                    // an empty entry's lane scores a padding block; 64 bit scores the empty sequence
+                    bool exact = o.accept() && db.entries[e].len != 0;
+                    if (type == kNW) {
+                        // exact when no value can reach either end of the int16 range
+                        const int64_t ml = (int64_t)std::min<size_t>(J.qs[q].len, db.entries[e].len);
+                        const int64_t span = 3 * gap + gap * (int64_t)(J.qs[q].len + db.entries[e].len + 4);
+                        const int64_t up = std::max<int64_t>(mi.maxv, 0) * ml;
+                        const int64_t lowest = -span - std::max<int64_t>(-mi.minv, 0) * ml;
+                        if (!(lowest > -32768 + gap + 1 && up < 32767)) exact = false;
+                    }
+                    if (!exact) {
+                        o.d = -o.dur();
+                        J.need64[e] = 1;
+                    }
+                    J.t16[at(q, e)] = o;
+                }
+        });
     });
     for (size_t j = 0; j < jobs.size(); ++j) {
         Job& J = jobs[j];
         const size_t nq = J.qs.size();
-        const int bw = J.rq.bit_width;
         double& ms_re = ms_re_job[j];
         uint64_t& computed = computed_job[j];
-        std::vector<std::pair<uint32_t, uint32_t>> p64;
-        for (uint32_t e = 0; e < N; ++e)
-            if (J.need64[e])
-                for (size_t q = 0; q < nq; ++q) p64.push_back({(uint32_t)q, e});
-        if (!p64.empty()) {
-            std::vector<int64_t> o64;
-            gpu_exact(ex, J.rq.type, J.qs, p64, o64, &ms_re, &computed);
-            for (size_t k = 0; k < p64.size(); ++k) J.s64[at(p64[k].first, p64[k].second)] = o64[k];
+        // pairs of the exact tier: entry by entry, every query of each
+        J.x_entries.clear();
+        J.x64.clear();
+        const uint8_t* nd = J.need64.data();
+        for (uint32_t e = 0; e < N;) {
+            uint64_t w;
+            if (e + 8 <= N && (memcpy(&w, nd + e, 8), w == 0)) {
+                e += 8;
+                continue;
+            }
+            if (nd[e]) J.x_entries.push_back(e);
+            ++e;
         }
-        if (bw == BIT_WIDTH_64)
-            for (size_t q = 0; q < nq; ++q)
-                for (uint32_t e = 0; e < N; ++e)
-                    if (!J.need64[e]) J.s64[at(q, e)] = J.t16[at(q, e)].score;
+        if (!J.x_entries.empty()) {
+            std::vector<std::pair<uint32_t, uint32_t>> p64;
+            p64.reserve(J.x_entries.size() * nq);
+            for (uint32_t e : J.x_entries)
+                for (size_t q = 0; q < nq; ++q) p64.push_back({(uint32_t)q, e});
+            gpu_exact(ex, J.rq.type, J.qs, p64, J.x64, &ms_re, &computed);
+        }
         J.st.ms_gpu_requeue = ms_re;
         J.st.cells_computed = computed;
     }

@@ -1,11 +1,14 @@
 // util.cpp -- messages, fatal errors, process-wide configuration and the
 // host worker pool.
+#include <algorithm>
 #include <atomic>
 #include <condition_variable>
 #include <cstdarg>
 #include <cstdio>
 #include <cstdlib>
 #include <deque>
+#include <iterator>
+#include <map>
 #include <memory>
 #include <mutex>
 #include <thread>
@@ -102,6 +105,61 @@ private:
 void parallel_for(size_t n, const std::function<void(size_t)>& fn) { WorkerPool::get().run(n, fn); }
 
 size_t parallel_width() { return WorkerPool::get().width(); }
+
+namespace {
+
+struct BlockCache {
+    static constexpr size_t kMaxCached = size_t(4) << 30;
+    std::mutex mu;
+    std::multimap<size_t, void*> free;  // capacity -> block
+    size_t cached = 0;
+    static BlockCache& get() {
+        static BlockCache* c = new BlockCache();  // never destroyed: blocks may be released during teardown
+        return *c;
+    }
+};
+
+}  // namespace
+
+void* host_block_acquire(size_t bytes, size_t* capacity) {
+    bytes = std::max<size_t>(bytes, 64);
+    BlockCache& bc = BlockCache::get();
+    {
+        std::lock_guard<std::mutex> lk(bc.mu);
+        // the smallest cached block that fits, if it is at most twice the request
+        auto it = bc.free.lower_bound(bytes);
+        if (it != bc.free.end() && it->first <= 2 * bytes) {
+            void* p = it->second;
+            *capacity = it->first;
+            bc.cached -= it->first;
+            bc.free.erase(it);
+            return p;
+        }
+    }
+    void* p = malloc(bytes);
+    if (!p) fatal("Unable to allocate enough memory: %zu.", bytes);
+    *capacity = bytes;
+    return p;
+}
+
+void host_block_release(void* p, size_t capacity) {
+    if (!p) return;
+    BlockCache& bc = BlockCache::get();
+    std::vector<void*> drop;
+    {
+        std::lock_guard<std::mutex> lk(bc.mu);
+        bc.free.emplace(capacity, p);
+        bc.cached += capacity;
+        // over the limit: give back the largest blocks
+        while (bc.cached > BlockCache::kMaxCached && !bc.free.empty()) {
+            auto it = std::prev(bc.free.end());
+            bc.cached -= it->first;
+            drop.push_back(it->second);
+            bc.free.erase(it);
+        }
+    }
+    for (void* d : drop) ::free(d);
+}
 
 Config& config() {
     static Config c;
