@@ -361,7 +361,8 @@ __device__ __forceinline__ void run_task(const ScoreParams& P, uint32_t gbase0, 
                 }
             }
         };
-        if (MODE == 1 && __any(lane_last && (ls_lo || ls_hi))) columns(Tag<true>{});
+        // only the lane holding the last query row records a score cell
+        if (MODE == 1 && __any(lane_last && g == last_g && (ls_lo || ls_hi))) columns(Tag<true>{});
         else columns(Tag<false>{});
         // overflow predicates at the end of the block: a packed test here (in
         // the cell loop's block, so the running reductions stay in the loop),
