@@ -156,6 +156,15 @@ void ssa_amd_get_matrix(int64_t* out);
 /* Name of the GPU architecture the library runs on (e.g. "gfx950:sramecc+:xnack-"). */
 const char* ssa_amd_device_arch(void);
 
+/* Self-check of the host replay of the reference's lane schedule and heap
+ * (host only, no GPU call): `cases` random sets of per-pair tier results
+ * (overflows at every width, heavy score ties, 1-6 query buffers, hit counts
+ * 1-5000), replayed once skipping the chunks that cannot change the heap and
+ * once in full.  Returns the number of cases whose hits (scores, IDs, strands,
+ * frames, query buffers, order) or overflow counts differ: 0 is a pass.  No
+ * counterpart in the reference. */
+int ssa_amd_selfcheck_replay(uint64_t seed, int cases);
+
 #ifdef __cplusplus
 }
 #endif

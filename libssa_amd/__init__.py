@@ -120,6 +120,7 @@ def _load():
         "ssa_amd_autotune": (I, [I, _C.POINTER(_C.c_size_t), S, I, _C.POINTER(I), _C.POINTER(I),
                                  _C.POINTER(_C.c_double)]),
         "ssa_amd_autotune_reset": (None, []),
+        "ssa_amd_selfcheck_replay": (I, [_C.c_uint64, I]),
         "ssa_amd_server_start": (P, [I, S, S]),
         "ssa_amd_server_submit": (_C.c_int64, [P, _C.POINTER(_Request)]),
         "ssa_amd_server_wait": (_C.POINTER(_AlignmentList), [P, _C.c_int64, _C.POINTER(Stats),
@@ -323,3 +324,9 @@ def autotune(search_type, lengths, copies=1):
 
 
 def autotune_reset(): lib.ssa_amd_autotune_reset()
+
+
+def selfcheck_replay(seed, cases):
+    """Host-only check of the replay's chunk filter against the full
+    simulation; returns the number of differing cases (0 = pass)."""
+    return lib.ssa_amd_selfcheck_replay(seed, cases)

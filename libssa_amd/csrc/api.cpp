@@ -205,6 +205,8 @@ int ssa_amd_autotune(int search_type, const size_t* lengths, size_t n, int copie
 
 void ssa_amd_autotune_reset(void) { autotune_reset(); }
 
+int ssa_amd_selfcheck_replay(uint64_t seed, int cases) { return cases > 0 ? replay_selfcheck(seed, cases) : 0; }
+
 int ssa_amd_db_pack(const char* path) { return write_packed_db(path); }
 
 void ssa_amd_get_matrix(int64_t* out) { memcpy(out, matrix().s, sizeof(matrix().s)); }

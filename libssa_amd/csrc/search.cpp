@@ -22,6 +22,7 @@
 #include <memory>
 #include <mutex>
 #include <queue>
+#include <random>
 #include <thread>
 
 #include "dbpack.h"
@@ -1221,9 +1222,10 @@ int64_t score64(const Job& J, size_t q, uint32_t e, uint32_t N) {
 // threshold per chunk from the earlier chunks' largest inserts (which do not
 // depend on order), runs the lane schedules only for chunks with two or more
 // inserts above it, and feeds the heap only those inserts, in the reference's
-// order.
-void job_replay(Job& J, const std::vector<std::pair<uint32_t, uint32_t>>& chunks, uint32_t N, bool threads) {
-    DeviceDB& db = g_db;
+// order.  filter = false simulates every chunk in full (the self-check's
+// reference).
+void job_replay(Job& J, const std::vector<DbEntry>& entries, const std::vector<std::pair<uint32_t, uint32_t>>& chunks,
+                uint32_t N, bool threads, bool filter = true) {
     const int64_t t_replay = now_us();
     const int W8 = lanes8(), W16 = lanes16();
     const size_t nq = J.qs.size();
@@ -1323,7 +1325,7 @@ void job_replay(Job& J, const std::vector<std::pair<uint32_t, uint32_t>>& chunks
         std::priority_queue<int64_t, std::vector<int64_t>, std::greater<int64_t>> best;
         for (size_t c = 0; c < nc; ++c) {
             const int64_t* tv = &top[c * kTop];
-            full[c] = best.size() >= k;
+            full[c] = filter && best.size() >= k;
             if (full[c]) {
                 thr[c] = best.top();
                 for (size_t i = 0; i < ntop[c]; ++i) above[c] += tv[i] > thr[c];
@@ -1404,7 +1406,7 @@ void job_replay(Job& J, const std::vector<std::pair<uint32_t, uint32_t>>& chunks
     for (size_t c = 0; c < nc; ++c)
         for (const Ins& x : per_chunk[c]) {
             if (heap.rejects(x.score)) continue;
-            const DbEntry& e = db.entries[x.e];
+            const DbEntry& e = entries[x.e];
             heap.add(Elem{e.id, e.frame, e.strand, x.q, x.score});
         }
     J.hits = heap.sorted();
@@ -1849,9 +1851,9 @@ void run_batch(Exec& ex, const std::vector<SearchRequest>& reqs, std::vector<Sea
 
     // ---- replay: searches in parallel, or chunks in parallel for few searches ----
     if (jobs.size() >= 4 && nthreads > 1) {
-        parallel_for(jobs.size(), [&](size_t j) { job_replay(jobs[j], chunks, N, false); });
+        parallel_for(jobs.size(), [&](size_t j) { job_replay(jobs[j], db.entries, chunks, N, false); });
     } else {
-        for (Job& J : jobs) job_replay(J, chunks, N, true);
+        for (Job& J : jobs) job_replay(J, db.entries, chunks, N, true);
     }
     for (Job& J : jobs)
         if (J.st.overflow_8_bit_count || J.st.overflow_16_bit_count)
@@ -1955,6 +1957,72 @@ int autotune(int type, const std::vector<size_t>& lengths, int copies, std::vect
 void autotune_reset() {
     std::lock_guard<std::mutex> lk(tuned_mutex());
     tuned_shapes().clear();
+}
+
+int replay_selfcheck(uint64_t seed, int cases) {
+    std::mt19937_64 rng(seed);
+    auto R = [&](uint64_t n) { return (uint64_t)(rng() % n); };
+    int bad = 0;
+    for (int t = 0; t < cases; ++t) {
+        static const int kWidths[3] = {BIT_WIDTH_8, BIT_WIDTH_16, BIT_WIDTH_64};
+        static const size_t kQueries[3] = {1, 2, 6};
+        static const size_t kHits[7] = {1, 2, 3, 5, 50, 100, 5000};
+        Job J;
+        memset(&J.st, 0, sizeof J.st);
+        J.rq = SearchRequest{kSW, nullptr, kHits[R(7)], kWidths[R(3)], COMPUTE_SCORE};
+        const size_t nq = kQueries[R(3)];
+        J.qs.assign(nq, QueryBuf{nullptr, 1, 0, 0});
+        const uint32_t N = 1 + (uint32_t)R(3000);
+        const int range = R(2) ? (int)(2 + R(40)) : 1000;  // narrow ranges: many equal scores
+        std::vector<DbEntry> entries(N);
+        for (uint32_t e = 0; e < N; ++e)
+            entries[e] = DbEntry{e / (1 + R(2)), 1, (uint8_t)R(2), (uint8_t)R(3), 0};
+        J.t8.reset(nq * N);
+        J.t16.reset(nq * N);
+        J.in16.reset(N);
+        J.in16.fill_zero();
+        J.need64.reset(N);
+        J.need64.fill_zero();
+        const uint64_t p8 = R(2) ? 20 : 300, p16 = R(2) ? 50 : 500;  // overflow rates per 1000
+        for (uint32_t e = 0; e < N; ++e)
+            for (size_t q = 0; q < nq; ++q) {
+                const int32_t d = 1 + (int32_t)R(12);
+                const TierOut o{R(1000) < p8 ? -d : d, (int32_t)R(range)};
+                J.t8[q * N + e] = o;
+                if (J.rq.bit_width == BIT_WIDTH_8 && !o.accept()) J.in16[e] = 1;
+            }
+        if (J.rq.bit_width != BIT_WIDTH_8) memset(J.in16.data(), 1, N);
+        for (uint32_t e = 0; e < N; ++e)
+            for (size_t q = 0; q < nq; ++q) {
+                const int32_t d = 1 + (int32_t)R(12);
+                const TierOut o{R(1000) < p16 ? -d : d, (int32_t)R(range) + 50};
+                J.t16[q * N + e] = o;
+                if (J.in16[e] && !o.accept()) J.need64[e] = 1;
+            }
+        for (uint32_t e = 0; e < N; ++e)
+            if (J.need64[e]) {
+                J.x_entries.push_back(e);
+                for (size_t q = 0; q < nq; ++q) J.x64.push_back((int64_t)R(range) + 100);
+            }
+        const size_t cs = 1 + R(400);
+        std::vector<std::pair<uint32_t, uint32_t>> chunks;
+        for (uint32_t c0 = 0; c0 < N; c0 += (uint32_t)cs) chunks.push_back({c0, (uint32_t)std::min<size_t>(N, c0 + cs)});
+        const bool threads = R(2);
+        job_replay(J, entries, chunks, N, threads, false);
+        const std::vector<Elem> full = J.hits;
+        const ssa_amd_stats_t st_full = J.st;
+        memset(&J.st, 0, sizeof J.st);
+        job_replay(J, entries, chunks, N, !threads, true);
+        bool same = full.size() == J.hits.size() && st_full.overflow_8_bit_count == J.st.overflow_8_bit_count &&
+                    st_full.overflow_16_bit_count == J.st.overflow_16_bit_count;
+        for (size_t i = 0; same && i < full.size(); ++i) {
+            const Elem &a = full[i], &b = J.hits[i];
+            same = a.score == b.score && a.db_id == b.db_id && a.query_id == b.query_id && a.db_frame == b.db_frame &&
+                   a.db_strand == b.db_strand;
+        }
+        bad += !same;
+    }
+    return bad;
 }
 
 p_alignment_list run_search(SearchType type, p_query query, size_t hitcount, int bit_width, int align_type) {

@@ -64,6 +64,12 @@ struct AutotuneResult {
 int autotune(int type, const std::vector<size_t>& lengths, int copies, std::vector<AutotuneResult>& out);
 void autotune_reset();
 
+// Self-check of the host replay: random per-pair tier results (overflows at
+// every width, heavy score ties, 1-6 query buffers, hit counts 1-5000),
+// replayed with the chunk filter and in full; returns the number of cases
+// whose hits or overflow counts differ.  Host only.
+int replay_selfcheck(uint64_t seed, int cases);
+
 class Exec;
 // Runs the searches as one batch on the context's stream: every result is
 // what the search returns alone.  one_graph: the first-tier launches of all
