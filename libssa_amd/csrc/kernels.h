@@ -47,6 +47,23 @@ struct QueryDesc {
                        // exact tier: int32 units, profile [qlen][32 codes]
     int32_t lb_off;    // in uint32 units: NW left boundary [pass*G*R rows][2] packed (H, E)
     int32_t pair_off;  // result index of (this query, entry position 0)
+    int32_t seg_off;   // packed query: its lanes' segments in ScoreLaunch::segs; -1 otherwise
+    int32_t pad;
+};
+
+// A packed query (SW): the rows of several queries laid end to end, each
+// query starting at a lane boundary and padded to whole lanes, scored as one
+// query of many passes.  Lane g of pass p takes its segment from
+// segs[seg_off + p * G + g]: the result index of its query's entry position 0
+// and the flags below.  A segment-top lane (its first row is its query's
+// first row) takes the matrix border instead of the row above; a
+// segment-bottom lane (the last lane of its query in the pass, or the pass's
+// last lane) issues the pair's score; a lane past the last query writes
+// nothing.
+constexpr int32_t kSegTop = 1, kSegBottom = 2, kSegNone = 4;
+struct SegLane {
+    int32_t pair_off;
+    int32_t flags;
 };
 
 // Per-launch constants in the kernel's value representation: every 16-bit
@@ -93,6 +110,7 @@ struct ScoreLaunch {
     const int32_t* slot_first;
     const int32_t* slot_entries;
     PairResult* results;
+    const SegLane* segs;
 };
 
 // one (stream, entry) of the residue interleaving kernel

@@ -144,13 +144,14 @@ __device__ __forceinline__ void finish_pair(PairResult* results, int pair, bool 
 // stage(p) loads pass p's into buffer p & 1 once every lane has left pass
 // p - 2.  Steps run two per iteration (an even count; an extra step is idle)
 // so that the one-step-ahead loads alternate between two register sets.
-template <int R, int G, int MODE, bool MULTI, class Stage>
+template <int R, int G, int MODE, bool MULTI, bool SEG, class Stage>
 __device__ __forceinline__ void run_task(const ScoreParams& P, uint32_t gbase0, const char* sp,
                                          const uint32_t* slbh0, const uint32_t* slbe0,
                                          const uint2* __restrict__ wres, uint4* __restrict__ wbnd,
                                          const int32_t* __restrict__ slot_entries, int pair_off,
-                                         PairResult* __restrict__ results, int sb_lo, int sb_hi, int k, int g,
-                                         int nbw, int nbp, int npass, int last_g, int last_r, const Stage& stage) {
+                                         const SegLane* __restrict__ segs, PairResult* __restrict__ results,
+                                         int sb_lo, int sb_hi, int k, int g, int nbw, int nbp, int npass, int last_g,
+                                         int last_r, const Stage& stage) {
     constexpr int NG = 64 / G;
     constexpr int PR = G * R;
     constexpr int LPR = Log2<PR>::value;
@@ -166,6 +167,21 @@ __device__ __forceinline__ void run_task(const ScoreParams& P, uint32_t gbase0, 
     const bool top_lane = g == 0;
     // the lane's pass and block within it (negative before the lane's first block)
     int pass = 0, b = -g;
+    // SEG: the lane's segment in its pass (a plain query: one segment per pass)
+    int seg_po = pair_off;
+    bool seg_top = top_lane, seg_bot = g == G - 1, seg_none = false;
+    auto load_seg = [&](int p) {
+        if constexpr (SEG) {
+            if (p < npass) {
+                const SegLane sl = segs[p * G + g];
+                seg_po = sl.pair_off;
+                seg_top = (sl.flags & kSegTop) != 0;
+                seg_bot = (sl.flags & kSegBottom) != 0;
+                seg_none = (sl.flags & kSegNone) != 0;
+            }
+        }
+    };
+    load_seg(0);
     uint32_t gbase = gbase0;  // the lane's rows in the profile buffer of its pass
     const uint32_t* slbh = slbh0;
     const uint32_t* slbe = slbe0;
@@ -204,7 +220,7 @@ __device__ __forceinline__ void run_task(const ScoreParams& P, uint32_t gbase0, 
         const bool lane_last = pass == npass - 1;
         const bool valid = pass < npass && (unsigned)b < (unsigned)nbw;
         const bool bot_out = g == G - 1 && pass < npass - 1;  // bottom row to the next pass
-        const bool score_lane = MODE == 0 ? g == G - 1 : (lane_last && g == last_g);
+        const bool score_lane = MODE == 0 ? seg_bot : (lane_last && g == last_g);
         // the lane's next block, in this pass or (MULTI) at the start of the next
         const bool wrap = MULTI && b + 1 == nbp;
         const int b_n = wrap ? 0 : b + 1, pass_n = wrap ? pass + 1 : pass;
@@ -223,14 +239,14 @@ __device__ __forceinline__ void run_task(const ScoreParams& P, uint32_t gbase0, 
         if (__any(st_lo || st_hi)) {
             const uint32_t m = (st_lo ? 0x0000ffffu : 0u) | (st_hi ? 0xffff0000u : 0u);
             if (st_lo) {
-                pair_lo = pair_off + slot_entries[sb_lo + kp_lo];
+                pair_lo = seg_none ? -1 : seg_po + slot_entries[sb_lo + kp_lo];
                 kp_lo++;
                 blk_lo = 0;
                 fa_lo = kNoFlag;
                 fb_lo = kNoFlag;
             }
             if (st_hi) {
-                pair_hi = pair_off + slot_entries[sb_hi + kp_hi];
+                pair_hi = seg_none ? -1 : seg_po + slot_entries[sb_hi + kp_hi];
                 kp_hi++;
                 blk_hi = 0;
                 fa_hi = kNoFlag;
@@ -296,6 +312,10 @@ __device__ __forceinline__ void run_task(const ScoreParams& P, uint32_t gbase0, 
         for (int c = 0; c < 4; ++c) {
             tbh[c] = shift_down<G>(toph[c], obh[c], top_lane);
             tbf[c] = shift_down<G>(topf[c], obf[c], top_lane);
+            if (SEG && seg_top) {  // a packed query's first row: the SW matrix border
+                tbh[c] = as_u(Z);
+                tbf[c] = as_u(Z);
+            }
         }
 
         s16x2 hmn = as_v(0u), hmx = as_v(0u);  // NW trackers, reset value of the reference
@@ -415,7 +435,8 @@ __device__ __forceinline__ void run_task(const ScoreParams& P, uint32_t gbase0, 
         }
         if (any_last) {
             if (MODE == 0) {  // the lane above finished the same pair one step earlier
-                const uint32_t scin = shift_down<G>(as_u(Z), scout, top_lane);
+                uint32_t scin = shift_down<G>(as_u(Z), scout, top_lane);
+                if (SEG && seg_top) scin = as_u(Z);
                 scout = as_u(vmax(S, as_v(scin)));
             }
             const s16x2 fin = MODE == 0 ? as_v(scout) : cap;
@@ -442,6 +463,7 @@ __device__ __forceinline__ void run_task(const ScoreParams& P, uint32_t gbase0, 
                 kp_lo = 0;
                 kp_hi = 0;
                 gbase ^= kProfBytes;
+                load_seg(pass);
                 const bool odd = pass & 1;
                 slbh = odd ? slbh0 + (PR + 1) : slbh0;
                 slbe = odd ? slbe0 + PR : slbe0;
@@ -538,12 +560,18 @@ __global__ __launch_bounds__(64 * kWavesPerWG) void score_kernel(const ScoreLaun
             load_pass(pass);
             __syncthreads();
         };
-        if (npass > 1)
-            run_task<R, G, MODE, true>(P, gbase, sp, slbh, slbe, wres, wbnd, L.slot_entries, Q.pair_off, L.results,
-                                       sb_lo, sb_hi, k, g, Wv.nblocks, nbp, npass, last_g, last_r, stage);
+        if (MODE == 0 && Q.seg_off >= 0)
+            run_task<R, G, MODE, true, MODE == 0>(P, gbase, sp, slbh, slbe, wres, wbnd, L.slot_entries, Q.pair_off,
+                                                  L.segs + Q.seg_off, L.results, sb_lo, sb_hi, k, g, Wv.nblocks,
+                                                  nbp, npass, last_g, last_r, stage);
+        else if (npass > 1)
+            run_task<R, G, MODE, true, false>(P, gbase, sp, slbh, slbe, wres, wbnd, L.slot_entries, Q.pair_off,
+                                              nullptr, L.results, sb_lo, sb_hi, k, g, Wv.nblocks, nbp, npass,
+                                              last_g, last_r, stage);
         else
-            run_task<R, G, MODE, false>(P, gbase, sp, slbh, slbe, wres, wbnd, L.slot_entries, Q.pair_off, L.results,
-                                        sb_lo, sb_hi, k, g, Wv.nblocks, nbp, npass, last_g, last_r, stage);
+            run_task<R, G, MODE, false, false>(P, gbase, sp, slbh, slbe, wres, wbnd, L.slot_entries, Q.pair_off,
+                                               nullptr, L.results, sb_lo, sb_hi, k, g, Wv.nblocks, nbp, npass,
+                                               last_g, last_r, stage);
     }
 }
 

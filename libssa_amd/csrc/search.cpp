@@ -625,6 +625,8 @@ struct TierRun {
     std::vector<QueryDesc> qd;
     std::vector<int16_t> prof;
     std::vector<uint32_t> lbv;
+    std::vector<SegLane> segs;
+    bool packed = false;  // SW with several queries: scored as one packed query
 
     ~TierRun() { local.release(); }
     bool empty() const { return ps == nullptr || ps->entries.empty() || qs.empty(); }
@@ -642,8 +644,10 @@ void tier_prepare(Exec& ex, TierRun& R) {
     if (R.empty()) return;
     R.out = (PairResult*)ex.pinned(R.pfx + "results", R.nout * sizeof(PairResult));
 
-    size_t qmax = 0;
-    for (const QueryBuf& q : R.qs) qmax = std::max(qmax, q.len);
+    // the query length the shape is chosen for: a packed query's total
+    size_t qmax = 0, qsum = 0;
+    for (const QueryBuf& q : R.qs) qmax = std::max(qmax, q.len), qsum += q.len;
+    if (R.type == kSW && nq > 1) qmax = qsum;
     if (R.sh.lanes == 0) {
         int64_t total_blocks = 0, max_blocks = 0;
         for (uint32_t e : ps.entries) {
@@ -659,11 +663,15 @@ void tier_prepare(Exec& ex, TierRun& R) {
 
     // ---- profiles [pass][code][row], left boundaries, per-query descriptors ----
     R.qd.assign(nq, QueryDesc{});
+    for (QueryDesc& d : R.qd) d.seg_off = -1;
     R.prof.clear();
     R.lbv.clear();
+    R.segs.clear();
     std::vector<int> npass(nq);
     const int64_t QRw = T.wrap(go + ge), Rw = T.wrap(ge);
-    for (size_t qi = 0; qi < nq; ++qi) {
+    // SW with several queries: packed queries (below)
+    R.packed = R.type == kSW && nq > 1;
+    for (size_t qi = 0; qi < nq && !R.packed; ++qi) {
         const QueryBuf& q = R.qs[qi];
         npass[qi] = (int)((q.len + PR - 1) / PR);
         const size_t rows = (size_t)npass[qi] * PR;
@@ -733,25 +741,116 @@ void tier_prepare(Exec& ex, TierRun& R) {
     }
     const StreamLayout* L = R.L;
 
-    // ---- tasks: every query against every group of 4 waves, costliest first ----
     const int nw = L->nwaves;
     if ((uint64_t)nq * n >= (uint64_t)INT32_MAX) fatal("libssa_amd: too many (query, entry) pairs in one launch");
     const int nt_q = nw / kWavesPerWG;
-    R.tasks.clear();
-    R.tasks.reserve((size_t)nq * nt_q);
-    std::vector<int64_t> cost;
-    cost.reserve((size_t)nq * nt_q);
-    bool multipass = false;
-    for (size_t qi = 0; qi < nq; ++qi) {
-        R.qd[qi].pair_off = (int32_t)(qi * n);
-        multipass |= npass[qi] > 1;
+    const int cap = score_resident_workgroups(sh.lanes, sh.rows, R.type == kSW ? 0 : 1);
+
+    // ---- packed queries (SW, several queries) ----
+    // Each query starts at a lane boundary and is padded to whole lanes (the
+    // padding rows score the lane minimum and never raise a maximum), so that
+    // short queries fill passes.  The queries are taken in order into packed
+    // queries of at most sq_max passes: a task runs all passes of its waves
+    // one after the other, so its length is passes x its longest wave, and
+    // the tasks over the longest waves (the longest DB entries) must stay a
+    // small part of a workgroup's share of the launch, or they run on alone
+    // after the rest has finished.
+    std::vector<int> sq_npass;
+    if (R.packed) {
+        int64_t per_pass = 0, longest = 1;  // task steps per pass: all tasks, the longest task
         for (int t = 0; t < nt_q; ++t) {
-            R.tasks.push_back(ScoreTask{(int32_t)qi, t * kWavesPerWG});
-            cost.push_back((int64_t)L->wave_nblocks[t * kWavesPerWG] * npass[qi] + sh.lanes);
+            per_pass += L->wave_nblocks[t * kWavesPerWG] + sh.lanes;
+            longest = std::max<int64_t>(longest, L->wave_nblocks[t * kWavesPerWG] + sh.lanes);
         }
-        for (int w = 0; w < nw; ++w) R.computed += (uint64_t)L->wave_nblocks[w] * 4 * spw * (uint64_t)npass[qi] * PR;
+        std::vector<int64_t> qrows(nq);
+        int64_t total = 0;
+        for (size_t qi = 0; qi < nq; ++qi) {
+            qrows[qi] = (int64_t)((R.qs[qi].len + sh.rows - 1) / sh.rows) * sh.rows;
+            total += qrows[qi];
+        }
+        const int64_t total_passes = std::max<int64_t>(1, (total + PR - 1) / PR);
+        const int64_t sq_max =
+            std::max<int64_t>(1, total_passes * per_pass / (4 * (int64_t)std::max(cap, 1) * longest));
+        auto build_sq = [&](size_t q0, size_t q1) {
+            std::vector<int64_t> start(q1 - q0);
+            int64_t rows = 0;
+            for (size_t qi = q0; qi < q1; ++qi) {
+                start[qi - q0] = rows;
+                rows += qrows[qi];
+            }
+            const int np = (int)((rows + PR - 1) / PR);
+            if (np == 0) return;  // empty queries only: their results keep the initial values
+            if ((int64_t)R.prof.size() + (int64_t)np * 32 * PR >= (int64_t)INT32_MAX)
+                fatal("libssa_amd: packed queries too long");
+            std::vector<int32_t> row_q((size_t)np * PR, -1);  // query of each row (-1 past the last)
+            for (size_t qi = q0; qi < q1; ++qi)
+                for (int64_t i = start[qi - q0]; i < start[qi - q0] + qrows[qi]; ++i) row_q[i] = (int32_t)qi;
+            QueryDesc d{};
+            d.qlen = (int32_t)std::min<int64_t>(rows, INT32_MAX);
+            d.qp_off = (int32_t)R.prof.size();
+            d.seg_off = (int32_t)R.segs.size();
+            R.prof.resize(R.prof.size() + (size_t)np * 32 * PR);
+            int16_t* pf = R.prof.data() + d.qp_off;
+            for (int p = 0; p < np; ++p)
+                for (int c = 0; c < kDim; ++c)
+                    for (int r = 0; r < PR; ++r) {
+                        const int64_t i = (int64_t)p * PR + r;
+                        const int32_t qi = row_q[i];
+                        const int64_t k = qi >= 0 ? i - start[qi - q0] : 0;
+                        int32_t v = -32768;
+                        if (qi >= 0 && k < (int64_t)R.qs[qi].len) v = T.repr(T.wrap(M.at(c, R.qs[qi].seq[k])));
+                        pf[((size_t)p * 32 + c) * PR + r] = (int16_t)v;
+                    }
+            for (int p = 0; p < np; ++p)
+                for (int g = 0; g < sh.lanes; ++g) {
+                    const int64_t i = (int64_t)p * PR + (int64_t)g * sh.rows;
+                    const int32_t qi = row_q[i];
+                    if (qi < 0) {
+                        R.segs.push_back(SegLane{0, kSegNone});
+                        continue;
+                    }
+                    const bool top = i == start[qi - q0];
+                    const bool bottom = g == sh.lanes - 1 || row_q[i + sh.rows] != qi;
+                    R.segs.push_back(SegLane{(int32_t)(qi * n), (top ? kSegTop : 0) | (bottom ? kSegBottom : 0)});
+                }
+            R.qd.push_back(d);
+            sq_npass.push_back(np);
+        };
+        size_t q0 = 0;
+        int64_t rows = 0;
+        for (size_t qi = 0; qi < nq; ++qi) {
+            if (qi > q0 && rows + qrows[qi] > sq_max * PR) {
+                build_sq(q0, qi);
+                q0 = qi;
+                rows = 0;
+            }
+            rows += qrows[qi];
+        }
+        build_sq(q0, nq);
     }
-    if (nq > 1) {
+
+    // ---- tasks: every (packed) query against every group of 4 waves, costliest first ----
+    R.tasks.clear();
+    std::vector<int64_t> cost;
+    bool multipass = false;
+    for (size_t qi = 0; qi < nq; ++qi) R.qd[qi].pair_off = (int32_t)(qi * n);
+    // a query of no rows gets no tasks: its results keep the initial values
+    // (SW: score 0, no flag)
+    auto add_tasks = [&](int32_t query, int np) {
+        if (np <= 0) return;
+        multipass |= np > 1;
+        for (int t = 0; t < nt_q; ++t) {
+            R.tasks.push_back(ScoreTask{query, t * kWavesPerWG});
+            cost.push_back((int64_t)L->wave_nblocks[t * kWavesPerWG] * np + sh.lanes);
+        }
+        for (int w = 0; w < nw; ++w) R.computed += (uint64_t)L->wave_nblocks[w] * 4 * spw * (uint64_t)np * PR;
+    };
+    if (R.packed) {
+        for (size_t s = 0; s < sq_npass.size(); ++s) add_tasks((int32_t)(nq + s), sq_npass[s]);
+    } else {
+        for (size_t qi = 0; qi < nq; ++qi) add_tasks((int32_t)qi, npass[qi]);
+    }
+    if (R.tasks.size() > (size_t)nt_q) {
         std::vector<uint32_t> ord(R.tasks.size());
         std::iota(ord.begin(), ord.end(), 0u);
         std::stable_sort(ord.begin(), ord.end(), [&](uint32_t a, uint32_t b) { return cost[a] > cost[b]; });
@@ -759,7 +858,6 @@ void tier_prepare(Exec& ex, TierRun& R) {
         for (size_t i = 0; i < ord.size(); ++i) sorted[i] = R.tasks[ord[i]];
         R.tasks.swap(sorted);
     }
-    const int cap = score_resident_workgroups(sh.lanes, sh.rows, R.type == kSW ? 0 : 1);
     R.grid = std::max(1, std::min<int>((int)R.tasks.size(), cap > 0 ? cap : (int)R.tasks.size()));
     int32_t max_nb = 0;
     for (int w = 0; w < nw; ++w) max_nb = std::max(max_nb, L->wave_nblocks[w]);
@@ -773,6 +871,7 @@ void tier_prepare(Exec& ex, TierRun& R) {
     auto* d_qd = (QueryDesc*)ex.arena(name("qdesc"), R.qd.size() * sizeof(QueryDesc));
     auto* d_prof = (int16_t*)ex.arena(name("prof"), std::max<size_t>(R.prof.size(), 8) * sizeof(int16_t));
     auto* d_lb = (uint32_t*)ex.arena(name("lb"), std::max<size_t>(R.lbv.size(), 1) * sizeof(uint32_t));
+    auto* d_segs = (SegLane*)ex.arena(name("segs"), std::max<size_t>(R.segs.size(), 1) * sizeof(SegLane));
     auto* d_bnd = (uint4*)ex.arena(
         name("bnd"), (size_t)std::max<int64_t>((int64_t)R.grid * kWavesPerWG * bnd_per_wave, 1) * sizeof(uint4));
     auto* d_counter = (int32_t*)ex.arena(name("counter"), sizeof(int32_t));
@@ -781,6 +880,7 @@ void tier_prepare(Exec& ex, TierRun& R) {
     ex.upload(d_qd, R.qd.data(), R.qd.size() * sizeof(QueryDesc));
     ex.upload(d_prof, R.prof.data(), R.prof.size() * sizeof(int16_t));
     ex.upload(d_lb, R.lbv.data(), R.lbv.size() * sizeof(uint32_t));
+    ex.upload(d_segs, R.segs.data(), R.segs.size() * sizeof(SegLane));
     ScoreLaunch& K = R.K;
     K.waves = L->d_waves;
     K.tasks = d_tasks;
@@ -796,6 +896,7 @@ void tier_prepare(Exec& ex, TierRun& R) {
     K.slot_first = L->d_slot_first;
     K.slot_entries = L->d_slot_entries;
     K.results = R.d_out;
+    K.segs = d_segs;
 }
 
 void tier_launch(hipStream_t st, TierRun& R) {
@@ -1524,10 +1625,12 @@ void run_batch(Exec& ex, const std::vector<SearchRequest>& reqs, std::vector<Sea
         size_t qmax = 0;
         for (const QueryBuf& q : J.qs) qmax = std::max(qmax, q.len);
         J.shape = choose_shape(qmax, total_blocks, max_blocks, J.rq.type == kSW ? 0 : 1);
+        // SW searches of one tier share one launch as a packed query whatever
+        // their lengths; NW searches share a launch per kernel shape
         int gi = -1;
         for (size_t g = 0; g < groups.size(); ++g)
             if (groups[g].type == J.rq.type && groups[g].bits == J.first_bits &&
-                groups[g].sh.lanes == J.shape.lanes && groups[g].sh.rows == J.shape.rows)
+                (J.rq.type == kSW || (groups[g].sh.lanes == J.shape.lanes && groups[g].sh.rows == J.shape.rows)))
                 gi = (int)g;
         if (gi < 0) {
             groups.push_back(Group{J.rq.type, J.first_bits, J.shape, {}});
@@ -1535,6 +1638,14 @@ void run_batch(Exec& ex, const std::vector<SearchRequest>& reqs, std::vector<Sea
         }
         J.group = gi;
         groups[gi].jobs.push_back(j);
+    }
+    for (Group& G : groups) {
+        if (G.type != kSW) continue;
+        size_t nqg = 0, qsum = 0;
+        for (size_t j : G.jobs)
+            for (const QueryBuf& q : jobs[j].qs) ++nqg, qsum += q.len;
+        if (nqg > 1) G.sh = choose_shape(qsum, total_blocks, max_blocks, 0);
+        for (size_t j : G.jobs) jobs[j].shape = G.sh;
     }
     std::vector<std::unique_ptr<TierRun>> runs(groups.size());
     for (size_t g = 0; g < groups.size(); ++g) {
@@ -1733,7 +1844,8 @@ void run_batch(Exec& ex, const std::vector<SearchRequest>& reqs, std::vector<Sea
             const Shape sh = choose_shape(qmax, total_blocks, max_blocks, J.rq.type == kSW ? 0 : 1);
             int gi = -1;
             for (size_t g = 0; g < rgroups.size(); ++g)
-                if (rgroups[g].type == J.rq.type && rgroups[g].sh.lanes == sh.lanes && rgroups[g].sh.rows == sh.rows)
+                if (rgroups[g].type == J.rq.type &&
+                    (J.rq.type == kSW || (rgroups[g].sh.lanes == sh.lanes && rgroups[g].sh.rows == sh.rows)))
                     gi = (int)g;
             if (gi < 0) {
                 rgroups.push_back(Group{J.rq.type, 16, sh, {}});
@@ -1741,6 +1853,13 @@ void run_batch(Exec& ex, const std::vector<SearchRequest>& reqs, std::vector<Sea
             }
             J.rgroup = gi;
             rgroups[gi].jobs.push_back(j);
+        }
+        for (Group& G : rgroups) {
+            if (G.type != kSW) continue;
+            size_t nqg = 0, qsum = 0;
+            for (size_t j : G.jobs)
+                for (const QueryBuf& q : jobs[j].qs) ++nqg, qsum += q.len;
+            if (nqg > 1) G.sh = choose_shape(qsum, total_blocks, max_blocks, 0);
         }
         for (size_t g = 0; g < rgroups.size(); ++g) {
             TierRun R;

@@ -226,3 +226,46 @@ def test_packed_db_other_symbol_type_maps_from_residues(tmp_path):
         assert key(ssa.sw_align(q, 10, 16, ssa.COMPUTE_ALIGNMENT)) == want
     finally:
         ssa.free_sequence(q)
+
+
+SW_SHAPES = [(8, 4), (16, 4), (64, 4), (8, 8), (32, 8), (64, 8), (8, 16), (16, 16), (32, 16), (8, 32), (16, 32)]
+
+
+@pytest.mark.parametrize("shape", SW_SHAPES)
+def test_packed_sw_batch_matches_oracle(shape):
+    """The SW searches of a batch run as one packed query: their rows end to
+    end, each query from a lane boundary.  Queries shorter than a lane, of
+    exactly one lane, ending one row into a lane, filling a pass exactly and
+    crossing pass boundaries, at 8/16/64 bit, every SW kernel shape; each list
+    and its overflow counters must be the oracle's."""
+    lanes, rows = shape
+    pr = lanes * rows
+    rnd = random.Random(7000 + lanes * 100 + rows)
+    base = "".join(rnd.choice(AA) for _ in range(900))
+    db = rand_db(rnd, O.AMINOACID, 160, 1, 300, homolog_of=base, frac=0.25)
+    configure(O.AMINOACID, 1, (-10, -1), ("builtin", "blosum62"))
+    lens = sorted({1, rows - 1 or 1, rows, rows + 1, pr - 1, pr, pr + 3, min(2 * pr + 5, 900), rnd.randint(1, 300)})
+    qraws = []
+    for n in lens:
+        o = rnd.randint(0, 900 - n)
+        qraws.append(base[o:o + n] if rnd.random() < 0.7 else "".join(rnd.choice(AA) for _ in range(n)))
+    with tempfile.TemporaryDirectory() as d:
+        path = os.path.join(d, "db.fas")
+        write_fasta(path, db)
+        ssa.init_db(path)
+        qs = [ssa.init_sequence_fasta(ssa.READ_FROM_STRING, s) for s in qraws]
+        widths = [(8, 16, 64)[i % 3] for i in range(len(qs))]
+        reqs = [(ssa.SEARCH_SW, q, rnd.choice([3, 10, 40]), bw, ssa.COMPUTE_SCORE) for q, bw in zip(qs, widths)]
+        ssa.set_kernel_shape(lanes, rows)
+        try:
+            lists, stats = ssa.batch_align(reqs)
+        finally:
+            ssa.set_kernel_shape(0, 0)
+        for q in qs:
+            ssa.free_sequence(q)
+    m = O.matrix_builtin("blosum62")
+    for (_, _, k, bw, _), raw, got, st in zip(reqs, qraws, lists, stats):
+        want, ost = O.search(O.SW, bw, O.AMINOACID, 1, [O.map_query(O.AMINOACID, raw)], db, k, m, -10, -1)
+        assert [(a.score, a.db_id) for a in got] == [(s, i) for s, i, _, _ in want], (len(raw), bw)
+        assert (st.overflow_8_bit_count, st.overflow_16_bit_count) == (ost.overflow_8, ost.overflow_16)
+        assert (st.lanes_per_stream, st.rows_per_lane) == (lanes, rows)

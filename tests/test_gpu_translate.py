@@ -200,3 +200,12 @@ def test_translated_db_too_short_exits():
         r = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, timeout=120)
         assert r.returncode == 1
         assert "Unable to re-allocate enough memory" in r.stderr + r.stdout
+
+
+@pytest.mark.parametrize("bw", [8, 16, 64])
+def test_translated_query_with_empty_frames(bw):
+    # a 4-nt query: frames of 1, 1 and 0 residues on each strand; the six
+    # buffers run as one packed query, the empty ones score 0 against every entry
+    rnd = random.Random(4242 + bw)
+    db = [rand_aa(rnd, rnd.randint(1, 60)) for _ in range(120)]
+    check(O.SW, bw, O.TRANS_QUERY, 3, (1, 1), "ACGT", db, 30, ("builtin", "BLOSUM62"), (-10, -1))
