@@ -435,11 +435,29 @@ Shape choose_shape(size_t qlen_max, int64_t total_blocks, int64_t max_blocks, in
 }
 
 struct PairSet {
-    // pairs (q, e) for all queries q and the entries listed (DB entry indices)
+    // pairs (q, e) for all queries q and the entries listed (DB entry
+    // indices): the set's own list, or a shared one (ext)
     std::vector<uint32_t> entries;
+    const std::vector<uint32_t>* ext = nullptr;
     size_t nq = 0;
-    size_t index(size_t q, size_t k) const { return q * entries.size() + k; }
-    size_t size() const { return nq * entries.size(); }
+    const std::vector<uint32_t>& list() const { return ext ? *ext : entries; }
+    size_t index(size_t q, size_t k) const { return q * list().size() + k; }
+    size_t size() const { return nq * list().size(); }
+};
+
+// What a search needs of the loaded DB beyond its entries, for one chunk
+// size: the chunks' entry ranges (whole chunks of DB IDs until the first
+// chunk without entries, as search_8/16/64 stop there,
+// src/algo/8/search_8.c:135-143), the entries they hold (0 .. N-1) and their
+// totals.  Computed once per DB generation and chunk size.
+struct DbChunks {
+    uint64_t generation = ~0ull;
+    size_t chunk_size = 0;
+    std::vector<std::pair<uint32_t, uint32_t>> chunks;
+    uint32_t N = 0;
+    std::vector<uint32_t> all;  // 0 .. N-1
+    int64_t total_blocks = 0, max_blocks = 0;
+    uint64_t total_residues = 0;
 };
 
 // Streams of one entry set for one lane-group width: entries are dealt
@@ -475,12 +493,13 @@ struct StreamLayout {
 void build_layout(Exec& ex, StreamLayout& L, int lanes, const PairSet& ps, const std::vector<uint32_t>& order) {
     DeviceDB& db = g_db;
     const int ng = 64 / lanes, spw = 2 * ng;
-    const size_t n = ps.entries.size();
+    const std::vector<uint32_t>& pe_list = ps.list();
+    const size_t n = pe_list.size();
     L.lanes = lanes;
     std::vector<int32_t> nbk(n);
     int64_t total = 0;
     for (size_t k = 0; k < n; ++k) {
-        nbk[k] = entry_blocks(db.entries[ps.entries[k]].len);
+        nbk[k] = entry_blocks(db.entries[pe_list[k]].len);
         total += nbk[k];
     }
     const int64_t per_wave = std::max<int64_t>(1, ((int64_t)target_waves()));
@@ -540,7 +559,7 @@ void build_layout(Exec& ex, StreamLayout& L, int lanes, const PairSet& ps, const
                 if (i < n) {
                     const uint32_t at = order[i];
                     L.slot_entries.push_back(at);
-                    const DbEntry& de = db.entries[ps.entries[at]];
+                    const DbEntry& de = db.entries[pe_list[at]];
                     PackEntry p;
                     p.code_off = de.code_off;
                     p.dst = (off + pos * ng + k) * 2 + h;
@@ -629,7 +648,7 @@ struct TierRun {
     bool packed = false;  // SW with several queries: scored as one packed query
 
     ~TierRun() { local.release(); }
-    bool empty() const { return ps == nullptr || ps->entries.empty() || qs.empty(); }
+    bool empty() const { return ps == nullptr || ps->list().empty() || qs.empty(); }
 };
 
 void tier_prepare(Exec& ex, TierRun& R) {
@@ -639,7 +658,7 @@ void tier_prepare(Exec& ex, TierRun& R) {
     const Tier& T = R.T;
     const int go = cf.gap_open, ge = cf.gap_extend;
     const PairSet& ps = *R.ps;
-    const size_t n = ps.entries.size(), nq = R.qs.size();
+    const size_t n = ps.list().size(), nq = R.qs.size();
     R.nout = ps.size();
     if (R.empty()) return;
     R.out = (PairResult*)ex.pinned(R.pfx + "results", R.nout * sizeof(PairResult));
@@ -650,7 +669,7 @@ void tier_prepare(Exec& ex, TierRun& R) {
     if (R.type == kSW && nq > 1) qmax = qsum;
     if (R.sh.lanes == 0) {
         int64_t total_blocks = 0, max_blocks = 0;
-        for (uint32_t e : ps.entries) {
+        for (uint32_t e : ps.list()) {
             const int64_t nb = entry_blocks(db.entries[e].len);
             total_blocks += nb;
             max_blocks = std::max(max_blocks, nb);
@@ -734,7 +753,7 @@ void tier_prepare(Exec& ex, TierRun& R) {
         std::vector<uint32_t> order(n);
         std::iota(order.begin(), order.end(), 0u);
         std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) {
-            return db.entries[ps.entries[a]].len > db.entries[ps.entries[b]].len;
+            return db.entries[ps.list()[a]].len > db.entries[ps.list()[b]].len;
         });
         build_layout(ex, R.local, sh.lanes, ps, order);
         R.L = &R.local;
@@ -1550,6 +1569,56 @@ void job_list(Job& J, std::vector<p_alignment>& al_out, std::vector<SearchType>&
 
 }  // namespace
 
+// The chunk view of the loaded DB for one chunk size, cached per DB
+// generation (the last few chunk sizes are kept; entries are never freed
+// while a search may hold a reference, only when the DB changes).
+const DbChunks& db_chunks(const DeviceDB& db, size_t cs) {
+    static std::mutex mu;
+    static std::list<DbChunks> cache;
+    std::lock_guard<std::mutex> lk(mu);
+    for (auto it = cache.begin(); it != cache.end();) {
+        if (it->generation == db.generation && it->chunk_size == cs) return *it;
+        // a DB that changed cannot be in use by a search still running
+        // (the DB is replaced only between searches)
+        if (it->generation != db.generation) it = cache.erase(it);
+        else ++it;
+    }
+    cache.emplace_back();
+    DbChunks& d = cache.back();
+    d.generation = db.generation;
+    d.chunk_size = cs;
+    for (size_t start = 0; start < db.count; start += cs) {
+        const size_t stop = std::min(db.count, start + cs);
+        const uint32_t e0 = db.first_entry[start], e1 = db.first_entry[stop];
+        if (e0 == e1) break;
+        d.chunks.push_back({e0, e1});
+    }
+    d.N = d.chunks.empty() ? 0 : d.chunks.back().second;
+    d.all.resize(d.N);
+    std::iota(d.all.begin(), d.all.end(), 0u);
+    for (uint32_t e = 0; e < d.N; ++e) {
+        const int64_t nb = entry_blocks(db.entries[e].len);
+        d.total_blocks += nb;
+        d.max_blocks = std::max(d.max_blocks, nb);
+        d.total_residues += db.entries[e].len;
+    }
+    return d;
+}
+
+// every entry of the loaded DB, 0 .. entries - 1
+const std::vector<uint32_t>& db_all_entries(const DeviceDB& db) {
+    static std::mutex mu;
+    static uint64_t gen = ~0ull;
+    static std::vector<uint32_t> all;
+    std::lock_guard<std::mutex> lk(mu);
+    if (gen != db.generation || all.size() != db.entries.size()) {
+        all.resize(db.entries.size());
+        std::iota(all.begin(), all.end(), 0u);
+        gen = db.generation;
+    }
+    return all;
+}
+
 void run_batch(Exec& ex, const std::vector<SearchRequest>& reqs, std::vector<SearchResult>& results,
                bool one_graph) {
     const int64_t t_start = now_us();
@@ -1577,30 +1646,15 @@ void run_batch(Exec& ex, const std::vector<SearchRequest>& reqs, std::vector<Sea
     }
     DeviceDB& db = *dbp;
 
-    // entries processed: whole chunks of DB IDs until the first chunk without
-    // entries (search_8/16/64 stop there, src/algo/8/search_8.c:135-143)
-    const size_t cs = cf.chunk_size ? cf.chunk_size : kDefaultChunkSize;
-    std::vector<std::pair<uint32_t, uint32_t>> chunks;  // entry ranges
-    for (size_t start = 0; start < db.count; start += cs) {
-        const size_t stop = std::min(db.count, start + cs);
-        const uint32_t e0 = db.first_entry[start], e1 = db.first_entry[stop];
-        if (e0 == e1) break;
-        chunks.push_back({e0, e1});
-    }
-    const uint32_t N = chunks.empty() ? 0 : chunks.back().second;
+    const DbChunks& dc = db_chunks(db, cf.chunk_size ? cf.chunk_size : kDefaultChunkSize);
+    const std::vector<std::pair<uint32_t, uint32_t>>& chunks = dc.chunks;  // entry ranges
+    const uint32_t N = dc.N;
     auto at = [N](size_t q, size_t e) { return q * N + e; };
     PairSet all;
-    all.entries.resize(N);
-    std::iota(all.entries.begin(), all.entries.end(), 0u);
+    all.ext = &dc.all;
     const bool full_db = N == db.entries.size();
-    int64_t total_blocks = 0, max_blocks = 0;
-    uint64_t total_residues = 0;
-    for (uint32_t e = 0; e < N; ++e) {
-        const int64_t nb = entry_blocks(db.entries[e].len);
-        total_blocks += nb;
-        max_blocks = std::max(max_blocks, nb);
-        total_residues += db.entries[e].len;
-    }
+    const int64_t total_blocks = dc.total_blocks, max_blocks = dc.max_blocks;
+    const uint64_t total_residues = dc.total_residues;
 
     const MatrixInfo mi = matrix_info();
     const int go = cf.gap_open, ge = cf.gap_extend;
@@ -2018,8 +2072,7 @@ int autotune(int type, const std::vector<size_t>& lengths, int copies, std::vect
     if (N == 0) return 0;
     copies = std::max(1, copies);
     PairSet all;
-    all.entries.resize(N);
-    std::iota(all.entries.begin(), all.entries.end(), 0u);
+    all.ext = &db_all_entries(db);
     // query symbols of the current alphabet (codes of the standard residues)
     std::vector<uint8_t> alphabet;
     const char* syms = cf.symtype == NUCLEOTIDE ? "ACGT" : "ARNDCQEGHILKMFPSTWYV";
