@@ -25,6 +25,9 @@ typedef struct {
     /* work and time of the last sw_align / nw_align call */
     uint64_t cells;           /* sum over (query, DB entry) pairs of |q| * |d| in the first tier */
     uint64_t cells_computed;  /* cells the GPU computed in all tiers, padding included */
+    uint64_t cells_scored;    /* cells of the query buffers the first tier scored (a both-strand
+                                 search whose reverse strand is read from the forward strand's
+                                 reverse-complement twins scores half of `cells`) */
     int lanes_per_stream;     /* score kernel shape of the first tier: lanes per stream pair */
     int rows_per_lane;        /* and query rows per lane */
     double ms_gpu_first;      /* first-tier kernel time (HIP events) */

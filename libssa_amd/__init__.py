@@ -74,7 +74,8 @@ class Stats(_C.Structure):
     """ssa_amd_stats_t (include/libssa_amd.h)."""
     _fields_ = [("chunk_count", _C.c_size_t), ("seq_count", _C.c_size_t),
                 ("overflow_8_bit_count", _C.c_size_t), ("overflow_16_bit_count", _C.c_size_t),
-                ("cells", _C.c_uint64), ("cells_computed", _C.c_uint64), ("lanes_per_stream", _C.c_int), ("rows_per_lane", _C.c_int),
+                ("cells", _C.c_uint64), ("cells_computed", _C.c_uint64), ("cells_scored", _C.c_uint64),
+                ("lanes_per_stream", _C.c_int), ("rows_per_lane", _C.c_int),
                 ("ms_gpu_first", _C.c_double), ("ms_gpu_requeue", _C.c_double), ("ms_replay", _C.c_double),
                 ("ms_traceback", _C.c_double), ("ms_total", _C.c_double)]
 

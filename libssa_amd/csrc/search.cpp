@@ -458,6 +458,7 @@ struct DbChunks {
     std::vector<uint32_t> all;  // 0 .. N-1
     int64_t total_blocks = 0, max_blocks = 0;
     uint64_t total_residues = 0;
+    bool twins = false;  // entries 2k, 2k+1 are a sequence and its reverse complement
 };
 
 // Streams of one entry set for one lane-group width: entries are dealt
@@ -1284,6 +1285,8 @@ struct Job {
     size_t q_off = 0;  // first query of this job in its first-tier group
     int group = -1;
     bool requeue_full = false;  // 16-bit re-run over the whole DB (most entries overflowed at 8 bit)
+    bool twin = false;          // both strands: the reverse strand's first tier read from the forward's
+    size_t nrun = 0;            // query buffers the first tier scores
     bool requeue_sub = false;   // 16-bit re-run over the overflowed entries only
     size_t rq_off = 0;          // first query of this job in its re-run group
     int rgroup = -1;
@@ -1602,7 +1605,44 @@ const DbChunks& db_chunks(const DeviceDB& db, size_t cs) {
         d.max_blocks = std::max(d.max_blocks, nb);
         d.total_residues += db.entries[e].len;
     }
+    d.twins = db.symtype == NUCLEOTIDE && d.N % 2 == 0 && d.N > 0;
+    for (uint32_t e = 0; d.twins && e < d.N; e += 2) {
+        const DbEntry &f = db.entries[e], &r = db.entries[e + 1];
+        d.twins = f.strand == 0 && r.strand == 1 && f.id == r.id && f.len == r.len;
+    }
     return d;
+}
+
+// A both-strand nucleotide search whose reverse query strand need not be
+// scored: the alignment score is unchanged when both sequences are reversed,
+// and, with a matrix that scores complements alike, when both are
+// complemented, so (reverse query, entry) scores as (forward query, the
+// entry's reverse-complement twin).  The replay also uses each pair's
+// overflow block, which differs between the two directions, so this holds
+// only where no int16 value of any pair can reach either end of the range
+// (the bound the 64-bit shortcut uses): then no pair is flagged and the
+// 16-bit results are exact.  8-bit searches keep both strands (their flags
+// are the point).
+bool strand_twins(const Job& J, const DbChunks& dc, const MatrixInfo& mi) {
+    const Config& cf = config();
+    if (!dc.twins || cf.symtype != NUCLEOTIDE || J.qs.size() != 2 || J.first_bits != 16 ||
+        J.rq.bit_width == BIT_WIDTH_8)
+        return false;
+    const QueryBuf &f = J.qs[0], &r = J.qs[1];
+    if (f.strand != 0 || r.strand != 1 || f.len != r.len || f.len == 0) return false;
+    for (size_t i = 0; i < f.len; ++i)
+        if (r.seq[i] != (uint8_t)complement_nt(f.seq[f.len - 1 - i])) return false;
+    const Matrix& M = matrix();
+    for (int a = 0; a < 16; ++a)
+        for (int b = 0; b < 16; ++b)
+            if (M.at(complement_nt(a), complement_nt(b)) != M.at(a, b)) return false;
+    const int64_t gap = std::abs((int64_t)cf.gap_open) + std::abs((int64_t)cf.gap_extend);
+    const int64_t dmax = 4 * dc.max_blocks;
+    const int64_t ml = std::min<int64_t>((int64_t)f.len, dmax);
+    const int64_t span = 3 * gap + gap * ((int64_t)f.len + dmax + 4);
+    const int64_t up = std::max<int64_t>(mi.maxv, 0) * ml;
+    const int64_t lowest = -span - std::max<int64_t>(-mi.minv, 0) * ml;
+    return lowest > -32768 + gap + 1 && up < 32767;
 }
 
 // every entry of the loaded DB, 0 .. entries - 1
@@ -1675,6 +1715,10 @@ void run_batch(Exec& ex, const std::vector<SearchRequest>& reqs, std::vector<Sea
         if (J.rq.bit_width == BIT_WIDTH_8) J.first_bits = J.sw_single ? 16 : 8;
         else if (J.rq.bit_width == BIT_WIDTH_16) J.first_bits = 16;
         else J.first_bits = mi.fits16 ? 16 : 0;
+        J.twin = strand_twins(J, dc, mi);
+        J.nrun = J.twin ? 1 : nq;
+        if (J.first_bits != 0)
+            for (size_t q = 0; q < J.nrun; ++q) J.st.cells_scored += (uint64_t)J.qs[q].len * total_residues;
         if (J.first_bits == 0 || N == 0 || nq == 0) continue;
         size_t qmax = 0;
         for (const QueryBuf& q : J.qs) qmax = std::max(qmax, q.len);
@@ -1697,7 +1741,7 @@ void run_batch(Exec& ex, const std::vector<SearchRequest>& reqs, std::vector<Sea
         if (G.type != kSW) continue;
         size_t nqg = 0, qsum = 0;
         for (size_t j : G.jobs)
-            for (const QueryBuf& q : jobs[j].qs) ++nqg, qsum += q.len;
+            for (size_t q = 0; q < jobs[j].nrun; ++q) ++nqg, qsum += jobs[j].qs[q].len;
         if (nqg > 1) G.sh = choose_shape(qsum, total_blocks, max_blocks, 0);
         for (size_t j : G.jobs) jobs[j].shape = G.sh;
     }
@@ -1712,7 +1756,7 @@ void run_batch(Exec& ex, const std::vector<SearchRequest>& reqs, std::vector<Sea
         R.sorted_all = full_db ? &db.by_length : nullptr;
         for (size_t j : groups[g].jobs) {
             jobs[j].q_off = R.qs.size();
-            R.qs.insert(R.qs.end(), jobs[j].qs.begin(), jobs[j].qs.end());
+            R.qs.insert(R.qs.end(), jobs[j].qs.begin(), jobs[j].qs.begin() + jobs[j].nrun);
         }
         PairSet* ps = new PairSet(all);
         ps->nq = R.qs.size();
@@ -1794,12 +1838,13 @@ void run_batch(Exec& ex, const std::vector<SearchRequest>& reqs, std::vector<Sea
         uint64_t& computed = computed_job[j];
         if (J.group >= 0) {
             TierRun& R = *runs[J.group];
-            computed += R.computed * nq / std::max<size_t>(R.qs.size(), 1);
+            computed += R.computed * J.nrun / std::max<size_t>(R.qs.size(), 1);
             J.st.lanes_per_stream = R.sh.lanes;
             J.st.rows_per_lane = R.sh.rows;
         }
         auto raw = [&](size_t q, uint32_t e) -> const PairResult& {
             const TierRun& R = *runs[J.group];
+            if (J.twin && q == 1) return R.out[R.ps->index(J.q_off, e ^ 1u)];
             return R.out[R.ps->index(J.q_off + q, e)];
         };
         const int bw = J.rq.bit_width;
@@ -1955,6 +2000,7 @@ void run_batch(Exec& ex, const std::vector<SearchRequest>& reqs, std::vector<Sea
         const int bw = J.rq.bit_width;
         auto raw = [&](size_t q, uint32_t e) -> const PairResult& {
             const TierRun& R = *runs[J.group];
+            if (J.twin && q == 1) return R.out[R.ps->index(J.q_off, e ^ 1u)];
             return R.out[R.ps->index(J.q_off + q, e)];
         };
         // a 64-bit search scores with the 16-bit pass where it is provably

@@ -281,3 +281,42 @@ def test_stream_packing_does_not_change_results(waves):
     finally:
         ssa.set_stream_waves(0)
         ssa.set_kernel_shape(0, 0)
+
+
+def _revcomp_nt(s):
+    return "".join({"A": "T", "C": "G", "G": "C", "T": "A"}[c] for c in reversed(s))
+
+
+@pytest.mark.parametrize("t,bw", [(O.NW, 16), (O.NW, 64), (O.SW, 16), (O.SW, 64)])
+def test_both_strands_reverse_from_forward_twins(t, bw):
+    """Both-strand DNA at 16/64 bit where no int16 value can reach either end
+    of the range: only the forward query strand is scored and the reverse
+    strand's results are read from each entry's reverse-complement twin.  Hit
+    lists (scores, IDs, strands, query strands), counters and CIGARs equal the
+    oracle's, which scores both strands in full; the scored cells show that
+    only one strand ran."""
+    rnd = random.Random(3100 + bw + 7 * t)
+    q = "".join(rnd.choice("ACGT") for _ in range(250))
+    db = []
+    for i in range(400):
+        n = rnd.randint(150, 250)
+        s = "".join(rnd.choice("ACGT") for _ in range(n))
+        if i % 40 == 0:  # planted copies of the query, half reverse-complemented
+            s = q[rnd.randint(0, 60):] if i % 80 else _revcomp_nt(q)[rnd.randint(0, 60):]
+        db.append(s)
+    check(t, bw, O.NUCLEOTIDE, 3, q, db, 25, ("const", (2, -4)), (-20, -2), align=True)
+    st = ssa.last_stats()
+    assert 2 * st.cells_scored == st.cells
+
+
+@pytest.mark.parametrize("t", [O.NW, O.SW])
+def test_both_strands_both_scored_past_int16_bound(t):
+    """The same search with a query and entries long enough that the int16
+    bound fails: both query strands are scored."""
+    rnd = random.Random(3200 + t)
+    q = "".join(rnd.choice("ACGT") for _ in range(700))
+    db = ["".join(rnd.choice("ACGT") for _ in range(rnd.randint(700, 900))) for _ in range(120)]
+    db[7] = _revcomp_nt(q)
+    check(t, 16, O.NUCLEOTIDE, 3, q, db, 10, ("const", (2, -4)), (-20, -2))
+    st = ssa.last_stats()
+    assert st.cells_scored == st.cells
