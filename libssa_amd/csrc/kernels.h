@@ -88,7 +88,6 @@ struct PairResult {
     int32_t score;
     int32_t flag_a;
     int32_t flag_b;
-    int32_t pad;
 };
 
 // Arguments of one score-kernel launch.  The grid is persistent: each

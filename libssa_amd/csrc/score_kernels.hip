@@ -583,7 +583,7 @@ __global__ void fill_u32_kernel(uint32_t* __restrict__ p, int64_t n, uint32_t v)
 // Sets every pair's result to the neutral value of the atomic combination.
 __global__ void init_results_kernel(PairResult* __restrict__ r, int64_t n, int32_t score) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) r[i] = PairResult{score, kNoFlag, kNoFlag, 0};
+    if (i < n) r[i] = PairResult{score, kNoFlag, kNoFlag};
 }
 
 // ---------------------------------------------------------------------------

@@ -459,6 +459,7 @@ struct DbChunks {
     int64_t total_blocks = 0, max_blocks = 0;
     uint64_t total_residues = 0;
     bool twins = false;  // entries 2k, 2k+1 are a sequence and its reverse complement
+    std::vector<int32_t> nb;  // blocks per entry (entry_blocks)
 };
 
 // Streams of one entry set for one lane-group width: entries are dealt
@@ -999,8 +1000,7 @@ void gpu_exact(Exec& gpu, SearchType type, const std::vector<QueryBuf>& qs,
 }
 
 // Decodes one tier result into (duration, accept, score).
-TierOut decode(SearchType type, const Tier& T, const PairResult& r, uint32_t len, bool sw16_as_8) {
-    const int32_t nb = entry_blocks(len);
+TierOut decode(SearchType type, const Tier& T, const PairResult& r, int32_t nb, bool sw16_as_8) {
     int32_t flag, score;
     bool accept;
     if (type == kSW) {
@@ -1375,6 +1375,12 @@ void job_replay(Job& J, const std::vector<DbEntry>& entries, const std::vector<s
     // in the chunk's 16-bit list, which every query scans; each 16-bit
     // overflow met in that scan adds the entry to the exact list once more,
     // which every query scans again.  Also counts the overflows.
+    const TierOut* p8[6] = {};
+    const TierOut* p16[6] = {};
+    for (size_t q = 0; q < nq && q < 6; ++q) {
+        if (!J.t8.empty()) p8[q] = J.t8.data() + at(q, 0);
+        if (!J.t16.empty()) p16[q] = J.t16.data() + at(q, 0);
+    }
     auto inserts = [&](size_t c, size_t& ov8, size_t& ov16, auto&& f) {
         for (uint32_t e = chunks[c].first; e < chunks[c].second; ++e) {
             if (bit_width == BIT_WIDTH_64) {
@@ -1385,7 +1391,7 @@ void job_replay(Job& J, const std::vector<DbEntry>& entries, const std::vector<s
             if (bit_width == BIT_WIDTH_8) {
                 m8 = 0;
                 for (size_t q = 0; q < nq; ++q) {
-                    const TierOut o = J.t8[at(q, e)];
+                    const TierOut o = p8[q][e];
                     if (o.accept()) f(q, e, (int64_t)o.score, (size_t)1);
                     else ++m8;
                 }
@@ -1394,7 +1400,7 @@ void job_replay(Job& J, const std::vector<DbEntry>& entries, const std::vector<s
             }
             size_t m16 = 0;
             for (size_t q = 0; q < nq; ++q) {
-                const TierOut o = J.t16[at(q, e)];
+                const TierOut o = p16[q][e];
                 if (o.accept()) f(q, e, (int64_t)o.score, m8);
                 else ++m16;
             }
@@ -1599,8 +1605,10 @@ const DbChunks& db_chunks(const DeviceDB& db, size_t cs) {
     d.N = d.chunks.empty() ? 0 : d.chunks.back().second;
     d.all.resize(d.N);
     std::iota(d.all.begin(), d.all.end(), 0u);
+    d.nb.resize(d.N);
     for (uint32_t e = 0; e < d.N; ++e) {
         const int64_t nb = entry_blocks(db.entries[e].len);
+        d.nb[e] = (int32_t)nb;
         d.total_blocks += nb;
         d.max_blocks = std::max(d.max_blocks, nb);
         d.total_residues += db.entries[e].len;
@@ -1687,6 +1695,7 @@ void run_batch(Exec& ex, const std::vector<SearchRequest>& reqs, std::vector<Sea
     DeviceDB& db = *dbp;
 
     const DbChunks& dc = db_chunks(db, cf.chunk_size ? cf.chunk_size : kDefaultChunkSize);
+    const int32_t* nbk = dc.nb.data();  // blocks per entry
     const std::vector<std::pair<uint32_t, uint32_t>>& chunks = dc.chunks;  // entry ranges
     const uint32_t N = dc.N;
     auto at = [N](size_t q, size_t e) { return q * N + e; };
@@ -1842,26 +1851,36 @@ void run_batch(Exec& ex, const std::vector<SearchRequest>& reqs, std::vector<Sea
             J.st.lanes_per_stream = R.sh.lanes;
             J.st.rows_per_lane = R.sh.rows;
         }
-        auto raw = [&](size_t q, uint32_t e) -> const PairResult& {
+        // the first tier's results of query buffer q: entry e at base[e ^ flip]
+        // (a twin search reads its reverse strand from the forward strand's
+        // reverse-complement twins)
+        auto src = [&](size_t q, uint32_t& flip) -> const PairResult* {
             const TierRun& R = *runs[J.group];
-            if (J.twin && q == 1) return R.out[R.ps->index(J.q_off, e ^ 1u)];
-            return R.out[R.ps->index(J.q_off + q, e)];
+            flip = (J.twin && q == 1) ? 1u : 0u;
+            return R.out + R.ps->index(J.q_off + (flip ? 0 : q), 0);
         };
         const int bw = J.rq.bit_width;
+        uint8_t* in16 = J.in16.data();
+        uint8_t* need64 = J.need64.data();
         if (bw == BIT_WIDTH_8) {
             J.t8.reset(nq * N);
             if (J.sw_single) {
                 J.t16.reset(nq * N);
                 over_entries([&](uint32_t e0, uint32_t e1) {
-                    for (size_t q = 0; q < nq; ++q)
+                    if (e1 <= e0) return;
+                    for (size_t q = 0; q < nq; ++q) {
+                        uint32_t fl;
+                        const PairResult* base = src(q, fl);
+                        TierOut* t8 = J.t8.data() + at(q, 0);
+                        TierOut* t16 = J.t16.data() + at(q, 0);
                         for (uint32_t e = e0; e < e1; ++e) {
-                            const PairResult& r = raw(q, e);
-                            const TierOut o8 = decode(type, kTier16, r, db.entries[e].len, true);
-                            const TierOut o16 = decode(type, kTier16, r, db.entries[e].len, false);
-                            J.t8[at(q, e)] = o8;
-                            J.t16[at(q, e)] = o16;
-                            if (!o8.accept()) J.in16[e] = 1;
+                            const PairResult r = base[e ^ fl];
+                            const TierOut o8 = decode(type, kTier16, r, nbk[e], true);
+                            t8[e] = o8;
+                            t16[e] = decode(type, kTier16, r, nbk[e], false);
+                            if (!o8.accept()) in16[e] = 1;
                         }
+                    }
                     // an entry reaches the exact tier when some query overflows
                     // both tiers
                     for (size_t q = 0; q < nq; ++q)
@@ -1872,12 +1891,17 @@ void run_batch(Exec& ex, const std::vector<SearchRequest>& reqs, std::vector<Sea
                 size_t n16 = 0;
                 std::mutex mu;
                 over_entries([&](uint32_t e0, uint32_t e1) {
-                    for (size_t q = 0; q < nq; ++q)
+                    if (e1 <= e0) return;
+                    for (size_t q = 0; q < nq; ++q) {
+                        uint32_t fl;
+                        const PairResult* base = src(q, fl);
+                        TierOut* t8 = J.t8.data() + at(q, 0);
                         for (uint32_t e = e0; e < e1; ++e) {
-                            const TierOut o8 = decode(type, kTier8, raw(q, e), db.entries[e].len, false);
-                            J.t8[at(q, e)] = o8;
-                            if (!o8.accept()) J.in16[e] = 1;
+                            const TierOut o8 = decode(type, kTier8, base[e ^ fl], nbk[e], false);
+                            t8[e] = o8;
+                            if (!o8.accept()) in16[e] = 1;
                         }
+                    }
                     size_t n = 0;
                     for (uint32_t e = e0; e < e1; ++e) n += J.in16[e];
                     std::lock_guard<std::mutex> lk(mu);
@@ -1896,12 +1920,17 @@ void run_batch(Exec& ex, const std::vector<SearchRequest>& reqs, std::vector<Sea
             J.t16.reset(nq * N);
             if (N) memset(J.in16.data(), 1, N);
             over_entries([&](uint32_t e0, uint32_t e1) {
-                for (size_t q = 0; q < nq; ++q)
+                if (e1 <= e0) return;
+                for (size_t q = 0; q < nq; ++q) {
+                    uint32_t fl;
+                    const PairResult* base = src(q, fl);
+                    TierOut* t16 = J.t16.data() + at(q, 0);
                     for (uint32_t e = e0; e < e1; ++e) {
-                        const TierOut o = decode(type, kTier16, raw(q, e), db.entries[e].len, false);
-                        J.t16[at(q, e)] = o;
-                        if (!o.accept()) J.need64[e] = 1;
+                        const TierOut o = decode(type, kTier16, base[e ^ fl], nbk[e], false);
+                        t16[e] = o;
+                        if (!o.accept()) need64[e] = 1;
                     }
+                }
             });
         }
     });
@@ -1926,7 +1955,7 @@ void run_batch(Exec& ex, const std::vector<SearchRequest>& reqs, std::vector<Sea
         for (size_t q = 0; q < nq; ++q)
             for (size_t k = 0; k < sub.entries.size(); ++k) {
                 const uint32_t e = sub.entries[k];
-                const TierOut o = decode(J.rq.type, kTier16, R2.out[sub.index(q, k)], db.entries[e].len, false);
+                const TierOut o = decode(J.rq.type, kTier16, R2.out[sub.index(q, k)], nbk[e], false);
                 J.t16[at(q, e)] = o;
                 if (!o.accept()) J.need64[e] = 1;
             }
@@ -1985,7 +2014,7 @@ void run_batch(Exec& ex, const std::vector<SearchRequest>& reqs, std::vector<Sea
                     for (uint32_t e = 0; e < N; ++e)
                         if (J.in16[e]) {
                             const TierOut o =
-                                decode(J.rq.type, kTier16, R.out[ps.index(J.rq_off + q, e)], db.entries[e].len, false);
+                                decode(J.rq.type, kTier16, R.out[ps.index(J.rq_off + q, e)], nbk[e], false);
                             J.t16[at(q, e)] = o;
                             if (!o.accept()) J.need64[e] = 1;
                         }
@@ -2015,7 +2044,7 @@ void run_batch(Exec& ex, const std::vector<SearchRequest>& reqs, std::vector<Sea
         over_entries([&](uint32_t e0, uint32_t e1) {
             for (size_t q = 0; q < nq; ++q)
                 for (uint32_t e = e0; e < e1; ++e) {
-                    TierOut o = decode(type, kTier16, raw(q, e), db.entries[e].len, false);
+                    TierOut o = decode(type, kTier16, raw(q, e), nbk[e], false);
                     // an empty entry's lane scores a padding block; 64 bit scores the empty sequence
                     bool exact = o.accept() && db.entries[e].len != 0;
                     if (type == kNW) {
