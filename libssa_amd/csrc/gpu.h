@@ -91,6 +91,11 @@ private:
 // ---- kernel launchers (score_kernels.hip, traceback_kernels.hip, pack_kernels.hip) ----
 bool score_shape_supported(int lanes, int rows, int mode);
 int score_resident_workgroups(int lanes, int rows, int mode);  // persistent grid size of a shape
+// paired packed SW queries (score_kernel_pair): shapes, grid size, launch
+bool score_pair_supported(int lanes, int rows);
+int score_pair_resident_workgroups(int lanes, int rows);
+hipError_t launch_score_pair(int lanes, int rows, hipStream_t st, const ScoreLaunch& L, const ScoreParams& P,
+                             int max_wg);
 hipError_t launch_score(int lanes, int rows, int mode, hipStream_t st, const ScoreLaunch& L, const ScoreParams& P,
                         int max_wg);
 hipError_t launch_init_results(hipStream_t st, PairResult* r, int64_t n, int32_t score);

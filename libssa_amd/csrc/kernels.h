@@ -48,7 +48,7 @@ struct QueryDesc {
     int32_t lb_off;    // in uint32 units: NW left boundary [pass*G*R rows][2] packed (H, E)
     int32_t pair_off;  // result index of (this query, entry position 0)
     int32_t seg_off;   // packed query: its lanes' segments in ScoreLaunch::segs; -1 otherwise
-    int32_t pad;
+    int32_t seg_off_hi;  // paired packed queries: the hi half's segments (score_kernel_pair)
 };
 
 // A packed query (SW): the rows of several queries laid end to end, each

@@ -114,6 +114,30 @@ __device__ __forceinline__ void read_rows(const char* p, uint32_t* d) {
     for (int q = 0; q < (R >= 8 ? R / 8 : 1); ++q) read_part<R>(p, d, q);
 }
 
+// Part q of a paired profile (lo, hi int16 per row, one dword per row): rows
+// 8q .. 8q+7 (R >= 8) or rows 0 .. 3 (R = 4).
+template <int R>
+__device__ __forceinline__ void read_part_pair(const char* p, uint32_t* d, int q) {
+    if constexpr (R == 4) {
+        const uint4 v = *reinterpret_cast<const uint4*>(p);
+        d[0] = v.x;
+        d[1] = v.y;
+        d[2] = v.z;
+        d[3] = v.w;
+    } else {
+        const uint4 a = *reinterpret_cast<const uint4*>(p + 32 * q);
+        const uint4 b = *reinterpret_cast<const uint4*>(p + 32 * q + 16);
+        d[8 * q + 0] = a.x;
+        d[8 * q + 1] = a.y;
+        d[8 * q + 2] = a.z;
+        d[8 * q + 3] = a.w;
+        d[8 * q + 4] = b.x;
+        d[8 * q + 5] = b.y;
+        d[8 * q + 6] = b.z;
+        d[8 * q + 7] = b.w;
+    }
+}
+
 // the packed (lo, hi) score pair of row r from the two halves' profile dwords
 __device__ __forceinline__ s16x2 pair_row(const uint32_t* lo, const uint32_t* hi, int r) {
     return as_v(__builtin_amdgcn_perm(hi[r / 2], lo[r / 2], (r & 1) ? 0x07060302u : 0x05040100u));
@@ -144,18 +168,25 @@ __device__ __forceinline__ void finish_pair(PairResult* results, int pair, bool 
 // stage(p) loads pass p's into buffer p & 1 once every lane has left pass
 // p - 2.  Steps run two per iteration (an even count; an extra step is idle)
 // so that the one-step-ahead loads alternate between two register sets.
-template <int R, int G, int MODE, bool MULTI, bool SEG, class Stage>
+//
+// PAIR (SW, packed queries): both halves of a lane group walk the same
+// stream (the layout gives the lo and hi slots the same entries) for two
+// packed queries; the staged profile holds the two queries' rows interleaved,
+// one dword (lo, hi) per row, so no v_perm is needed, and each half has its
+// own segment table.
+template <int R, int G, int MODE, bool MULTI, bool SEG, bool PAIR, class Stage>
 __device__ __forceinline__ void run_task(const ScoreParams& P, uint32_t gbase0, const char* sp,
                                          const uint32_t* slbh0, const uint32_t* slbe0,
                                          const uint2* __restrict__ wres, uint4* __restrict__ wbnd,
                                          const int32_t* __restrict__ slot_entries, int pair_off,
-                                         const SegLane* __restrict__ segs, PairResult* __restrict__ results,
-                                         int sb_lo, int sb_hi, int k, int g, int nbw, int nbp, int npass, int last_g,
-                                         int last_r, const Stage& stage) {
+                                         const SegLane* __restrict__ segs, const SegLane* __restrict__ segs_hi,
+                                         PairResult* __restrict__ results, int sb_lo, int sb_hi, int k, int g,
+                                         int nbw, int nbp, int npass, int last_g, int last_r, const Stage& stage) {
+    static_assert(!PAIR || (SEG && MULTI && MODE == 0), "paired queries are packed SW queries");
     constexpr int NG = 64 / G;
     constexpr int PR = G * R;
-    constexpr int LPR = Log2<PR>::value;
-    constexpr uint32_t kProfBytes = 32 * PR * 2;  // one staged profile buffer
+    constexpr int LPR = Log2<PR>::value + (PAIR ? 1 : 0);  // log2 of a profile code row's int16 count
+    constexpr uint32_t kProfBytes = 32 * PR * (PAIR ? 4 : 2);  // one staged profile buffer
     const s16x2 QR = as_v(P.qr), RR = as_v(P.r), Z = as_v(P.zero);
     // per-half thresholds of the overflow predicates; a half whose predicate
     // cannot fire (flag taken, or no pair running) holds the neutral value
@@ -167,17 +198,23 @@ __device__ __forceinline__ void run_task(const ScoreParams& P, uint32_t gbase0, 
     const bool top_lane = g == 0;
     // the lane's pass and block within it (negative before the lane's first block)
     int pass = 0, b = -g;
-    // SEG: the lane's segment in its pass (a plain query: one segment per pass)
-    int seg_po = pair_off;
-    bool seg_top = top_lane, seg_bot = g == G - 1, seg_none = false;
+    // SEG: the lane's segment in its pass, per half (a plain query: one
+    // segment per pass); seg_top holds the halves that start a query there
+    int po_lo = pair_off, po_hi = pair_off;
+    uint32_t seg_top = 0u;
+    bool bot_lo = g == G - 1, bot_hi = g == G - 1, none_lo = false, none_hi = false;
     auto load_seg = [&](int p) {
         if constexpr (SEG) {
             if (p < npass) {
                 const SegLane sl = segs[p * G + g];
-                seg_po = sl.pair_off;
-                seg_top = (sl.flags & kSegTop) != 0;
-                seg_bot = (sl.flags & kSegBottom) != 0;
-                seg_none = (sl.flags & kSegNone) != 0;
+                const SegLane sh = PAIR ? segs_hi[p * G + g] : sl;
+                po_lo = sl.pair_off;
+                po_hi = sh.pair_off;
+                seg_top = ((sl.flags & kSegTop) ? 0x0000ffffu : 0u) | ((sh.flags & kSegTop) ? 0xffff0000u : 0u);
+                bot_lo = (sl.flags & kSegBottom) != 0;
+                bot_hi = (sh.flags & kSegBottom) != 0;
+                none_lo = (sl.flags & kSegNone) != 0;
+                none_hi = (sh.flags & kSegNone) != 0;
             }
         }
     };
@@ -214,13 +251,20 @@ __device__ __forceinline__ void run_task(const ScoreParams& P, uint32_t gbase0, 
                     uint4& tn1) {
         constexpr bool FIRST = decltype(first_tag)::value;
         // profile dwords of the current column (lo, hi halves); column 0's now
-        uint32_t pl[R / 2], ph[R / 2];
-        read_rows<R>(sp + colofs(w.x, 0), pl);
-        read_rows<R>(sp + colofs(w.y, 0), ph);
+        // (PAIR: pl holds one dword per row, both halves' scores)
+        uint32_t pl[PAIR ? R : R / 2], ph[PAIR ? 1 : R / 2];
+        if constexpr (PAIR) {
+#pragma unroll
+            for (int q = 0; q < (R >= 8 ? R / 8 : 1); ++q) read_part_pair<R>(sp + colofs(w.x, 0), pl, q);
+        } else {
+            read_rows<R>(sp + colofs(w.x, 0), pl);
+            read_rows<R>(sp + colofs(w.y, 0), ph);
+        }
         const bool lane_last = pass == npass - 1;
         const bool valid = pass < npass && (unsigned)b < (unsigned)nbw;
         const bool bot_out = g == G - 1 && pass < npass - 1;  // bottom row to the next pass
-        const bool score_lane = MODE == 0 ? seg_bot : (lane_last && g == last_g);
+        const bool score_lo = MODE == 0 ? bot_lo : (lane_last && g == last_g);
+        const bool score_hi = MODE == 0 ? bot_hi : (lane_last && g == last_g);
         // the lane's next block, in this pass or (MULTI) at the start of the next
         const bool wrap = MULTI && b + 1 == nbp;
         const int b_n = wrap ? 0 : b + 1, pass_n = wrap ? pass + 1 : pass;
@@ -239,14 +283,14 @@ __device__ __forceinline__ void run_task(const ScoreParams& P, uint32_t gbase0, 
         if (__any(st_lo || st_hi)) {
             const uint32_t m = (st_lo ? 0x0000ffffu : 0u) | (st_hi ? 0xffff0000u : 0u);
             if (st_lo) {
-                pair_lo = seg_none ? -1 : seg_po + slot_entries[sb_lo + kp_lo];
+                pair_lo = none_lo ? -1 : po_lo + slot_entries[sb_lo + kp_lo];
                 kp_lo++;
                 blk_lo = 0;
                 fa_lo = kNoFlag;
                 fb_lo = kNoFlag;
             }
             if (st_hi) {
-                pair_hi = seg_none ? -1 : seg_po + slot_entries[sb_hi + kp_hi];
+                pair_hi = none_hi ? -1 : po_hi + slot_entries[sb_hi + kp_hi];
                 kp_hi++;
                 blk_hi = 0;
                 fa_hi = kNoFlag;
@@ -312,9 +356,9 @@ __device__ __forceinline__ void run_task(const ScoreParams& P, uint32_t gbase0, 
         for (int c = 0; c < 4; ++c) {
             tbh[c] = shift_down<G>(toph[c], obh[c], top_lane);
             tbf[c] = shift_down<G>(topf[c], obf[c], top_lane);
-            if (SEG && seg_top) {  // a packed query's first row: the SW matrix border
-                tbh[c] = as_u(Z);
-                tbf[c] = as_u(Z);
+            if (SEG) {  // a packed query's first row: the SW matrix border
+                tbh[c] = (seg_top & as_u(Z)) | (~seg_top & tbh[c]);
+                tbf[c] = (seg_top & as_u(Z)) | (~seg_top & tbf[c]);
             }
         }
 
@@ -343,18 +387,27 @@ __device__ __forceinline__ void run_task(const ScoreParams& P, uint32_t gbase0, 
             for (int c = 0; c < 4; ++c) {
                 s16x2 f = as_v(tbf[c]);
                 s16x2 fR = adds(f, RR);
-                s16x2 v = pair_row(pl, ph, 0);
+                s16x2 v;
+                if constexpr (PAIR) v = as_v(pl[0]);
+                else v = pair_row(pl, ph, 0);
                 s16x2 h1 = adds(htop, v);  // row 0: diagonal H from the row above
                 htop = as_v(tbh[c]);
                 s16x2 hq = Z;
 #pragma unroll
                 for (int r = 0; r < R; ++r) {
                     const bool more = r + 1 < R;
-                    if (more) v = pair_row(pl, ph, r + 1);
+                    if (more) {
+                        if constexpr (PAIR) v = as_v(pl[r + 1 < R ? r + 1 : 0]);
+                        else v = pair_row(pl, ph, r + 1);
+                    }
                     // the next column's profile rows, once this column's dwords are consumed
                     if (c < 3 && ((R == 4 && r == 3) || (R >= 8 && r % 8 == 7))) {
-                        read_part<R>(sp + colofs(w.x, c + 1), pl, r / 8);
-                        read_part<R>(sp + colofs(w.y, c + 1), ph, r / 8);
+                        if constexpr (PAIR) {
+                            read_part_pair<R>(sp + colofs(w.x, c + 1), pl, r / 8);
+                        } else {
+                            read_part<R>(sp + colofs(w.x, c + 1), pl, r / 8);
+                            read_part<R>(sp + colofs(w.y, c + 1), ph, r / 8);
+                        }
                     }
                     s16x2 h = vmax(h1, f);
                     const s16x2 eR = adds(El[r], RR);
@@ -436,18 +489,18 @@ __device__ __forceinline__ void run_task(const ScoreParams& P, uint32_t gbase0, 
         if (any_last) {
             if (MODE == 0) {  // the lane above finished the same pair one step earlier
                 uint32_t scin = shift_down<G>(as_u(Z), scout, top_lane);
-                if (SEG && seg_top) scin = as_u(Z);
+                if (SEG) scin = (seg_top & as_u(Z)) | (~seg_top & scin);
                 scout = as_u(vmax(S, as_v(scin)));
             }
             const s16x2 fin = MODE == 0 ? as_v(scout) : cap;
             uint32_t m = 0u;
             if (ls_lo && pair_lo >= 0) {
-                finish_pair<MODE>(results, pair_lo, score_lane, lo16(fin), fa_lo, fb_lo);
+                finish_pair<MODE>(results, pair_lo, score_lo, lo16(fin), fa_lo, fb_lo);
                 pair_lo = -1;
                 m |= 0x0000ffffu;
             }
             if (ls_hi && pair_hi >= 0) {
-                finish_pair<MODE>(results, pair_hi, score_lane, hi16(fin), fa_hi, fb_hi);
+                finish_pair<MODE>(results, pair_hi, score_hi, hi16(fin), fa_hi, fb_hi);
                 pair_hi = -1;
                 m |= 0xffff0000u;
             }
@@ -561,17 +614,74 @@ __global__ __launch_bounds__(64 * kWavesPerWG) void score_kernel(const ScoreLaun
             __syncthreads();
         };
         if (MODE == 0 && Q.seg_off >= 0)
-            run_task<R, G, MODE, true, MODE == 0>(P, gbase, sp, slbh, slbe, wres, wbnd, L.slot_entries, Q.pair_off,
-                                                  L.segs + Q.seg_off, L.results, sb_lo, sb_hi, k, g, Wv.nblocks,
-                                                  nbp, npass, last_g, last_r, stage);
+            run_task<R, G, MODE, true, MODE == 0, false>(P, gbase, sp, slbh, slbe, wres, wbnd, L.slot_entries,
+                                                         Q.pair_off, L.segs + Q.seg_off, nullptr, L.results, sb_lo,
+                                                         sb_hi, k, g, Wv.nblocks, nbp, npass, last_g, last_r, stage);
         else if (npass > 1)
-            run_task<R, G, MODE, true, false>(P, gbase, sp, slbh, slbe, wres, wbnd, L.slot_entries, Q.pair_off,
-                                              nullptr, L.results, sb_lo, sb_hi, k, g, Wv.nblocks, nbp, npass,
-                                              last_g, last_r, stage);
+            run_task<R, G, MODE, true, false, false>(P, gbase, sp, slbh, slbe, wres, wbnd, L.slot_entries,
+                                                     Q.pair_off, nullptr, nullptr, L.results, sb_lo, sb_hi, k, g,
+                                                     Wv.nblocks, nbp, npass, last_g, last_r, stage);
         else
-            run_task<R, G, MODE, false, false>(P, gbase, sp, slbh, slbe, wres, wbnd, L.slot_entries, Q.pair_off,
-                                               nullptr, L.results, sb_lo, sb_hi, k, g, Wv.nblocks, nbp, npass,
-                                               last_g, last_r, stage);
+            run_task<R, G, MODE, false, false, false>(P, gbase, sp, slbh, slbe, wres, wbnd, L.slot_entries,
+                                                      Q.pair_off, nullptr, nullptr, L.results, sb_lo, sb_hi, k, g,
+                                                      Wv.nblocks, nbp, npass, last_g, last_r, stage);
+    }
+}
+
+// Paired packed SW queries (run_task's PAIR): two packed queries share every
+// stream, one per half.  The staged profile interleaves their rows (one
+// dword per row), twice the bytes per pass of score_kernel, so a pass holds at
+// most 256 rows.
+template <int R, int G>
+__global__ __launch_bounds__(64 * kWavesPerWG) void score_kernel_pair(const ScoreLaunch L, const ScoreParams P) {
+    constexpr int PR = G * R;
+    static_assert((1 << Log2<PR>::value) == PR, "rows per pass must be a power of two");
+    static_assert(PR <= 256, "two staged paired profiles of 32 codes x PR rows must fit the LDS budget");
+    __shared__ __align__(16) int16_t sprof[2 * 2 * 32 * PR];  // paired profiles of passes p (even, odd)
+    __shared__ uint32_t slb_unused[1];                         // (the NW left border of score_kernel)
+    __shared__ int s_task;
+
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int k = lane / G, g = lane % G;
+    const uint32_t gbase = (uint32_t)(g * R * 4);  // byte offset of the lane's rows in a paired profile code row
+    const char* sp = reinterpret_cast<const char*>(sprof);
+    uint4* __restrict__ wbnd = L.bnd + ((int64_t)blockIdx.x * kWavesPerWG + wv) * L.bnd_per_wave;
+
+    for (;;) {
+        __syncthreads();  // the previous task is done with s_task and the staged profiles
+        if (threadIdx.x == 0) s_task = atomicAdd(L.counter, 1);
+        __syncthreads();
+        const int task = s_task;
+        if (task >= L.ntasks) break;  // the same for the whole workgroup
+        const ScoreTask T = L.tasks[task];
+        const ScoreWave Wv = L.waves[T.wave0 + wv];
+        const QueryDesc Q = L.queries[T.query];
+        const int npass = (Q.qlen + PR - 1) / PR;
+        int nbmax = 0;
+#pragma unroll
+        for (int i = 0; i < kWavesPerWG; ++i) nbmax = max(nbmax, L.waves[T.wave0 + i].nblocks);
+        if (nbmax == 0) continue;
+        const int nbp = npass > 1 ? (max(nbmax, G + 2) + 1) & ~1 : (nbmax + 1) & ~1;
+        const uint2* __restrict__ wres = L.res + Wv.res_off;
+        const int sb_lo = L.slot_first[Wv.slot_base + 2 * k];
+        const int sb_hi = L.slot_first[Wv.slot_base + 2 * k + 1];
+        auto load_pass = [&](int pass) {
+            const int buf = pass & 1;
+            const uint4* src = reinterpret_cast<const uint4*>(L.qp + Q.qp_off + (int64_t)pass * 2 * 32 * PR);
+            uint4* dst = reinterpret_cast<uint4*>(sprof + buf * 2 * 32 * PR);
+            for (int i = threadIdx.x; i < 2 * 32 * PR / 8; i += 64 * kWavesPerWG) dst[i] = src[i];
+        };
+        load_pass(0);
+        if (npass > 1) load_pass(1);
+        __syncthreads();
+        auto stage = [&](int pass) {
+            __syncthreads();  // every lane of the workgroup has left pass - 2
+            load_pass(pass);
+            __syncthreads();
+        };
+        run_task<R, G, 0, true, true, true>(P, gbase, sp, slb_unused, slb_unused, wres, wbnd, L.slot_entries,
+                                            Q.pair_off, L.segs + Q.seg_off, L.segs + Q.seg_off_hi, L.results, sb_lo,
+                                            sb_hi, k, g, Wv.nblocks, nbp, npass, 0, 0, stage);
     }
 }
 
@@ -740,6 +850,59 @@ hipError_t launch_score(int lanes, int rows, int mode, hipStream_t st, const Sco
     if (lanes == GG && rows == RR)                                                                   \
         return mode == 0 ? launch_shape<RR, GG, 0>(st, L, P, max_wg) : launch_shape<RR, GG, 1>(st, L, P, max_wg);
     SSA_SHAPES(SSA_CASE)
+#undef SSA_CASE
+    return hipErrorInvalidValue;
+}
+
+template <int R, int G>
+static hipError_t launch_pair_t(hipStream_t st, const ScoreLaunch& L, const ScoreParams& P, int max_wg) {
+    const int nwg = std::max(1, std::min(L.ntasks, max_wg));
+    hipLaunchKernelGGL(fill_u32_kernel, dim3(1), dim3(64), 0, st, reinterpret_cast<uint32_t*>(L.counter),
+                       (int64_t)1, 0u);
+    hipLaunchKernelGGL((score_kernel_pair<R, G>), dim3(nwg), dim3(64 * kWavesPerWG), 0, st, L, P);
+    return hipGetLastError();
+}
+
+template <int R, int G>
+static int resident_pair_t() {
+    static int cap = 0;
+    if (!cap) {
+        int per_cu = 0, dev = 0;
+        hipDeviceProp_t prop;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, score_kernel_pair<R, G>, 64 * kWavesPerWG, 0) !=
+                hipSuccess ||
+            hipGetDevice(&dev) != hipSuccess || hipGetDeviceProperties(&prop, dev) != hipSuccess)
+            return 0;
+        cap = std::max(1, per_cu) * prop.multiProcessorCount;
+    }
+    return cap;
+}
+
+// shapes of the paired kernel (lanes per stream pair, rows per lane)
+#define SSA_PAIR_SHAPES(X) X(8, 16) X(8, 32) X(16, 8) X(16, 16) X(32, 8)
+
+bool score_pair_supported(int lanes, int rows) {
+#define SSA_CASE(GG, RR) \
+    if (lanes == GG && rows == RR) return true;
+    SSA_PAIR_SHAPES(SSA_CASE)
+#undef SSA_CASE
+    return false;
+}
+
+int score_pair_resident_workgroups(int lanes, int rows) {
+#define SSA_CASE(GG, RR) \
+    if (lanes == GG && rows == RR) return resident_pair_t<RR, GG>();
+    SSA_PAIR_SHAPES(SSA_CASE)
+#undef SSA_CASE
+    return 0;
+}
+
+hipError_t launch_score_pair(int lanes, int rows, hipStream_t st, const ScoreLaunch& L, const ScoreParams& P,
+                             int max_wg) {
+    if (L.ntasks <= 0) return hipSuccess;
+#define SSA_CASE(GG, RR) \
+    if (lanes == GG && rows == RR) return launch_pair_t<RR, GG>(st, L, P, max_wg);
+    SSA_PAIR_SHAPES(SSA_CASE)
 #undef SSA_CASE
     return hipErrorInvalidValue;
 }

@@ -468,6 +468,7 @@ struct DbChunks {
 // runs of 2*64/G per wave, so the waves that start first are the longest.
 struct StreamLayout {
     int lanes = 0;
+    bool paired = false;  // the lo and hi streams of each lane group hold the same entries
     int nwaves = 0;                      // a multiple of kWavesPerWG
     std::vector<int32_t> wave_nblocks;   // per wave
     std::vector<int64_t> wave_res_off;   // per wave, uint2 units
@@ -492,12 +493,16 @@ struct StreamLayout {
     }
 };
 
-void build_layout(Exec& ex, StreamLayout& L, int lanes, const PairSet& ps, const std::vector<uint32_t>& order) {
+void build_layout(Exec& ex, StreamLayout& L, int lanes, const PairSet& ps, const std::vector<uint32_t>& order,
+                  bool paired) {
     DeviceDB& db = g_db;
     const int ng = 64 / lanes, spw = 2 * ng;
+    // distinct streams per wave: both halves of a lane group, or (paired) one
+    const int gsz = paired ? ng : spw;
     const std::vector<uint32_t>& pe_list = ps.list();
     const size_t n = pe_list.size();
     L.lanes = lanes;
+    L.paired = paired;
     std::vector<int32_t> nbk(n);
     int64_t total = 0;
     for (size_t k = 0; k < n; ++k) {
@@ -505,11 +510,11 @@ void build_layout(Exec& ex, StreamLayout& L, int lanes, const PairSet& ps, const
         total += nbk[k];
     }
     const int64_t per_wave = std::max<int64_t>(1, ((int64_t)target_waves()));
-    const int64_t B = std::max<int64_t>(16 * (int64_t)lanes, (total + spw * per_wave - 1) / (spw * per_wave));
+    const int64_t B = std::max<int64_t>(16 * (int64_t)lanes, (total + gsz * per_wave - 1) / (gsz * per_wave));
     int64_t ns = std::max<int64_t>(1, (total + B - 1) / B);
-    const int64_t unit = (int64_t)spw * kWavesPerWG;
+    const int64_t unit = (int64_t)gsz * kWavesPerWG;
     ns = (ns + unit - 1) / unit * unit;
-    const int nw = (int)(ns / spw);
+    const int nw = (int)(ns / gsz);
 
     // Lockstep groups: the entries in length order (longest first) are cut
     // into groups of spw; a wave runs its groups one after the other, stream
@@ -520,7 +525,7 @@ void build_layout(Exec& ex, StreamLayout& L, int lanes, const PairSet& ps, const
     // streams starts a pair; consecutive entries in length order differ by
     // little, so the padding is small.  Groups go longest first to the least
     // loaded wave (LPT).
-    const size_t ngroups = (n + spw - 1) / spw;
+    const size_t ngroups = (n + gsz - 1) / gsz;
     std::vector<int64_t> wload(nw, 0);
     std::vector<std::vector<uint32_t>> wgroups(nw);
     {
@@ -531,7 +536,7 @@ void build_layout(Exec& ex, StreamLayout& L, int lanes, const PairSet& ps, const
             Item it = pq.top();
             pq.pop();
             wgroups[it.second].push_back((uint32_t)gi);
-            it.first += nbk[order[gi * spw]];  // the group's first entry is its longest
+            it.first += nbk[order[gi * gsz]];  // the group's first entry is its longest
             wload[it.second] = it.first;
             pq.push(it);
         }
@@ -557,7 +562,7 @@ void build_layout(Exec& ex, StreamLayout& L, int lanes, const PairSet& ps, const
             const int k = sl / 2, h = sl % 2;
             int64_t pos = 0;
             for (uint32_t gi : gs) {
-                const size_t i = gi * spw + sl;
+                const size_t i = gi * gsz + (paired ? k : sl);
                 if (i < n) {
                     const uint32_t at = order[i];
                     L.slot_entries.push_back(at);
@@ -571,7 +576,7 @@ void build_layout(Exec& ex, StreamLayout& L, int lanes, const PairSet& ps, const
                     p.pad = 0;
                     pe.push_back(p);
                 }
-                pos += nbk[order[gi * spw]];
+                pos += nbk[order[gi * gsz]];
             }
             L.slot_first[(size_t)w * spw + sl + 1] = (int32_t)L.slot_entries.size();
         }
@@ -647,7 +652,8 @@ struct TierRun {
     std::vector<int16_t> prof;
     std::vector<uint32_t> lbv;
     std::vector<SegLane> segs;
-    bool packed = false;  // SW with several queries: scored as one packed query
+    bool packed = false;  // SW with several queries: scored as packed queries
+    bool paired = false;  // packed queries two by two, one per half of every lane group (score_kernel_pair)
 
     ~TierRun() { local.release(); }
     bool empty() const { return ps == nullptr || ps->list().empty() || qs.empty(); }
@@ -684,14 +690,16 @@ void tier_prepare(Exec& ex, TierRun& R) {
 
     // ---- profiles [pass][code][row], left boundaries, per-query descriptors ----
     R.qd.assign(nq, QueryDesc{});
-    for (QueryDesc& d : R.qd) d.seg_off = -1;
+    for (QueryDesc& d : R.qd) d.seg_off = d.seg_off_hi = -1;
     R.prof.clear();
     R.lbv.clear();
     R.segs.clear();
     std::vector<int> npass(nq);
     const int64_t QRw = T.wrap(go + ge), Rw = T.wrap(ge);
-    // SW with several queries: packed queries (below)
+    // SW with several queries: packed queries (below), two by two where the
+    // shape has a paired kernel
     R.packed = R.type == kSW && nq > 1;
+    R.paired = R.packed && score_pair_supported(sh.lanes, sh.rows);
     for (size_t qi = 0; qi < nq && !R.packed; ++qi) {
         const QueryBuf& q = R.qs[qi];
         npass[qi] = (int)((q.len + PR - 1) / PR);
@@ -744,11 +752,11 @@ void tier_prepare(Exec& ex, TierRun& R) {
     if (R.sorted_all) {
         std::lock_guard<std::mutex> lk(layout_mutex());
         for (StreamLayout& c : layout_cache())
-            if (c.lanes == sh.lanes && c.generation == db.generation) R.L = &c;
+            if (c.lanes == sh.lanes && c.paired == R.paired && c.generation == db.generation) R.L = &c;
         if (!R.L) {
             layout_cache().emplace_back();
             R.L = &layout_cache().back();
-            build_layout(ex, *R.L, sh.lanes, ps, *R.sorted_all);
+            build_layout(ex, *R.L, sh.lanes, ps, *R.sorted_all, R.paired);
             R.L->generation = db.generation;
         }
     } else {
@@ -757,7 +765,7 @@ void tier_prepare(Exec& ex, TierRun& R) {
         std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) {
             return db.entries[ps.list()[a]].len > db.entries[ps.list()[b]].len;
         });
-        build_layout(ex, R.local, sh.lanes, ps, order);
+        build_layout(ex, R.local, sh.lanes, ps, order, R.paired);
         R.L = &R.local;
     }
     const StreamLayout* L = R.L;
@@ -765,7 +773,8 @@ void tier_prepare(Exec& ex, TierRun& R) {
     const int nw = L->nwaves;
     if ((uint64_t)nq * n >= (uint64_t)INT32_MAX) fatal("libssa_amd: too many (query, entry) pairs in one launch");
     const int nt_q = nw / kWavesPerWG;
-    const int cap = score_resident_workgroups(sh.lanes, sh.rows, R.type == kSW ? 0 : 1);
+    const int cap = R.paired ? score_pair_resident_workgroups(sh.lanes, sh.rows)
+                             : score_resident_workgroups(sh.lanes, sh.rows, R.type == kSW ? 0 : 1);
 
     // ---- packed queries (SW, several queries) ----
     // Each query starts at a lane boundary and is padded to whole lanes (the
@@ -789,65 +798,117 @@ void tier_prepare(Exec& ex, TierRun& R) {
             qrows[qi] = (int64_t)((R.qs[qi].len + sh.rows - 1) / sh.rows) * sh.rows;
             total += qrows[qi];
         }
-        const int64_t total_passes = std::max<int64_t>(1, (total + PR - 1) / PR);
+        const int halves = R.paired ? 2 : 1;
+        const int64_t total_passes = std::max<int64_t>(1, (total + (int64_t)halves * PR - 1) / ((int64_t)halves * PR));
         const int64_t sq_max =
             std::max<int64_t>(1, total_passes * per_pass / (4 * (int64_t)std::max(cap, 1) * longest));
-        auto build_sq = [&](size_t q0, size_t q1) {
-            std::vector<int64_t> start(q1 - q0);
+        // a packed query: the rows of the listed queries end to end
+        struct Rows {
+            std::vector<uint32_t> qs;
+            std::vector<int64_t> start;
             int64_t rows = 0;
-            for (size_t qi = q0; qi < q1; ++qi) {
-                start[qi - q0] = rows;
+        };
+        auto rows_of = [&](const std::vector<uint32_t>& list) {
+            Rows r;
+            r.qs = list;
+            for (uint32_t qi : list) {
+                r.start.push_back(r.rows);
+                r.rows += qrows[qi];
+            }
+            return r;
+        };
+        // cuts a list of queries (in order) into packed queries of at most sq_max passes
+        auto cut = [&](const std::vector<uint32_t>& list) {
+            std::vector<std::vector<uint32_t>> out;
+            std::vector<uint32_t> cur;
+            int64_t rows = 0;
+            for (uint32_t qi : list) {
+                if (!cur.empty() && rows + qrows[qi] > sq_max * PR) {
+                    out.push_back(cur);
+                    cur.clear();
+                    rows = 0;
+                }
+                cur.push_back(qi);
                 rows += qrows[qi];
             }
-            const int np = (int)((rows + PR - 1) / PR);
-            if (np == 0) return;  // empty queries only: their results keep the initial values
-            if ((int64_t)R.prof.size() + (int64_t)np * 32 * PR >= (int64_t)INT32_MAX)
-                fatal("libssa_amd: packed queries too long");
-            std::vector<int32_t> row_q((size_t)np * PR, -1);  // query of each row (-1 past the last)
-            for (size_t qi = q0; qi < q1; ++qi)
-                for (int64_t i = start[qi - q0]; i < start[qi - q0] + qrows[qi]; ++i) row_q[i] = (int32_t)qi;
-            QueryDesc d{};
-            d.qlen = (int32_t)std::min<int64_t>(rows, INT32_MAX);
-            d.qp_off = (int32_t)R.prof.size();
-            d.seg_off = (int32_t)R.segs.size();
-            R.prof.resize(R.prof.size() + (size_t)np * 32 * PR);
-            int16_t* pf = R.prof.data() + d.qp_off;
+            if (!cur.empty()) out.push_back(cur);
+            return out;
+        };
+        // the profile values and segments of one half of a packed query
+        auto half = [&](const Rows& rw, int np, int h, int16_t* pf, std::vector<SegLane>& segs) {
+            std::vector<int32_t> row_q((size_t)np * PR + sh.rows, -1);  // list position of each row
+            for (size_t x = 0; x < rw.qs.size(); ++x)
+                for (int64_t i = rw.start[x]; i < rw.start[x] + qrows[rw.qs[x]]; ++i) row_q[i] = (int32_t)x;
             for (int p = 0; p < np; ++p)
                 for (int c = 0; c < kDim; ++c)
                     for (int r = 0; r < PR; ++r) {
                         const int64_t i = (int64_t)p * PR + r;
-                        const int32_t qi = row_q[i];
-                        const int64_t k = qi >= 0 ? i - start[qi - q0] : 0;
+                        const int32_t x = row_q[i];
                         int32_t v = -32768;
-                        if (qi >= 0 && k < (int64_t)R.qs[qi].len) v = T.repr(T.wrap(M.at(c, R.qs[qi].seq[k])));
-                        pf[((size_t)p * 32 + c) * PR + r] = (int16_t)v;
+                        if (x >= 0) {
+                            const QueryBuf& q = R.qs[rw.qs[x]];
+                            const int64_t k = i - rw.start[x];
+                            if (k < (int64_t)q.len) v = T.repr(T.wrap(M.at(c, q.seq[k])));
+                        }
+                        pf[(((size_t)p * 32 + c) * PR + r) * halves + h] = (int16_t)v;
                     }
             for (int p = 0; p < np; ++p)
                 for (int g = 0; g < sh.lanes; ++g) {
                     const int64_t i = (int64_t)p * PR + (int64_t)g * sh.rows;
-                    const int32_t qi = row_q[i];
-                    if (qi < 0) {
-                        R.segs.push_back(SegLane{0, kSegNone});
+                    const int32_t x = row_q[i];
+                    if (x < 0) {
+                        segs.push_back(SegLane{0, kSegNone});
                         continue;
                     }
-                    const bool top = i == start[qi - q0];
-                    const bool bottom = g == sh.lanes - 1 || row_q[i + sh.rows] != qi;
-                    R.segs.push_back(SegLane{(int32_t)(qi * n), (top ? kSegTop : 0) | (bottom ? kSegBottom : 0)});
+                    const bool top = i == rw.start[x];
+                    const bool bottom = g == sh.lanes - 1 || row_q[i + sh.rows] != x;
+                    segs.push_back(
+                        SegLane{(int32_t)(rw.qs[x] * n), (top ? kSegTop : 0) | (bottom ? kSegBottom : 0)});
                 }
+        };
+        // one packed query (lo), or two sharing the streams (lo, hi)
+        auto build = [&](const std::vector<uint32_t>& lo, const std::vector<uint32_t>& hi) {
+            const Rows rl = rows_of(lo), rh = rows_of(hi);
+            const int64_t rows = std::max(rl.rows, rh.rows);
+            const int np = (int)((rows + PR - 1) / PR);
+            if (np == 0) return;  // empty queries only: their results keep the initial values
+            if ((int64_t)R.prof.size() + (int64_t)np * 32 * PR * halves >= (int64_t)INT32_MAX)
+                fatal("libssa_amd: packed queries too long");
+            QueryDesc d{};
+            d.qlen = (int32_t)std::min<int64_t>(rows, INT32_MAX);
+            d.qp_off = (int32_t)R.prof.size();
+            R.prof.resize(R.prof.size() + (size_t)np * 32 * PR * halves);
+            int16_t* pf = R.prof.data() + d.qp_off;
+            d.seg_off = (int32_t)R.segs.size();
+            half(rl, np, 0, pf, R.segs);
+            d.seg_off_hi = (int32_t)R.segs.size();
+            if (R.paired) half(rh, np, 1, pf, R.segs);
             R.qd.push_back(d);
             sq_npass.push_back(np);
         };
-        size_t q0 = 0;
-        int64_t rows = 0;
-        for (size_t qi = 0; qi < nq; ++qi) {
-            if (qi > q0 && rows + qrows[qi] > sq_max * PR) {
-                build_sq(q0, qi);
-                q0 = qi;
-                rows = 0;
+        std::vector<uint32_t> all_q(nq);
+        std::iota(all_q.begin(), all_q.end(), 0u);
+        if (!R.paired) {
+            for (const auto& sq : cut(all_q)) build(sq, {});
+        } else {
+            // two halves of nearly equal rows (longest queries first, each to
+            // the half with fewer rows), cut alike and paired in order
+            std::stable_sort(all_q.begin(), all_q.end(), [&](uint32_t a, uint32_t b) { return qrows[a] > qrows[b]; });
+            std::vector<uint32_t> lo, hi;
+            int64_t rlo = 0, rhi = 0;
+            for (uint32_t qi : all_q) {
+                if (rlo <= rhi) {
+                    lo.push_back(qi);
+                    rlo += qrows[qi];
+                } else {
+                    hi.push_back(qi);
+                    rhi += qrows[qi];
+                }
             }
-            rows += qrows[qi];
+            const auto slo = cut(lo), shi = cut(hi);
+            for (size_t x = 0; x < std::max(slo.size(), shi.size()); ++x)
+                build(x < slo.size() ? slo[x] : std::vector<uint32_t>(), x < shi.size() ? shi[x] : std::vector<uint32_t>());
         }
-        build_sq(q0, nq);
     }
 
     // ---- tasks: every (packed) query against every group of 4 waves, costliest first ----
@@ -923,7 +984,8 @@ void tier_prepare(Exec& ex, TierRun& R) {
 void tier_launch(hipStream_t st, TierRun& R) {
     if (R.empty()) return;
     SSA_HIP(launch_init_results(st, R.d_out, (int64_t)R.nout, -32768));
-    SSA_HIP(launch_score(R.sh.lanes, R.sh.rows, R.type == kSW ? 0 : 1, st, R.K, R.P, R.grid));
+    if (R.paired) SSA_HIP(launch_score_pair(R.sh.lanes, R.sh.rows, st, R.K, R.P, R.grid));
+    else SSA_HIP(launch_score(R.sh.lanes, R.sh.rows, R.type == kSW ? 0 : 1, st, R.K, R.P, R.grid));
 }
 
 void tier_finish(Exec& ex, TierRun& R) {

@@ -228,16 +228,18 @@ def test_packed_db_other_symbol_type_maps_from_residues(tmp_path):
         ssa.free_sequence(q)
 
 
-SW_SHAPES = [(8, 4), (16, 4), (64, 4), (8, 8), (32, 8), (64, 8), (8, 16), (16, 16), (32, 16), (8, 32), (16, 32)]
+SW_SHAPES = [(8, 4), (16, 4), (64, 4), (8, 8), (16, 8), (32, 8), (64, 8), (8, 16), (16, 16), (32, 16), (8, 32), (16, 32)]
 
 
 @pytest.mark.parametrize("shape", SW_SHAPES)
 def test_packed_sw_batch_matches_oracle(shape):
-    """The SW searches of a batch run as one packed query: their rows end to
-    end, each query from a lane boundary.  Queries shorter than a lane, of
-    exactly one lane, ending one row into a lane, filling a pass exactly and
-    crossing pass boundaries, at 8/16/64 bit, every SW kernel shape; each list
-    and its overflow counters must be the oracle's."""
+    """The SW searches of a batch run as packed queries: their rows end to
+    end, each query from a lane boundary, and, for shapes of at most 256 rows
+    per pass, two packed queries sharing every stream (one per half).  Queries
+    shorter than a lane, of exactly one lane, ending one row into a lane,
+    filling a pass exactly and crossing pass boundaries, an odd number of
+    them, at 8/16/64 bit, every SW kernel shape; each list and its overflow
+    counters must be the oracle's."""
     lanes, rows = shape
     pr = lanes * rows
     rnd = random.Random(7000 + lanes * 100 + rows)
