@@ -372,7 +372,10 @@ const std::vector<Shape>& kernel_shapes(int type) {
     return type == 0 ? sw : nw;
 }
 
-Shape choose_shape(size_t qlen_max, int64_t total_blocks, int64_t max_blocks, int type) {
+// pair: packed SW queries of several buffers, which run paired (without the
+// v_perm per cell pair) on the shapes with a paired kernel; only those are
+// considered.
+Shape choose_shape(size_t qlen_max, int64_t total_blocks, int64_t max_blocks, int type, bool pair) {
     const Shape ov = shape_override();
     // a forced shape of 32 rows per lane runs NW with 16 (the NW kernel has no 32-row shape)
     if (ov.lanes && ov.rows) return (type == 1 && ov.rows == 32) ? Shape{ov.lanes, 16} : ov;
@@ -395,6 +398,7 @@ Shape choose_shape(size_t qlen_max, int64_t total_blocks, int64_t max_blocks, in
             double bt = 1e300;
             for (const TunedShape& t : tuned_shapes()) {
                 if (t.type != type || t.qlen != near) continue;
+                if (pair && !score_pair_supported(t.sh.lanes, t.sh.rows)) continue;
                 const size_t pr = (size_t)t.sh.lanes * t.sh.rows;
                 const double p_q = (double)std::max<size_t>(1, (qlen_max + pr - 1) / pr);
                 const double p_t = (double)std::max<size_t>(1, (t.qlen + pr - 1) / pr);
@@ -407,9 +411,10 @@ Shape choose_shape(size_t qlen_max, int64_t total_blocks, int64_t max_blocks, in
             if (best.lanes) return best;
         }
     }
-    Shape best = kernel_shapes(type == 0 ? 0 : 1)[0];
+    Shape best = pair ? Shape{8, 32} : kernel_shapes(type == 0 ? 0 : 1)[0];
     double best_cost = 1e300;
     for (const Shape& sh : kernel_shapes(type == 0 ? 0 : 1)) {
+        if (pair && !score_pair_supported(sh.lanes, sh.rows)) continue;
         const int64_t pr = (int64_t)sh.lanes * sh.rows;
         const int64_t passes = std::max<int64_t>(1, ((int64_t)qlen_max + pr - 1) / pr);
         // measured per-row costs: short lanes pay the fixed per-step work (32
@@ -420,6 +425,8 @@ Shape choose_shape(size_t qlen_max, int64_t total_blocks, int64_t max_blocks, in
         // 512-row passes double-buffer 64 KiB of profile: two workgroups per
         // CU, which costs the 16-row shape its third wave per SIMD
         if (sh.lanes * sh.rows == 512 && sh.rows == 16) per_row *= 1.06;
+        // paired: no v_perm (one of the ten packed ops of a SW cell pair)
+        if (pair) per_row *= 0.9;
         const double waves = std::max<double>(target_waves(), 1.0);
         // the passes of a task run as one pipeline: G - 1 fill steps per task
         const double work =
@@ -682,7 +689,7 @@ void tier_prepare(Exec& ex, TierRun& R) {
             total_blocks += nb;
             max_blocks = std::max(max_blocks, nb);
         }
-        R.sh = choose_shape(qmax, total_blocks, max_blocks, R.type == kSW ? 0 : 1);
+        R.sh = choose_shape(qmax, total_blocks, max_blocks, R.type == kSW ? 0 : 1, R.type == kSW && nq > 1);
     }
     const Shape sh = R.sh;
     if (!score_shape_supported(sh.lanes, sh.rows, R.type == kSW ? 0 : 1)) fatal("libssa_amd: unsupported kernel shape");
@@ -1793,7 +1800,7 @@ void run_batch(Exec& ex, const std::vector<SearchRequest>& reqs, std::vector<Sea
         if (J.first_bits == 0 || N == 0 || nq == 0) continue;
         size_t qmax = 0;
         for (const QueryBuf& q : J.qs) qmax = std::max(qmax, q.len);
-        J.shape = choose_shape(qmax, total_blocks, max_blocks, J.rq.type == kSW ? 0 : 1);
+        J.shape = choose_shape(qmax, total_blocks, max_blocks, J.rq.type == kSW ? 0 : 1, false);
         // SW searches of one tier share one launch as a packed query whatever
         // their lengths; NW searches share a launch per kernel shape
         int gi = -1;
@@ -1813,7 +1820,7 @@ void run_batch(Exec& ex, const std::vector<SearchRequest>& reqs, std::vector<Sea
         size_t nqg = 0, qsum = 0;
         for (size_t j : G.jobs)
             for (size_t q = 0; q < jobs[j].nrun; ++q) ++nqg, qsum += jobs[j].qs[q].len;
-        if (nqg > 1) G.sh = choose_shape(qsum, total_blocks, max_blocks, 0);
+        if (nqg > 1) G.sh = choose_shape(qsum, total_blocks, max_blocks, 0, true);
         for (size_t j : G.jobs) jobs[j].shape = G.sh;
     }
     std::vector<std::unique_ptr<TierRun>> runs(groups.size());
@@ -2031,7 +2038,7 @@ void run_batch(Exec& ex, const std::vector<SearchRequest>& reqs, std::vector<Sea
             if (!J.requeue_full) continue;
             size_t qmax = 0;
             for (const QueryBuf& q : J.qs) qmax = std::max(qmax, q.len);
-            const Shape sh = choose_shape(qmax, total_blocks, max_blocks, J.rq.type == kSW ? 0 : 1);
+            const Shape sh = choose_shape(qmax, total_blocks, max_blocks, J.rq.type == kSW ? 0 : 1, false);
             int gi = -1;
             for (size_t g = 0; g < rgroups.size(); ++g)
                 if (rgroups[g].type == J.rq.type &&
@@ -2049,7 +2056,7 @@ void run_batch(Exec& ex, const std::vector<SearchRequest>& reqs, std::vector<Sea
             size_t nqg = 0, qsum = 0;
             for (size_t j : G.jobs)
                 for (const QueryBuf& q : jobs[j].qs) ++nqg, qsum += q.len;
-            if (nqg > 1) G.sh = choose_shape(qsum, total_blocks, max_blocks, 0);
+            if (nqg > 1) G.sh = choose_shape(qsum, total_blocks, max_blocks, 0, true);
         }
         for (size_t g = 0; g < rgroups.size(); ++g) {
             TierRun R;
