@@ -1355,6 +1355,7 @@ struct Job {
     int group = -1;
     bool requeue_full = false;  // 16-bit re-run over the whole DB (most entries overflowed at 8 bit)
     bool twin = false;          // both strands: the reverse strand's first tier read from the forward's
+    bool skip8 = false;         // NW at 8 bit whose every pair provably overflows in its first block
     size_t nrun = 0;            // query buffers the first tier scores
     bool requeue_sub = false;   // 16-bit re-run over the overflowed entries only
     size_t rq_off = 0;          // first query of this job in its re-run group
@@ -1690,6 +1691,41 @@ const DbChunks& db_chunks(const DeviceDB& db, size_t cs) {
     return d;
 }
 
+// NW at 8 bit flags every pair in its first block once the query is long
+// enough: down column 0 the left border and the vertical gap from row 0 fall
+// by a gap extension per row, and soon a cell lies below the overflow
+// threshold (I_MIN - gapO - gapE - 1, src/algo/simd/search_simd_nw.c:372).
+// The score kernel's column 0 is one chain of saturating max-plus steps down
+// the rows (the left border H and E staged per row, the top border's F
+// entering row 0, the diagonal from the row above's left border), monotone in
+// the profile values; run on the host with each row's best score over every
+// DB code, it bounds every entry's column 0 from above.  If the bound falls
+// below the threshold on some real row, every pair's first block is flagged
+// (a lane spends one block on it), which is all the 8-bit tier yields for an
+// overflowed pair, so the search records that and skips the 8-bit launch.
+bool nw8_first_block_flags(const QueryBuf& q) {
+    const Config& cf = config();
+    const Matrix& M = matrix();
+    const Tier& T = kTier8;
+    const int go = cf.gap_open, ge = cf.gap_extend;
+    const int64_t QR = T.wrap(go + ge), RR = T.wrap(ge);
+    const int64_t thr = T.wrap(T.imin() - go - ge - 1);
+    int64_t f = T.sat(T.wrap(go + ge) + QR);  // F entering row 0: the top border's, plus open
+    int64_t c = QR;                           // left border H of row i
+    int64_t hprev = 0;                        // left border H of row i - 1 (the corner for row 0)
+    for (size_t i = 0; i < q.len; ++i) {
+        int64_t v = INT64_MIN;
+        for (int d = 0; d < kDim; ++d) v = std::max(v, T.wrap(M.at(d, q.seq[i])));
+        int64_t h = std::max(T.sat(hprev + v), f);
+        h = std::max(h, T.sat(c + QR));  // E entering column 0: the left border's, plus open
+        if (h < thr) return true;
+        f = std::max(T.sat(f + RR), T.sat(h + QR));
+        hprev = c;
+        c = T.sat(c + RR);
+    }
+    return false;
+}
+
 // A both-strand nucleotide search whose reverse query strand need not be
 // scored: the alignment score is unchanged when both sequences are reversed,
 // and, with a matrix that scores complements alike, when both are
@@ -1795,9 +1831,11 @@ void run_batch(Exec& ex, const std::vector<SearchRequest>& reqs, std::vector<Sea
         else J.first_bits = mi.fits16 ? 16 : 0;
         J.twin = strand_twins(J, dc, mi);
         J.nrun = J.twin ? 1 : nq;
-        if (J.first_bits != 0)
+        J.skip8 = J.rq.type == kNW && J.first_bits == 8 && nq > 0 &&
+                  std::all_of(J.qs.begin(), J.qs.end(), [](const QueryBuf& q) { return nw8_first_block_flags(q); });
+        if (J.first_bits != 0 && !J.skip8)
             for (size_t q = 0; q < J.nrun; ++q) J.st.cells_scored += (uint64_t)J.qs[q].len * total_residues;
-        if (J.first_bits == 0 || N == 0 || nq == 0) continue;
+        if (J.first_bits == 0 || N == 0 || nq == 0 || J.skip8) continue;
         size_t qmax = 0;
         for (const QueryBuf& q : J.qs) qmax = std::max(qmax, q.len);
         J.shape = choose_shape(qmax, total_blocks, max_blocks, J.rq.type == kSW ? 0 : 1, false);
@@ -1956,6 +1994,13 @@ void run_batch(Exec& ex, const std::vector<SearchRequest>& reqs, std::vector<Sea
                         for (uint32_t e = e0; e < e1; ++e)
                             if (J.in16[e] && !J.t16[at(q, e)].accept()) J.need64[e] = 1;
                 });
+            } else if (J.skip8) {
+                // every pair overflows in its first block (nw8_first_block_flags)
+                TierOut* t8 = J.t8.data();
+                for (size_t i = 0; i < nq * N; ++i) t8[i] = TierOut{-1, 0};
+                if (N) memset(in16, 1, N);
+                J.t16.reset(nq * N);
+                J.requeue_full = N > 0;
             } else {
                 size_t n16 = 0;
                 std::mutex mu;

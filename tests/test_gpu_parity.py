@@ -320,3 +320,25 @@ def test_both_strands_both_scored_past_int16_bound(t):
     check(t, 16, O.NUCLEOTIDE, 3, q, db, 10, ("const", (2, -4)), (-20, -2))
     st = ssa.last_stats()
     assert st.cells_scored == st.cells
+
+
+@pytest.mark.parametrize("qlen,skipped", [(60, False), (140, True), (420, True)])
+def test_nw8_first_block_overflow_without_the_8bit_pass(qlen, skipped):
+    """NW at 8 bit: once the query is long enough, a bound on column 0 shows
+    that every pair overflows in its first block, and the 8-bit launch is
+    skipped (no cells scored by a first tier); hits, overflow counts and
+    CIGARs must still equal the oracle's, which runs the 8-bit tier."""
+    rnd = random.Random(4400 + qlen)
+    base = "".join(rnd.choice("ARNDCQEGHILKMFPSTWYV") for _ in range(qlen))
+    db = rand_db(rnd, O.AMINOACID, 300, 1, 500, homolog_of=base, frac=0.1)
+    check(O.NW, 8, O.AMINOACID, 1, base, db, 20, ("builtin", "blosum62"), (-10, -1), align=True)
+    st = ssa.last_stats()
+    assert (st.cells_scored == 0) == skipped
+
+
+def test_nw8_first_block_overflow_both_strands():
+    rnd = random.Random(4500)
+    q = "".join(rnd.choice("ACGT") for _ in range(180))
+    db = rand_db(rnd, O.NUCLEOTIDE, 200, 20, 300, homolog_of=q, frac=0.1)
+    check(O.NW, 8, O.NUCLEOTIDE, 3, q, db, 15, ("const", (2, -4)), (-20, -2))
+    assert ssa.last_stats().cells_scored == 0
