@@ -30,6 +30,9 @@ typedef struct {
                                  reverse-complement twins scores half of `cells`) */
     int lanes_per_stream;     /* score kernel shape of the first tier: lanes per stream pair */
     int rows_per_lane;        /* and query rows per lane */
+    int overflow_tracking;    /* 1 when the first tier's kernel tracked overflow (every 8-bit
+                                 tier and SW; NW at 16 bit only where some value could reach
+                                 either end of the int16 range), 0 when it provably cannot */
     double ms_gpu_first;      /* first-tier kernel time (HIP events) */
     double ms_gpu_requeue;    /* re-queued tiers (16-bit pass, 32-bit exact pass) */
     double ms_replay;         /* host replay of the reference's lane schedule and heap */

@@ -76,6 +76,7 @@ class Stats(_C.Structure):
                 ("overflow_8_bit_count", _C.c_size_t), ("overflow_16_bit_count", _C.c_size_t),
                 ("cells", _C.c_uint64), ("cells_computed", _C.c_uint64), ("cells_scored", _C.c_uint64),
                 ("lanes_per_stream", _C.c_int), ("rows_per_lane", _C.c_int),
+                ("overflow_tracking", _C.c_int),
                 ("ms_gpu_first", _C.c_double), ("ms_gpu_requeue", _C.c_double), ("ms_replay", _C.c_double),
                 ("ms_traceback", _C.c_double), ("ms_total", _C.c_double)]
 

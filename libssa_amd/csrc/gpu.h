@@ -91,6 +91,9 @@ private:
 // ---- kernel launchers (score_kernels.hip, traceback_kernels.hip, pack_kernels.hip) ----
 bool score_shape_supported(int lanes, int rows, int mode);
 int score_resident_workgroups(int lanes, int rows, int mode);  // persistent grid size of a shape
+// NW shapes with an untracked variant (mode 2: no value of any pair can reach
+// either end of the int16 range)
+bool score_nw_untracked_supported(int lanes, int rows);
 // paired packed SW queries (score_kernel_pair): shapes, grid size, launch
 bool score_pair_supported(int lanes, int rows);
 int score_pair_resident_workgroups(int lanes, int rows);

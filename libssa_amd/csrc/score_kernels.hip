@@ -1,6 +1,9 @@
 // score_kernels.hip -- gfx950 kernels of the database search.
 //
-// score_kernel<R, G, MODE>: SW (MODE 0) / NW (MODE 1) over packed int16 lanes.
+// score_kernel<R, G, MODE>: SW (MODE 0) / NW (MODE 1) over packed int16 lanes;
+// MODE 2 is NW without the overflow trackers, for 16-bit launches where the
+// host has shown that no value of any pair can reach either end of the int16
+// range (then no block can be flagged and the trackers are dead work).
 // Restates the per-lane arithmetic of the reference's width kernels
 // (src/algo/simd/search_simd_sw.c:172-277, search_simd_nw.c:180-312):
 // saturating adds, the H/E/F recurrence and the overflow predicates, but not
@@ -416,7 +419,7 @@ __device__ __forceinline__ void run_task(const ScoreParams& P, uint32_t gbase0, 
                     const s16x2 tt = adds(h, QR);
                     if (MODE == 0) {
                         S = vmax(S, h);
-                    } else {
+                    } else if (MODE == 1) {
                         hmn = vmin(hmn, h);
                         hmx = vmax(hmx, h);
                     }
@@ -435,7 +438,7 @@ __device__ __forceinline__ void run_task(const ScoreParams& P, uint32_t gbase0, 
             }
         };
         // only the lane holding the last query row records a score cell
-        if (MODE == 1 && __any(lane_last && g == last_g && (ls_lo || ls_hi))) columns(Tag<true>{});
+        if (MODE >= 1 && __any(lane_last && g == last_g && (ls_lo || ls_hi))) columns(Tag<true>{});
         else columns(Tag<false>{});
         // overflow predicates at the end of the block: a packed test here (in
         // the cell loop's block, so the running reductions stay in the loop),
@@ -443,8 +446,10 @@ __device__ __forceinline__ void run_task(const ScoreParams& P, uint32_t gbase0, 
         bool fire;
         if (MODE == 0) {
             fire = as_u(vmax(S, as_v(TA))) != TA || as_u(vmax(S, as_v(TB))) != TB;
-        } else {
+        } else if (MODE == 1) {
             fire = as_u(vmin(hmn, as_v(TA))) != TA || as_u(vmax(hmx, as_v(TB))) != TB;
+        } else {
+            fire = false;
         }
         if (__any(fire)) {
             if (MODE == 0) {
@@ -476,7 +481,7 @@ __device__ __forceinline__ void run_task(const ScoreParams& P, uint32_t gbase0, 
             wbnd[o + 1] = make_uint4(obh[2], obf[2], obh[3], obf[3]);
         }
 
-        if (MODE == 1 && FIRST) {  // the reference advances the top border by four columns per block
+        if (MODE >= 1 && FIRST) {  // the reference advances the top border by four columns per block
             fb[0] = adds(fb[3], RR);
             fb[1] = adds(fb[0], RR);
             fb[2] = adds(fb[1], RR);
@@ -554,8 +559,8 @@ __global__ __launch_bounds__(64 * kWavesPerWG) void score_kernel(const ScoreLaun
     static_assert((1 << Log2<PR>::value) == PR, "rows per pass must be a power of two");
     static_assert(PR <= 512, "two staged profiles of 32 codes x PR rows must fit the LDS budget");
     __shared__ __align__(16) int16_t sprof[2 * 32 * PR];      // profiles of passes p (even, odd)
-    __shared__ uint32_t slbh[MODE == 1 ? 2 * (PR + 1) : 1];  // NW left boundary H of rows base-1 .. base+PR-1
-    __shared__ uint32_t slbe[MODE == 1 ? 2 * PR : 1];        // NW left boundary E of rows base .. base+PR-1
+    __shared__ uint32_t slbh[MODE >= 1 ? 2 * (PR + 1) : 1];  // NW left boundary H of rows base-1 .. base+PR-1
+    __shared__ uint32_t slbe[MODE >= 1 ? 2 * PR : 1];        // NW left boundary E of rows base .. base+PR-1
     __shared__ int s_task;
 
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
@@ -594,7 +599,7 @@ __global__ __launch_bounds__(64 * kWavesPerWG) void score_kernel(const ScoreLaun
             const uint4* src = reinterpret_cast<const uint4*>(L.qp + Q.qp_off + (int64_t)pass * 32 * PR);
             uint4* dst = reinterpret_cast<uint4*>(sprof + buf * 32 * PR);
             for (int i = threadIdx.x; i < 32 * PR / 8; i += 64 * kWavesPerWG) dst[i] = src[i];
-            if (MODE == 1) {
+            if (MODE >= 1) {
                 const int base = pass * PR;
                 uint32_t* bh = slbh + buf * (PR + 1);
                 uint32_t* be = slbe + buf * PR;
@@ -819,15 +824,28 @@ bool score_shape_supported(int lanes, int rows, int mode) {
            (rows == 4 || rows == 8 || rows == 16 || (rows == 32 && mode == 0)) && lanes * rows <= 512;
 }
 
+// NW without trackers (MODE 2) exists for the common NW shapes only
+template <int R, int G>
+constexpr bool nw_untracked_shape() {
+    return (G == 8 && (R == 8 || R == 16)) || (G == 16 && (R == 8 || R == 16)) || (G == 32 && R == 8);
+}
+
+bool score_nw_untracked_supported(int lanes, int rows) {
+    return (lanes == 8 && (rows == 8 || rows == 16)) || (lanes == 16 && (rows == 8 || rows == 16)) ||
+           (lanes == 32 && rows == 8);
+}
+
 template <int R, int G, int MODE>
 static int resident_wg_shape() {
-    if constexpr (R == 32 && MODE == 1) return 0;
+    if constexpr (R == 32 && MODE >= 1) return 0;
+    else if constexpr (MODE == 2 && !nw_untracked_shape<R, G>()) return 0;
     else return resident_wg_t<R, G, MODE>();
 }
 
 template <int R, int G, int MODE>
 static hipError_t launch_shape(hipStream_t st, const ScoreLaunch& L, const ScoreParams& P, int max_wg) {
-    if constexpr (R == 32 && MODE == 1) return hipErrorInvalidValue;
+    if constexpr (R == 32 && MODE >= 1) return hipErrorInvalidValue;
+    else if constexpr (MODE == 2 && !nw_untracked_shape<R, G>()) return hipErrorInvalidValue;
     else return launch_score_t<R, G, MODE>(st, L, P, max_wg);
 }
 
@@ -836,8 +854,10 @@ static hipError_t launch_shape(hipStream_t st, const ScoreLaunch& L, const Score
     X(64, 4) X(64, 8)
 
 int score_resident_workgroups(int lanes, int rows, int mode) {
-#define SSA_CASE(GG, RR) \
-    if (lanes == GG && rows == RR) return mode == 0 ? resident_wg_shape<RR, GG, 0>() : resident_wg_shape<RR, GG, 1>();
+#define SSA_CASE(GG, RR)                                                                              \
+    if (lanes == GG && rows == RR)                                                                  \
+        return mode == 0 ? resident_wg_shape<RR, GG, 0>()                                           \
+                         : (mode == 1 ? resident_wg_shape<RR, GG, 1>() : resident_wg_shape<RR, GG, 2>());
     SSA_SHAPES(SSA_CASE)
 #undef SSA_CASE
     return 0;
@@ -848,7 +868,9 @@ hipError_t launch_score(int lanes, int rows, int mode, hipStream_t st, const Sco
     if (L.ntasks <= 0) return hipSuccess;
 #define SSA_CASE(GG, RR)                                                                               \
     if (lanes == GG && rows == RR)                                                                   \
-        return mode == 0 ? launch_shape<RR, GG, 0>(st, L, P, max_wg) : launch_shape<RR, GG, 1>(st, L, P, max_wg);
+        return mode == 0 ? launch_shape<RR, GG, 0>(st, L, P, max_wg)                                 \
+                         : (mode == 1 ? launch_shape<RR, GG, 1>(st, L, P, max_wg)                    \
+                                      : launch_shape<RR, GG, 2>(st, L, P, max_wg));
     SSA_SHAPES(SSA_CASE)
 #undef SSA_CASE
     return hipErrorInvalidValue;

@@ -301,6 +301,20 @@ MatrixInfo matrix_info() {
     return mi;
 }
 
+// True when no H, E or F value of an alignment of a query of qlen residues
+// with a DB entry of at most dmax residues can reach either end of the int16
+// range (with room for one more gap step): the 16-bit tier is then exact and
+// no overflow predicate can fire.
+bool int16_exact(size_t qlen, int64_t dmax, const MatrixInfo& mi) {
+    const Config& cf = config();
+    const int64_t gap = std::abs((int64_t)cf.gap_open) + std::abs((int64_t)cf.gap_extend);
+    const int64_t ml = std::min<int64_t>((int64_t)qlen, dmax);
+    const int64_t span = 3 * gap + gap * ((int64_t)qlen + dmax + 4);
+    const int64_t up = std::max<int64_t>(mi.maxv, 0) * ml;
+    const int64_t lowest = -span - std::max<int64_t>(-mi.minv, 0) * ml;
+    return lowest > -32768 + gap + 1 && up < 32767;
+}
+
 // result of one pair at one tier, decoded: d > 0 is a heap insert of score
 // after d blocks in a reference lane, d < 0 an overflow after -d blocks
 // (blocks >= 1 either way)
@@ -476,6 +490,7 @@ struct DbChunks {
 struct StreamLayout {
     int lanes = 0;
     bool paired = false;  // the lo and hi streams of each lane group hold the same entries
+    int32_t max_entry_blocks = 0;  // blocks of the longest entry
     int nwaves = 0;                      // a multiple of kWavesPerWG
     std::vector<int32_t> wave_nblocks;   // per wave
     std::vector<int64_t> wave_res_off;   // per wave, uint2 units
@@ -512,9 +527,11 @@ void build_layout(Exec& ex, StreamLayout& L, int lanes, const PairSet& ps, const
     L.paired = paired;
     std::vector<int32_t> nbk(n);
     int64_t total = 0;
+    L.max_entry_blocks = 0;
     for (size_t k = 0; k < n; ++k) {
         nbk[k] = entry_blocks(db.entries[pe_list[k]].len);
         total += nbk[k];
+        L.max_entry_blocks = std::max(L.max_entry_blocks, nbk[k]);
     }
     const int64_t per_wave = std::max<int64_t>(1, ((int64_t)target_waves()));
     const int64_t B = std::max<int64_t>(16 * (int64_t)lanes, (total + gsz * per_wave - 1) / (gsz * per_wave));
@@ -661,6 +678,8 @@ struct TierRun {
     std::vector<SegLane> segs;
     bool packed = false;  // SW with several queries: scored as packed queries
     bool paired = false;  // packed queries two by two, one per half of every lane group (score_kernel_pair)
+    bool untracked = false;  // NW 16-bit where no pair can overflow: the kernel without trackers
+    int mode() const { return type == kSW ? 0 : (untracked ? 2 : 1); }
 
     ~TierRun() { local.release(); }
     bool empty() const { return ps == nullptr || ps->list().empty() || qs.empty(); }
@@ -780,8 +799,16 @@ void tier_prepare(Exec& ex, TierRun& R) {
     const int nw = L->nwaves;
     if ((uint64_t)nq * n >= (uint64_t)INT32_MAX) fatal("libssa_amd: too many (query, entry) pairs in one launch");
     const int nt_q = nw / kWavesPerWG;
+    // NW at 16 bit where no value of any pair can reach either end of the
+    // range: no block can be flagged, so the kernel drops the trackers
+    {
+        const MatrixInfo mi = matrix_info();
+        R.untracked = R.type == kNW && T.bits == 16 && score_nw_untracked_supported(sh.lanes, sh.rows);
+        for (size_t qi = 0; qi < nq && R.untracked; ++qi)
+            R.untracked = int16_exact(R.qs[qi].len, 4 * (int64_t)L->max_entry_blocks, mi);
+    }
     const int cap = R.paired ? score_pair_resident_workgroups(sh.lanes, sh.rows)
-                             : score_resident_workgroups(sh.lanes, sh.rows, R.type == kSW ? 0 : 1);
+                             : score_resident_workgroups(sh.lanes, sh.rows, R.mode());
 
     // ---- packed queries (SW, several queries) ----
     // Each query starts at a lane boundary and is padded to whole lanes (the
@@ -992,7 +1019,7 @@ void tier_launch(hipStream_t st, TierRun& R) {
     if (R.empty()) return;
     SSA_HIP(launch_init_results(st, R.d_out, (int64_t)R.nout, -32768));
     if (R.paired) SSA_HIP(launch_score_pair(R.sh.lanes, R.sh.rows, st, R.K, R.P, R.grid));
-    else SSA_HIP(launch_score(R.sh.lanes, R.sh.rows, R.type == kSW ? 0 : 1, st, R.K, R.P, R.grid));
+    else SSA_HIP(launch_score(R.sh.lanes, R.sh.rows, R.mode(), st, R.K, R.P, R.grid));
 }
 
 void tier_finish(Exec& ex, TierRun& R) {
@@ -1749,13 +1776,7 @@ bool strand_twins(const Job& J, const DbChunks& dc, const MatrixInfo& mi) {
     for (int a = 0; a < 16; ++a)
         for (int b = 0; b < 16; ++b)
             if (M.at(complement_nt(a), complement_nt(b)) != M.at(a, b)) return false;
-    const int64_t gap = std::abs((int64_t)cf.gap_open) + std::abs((int64_t)cf.gap_extend);
-    const int64_t dmax = 4 * dc.max_blocks;
-    const int64_t ml = std::min<int64_t>((int64_t)f.len, dmax);
-    const int64_t span = 3 * gap + gap * ((int64_t)f.len + dmax + 4);
-    const int64_t up = std::max<int64_t>(mi.maxv, 0) * ml;
-    const int64_t lowest = -span - std::max<int64_t>(-mi.minv, 0) * ml;
-    return lowest > -32768 + gap + 1 && up < 32767;
+    return int16_exact(f.len, 4 * dc.max_blocks, mi);
 }
 
 // every entry of the loaded DB, 0 .. entries - 1
@@ -1957,6 +1978,7 @@ void run_batch(Exec& ex, const std::vector<SearchRequest>& reqs, std::vector<Sea
             computed += R.computed * J.nrun / std::max<size_t>(R.qs.size(), 1);
             J.st.lanes_per_stream = R.sh.lanes;
             J.st.rows_per_lane = R.sh.rows;
+            J.st.overflow_tracking = R.mode() != 2;
         }
         // the first tier's results of query buffer q: entry e at base[e ^ flip]
         // (a twin search reads its reverse strand from the forward strand's

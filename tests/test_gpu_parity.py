@@ -322,6 +322,31 @@ def test_both_strands_both_scored_past_int16_bound(t):
     assert st.cells_scored == st.cells
 
 
+UNTRACKED_NW_SHAPES = {(8, 8), (8, 16), (16, 8), (16, 16), (32, 8)}
+
+
+@pytest.mark.parametrize("shape", [(8, 8), (16, 16), (32, 8), (16, 4), (64, 4)])
+@pytest.mark.parametrize("long_case", [False, True])
+def test_nw16_without_overflow_tracking(shape, long_case):
+    """NW at 16 bit where the int16 bound (query length, longest entry, gap
+    and matrix extremes) rules out every overflow flag: the kernel variant
+    without h_min / h_max trackers runs on the shapes that have it; results,
+    overflow counts and CIGARs must equal the oracle's either way.  The long
+    case fails the bound and keeps the trackers."""
+    rnd = random.Random(4600 + shape[0] * 7 + shape[1] + (1000 if long_case else 0))
+    qlen, lo, hi, n = (1700, 1200, 1500, 40) if long_case else (230, 1, 400, 300)
+    q = "".join(rnd.choice("ARNDCQEGHILKMFPSTWYV") for _ in range(qlen))
+    db = rand_db(rnd, O.AMINOACID, n, lo, hi, homolog_of=q, frac=0.1)
+    ssa.set_kernel_shape(*shape)
+    try:
+        check(O.NW, 16, O.AMINOACID, 1, q, db, 15, ("builtin", "blosum62"), (-10, -1), align=True)
+        st = ssa.last_stats()
+    finally:
+        ssa.set_kernel_shape(0, 0)
+    assert (st.lanes_per_stream, st.rows_per_lane) == shape
+    assert st.overflow_tracking == (1 if long_case or shape not in UNTRACKED_NW_SHAPES else 0)
+
+
 @pytest.mark.parametrize("qlen,skipped", [(60, False), (140, True), (420, True)])
 def test_nw8_first_block_overflow_without_the_8bit_pass(qlen, skipped):
     """NW at 8 bit: once the query is long enough, a bound on column 0 shows
