@@ -226,7 +226,9 @@ __device__ __forceinline__ void run_task(const ScoreParams& P, uint32_t gbase0, 
     const uint32_t* slbh = slbh0;
     const uint32_t* slbe = slbe0;
     // byte offset of the lane's rows for column c's DB code in the staged profile
-    auto colofs = [&gbase](uint32_t word, int c) { return (((word >> (8 * c)) & 0x3eu) << LPR) + gbase; };
+    // (the code field lies above the lane's rows and below the buffer bit of
+    // gbase, so the two combine with an or)
+    auto colofs = [&gbase](uint32_t word, int c) { return (((word >> (8 * c)) & 0x3eu) << LPR) | gbase; };
 
     s16x2 Hl[R], El[R];
 #pragma unroll
