@@ -284,8 +284,10 @@ __device__ __forceinline__ void run_task(const ScoreParams& P, uint32_t gbase0, 
             wn = make_uint2(0, 0);
         }
 
-        const bool st_lo = (w.x & kResStart) != 0, st_hi = (w.y & kResStart) != 0;
-        if (__any(st_lo || st_hi)) {
+        // the wave-level tests read both halves' flags with one or; the
+        // per-half flags are only taken inside the rare branches
+        if (__any(((w.x | w.y) & kResStart) != 0)) {
+            const bool st_lo = (w.x & kResStart) != 0, st_hi = (w.y & kResStart) != 0;
             const uint32_t m = (st_lo ? 0x0000ffffu : 0u) | (st_hi ? 0xffff0000u : 0u);
             if (st_lo) {
                 pair_lo = none_lo ? -1 : po_lo + slot_entries[sb_lo + kp_lo];
@@ -368,8 +370,7 @@ __device__ __forceinline__ void run_task(const ScoreParams& P, uint32_t gbase0, 
         }
 
         s16x2 hmn = as_v(0u), hmx = as_v(0u);  // NW trackers, reset value of the reference
-        const bool ls_lo = (w.x & kResLast) != 0, ls_hi = (w.y & kResLast) != 0;
-        const bool any_last = __any(ls_lo || ls_hi);
+        const bool any_last = __any(((w.x | w.y) & kResLast) != 0);
         const int z_lo = (int)((w.x >> kResZShift) & 3u), z_hi = (int)((w.y >> kResZShift) & 3u);
         // the block's four columns; CAP also records the NW score cell (the
         // last query row at the pair's last column), a select per row on a
@@ -378,8 +379,8 @@ __device__ __forceinline__ void run_task(const ScoreParams& P, uint32_t gbase0, 
             constexpr bool CAP = decltype(cap_tag)::value;
             uint32_t cm_lo = 0u, cm_hi = 0u;
             if (CAP && g == last_g && lane_last) {
-                cm_lo = ls_lo ? 0x0000ffffu : 0u;
-                cm_hi = ls_hi ? 0xffff0000u : 0u;
+                cm_lo = (w.x & kResLast) ? 0x0000ffffu : 0u;
+                cm_hi = (w.y & kResLast) ? 0xffff0000u : 0u;
             }
             // The cell statements are written so that every dependent packed
             // op has an independent one in front of it (gfx950 needs a wait
@@ -440,7 +441,7 @@ __device__ __forceinline__ void run_task(const ScoreParams& P, uint32_t gbase0, 
             }
         };
         // only the lane holding the last query row records a score cell
-        if (MODE >= 1 && __any(lane_last && g == last_g && (ls_lo || ls_hi))) columns(Tag<true>{});
+        if (MODE >= 1 && __any(lane_last && g == last_g && ((w.x | w.y) & kResLast) != 0)) columns(Tag<true>{});
         else columns(Tag<false>{});
         // overflow predicates at the end of the block: a packed test here (in
         // the cell loop's block, so the running reductions stay in the loop),
@@ -494,6 +495,7 @@ __device__ __forceinline__ void run_task(const ScoreParams& P, uint32_t gbase0, 
             hb[3] = adds(hb[2], RR);
         }
         if (any_last) {
+            const bool ls_lo = (w.x & kResLast) != 0, ls_hi = (w.y & kResLast) != 0;
             if (MODE == 0) {  // the lane above finished the same pair one step earlier
                 uint32_t scin = shift_down<G>(as_u(Z), scout, top_lane);
                 if (SEG) scin = (seg_top & as_u(Z)) | (~seg_top & scin);
