@@ -105,7 +105,7 @@ def cpu_baseline(w, sample_frac):
         secs += ref.ref_bench_sw8(codes.ctypes.data, offs.ctypes.data, lens.ctypes.data, n, qc, len(qc), k, 1000,
                                   threads, ids.ctypes.data, sc.ctypes.data, C.byref(cnt))
         cells += len(q) * int(lens.sum())
-        tops.append(sorted(sc[:cnt.value].tolist(), reverse=True))
+        tops.append(list(zip(sc[:cnt.value].tolist(), ids[:cnt.value].tolist())))
     return dict(value=round(cells / secs / 1e9, 2), unit="GCUPS", cores=threads, kind="reference",
                 sample="%d of %d DB sequences (the first), all %d queries, reference AVX2 8-bit kernels with "
                        "16/64-bit cascade, %d threads, chunk 1000; %.2f s" % (n, len(w["db"]), len(w["queries"]),
@@ -355,13 +355,14 @@ def main():
         for h in qh:
             hits = search(h, w["k"], w["bw"], align)
             st.append(ssa.last_stats())
-            tops.append([a.score for a in hits])
+            tops.append([(a.score, a.db_id) for a in hits])
         return st, tops
 
     for _ in range(args.warmup):
         step()
     t0 = time.perf_counter()
     stats = []
+    tops = []
     for _ in range(args.steps):
         st, tops = step()
         stats.extend(st)
@@ -416,6 +417,18 @@ def main():
         copy_probe(args.copy_probe)
     if not args.no_cpu_baseline:
         cb, check = cpu_baseline(w, args.cpu_sample)
+        if cb is not None and check[0] == len(w["db"]):
+            # the reference ran on the whole DB: its top-k per query against
+            # this run's (scores as a multiset; the hits above the k-th score
+            # by ID; at the k-th score the reference's threads pick among ties
+            # in an order that depends on their timing)
+            same_scores, same_above = True, True
+            for mine, ref in zip(tops, check[1]):
+                same_scores &= sorted(x[0] for x in mine) == sorted(x[0] for x in ref)
+                kth = min(x[0] for x in ref) if ref else None
+                same_above &= sorted(x for x in mine if x[0] > kth) == sorted(x for x in ref if x[0] > kth)
+            cb["agrees_with_gpu"] = {"queries": len(tops), "top_k_scores_equal": same_scores,
+                                     "hits_above_kth_score_equal": same_above}
         out["cpu_baseline"] = cb
     print(json.dumps(out))
 
