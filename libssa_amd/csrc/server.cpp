@@ -8,8 +8,12 @@
 // while the batch stays within the cell budget, the batch size and the
 // worker's share of the device memory budget.  Batches of different workers
 // run on the GPU at the same time, so requests that arrive while others are
-// on the GPU start as soon as a worker and memory free up.  Every result is what the search returns alone
-// (run_batch's contract).
+// on the GPU start as soon as a worker and memory free up.  A worker joins
+// another one on the GPU only once the oldest waiting request has waited past
+// the assist limit: below saturation a request that arrives during a short
+// search then starts alone a little later instead of sharing the device (and
+// its time) with it, and under load every worker stays busy.  Every result is
+// what the search returns alone (run_batch's contract).
 #include <algorithm>
 #include <chrono>
 #include <condition_variable>
@@ -114,8 +118,17 @@ private:
             std::vector<int64_t> batch;
             {
                 std::unique_lock<std::mutex> lk(mu_);
-                cv_work_.wait(lk, [&] { return stopping_ || !queue_.empty(); });
-                if (queue_.empty()) return;  // stopping with nothing left
+                for (;;) {
+                    cv_work_.wait(lk, [&] { return stopping_ || !queue_.empty(); });
+                    if (queue_.empty()) return;  // stopping with nothing left
+                    if (busy_ == 0 || stopping_) break;
+                    // another worker is on the GPU: join it once the oldest
+                    // request has waited past the assist limit (a worker
+                    // that finishes wakes the others)
+                    const double waited = now_ms() - tickets_[queue_.front()].t_submit;
+                    if (waited >= assist_ms_) break;
+                    cv_work_.wait_for(lk, std::chrono::duration<double, std::milli>(assist_ms_ - waited));
+                }
                 // the oldest request first once it has waited past the aging
                 // limit, then the cheapest ones
                 size_t bytes = 0;
@@ -139,6 +152,7 @@ private:
                 const double t0 = now_ms();
                 for (int64_t id : batch) tickets_[id].t_start = t0;
                 ++batches_;
+                ++busy_;
                 requests_ += batch.size();
                 max_batch_seen_ = std::max<size_t>(max_batch_seen_, batch.size());
                 if (!queue_.empty()) cv_work_.notify_one();
@@ -159,8 +173,10 @@ private:
                     t.t_done = t1;
                     t.done = true;
                 }
+                --busy_;
             }
             cv_done_.notify_all();
+            cv_work_.notify_all();
         }
     }
 
@@ -173,6 +189,8 @@ private:
     size_t budget_ = 0, slot_budget_ = 0;
     double batch_cells_ = 2.5e11;  // GPU work per batch (about one average request, ~45 ms)
     double aging_ms_ = 750.0;    // a request waiting longer goes first
+    double assist_ms_ = 50.0;    // a request waiting longer brings another worker onto the GPU
+    int busy_ = 0;               // workers with a batch on the GPU
     int64_t next_id_ = 0;
     bool stopping_ = false;
     size_t batches_ = 0, requests_ = 0, max_batch_seen_ = 0;
