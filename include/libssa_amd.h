@@ -115,10 +115,12 @@ typedef struct {
 } ssa_amd_server_counters_t;
 
 /* Starts a request loop with `slots` workers (1-8), each with its own device
- * stream and buffers; a free worker takes the waiting requests, in arrival
- * order and at most `max_batch` of them, as one batch, admitting requests
- * while the batch's estimated device footprint fits its share of
- * `device_bytes` (0 = half the free device memory).  Uses the settings current
+ * stream and buffers; a free worker takes waiting requests (see
+ * ssa_amd_server_tune for the order), at most `max_batch` of them, as one
+ * batch, admitting requests while the batch's estimated device footprint fits
+ * its share of `device_bytes` (0 = half the free device memory).  A worker
+ * joins another one already on the GPU only once the oldest waiting request
+ * has waited 50 ms, so below saturation requests run alone.  Uses the settings current
  * at the time (scoring, gaps, symbol type, DB); they must not change while the
  * server runs.  Returns NULL if the arguments or the settings are invalid. */
 ssa_amd_server* ssa_amd_server_start(int slots, size_t max_batch, size_t device_bytes);
