@@ -192,9 +192,9 @@ def rand_db(rnd, symtype, n, lo, hi, homolog_of=None, frac=0.0):
     return out
 
 
-CASES = []
-for seed in range(24):
-    CASES.append(seed)
+# SSA_PARITY_SEEDS=N runs N random cases (default 24); cases past the first
+# 24 also draw longer queries, which take several passes of the score kernel
+CASES = list(range(int(os.environ.get("SSA_PARITY_SEEDS", "24"))))
 
 
 @pytest.mark.parametrize("seed", CASES)
@@ -206,7 +206,8 @@ def test_random_parity(seed):
     bw = [8, 16, 64][seed % 3 if seed < 12 else rnd.randrange(3)]
     isa = "avx2" if seed % 4 else "sse"
     alpha = "ACGT" if symtype == O.NUCLEOTIDE else "ARNDCQEGHILKMFPSTWYV"
-    qraw = "".join(rnd.choice(alpha) for _ in range(rnd.choice([5, 17, 33, 64, 100, 150])))
+    qlens = [5, 17, 33, 64, 100, 150] if seed < 24 else [5, 17, 33, 64, 100, 150, 300, 600, 1100]
+    qraw = "".join(rnd.choice(alpha) for _ in range(rnd.choice(qlens)))
     db = rand_db(rnd, symtype, rnd.randint(50, 700), 1, rnd.choice([40, 120, 400]), homolog_of=qraw, frac=0.1)
     if symtype == O.NUCLEOTIDE:
         matrix = ("const", rnd.choice([(1, -1), (2, -4), (5, -4), (2, -2)]))
