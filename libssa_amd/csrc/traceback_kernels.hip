@@ -302,13 +302,18 @@ __device__ void tb_region(const int32_t* smat, const uint8_t* qc, const uint8_t*
                         pp = H[k];
                         H[k] = hn;
                         h = hn;
+                        // the first cell in (column, row) order: a lane meets
+                        // the columns in order within a pass, but its later
+                        // passes revisit earlier columns (on later rows), so
+                        // an equal cell of a later pass wins when its column
+                        // comes first
                         if (FWD) {
-                            if (hn > bv) {  // strict: the first maximum of this lane in its order
+                            if (hn > bv || (hn == bv && hn > 0 && c < bcol)) {
                                 bv = hn;
                                 bcol = c;
                                 brow = r0 + k;
                             }
-                        } else if (!found && hn >= target) {
+                        } else if (hn >= target && (!found || c < bcol)) {
                             found = true;
                             bcol = c;
                             brow = r0 + k;

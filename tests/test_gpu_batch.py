@@ -271,3 +271,70 @@ def test_packed_sw_batch_matches_oracle(shape):
         assert [(a.score, a.db_id) for a in got] == [(s, i) for s, i, _, _ in want], (len(raw), bw)
         assert (st.overflow_8_bit_count, st.overflow_16_bit_count) == (ost.overflow_8, ost.overflow_16)
         assert (st.lanes_per_stream, st.rows_per_lane) == (lanes, rows)
+
+
+# SSA_BATCH_SEEDS=N runs N random batches (default 4)
+BATCH_CASES = list(range(int(os.environ.get("SSA_BATCH_SEEDS", "4"))))
+
+
+@pytest.mark.parametrize("seed", BATCH_CASES)
+def test_random_batch_matches_oracle(seed):
+    """A random batch — SW and NW, 8/16/64 bit, scores and CIGARs, protein or
+    DNA with one or both strands, queries from 1 residue to several passes —
+    against the oracle request by request: hits, strands, overflow and chunk
+    counters, CIGARs and regions."""
+    rnd = random.Random(31000 + seed)
+    symtype = O.AMINOACID if seed % 2 == 0 else O.NUCLEOTIDE
+    strands = 1 if symtype == O.AMINOACID else rnd.choice([1, 2, 3])
+    alpha = AA if symtype == O.AMINOACID else "ACGT"
+    base = "".join(rnd.choice(alpha) for _ in range(1200))
+    db = rand_db(rnd, symtype, rnd.randint(60, 300), 1, rnd.choice([40, 200, 500]), homolog_of=base, frac=0.2)
+    if symtype == O.AMINOACID:
+        matrix = ("builtin", rnd.choice(["blosum62", "blosum50", "pam30"]))
+    else:
+        matrix = ("const", rnd.choice([(1, -1), (2, -4), (5, -4)]))
+    gaps = rnd.choice([(-10, -1), (-4, -2), (-20, -2), (-3, -1)])
+    chunk = rnd.choice([1000, 37, 250])
+    configure(symtype, strands, gaps, matrix, chunk=chunk)
+    qraws = []
+    for _ in range(rnd.randint(1, 12)):
+        n = rnd.choice([1, 5, 30, 64, 100, 257, 513, 900])
+        o = rnd.randint(0, 1200 - n)
+        qraws.append(base[o:o + n] if rnd.random() < 0.6 else "".join(rnd.choice(alpha) for _ in range(n)))
+    with tempfile.TemporaryDirectory() as d:
+        path = os.path.join(d, "db.fas")
+        write_fasta(path, db)
+        ssa.init_db(path)
+        qs = [ssa.init_sequence_fasta(ssa.READ_FROM_STRING, s) for s in qraws]
+        reqs = [(rnd.choice([ssa.SEARCH_SW, ssa.SEARCH_NW]), q, rnd.choice([1, 5, 20, 60]), rnd.choice([8, 16, 64]),
+                 ssa.COMPUTE_ALIGNMENT if rnd.random() < 0.3 else ssa.COMPUTE_SCORE) for q in qs]
+        try:
+            lists, stats = ssa.batch_align(reqs)
+        finally:
+            for q in qs:
+                ssa.free_sequence(q)
+    m = O.matrix_builtin(matrix[1]) if matrix[0] == "builtin" else O.matrix_constant(*matrix[1])
+    qstrands = [s for s in range(2) if (s + 1) & strands] if symtype == O.NUCLEOTIDE else [0]
+    for (t, _, k, bw, al), raw, got, st in zip(reqs, qraws, lists, stats):
+        ot = O.SW if t == ssa.SEARCH_SW else O.NW
+        qbufs = O.query_buffers(symtype, strands, raw)
+        want, ost = O.search(ot, bw, symtype, strands, qbufs, db, k, m, gaps[0], gaps[1], chunk=chunk)
+        case = (seed, len(raw), ot, bw, k)
+        assert [(a.score, a.db_id, a.db_frame, qstrands.index(a.query_strand)) for a in got] == \
+            [(s, i, strand, qi) for s, i, strand, qi in want], case
+        assert (st.overflow_8_bit_count, st.overflow_16_bit_count) == (ost.overflow_8, ost.overflow_16), case
+        assert (st.chunk_count, st.seq_count) == (ost.chunk_count, ost.seq_count), case
+        if al == ssa.COMPUTE_ALIGNMENT:
+            for a, (s, i, strand, qi) in zip(got, want):
+                if ot == O.SW and s <= 0:
+                    continue
+                dseq = O.map_db(symtype, db[i])
+                if ot == O.SW and O.region_local(qbufs[qi], dseq, m, gaps[0], gaps[1]) is None:
+                    # the attached sequence (a reverse-strand hit carries the
+                    # forward one) has no positive cell: the reference's region
+                    # is undefined there, this build reports an empty alignment
+                    assert (a.alignment or "", a.align_q_start, a.align_q_end, a.align_d_start, a.align_d_end) == \
+                        ("", 0, 0, 0, 0), case
+                    continue
+                c, r = O.cigar(ot, qbufs[qi], dseq, m, gaps[0], gaps[1])
+                assert (a.alignment, a.align_q_start, a.align_q_end, a.align_d_start, a.align_d_end) == (c,) + tuple(r), case

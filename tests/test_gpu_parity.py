@@ -85,6 +85,12 @@ def check(t, bw, symtype, strands, qraw, db, k, matrix, gaps, isa="avx2", chunk=
             dseq = O.map_db(symtype, db[i])
             if t == O.SW and s <= 0:
                 continue
+            if t == O.SW and O.region_local(qbufs[qi], dseq, m, gaps[0], gaps[1]) is None:
+                # no positive cell against the attached sequence: the
+                # reference's region is undefined, this build reports none
+                assert (a.alignment or "", a.align_q_start, a.align_q_end, a.align_d_start, a.align_d_end) == \
+                    ("", 0, 0, 0, 0)
+                continue
             c, r = O.cigar(t, qbufs[qi], dseq, m, gaps[0], gaps[1])
             assert a.alignment == c
             assert (a.align_q_start, a.align_q_end, a.align_d_start, a.align_d_end) == r
@@ -217,6 +223,22 @@ def test_random_parity(seed):
     k = rnd.choice([1, 5, 10, 50, 200])
     chunk = rnd.choice([1000, 37, 250])
     check(t, bw, symtype, strands, qraw, db, k, matrix, gaps, isa=isa, chunk=chunk, align=(seed % 4 == 1))
+
+
+def test_traceback_tie_across_query_passes():
+    """Two equal best local alignments whose end cells sit on rows 512 apart
+    (the same lane of the traceback kernel, in its first and second pass of
+    512 query rows), the second pass's at the smaller DB column: the region
+    is the first maximum in (DB column, query row) order, the second pass's."""
+    rnd = random.Random(515)
+    m1 = "".join(rnd.choice("GT") for _ in range(30))
+    m2 = "".join(rnd.choice("GT") for _ in range(30))
+    q = "A" * 100 + m1 + "A" * 482 + m2 + "A" * 258
+    db = ["C" * 50 + m2 + "C" * 220 + m1 + "C" * 70]
+    db += ["".join(rnd.choice("AC") for _ in range(rnd.randint(10, 300))) for _ in range(20)]
+    got = check(O.SW, 16, O.NUCLEOTIDE, 1, q, db, 3, ("const", (1, -1)), (-3, -1), align=True)
+    assert (got[0].db_id, got[0].score, got[0].alignment) == (0, 30, "30M")
+    assert (got[0].align_q_end, got[0].align_d_end) == (641, 79)
 
 
 @pytest.mark.parametrize("shape", [(8, 4), (8, 16), (16, 8), (32, 4), (32, 16), (64, 4), (64, 8),
