@@ -142,6 +142,8 @@ def back_translate(rnd, prot, gencode):
 CASES = [(sym, strands, t, bw) for sym in (O.TRANS_QUERY, O.TRANS_DB, O.TRANS_BOTH) for strands in (1, 2, 3)
          for t in (O.SW, O.NW) for bw in (8, 16)]
 CASES += [(sym, 3, t, 64) for sym in (O.TRANS_QUERY, O.TRANS_DB, O.TRANS_BOTH) for t in (O.SW, O.NW)]
+# SSA_TRANSLATE_ROUNDS=N repeats every configuration N times with other seeds (default 1)
+CASES = CASES * int(os.environ.get("SSA_TRANSLATE_ROUNDS", "1"))
 
 
 @pytest.mark.parametrize("case", range(len(CASES)))
