@@ -225,6 +225,22 @@ def test_random_parity(seed):
     check(t, bw, symtype, strands, qraw, db, k, matrix, gaps, isa=isa, chunk=chunk, align=(seed % 4 == 1))
 
 
+@pytest.mark.parametrize("t", [O.SW, O.NW])
+@pytest.mark.parametrize("bw", [8, 16, 64])
+def test_edge_inputs(t, bw):
+    """Empty records among the sequences (the adapter skips them,
+    src/db_adapter.c:231-233, and the IDs keep counting them), one-residue
+    query and entries, a hit count above the number of entries, k = 1, and a
+    DB of empty records only."""
+    rnd = random.Random(61 + bw + 7 * t)
+    db = ["", "A", "", "MKV", "".join(rnd.choice("ARNDCQEGHILKMFPSTWYV") for _ in range(70)), "", "W"]
+    db += rand_db(rnd, O.AMINOACID, 20, 1, 60)
+    blosum = ("builtin", "blosum62")
+    check(t, bw, O.AMINOACID, 1, "M", db, 500, blosum, (-10, -1), align=True)
+    check(t, bw, O.AMINOACID, 1, "MKVW" * 10, db, 1, blosum, (-10, -1), align=True)
+    assert check(t, bw, O.AMINOACID, 1, "MKV", ["", "", ""], 5, blosum, (-10, -1)) == []
+
+
 def test_traceback_tie_across_query_passes():
     """Two equal best local alignments whose end cells sit on rows 512 apart
     (the same lane of the traceback kernel, in its first and second pass of
