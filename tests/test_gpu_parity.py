@@ -198,9 +198,11 @@ def rand_db(rnd, symtype, n, lo, hi, homolog_of=None, frac=0.0):
     return out
 
 
-# SSA_PARITY_SEEDS=N runs N random cases (default 24); cases past the first
-# 24 also draw longer queries, which take several passes of the score kernel
-CASES = list(range(int(os.environ.get("SSA_PARITY_SEEDS", "24"))))
+# SSA_PARITY_SEEDS=N runs N random cases (default 24) from seed
+# SSA_PARITY_SEED0 (default 0); cases past the first 24 also draw longer
+# queries, which take several passes of the score kernel
+_SEED0 = int(os.environ.get("SSA_PARITY_SEED0", "0"))
+CASES = list(range(_SEED0, _SEED0 + int(os.environ.get("SSA_PARITY_SEEDS", "24"))))
 
 
 @pytest.mark.parametrize("seed", CASES)
