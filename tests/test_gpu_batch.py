@@ -308,9 +308,13 @@ def test_random_batch_matches_oracle(seed):
         qs = [ssa.init_sequence_fasta(ssa.READ_FROM_STRING, s) for s in qraws]
         reqs = [(rnd.choice([ssa.SEARCH_SW, ssa.SEARCH_NW]), q, rnd.choice([1, 5, 20, 60]), rnd.choice([8, 16, 64]),
                  ssa.COMPUTE_ALIGNMENT if rnd.random() < 0.3 else ssa.COMPUTE_SCORE) for q in qs]
+        # past the default cases, a forced kernel shape (NW takes 16 rows where 32 are forced)
+        shape = rnd.choice(SW_SHAPES) if seed >= 4 else (0, 0)
+        ssa.set_kernel_shape(*shape)
         try:
             lists, stats = ssa.batch_align(reqs)
         finally:
+            ssa.set_kernel_shape(0, 0)
             for q in qs:
                 ssa.free_sequence(q)
     m = O.matrix_builtin(matrix[1]) if matrix[0] == "builtin" else O.matrix_constant(*matrix[1])
@@ -319,7 +323,7 @@ def test_random_batch_matches_oracle(seed):
         ot = O.SW if t == ssa.SEARCH_SW else O.NW
         qbufs = O.query_buffers(symtype, strands, raw)
         want, ost = O.search(ot, bw, symtype, strands, qbufs, db, k, m, gaps[0], gaps[1], chunk=chunk)
-        case = (seed, len(raw), ot, bw, k)
+        case = (seed, shape, len(raw), ot, bw, k)
         assert [(a.score, a.db_id, a.db_frame, qstrands.index(a.query_strand)) for a in got] == \
             [(s, i, strand, qi) for s, i, strand, qi in want], case
         assert (st.overflow_8_bit_count, st.overflow_16_bit_count) == (ost.overflow_8, ost.overflow_16), case
