@@ -200,7 +200,8 @@ def rand_db(rnd, symtype, n, lo, hi, homolog_of=None, frac=0.0):
 
 # SSA_PARITY_SEEDS=N runs N random cases (default 24) from seed
 # SSA_PARITY_SEED0 (default 0); cases past the first 24 also draw longer
-# queries, which take several passes of the score kernel
+# queries, which take several passes of the score kernel, and force a random
+# kernel shape
 _SEED0 = int(os.environ.get("SSA_PARITY_SEED0", "0"))
 CASES = list(range(_SEED0, _SEED0 + int(os.environ.get("SSA_PARITY_SEEDS", "24"))))
 
@@ -224,7 +225,14 @@ def test_random_parity(seed):
     gaps = rnd.choice([(-1, -1), (-10, -1), (-4, -2), (-20, -2), (-3, -1)])
     k = rnd.choice([1, 5, 10, 50, 200])
     chunk = rnd.choice([1000, 37, 250])
-    check(t, bw, symtype, strands, qraw, db, k, matrix, gaps, isa=isa, chunk=chunk, align=(seed % 4 == 1))
+    # past the default cases, a forced kernel shape (NW takes 16 rows where 32 are forced)
+    shapes = [(8, 4), (16, 4), (32, 4), (64, 4), (8, 8), (16, 8), (32, 8), (64, 8), (8, 16), (16, 16), (32, 16),
+              (8, 32), (16, 32)]
+    ssa.set_kernel_shape(*(rnd.choice(shapes) if seed >= 24 else (0, 0)))
+    try:
+        check(t, bw, symtype, strands, qraw, db, k, matrix, gaps, isa=isa, chunk=chunk, align=(seed % 4 == 1))
+    finally:
+        ssa.set_kernel_shape(0, 0)
 
 
 @pytest.mark.parametrize("t", [O.SW, O.NW])
