@@ -342,3 +342,40 @@ def test_random_batch_matches_oracle(seed):
                     continue
                 c, r = O.cigar(ot, qbufs[qi], dseq, m, gaps[0], gaps[1])
                 assert (a.alignment, a.align_q_start, a.align_q_end, a.align_d_start, a.align_d_end) == (c,) + tuple(r), case
+
+
+# SSA_SERVER_SEEDS=N runs N random serving rounds (default 2)
+SERVER_CASES = list(range(int(os.environ.get("SSA_SERVER_SEEDS", "2"))))
+
+
+@pytest.mark.parametrize("seed", SERVER_CASES)
+def test_random_server_results_equal_alone(protein_db, seed):
+    """Random requests from several submitting threads, with random gaps,
+    into a server of 1-4 workers and random batch limits: every result is
+    the one the search gives alone."""
+    _, db, qraws, qs = protein_db
+    rnd = random.Random(52000 + seed)
+    reqs = [(rnd.choice([ssa.SEARCH_SW, ssa.SEARCH_NW]), rnd.choice(qs), rnd.choice([1, 5, 20, 50]),
+             rnd.choice([8, 16, 64]), ssa.COMPUTE_ALIGNMENT if rnd.random() < 0.4 else ssa.COMPUTE_SCORE)
+            for _ in range(rnd.randint(5, 40))]
+    want = [key(alone(r)) for r in reqs]
+    srv = ssa.Server(slots=rnd.randint(1, 4), max_batch=rnd.choice([1, 2, 5, 16]))
+    try:
+        tickets = [None] * len(reqs)
+        nthreads = rnd.randint(1, 4)
+        delays = [rnd.choice([0.0, 0.0, 0.001, 0.01]) for _ in reqs]
+
+        def submit(part):
+            for i in range(part, len(reqs), nthreads):
+                tickets[i] = srv.submit(*reqs[i])
+                time.sleep(delays[i])
+
+        ths = [threading.Thread(target=submit, args=(p,)) for p in range(nthreads)]
+        for th in ths:
+            th.start()
+        for th in ths:
+            th.join()
+        got = [key(srv.wait(t)[0]) for t in tickets]
+    finally:
+        srv.stop()
+    assert got == want, seed
