@@ -273,8 +273,9 @@ def test_packed_sw_batch_matches_oracle(shape):
         assert (st.lanes_per_stream, st.rows_per_lane) == (lanes, rows)
 
 
-# SSA_BATCH_SEEDS=N runs N random batches (default 4)
-BATCH_CASES = list(range(int(os.environ.get("SSA_BATCH_SEEDS", "4"))))
+# SSA_BATCH_SEEDS=N runs N random batches (default 4) from seed SSA_BATCH_SEED0 (default 0)
+_BSEED0 = int(os.environ.get("SSA_BATCH_SEED0", "0"))
+BATCH_CASES = list(range(_BSEED0, _BSEED0 + int(os.environ.get("SSA_BATCH_SEEDS", "4"))))
 
 
 @pytest.mark.parametrize("seed", BATCH_CASES)
