@@ -135,6 +135,19 @@ bool packed_open(const char* path, PackedView* v, void** map, size_t* map_bytes,
         }
         for (uint64_t u = 0; u < h.unknown_count && !bad; ++u)
             if (v->unknown[u].id >= h.seq_count) bad = "unknown-symbol record out of range";
+        // every residue code below the alphabet's size (the kernels index
+        // 32-code profiles with them and keep flag bits above a doubled
+        // code), and the padding to whole 4-residue blocks zero
+        const unsigned limit = h.symtype == NUCLEOTIDE ? 16u : 32u;
+        for (uint64_t k = 0; k < h.entry_count && !bad; ++k) {
+            const EntryRec& e = v->entries[k];
+            const uint8_t* c = v->codes + e.code_off;
+            const uint64_t padded = ((uint64_t)e.len + 3) / 4 * 4;
+            for (uint64_t i = 0; i < e.len && !bad; ++i)
+                if (c[i] >= limit) bad = "residue code out of range";
+            for (uint64_t i = e.len; i < padded && !bad; ++i)
+                if (c[i] != 0) bad = "non-zero padding after an entry's codes";
+        }
     }
     if (bad) {
         munmap(m, bytes);

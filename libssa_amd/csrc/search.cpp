@@ -37,9 +37,13 @@ namespace ssa {
 namespace {
 
 uint64_t g_generation = 1;
+uint64_t g_build = 0;  // incremented at every rebuild of the device DB
 
 struct DeviceDB {
-    uint64_t generation = 0;
+    uint64_t generation = 0;  // the provider DB it was built from (init_db bumps it)
+    uint64_t build = 0;       // this build: changes whenever the entries are rebuilt, for
+                              // a new DB or new symbol type / strands / genetic code, so
+                              // every cache derived from the entries is keyed on it
     int symtype = -1, strands = -1, gencode = -1;
     size_t count = 0;
     std::vector<DbEntry> entries;
@@ -224,6 +228,7 @@ static DeviceDB& device_db() {
     release_device_db();
     DeviceDB& d = g_db;
     d.generation = g_generation;
+    d.build = ++g_build;
     d.symtype = db_type;
     d.strands = strands;
     d.gencode = gencode;
@@ -470,9 +475,9 @@ struct PairSet {
 // size: the chunks' entry ranges (whole chunks of DB IDs until the first
 // chunk without entries, as search_8/16/64 stop there,
 // src/algo/8/search_8.c:135-143), the entries they hold (0 .. N-1) and their
-// totals.  Computed once per DB generation and chunk size.
+// totals.  Computed once per device DB build and chunk size.
 struct DbChunks {
-    uint64_t generation = ~0ull;
+    uint64_t build = ~0ull;
     size_t chunk_size = 0;
     std::vector<std::pair<uint32_t, uint32_t>> chunks;
     uint32_t N = 0;
@@ -501,7 +506,7 @@ struct StreamLayout {
     ScoreWave* d_waves = nullptr;        // device copies of the wave table and the stream CSR
     int32_t* d_slot_first = nullptr;
     int32_t* d_slot_entries = nullptr;
-    uint64_t generation = 0;             // cached layouts: DB generation they belong to
+    uint64_t build = 0;                  // cached layouts: the device DB build they belong to
 
     void release() {
         if (d_res) (void)hipFree(d_res);
@@ -778,12 +783,12 @@ void tier_prepare(Exec& ex, TierRun& R) {
     if (R.sorted_all) {
         std::lock_guard<std::mutex> lk(layout_mutex());
         for (StreamLayout& c : layout_cache())
-            if (c.lanes == sh.lanes && c.paired == R.paired && c.generation == db.generation) R.L = &c;
+            if (c.lanes == sh.lanes && c.paired == R.paired && c.build == db.build) R.L = &c;
         if (!R.L) {
             layout_cache().emplace_back();
             R.L = &layout_cache().back();
             build_layout(ex, *R.L, sh.lanes, ps, *R.sorted_all, R.paired);
-            R.L->generation = db.generation;
+            R.L->build = db.build;
         }
     } else {
         std::vector<uint32_t> order(n);
@@ -1248,13 +1253,20 @@ std::vector<uint8_t> prepared_db_sequence(p_seqinfo s, int strand_field, int fra
     return v;
 }
 
+// Direction-matrix bytes one traceback launch may use: the hits of a batch
+// run in consecutive launches whose matrices fit this budget (a hit larger
+// than the budget runs alone), so the arena stays bounded however many long
+// hits a batch asks for.
+constexpr int64_t kTraceDirBudget = (int64_t)1 << 30;
+
 // Regions and CIGARs of the hits listed (types[k] = the search type of al[k]).
 void gpu_traceback(Exec& gpu, const std::vector<SearchType>& types, std::vector<p_alignment>& al, double* ms) {
     if (al.empty()) return;
     const Config& cf = config();
     std::vector<uint8_t> codes;
     std::vector<TraceTask> tasks(al.size());
-    int64_t dir_total = 0, hh_total = 0, ops_total = 0;
+    std::vector<int64_t> dir_bytes(al.size());
+    int64_t hh_total = 0, ops_total = 0;
     bool wave_ok = true;
     int64_t smax = 0;
     for (int i = 0; i < kDim * kDim; ++i) smax = std::max<int64_t>(smax, std::abs(matrix().s[i]));
@@ -1267,9 +1279,8 @@ void gpu_traceback(Exec& gpu, const std::vector<SearchType>& types, std::vector<
         codes.insert(codes.end(), (const uint8_t*)a->db_seq.seq, (const uint8_t*)a->db_seq.seq + a->db_seq.len);
         t.qlen = (int32_t)a->query.len;
         t.dlen = (int32_t)a->db_seq.len;
-        t.dir_off = dir_total;
-        dir_total += (((int64_t)t.qlen + 7) & ~(int64_t)7) * t.dlen + 8;
-        dir_total = (dir_total + 7) & ~(int64_t)7;  // 8-byte aligned per task (packed stores)
+        // 8-byte aligned per task (packed stores)
+        dir_bytes[k] = ((((int64_t)t.qlen + 7) & ~(int64_t)7) * t.dlen + 8 + 7) & ~(int64_t)7;
         t.hh_off = hh_total;
         // int64 scratch of the one-thread kernel, or the int32 carries and
         // last-row copies of the wave kernel, whichever is larger
@@ -1284,23 +1295,42 @@ void gpu_traceback(Exec& gpu, const std::vector<SearchType>& types, std::vector<
         ops_total += t.ops_cap;
         t.type = types[k] == kSW ? 0 : 1;
     }
+    // launches: consecutive hits whose direction matrices fit the budget
+    std::vector<std::pair<size_t, size_t>> launches;  // [first, last) hit
+    int64_t dir_max = 8;
+    for (size_t k = 0; k < al.size();) {
+        size_t e = k;
+        int64_t used = 0;
+        while (e < al.size() && (e == k || used + dir_bytes[e] <= kTraceDirBudget)) {
+            tasks[e].dir_off = used;
+            used += dir_bytes[e];
+            ++e;
+        }
+        dir_max = std::max(dir_max, used);
+        launches.push_back({k, e});
+        k = e;
+    }
     const int64_t t0 = now_us();
     auto* d_codes = (uint8_t*)gpu.arena("tb_codes", codes.size() + 1);
     auto* d_tasks = (TraceTask*)gpu.arena("tb_tasks", tasks.size() * sizeof(TraceTask));
     auto* d_mat = (int64_t*)gpu.arena("tb_mat", kDim * kDim * sizeof(int64_t));
-    auto* d_dirs = (uint8_t*)gpu.arena("tb_dirs", (size_t)dir_total);
+    auto* d_dirs = (uint8_t*)gpu.arena("tb_dirs", (size_t)dir_max);
     auto* d_hh = (int64_t*)gpu.arena("tb_hh", (size_t)hh_total * sizeof(int64_t));
     auto* d_ops = (int32_t*)gpu.arena("tb_ops", (size_t)ops_total * 2 * sizeof(int32_t));
     auto* d_res = (TraceResult*)gpu.arena("tb_res", tasks.size() * sizeof(TraceResult));
     gpu.upload(d_codes, codes.data(), codes.size());
     gpu.upload(d_tasks, tasks.data(), tasks.size() * sizeof(TraceTask));
     gpu.upload(d_mat, matrix().s, kDim * kDim * sizeof(int64_t));
-    if (wave_ok)
-        SSA_HIP(launch_traceback_wave((int)tasks.size(), gpu.stream(), d_tasks, d_codes, d_mat, cf.gap_open,
-                                      cf.gap_extend, d_dirs, d_hh, d_ops, d_res));
-    else
-        SSA_HIP(launch_traceback((int)tasks.size(), gpu.stream(), d_tasks, d_codes, d_mat, cf.gap_open,
-                                 cf.gap_extend, d_dirs, d_hh, d_ops, d_res));
+    // launches in stream order share the direction arena
+    for (const auto& lr : launches) {
+        const int n = (int)(lr.second - lr.first);
+        if (wave_ok)
+            SSA_HIP(launch_traceback_wave(n, gpu.stream(), d_tasks + lr.first, d_codes, d_mat, cf.gap_open,
+                                          cf.gap_extend, d_dirs, d_hh, d_ops, d_res + lr.first));
+        else
+            SSA_HIP(launch_traceback(n, gpu.stream(), d_tasks + lr.first, d_codes, d_mat, cf.gap_open,
+                                     cf.gap_extend, d_dirs, d_hh, d_ops, d_res + lr.first));
+    }
     std::vector<TraceResult> res(tasks.size());
     std::vector<int32_t> ops((size_t)ops_total * 2);
     gpu.download(res.data(), d_res, res.size() * sizeof(TraceResult));
@@ -1675,23 +1705,24 @@ void job_list(Job& J, std::vector<p_alignment>& al_out, std::vector<SearchType>&
 
 }  // namespace
 
-// The chunk view of the loaded DB for one chunk size, cached per DB
-// generation (the last few chunk sizes are kept; entries are never freed
-// while a search may hold a reference, only when the DB changes).
+// The chunk view of the loaded DB for one chunk size, cached per device DB
+// build (a new DB, or new symbol type, strands or genetic code, rebuilds the
+// entries); the last few chunk sizes are kept, and entries are never freed
+// while a search may hold a reference, only when the DB is rebuilt.
 const DbChunks& db_chunks(const DeviceDB& db, size_t cs) {
     static std::mutex mu;
     static std::list<DbChunks> cache;
     std::lock_guard<std::mutex> lk(mu);
     for (auto it = cache.begin(); it != cache.end();) {
-        if (it->generation == db.generation && it->chunk_size == cs) return *it;
-        // a DB that changed cannot be in use by a search still running
-        // (the DB is replaced only between searches)
-        if (it->generation != db.generation) it = cache.erase(it);
+        if (it->build == db.build && it->chunk_size == cs) return *it;
+        // a DB that was rebuilt cannot be in use by a search still running
+        // (the DB is rebuilt only between searches)
+        if (it->build != db.build) it = cache.erase(it);
         else ++it;
     }
     cache.emplace_back();
     DbChunks& d = cache.back();
-    d.generation = db.generation;
+    d.build = db.build;
     d.chunk_size = cs;
     for (size_t start = 0; start < db.count; start += cs) {
         const size_t stop = std::min(db.count, start + cs);
@@ -1782,13 +1813,13 @@ bool strand_twins(const Job& J, const DbChunks& dc, const MatrixInfo& mi) {
 // every entry of the loaded DB, 0 .. entries - 1
 const std::vector<uint32_t>& db_all_entries(const DeviceDB& db) {
     static std::mutex mu;
-    static uint64_t gen = ~0ull;
+    static uint64_t build = ~0ull;
     static std::vector<uint32_t> all;
     std::lock_guard<std::mutex> lk(mu);
-    if (gen != db.generation || all.size() != db.entries.size()) {
+    if (build != db.build || all.size() != db.entries.size()) {
         all.resize(db.entries.size());
         std::iota(all.begin(), all.end(), 0u);
-        gen = db.generation;
+        build = db.build;
     }
     return all;
 }

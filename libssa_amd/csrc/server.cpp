@@ -46,6 +46,7 @@ struct Ticket {
     double cells = 0;  // estimated GPU work
     double t_submit = 0, t_start = 0, t_done = 0;
     bool done = false;
+    bool claimed = false;  // a thread waits for the result
 };
 
 }  // namespace
@@ -90,11 +91,16 @@ public:
         return id;
     }
 
+    // One waiter takes a ticket's result: the first to wait claims it, and
+    // any other waiter on the same ticket returns false at once.  The ticket
+    // is looked up again after every wake-up (the map may rehash meanwhile).
     bool wait(int64_t id, SearchResult* out, double* latency_ms, double* queue_ms) {
         std::unique_lock<std::mutex> lk(mu_);
         auto it = tickets_.find(id);
-        if (it == tickets_.end()) return false;
-        cv_done_.wait(lk, [&] { return it->second.done; });
+        if (it == tickets_.end() || it->second.claimed) return false;
+        it->second.claimed = true;
+        cv_done_.wait(lk, [&] { return tickets_.at(id).done; });
+        it = tickets_.find(id);
         *out = it->second.res;
         if (latency_ms) *latency_ms = it->second.t_done - it->second.t_submit;
         if (queue_ms) *queue_ms = it->second.t_start - it->second.t_submit;
@@ -233,20 +239,30 @@ double request_cells(const SearchRequest& rq, double residues) {
     return qlen * residues * ((rq.type == kNW && rq.bit_width == BIT_WIDTH_8) ? 2.0 : 1.0);
 }
 
-double db_entry_residues() {
-    double res = 0;
+// residues of the DB's search entries, and the longest DB sequence
+struct DbSize {
+    double residues = 0;
+    size_t max_len = 0;
+};
+
+DbSize db_size() {
+    DbSize d;
     const size_t n = ssa_db_get_sequence_count();
     for (size_t i = 0; i < n; ++i) {
         p_seqinfo s = ssa_db_get_sequence(i);
         if (!s) break;
-        res += (double)s->seqlen;
+        d.residues += (double)s->seqlen;
+        d.max_len = std::max<size_t>(d.max_len, s->seqlen);
     }
-    return res * entry_factors().second;
+    d.residues *= entry_factors().second;
+    return d;
 }
 
 // device bytes one request needs in a batch: pair results and pass boundaries
-// of its first tier, and the traceback matrices of its hits
-size_t request_footprint(const SearchRequest& rq) {
+// of its first tier, and the direction matrices of its hits' traceback (hits
+// against the DB's longest sequence, at most one traceback launch's budget:
+// the hits of a batch run in launches that fit it)
+size_t request_footprint(const SearchRequest& rq, size_t db_max_len) {
     const size_t n = ssa_db_get_sequence_count();
     const size_t entries = (size_t)((double)n * entry_factors().first);
     const std::vector<size_t> ql = query_lengths(rq.query);
@@ -255,7 +271,9 @@ size_t request_footprint(const SearchRequest& rq) {
     for (size_t l : ql) qlen = std::max(qlen, l);
     const size_t mean_len = 400;  // a planning figure, not a limit
     size_t bytes = nq * entries * sizeof(PairResult) + nq * entries * (mean_len / 4) * 16;
-    if (rq.align_type != COMPUTE_SCORE) bytes += rq.hitcount * qlen * mean_len * 4;
+    if (rq.align_type != COMPUTE_SCORE)
+        bytes += std::min<size_t>(rq.hitcount * ((qlen + 7) & ~(size_t)7) * std::max<size_t>(db_max_len, 1),
+                                  (size_t)1 << 30);
     return bytes;
 }
 
@@ -267,7 +285,7 @@ extern "C" {
 
 struct ssa_amd_server {
     Server* s;
-    double residues;  // of the DB at start
+    DbSize db;  // of the DB at start
 };
 
 ssa_amd_server* ssa_amd_server_start(int slots, size_t max_batch, size_t device_bytes) {
@@ -276,7 +294,7 @@ ssa_amd_server* ssa_amd_server_start(int slots, size_t max_batch, size_t device_
     const Config& c = config();
     if (!matrix().constant && c.symtype == NUCLEOTIDE) return nullptr;
     ssa_amd_server* h = new ssa_amd_server;
-    h->residues = db_entry_residues();
+    h->db = db_size();
     h->s = new Server(slots, max_batch, device_bytes);
     return h;
 }
@@ -289,7 +307,7 @@ int64_t ssa_amd_server_submit(ssa_amd_server* h, const ssa_amd_request_t* r) {
     if (r->align_type != COMPUTE_SCORE && r->align_type != COMPUTE_ALIGNMENT) return -1;
     if (r->bit_width == BIT_WIDTH_8 && config().simd_mode < COMPUTE_ON_SSE41) return -1;
     const SearchRequest rq{r->search_type == 0 ? kSW : kNW, r->query, r->hitcount, r->bit_width, r->align_type};
-    return h->s->submit(rq, request_footprint(rq), request_cells(rq, h->residues));
+    return h->s->submit(rq, request_footprint(rq, h->db.max_len), request_cells(rq, h->db.residues));
 }
 
 p_alignment_list ssa_amd_server_wait(ssa_amd_server* h, int64_t ticket, ssa_amd_stats_t* stats, double* latency_ms,

@@ -256,6 +256,27 @@ def cigar(type_, a, b, m, go, ge, region=None):
 
 
 # ---------------------------------------------------------------- reference
+def ref_available():
+    """True where oracle/_ref was built (decided without opening it)."""
+    return os.path.exists(REF_SO)
+
+
+class LazyRef:
+    """The reference's compiled library, opened on the first call through it:
+    collecting a test module opens nothing, so a run that selects none of its
+    tests (a GPU run, for one) never maps the library."""
+
+    def __init__(self):
+        self._lib = None
+
+    def __getattr__(self, name):
+        if self._lib is None:
+            self._lib = ref_lib()
+            if self._lib is None:
+                raise RuntimeError("oracle/_ref not built")
+        return getattr(self._lib, name)
+
+
 def ref_lib():
     """The reference's own compiled kernels, or None when oracle/_ref was not built."""
     if not os.path.exists(REF_SO):

@@ -175,10 +175,32 @@ def test_corrupt_packed_files_are_rejected(tmp_path):
         (h["off_bylen"], struct.pack("<I", bylen[1])),                      # not a permutation
         (h["off_seqs"] + 16, struct.pack("<Q", 10 ** 9)),                   # residues past the end
     ]
+    # residue codes: the first entry's first code, a code in the middle of a
+    # long entry, and a padding byte after an entry whose length is not a
+    # multiple of 4 (a code of 64 or more would forge the kernels' flag bits)
+    e0 = ents[0]
+    long_e = max(ents, key=lambda e: e[1])
+    odd = next(e for e in ents if e[1] % 4)
+    cases += [
+        (h["off_codes"] + e0[5], bytes([32])),                              # first code out of range
+        (h["off_codes"] + long_e[5] + long_e[1] // 2, bytes([0x40])),       # a code with a flag bit
+        (h["off_codes"] + odd[5] + odd[1], bytes([1])),                     # non-zero padding
+    ]
     for at, data in cases:
         corrupt(good, bad, at, data)
         assert ssa.lib.ssa_db_init(bad.encode()) == -1, (at, data)
         assert ssa.lib.ssa_db_get_sequence_count() == 0
+    # a nucleotide pack: codes of 16 or more are outside the nucleotide alphabet
+    setup(O.NUCLEOTIDE, 3, tmp_path, random_db(rnd, O.NUCLEOTIDE, 30))
+    ssa.set_output_mode(ssa.OUTPUT_SILENT)
+    nt_good = str(tmp_path / "nt.ssapack")
+    ssa.pack_db(nt_good)
+    _, hn, _, _, ents_n, _, _, _, _ = parse(nt_good)
+    corrupt(nt_good, bad, hn["off_codes"] + ents_n[1][5], bytes([16]))
+    assert ssa.lib.ssa_db_init(bad.encode()) == -1
+    corrupt(nt_good, bad, hn["off_codes"] + ents_n[1][5], bytes([15]))    # the largest valid code
+    assert ssa.lib.ssa_db_init(bad.encode()) == 0
+    ssa.init_symbol_translation(O.AMINOACID, 1, 1, 1)
     open(bad, "wb").write(open(good, "rb").read()[:-8])                     # truncated
     assert ssa.lib.ssa_db_init(bad.encode()) == -1
     assert ssa.lib.ssa_db_init(good.encode()) == 0

@@ -15,7 +15,7 @@ import _oracle as O
 ONE_SEQ = "ATGCCCAAGCTGAATAGCGTAGAGGGGTTTTCATCATTTGAGGACGATGTATAA"
 AA = "-ABCDEFGHIKLMNPQRSTVWXYZU*OJ"
 VALID = [1, 2, 3, 4, 5, 6, 9, 10, 11, 12, 13, 14, 15, 16, 21, 22, 23]
-REF = O.ref_lib()
+REF = O.LazyRef()
 
 
 def nt(s):
@@ -147,7 +147,7 @@ def test_search_trans_db_empty_frames(bw, t):
 ALL_CODONS = "".join(a + b + c for a in "-ACMGRSVTWYHKDBN" for b in "-ACMGRSVTWYHKDBN" for c in "-ACMGRSVTWYHKDBN")
 
 
-@pytest.mark.skipif(REF is None, reason="oracle/_ref not built (reference sources absent)")
+@pytest.mark.skipif(not O.ref_available(), reason="oracle/_ref not built (reference sources absent)")
 @pytest.mark.parametrize("gencode", VALID)
 def test_every_codon_matches_reference(gencode):
     # all 16^3 nt16 codons of the table in frame 0, both tables (query, DB)
@@ -157,7 +157,7 @@ def test_every_codon_matches_reference(gencode):
     assert O.ref_translate(REF, 1, 1, gencode, d, 0, 0) == mine
 
 
-@pytest.mark.skipif(REF is None, reason="oracle/_ref not built (reference sources absent)")
+@pytest.mark.skipif(not O.ref_available(), reason="oracle/_ref not built (reference sources absent)")
 @pytest.mark.parametrize("seed", range(10))
 def test_random_frames_match_reference(seed):
     rnd = random.Random(seed)
@@ -171,7 +171,7 @@ def test_random_frames_match_reference(seed):
                 assert O.ref_translate(REF, 1, gq, gd, d, strand, f) == O.translate(O.translate_table(gd), d, strand, f)
 
 
-@pytest.mark.skipif(REF is None, reason="oracle/_ref not built (reference sources absent)")
+@pytest.mark.skipif(not O.ref_available(), reason="oracle/_ref not built (reference sources absent)")
 @pytest.mark.parametrize("seed", range(20))
 def test_lane_kernels_with_empty_entries_match_reference(seed):
     # translated DBs hand the lane kernels entries with no residue; the
@@ -200,7 +200,7 @@ def test_lane_kernels_with_empty_entries_match_reference(seed):
                 assert [i for i, k, s in ev if k == 1] == ref_ovf, (t, width, avx2)
 
 
-@pytest.mark.skipif(REF is None, reason="oracle/_ref not built (reference sources absent)")
+@pytest.mark.skipif(not O.ref_available(), reason="oracle/_ref not built (reference sources absent)")
 def test_full64_empty_db_matches_reference():
     m = O.matrix_builtin("blosum62")
     REF.ref_matrix_builtin(b"blosum62")

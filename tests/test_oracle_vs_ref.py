@@ -14,8 +14,8 @@ import pytest
 
 import _oracle as O
 
-REF = O.ref_lib()
-pytestmark = pytest.mark.skipif(REF is None, reason="oracle/_ref not built (reference sources absent)")
+REF = O.LazyRef()
+pytestmark = pytest.mark.skipif(not O.ref_available(), reason="oracle/_ref not built (reference sources absent)")
 
 NAMES = ["blosum45", "blosum50", "blosum62", "blosum80", "blosum90", "pam30", "pam70", "pam250"]
 
