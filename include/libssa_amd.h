@@ -148,6 +148,44 @@ void ssa_amd_server_counters(ssa_amd_server* server, ssa_amd_server_counters_t* 
 /* Finishes every queued request, then stops the workers; results not taken are freed. */
 void ssa_amd_server_stop(ssa_amd_server* server);
 
+/* ---- a search that allocates no device memory ---- */
+
+/* One hit of ssa_amd_align_noalloc: the fields of alignment_t that a
+ * score-only search fills, with the same values (db_strand and db_frame as
+ * alignment_t.db_seq.strand and .frame, which the reference fills swapped,
+ * src/algo/aligner.c:76-77). */
+typedef struct {
+    size_t db_id;
+    int db_strand;
+    int db_frame;
+    int query_strand;
+    int query_frame;
+    long score;
+} ssa_amd_hit_t;
+
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): the search sw_align (search_type 0) or nw_align (1)
+ * runs with COMPUTE_SCORE, and its hits -- the same hits, in the same order,
+ * as that call's list -- are written to hits[0 .. *nhits), which holds
+ * hitcount entries.  The call allocates and frees no device or pinned host
+ * memory: every per-call device buffer (pair results, profiles, task lists,
+ * pass boundaries, counters) is a piece of one file-scope device array of
+ * 128 MiB that each GPU the library runs on holds until the process ends; its
+ * contents persist between calls, and the kernels clear what must start at
+ * zero.  Host working memory comes from the process heap.  The DB must already
+ * be resident on the GPU with the current settings, with the stream layout of
+ * at least one kernel shape: one sw_align or nw_align over the loaded DB
+ * leaves both.  Queries longer than one pass of the kernel run on as many
+ * workgroups as the array holds pass boundaries for.  Calls are serialised
+ * within the process (they share the array; calls that overlapped on one GPU
+ * would race).  Returns 0; -1 for invalid arguments; -2 when scoring is not
+ * initialised; -3 when the DB or a stream layout is not resident (run sw_align
+ * or nw_align once first); -4 when the search needs more than the array holds
+ * (an error message says which). */
+int ssa_amd_align_noalloc(int search_type, p_query query, size_t hitcount, int bit_width, ssa_amd_hit_t* hits,
+                          size_t* nhits);
+
 /* Writes the DB the plugin currently serves to a pre-packed file: the
  * sequences, their codes mapped for the symbol type in force (nucleotide DBs
  * with both strands' entries), the entry table and the entries in length

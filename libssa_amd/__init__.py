@@ -64,6 +64,12 @@ class _Request(_C.Structure):
 SEARCH_SW, SEARCH_NW = 0, 1
 
 
+class Hit(_C.Structure):
+    """One hit of align_noalloc (ssa_amd_hit_t)."""
+    _fields_ = [("db_id", _C.c_size_t), ("db_strand", _C.c_int), ("db_frame", _C.c_int),
+                ("query_strand", _C.c_int), ("query_frame", _C.c_int), ("score", _C.c_long)]
+
+
 class ServerCounters(_C.Structure):
     """ssa_amd_server_counters_t (include/libssa_amd.h)."""
     _fields_ = [("batches", _C.c_size_t), ("requests", _C.c_size_t), ("max_batch_seen", _C.c_size_t),
@@ -130,6 +136,7 @@ def _load():
         "ssa_amd_server_counters": (None, [P, _C.POINTER(ServerCounters)]),
         "ssa_amd_server_tune": (None, [P, _C.c_double, _C.c_double]),
         "ssa_amd_server_stop": (None, [P]),
+        "ssa_amd_align_noalloc": (I, [I, P, S, I, _C.POINTER(Hit), _C.POINTER(_C.c_size_t)]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -326,6 +333,15 @@ def autotune(search_type, lengths, copies=1):
 
 
 def autotune_reset(): lib.ssa_amd_autotune_reset()
+
+
+def align_noalloc(search_type, query, hitcount, bit_width):
+    """sw_align / nw_align with COMPUTE_SCORE through ssa_amd_align_noalloc
+    (no device allocation).  Returns (code, [Hit]); code 0 on success."""
+    hits = (Hit * max(hitcount, 1))()
+    n = _C.c_size_t()
+    rc = lib.ssa_amd_align_noalloc(search_type, query, hitcount, bit_width, hits, _C.byref(n))
+    return rc, [hits[i] for i in range(n.value)]
 
 
 def selfcheck_replay(seed, cases):

@@ -1,4 +1,5 @@
 // api.cpp -- the libssa.h entry points (reference: src/libssa.c:41-271).
+#include <algorithm>
 #include <cstdlib>
 #include <cstring>
 #include <vector>
@@ -182,6 +183,25 @@ int ssa_amd_batch_align(const ssa_amd_request_t* reqs, size_t n, p_alignment_lis
         if (stats) stats[i] = res[i].stats;
     }
     last_stats() = res[n - 1].stats;
+    return 0;
+}
+
+int ssa_amd_align_noalloc(int search_type, p_query query, size_t hitcount, int bit_width, ssa_amd_hit_t* hits,
+                          size_t* nhits) {
+    if (nhits) *nhits = 0;
+    if (search_type != 0 && search_type != 1) return -1;
+    if (!query || !nhits || (hitcount && !hits)) return -1;
+    if (bit_width != BIT_WIDTH_8 && bit_width != BIT_WIDTH_16 && bit_width != BIT_WIDTH_64) return -1;
+    const Config& c = config();
+    if (bit_width == BIT_WIDTH_8 && c.simd_mode < COMPUTE_ON_SSE41) return -1;
+    if (!matrix().valid) return -2;
+    if (!matrix().constant && c.symtype == NUCLEOTIDE) return -2;
+    std::vector<ssa_amd_hit_t> got;
+    const int rc = run_search_noalloc(search_type == 0 ? kSW : kNW, query, hitcount, bit_width, got);
+    if (rc != 0) return rc;
+    const size_t n = std::min(got.size(), hitcount);
+    if (n) memcpy(hits, got.data(), n * sizeof(ssa_amd_hit_t));
+    *nhits = n;
     return 0;
 }
 

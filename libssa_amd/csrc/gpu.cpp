@@ -1,6 +1,7 @@
 // gpu.cpp -- device context: stream, growable arenas, timing.
 #include "gpu.h"
 
+#include <algorithm>
 #include <cstdio>
 #include <cstring>
 
@@ -58,6 +59,47 @@ Gpu::Gpu() {
     cus_ = prop.multiProcessorCount;
     arch_ = prop.gcnArchName;
     exec_ = new Exec();
+    // the no-allocation entry point's context: its own stream, arenas in the
+    // device's copy of the scratch array, resolved here once with the device
+    // current
+    size_t scratch_bytes = 0;
+    void* scratch = scratch_setup(&scratch_bytes);
+    fixed_ = new Exec();
+    fixed_->use_region(scratch, scratch_bytes);
+}
+
+void* Exec::arena(const std::string& name, size_t bytes) {
+    if (!region_) return arenas_[name].get(bytes);
+    auto it = pieces_.find(name);
+    if (it != pieces_.end() && it->second.second >= bytes) return region_ + it->second.first;
+    const size_t need = (std::max<size_t>(bytes, 1) + 255) & ~(size_t)255;
+    if (need > region_bytes_ - region_used_)
+        throw FixedRegionError{-4, "the search needs more device memory than the no-allocation scratch holds"};
+    const size_t off = region_used_;
+    region_used_ += need;
+    pieces_[name] = std::make_pair(off, need);
+    return region_ + off;
+}
+
+void* Exec::pinned(const std::string& name, size_t bytes) {
+    if (!region_) return pinned_[name].get(bytes);
+    HostBlock& b = host_[name];
+    if (bytes > b.cap) {
+        if (b.p) host_block_release(b.p, b.cap);
+        b.p = host_block_acquire(bytes, &b.cap);
+    }
+    return b.p;
+}
+
+void Exec::use_region(void* base, size_t bytes) {
+    region_ = (uint8_t*)base;
+    region_bytes_ = bytes;
+    begin_call();
+}
+
+void Exec::begin_call() {
+    region_used_ = 0;
+    pieces_.clear();
 }
 
 Gpu& Gpu::get() {
@@ -96,6 +138,10 @@ void Exec::release_all() {
     arenas_.clear();
     for (auto& kv : pinned_) kv.second.release();
     pinned_.clear();
+    for (auto& kv : host_) host_block_release(kv.second.p, kv.second.cap);
+    host_.clear();
+    pieces_.clear();
+    region_used_ = 0;
 }
 
 GpuTimer::GpuTimer(hipStream_t st) : st_(st) {

@@ -33,9 +33,22 @@ struct DeviceArena {
     void release();
 };
 
+// Thrown by a fixed-region context (Exec::use_region) when a call needs more
+// device memory than the region holds, or device state it may not build.
+struct FixedRegionError {
+    int code;            // -3: device state missing, -4: region too small
+    const char* what;
+};
+
 // A device work context: one stream and the growable buffers of the work
 // issued on it.  Work on different contexts may run concurrently; one context
 // is used by one host thread at a time.
+//
+// A context given a fixed region (use_region) allocates nothing: arena()
+// hands out consecutive 256-byte aligned pieces of the region, the same piece
+// for a name asked again within one call, until begin_call() starts over;
+// pinned() hands out pageable host blocks.  Code that would otherwise build
+// device state asks fixed() and takes another route or throws.
 class Exec {
 public:
     Exec();
@@ -44,24 +57,46 @@ public:
     Exec& operator=(const Exec&) = delete;
     hipStream_t stream() const { return stream_; }
     hipStream_t aux(size_t i);  // extra streams for graph branches (4, created on first use)
-    void* arena(const std::string& name, size_t bytes) { return arenas_[name].get(bytes); }
-    void* pinned(const std::string& name, size_t bytes) { return pinned_[name].get(bytes); }
+    void* arena(const std::string& name, size_t bytes);
+    void* pinned(const std::string& name, size_t bytes);
     void upload(void* dst, const void* src, size_t bytes);
     void download(void* dst, const void* src, size_t bytes);
     void sync();
     void release_all();
+
+    void use_region(void* base, size_t bytes);  // from now on, arena() carves this region
+    bool fixed() const { return region_ != nullptr; }
+    void begin_call();                          // fixed: the region is free again
+    // fixed: every piece is free again (the caller has synchronised the
+    // stream and keeps no device pointer of the pieces); otherwise nothing
+    void recycle() {
+        if (region_) begin_call();
+    }
+    size_t region_left() const { return region_bytes_ - region_used_; }
 
 private:
     hipStream_t stream_ = nullptr;
     hipStream_t aux_[4] = {nullptr, nullptr, nullptr, nullptr};
     std::unordered_map<std::string, DeviceArena> arenas_;
     std::unordered_map<std::string, PinnedArena> pinned_;
+    // fixed region: base, size, bytes handed out this call, pieces by name
+    uint8_t* region_ = nullptr;
+    size_t region_bytes_ = 0, region_used_ = 0;
+    std::unordered_map<std::string, std::pair<size_t, size_t>> pieces_;  // name -> (offset, bytes)
+    struct HostBlock {
+        void* p = nullptr;
+        size_t cap = 0;
+    };
+    std::unordered_map<std::string, HostBlock> host_;
 };
 
 class Gpu {
 public:
     static Gpu& get();  // initialises the device on first use
     Exec& exec() { return *exec_; }  // the context of the libssa.h entry points
+    // the context of the no-allocation entry point: its arenas are the
+    // device's copy of the scratch array (score_kernels.hip)
+    Exec& fixed_exec() { return *fixed_; }
     int device() const { return device_; }
     std::string arch() const { return arch_; }
     int cu_count() const { return cus_; }
@@ -72,6 +107,7 @@ private:
     int cus_ = 0;
     std::string arch_;
     Exec* exec_ = nullptr;
+    Exec* fixed_ = nullptr;
 };
 
 // event pair timing a span of work on one stream
@@ -114,5 +150,11 @@ hipError_t launch_traceback(int ntasks, hipStream_t st, const TraceTask* tasks, 
 
 hipError_t launch_pack(int nentries, hipStream_t st, const PackEntry* entries, const uint8_t* codes,
                        uint32_t* res, int64_t res_words);
+
+// The device scratch array of the no-allocation entry point (a file-scope
+// __device__ array in score_kernels.hip, one copy per device): resolves the
+// current device's copy once (hipGetSymbolAddress) and returns its address
+// and size.  Host code uses only this address, never the array's name.
+void* scratch_setup(size_t* bytes);
 
 }  // namespace ssa

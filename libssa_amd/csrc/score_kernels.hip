@@ -42,6 +42,7 @@
 
 #include <algorithm>
 
+#include "common.h"
 #include "kernels.h"
 
 namespace ssa {
@@ -950,6 +951,26 @@ hipError_t launch_pack(int nentries, hipStream_t st, const PackEntry* entries, c
     if (nentries <= 0) return hipSuccess;
     hipLaunchKernelGGL(pack_kernel, dim3(nentries), dim3(64), 0, st, entries, codes, res);
     return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Scratch of the no-allocation entry point: every per-call device buffer of a
+// search run through it (pair results, task list, profile, pass boundaries,
+// counters) is a piece of this array.  Each device holds its own copy from
+// the time the library's code object is loaded until the process ends.
+// ---------------------------------------------------------------------------
+constexpr size_t kScratchBytes = size_t(128) << 20;
+static __device__ __align__(256) uint8_t scratch[kScratchBytes];
+
+void* scratch_setup(size_t* bytes) {
+    // the copy of the device that is current (the caller's set-up runs once
+    // per device with it current); host code passes this address to kernels
+    // and copies, never the array's name, which is a host-side stand-in
+    void* ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) fatal("HIP error %d (%s) resolving the device scratch array", (int)e, hipGetErrorString(e));
+    *bytes = kScratchBytes;
+    return ptr;
 }
 
 hipError_t launch_exact(int npairs, hipStream_t st, const ExactPair* pairs, const QueryDesc* queries,

@@ -212,8 +212,10 @@ static void load_packed(const PackedView& pv, bool keep_reverse, DeviceDB& d) {
 
 // Reads the plugin DB once, maps it (src/db_adapter.c:47-110) and keeps the
 // codes resident on the GPU until the DB or the symbol settings change.  A
-// packed file opened by the built-in provider is uploaded as it is.
-static DeviceDB& device_db() {
+// packed file opened by the built-in provider is uploaded as it is.  With
+// may_build false (the no-allocation path) a DB that is not resident for the
+// current settings is an error instead.
+static DeviceDB& device_db(bool may_build = true) {
     const Config& cf = config();
     size_t count = ssa_db_get_sequence_count();
     // the DB entries depend on the symbol type of the DB sequences, the
@@ -225,6 +227,9 @@ static DeviceDB& device_db() {
     if (g_db.generation == g_generation && g_db.symtype == db_type && g_db.strands == strands &&
         g_db.gencode == gencode && g_db.count == count && g_db.d_codes)
         return g_db;
+    if (!may_build)
+        throw FixedRegionError{-3, "the DB is not resident on the GPU with the current settings; run sw_align or "
+                                   "nw_align once first"};
     release_device_db();
     DeviceDB& d = g_db;
     d.generation = g_generation;
@@ -391,13 +396,39 @@ const std::vector<Shape>& kernel_shapes(int type) {
     return type == 0 ? sw : nw;
 }
 
+// true when the whole-DB stream layout of this lane width (paired or not) is
+// cached for the current device DB build
+bool layout_cached(int lanes, bool paired);
+
 // pair: packed SW queries of several buffers, which run paired (without the
 // v_perm per cell pair) on the shapes with a paired kernel; only those are
-// considered.
-Shape choose_shape(size_t qlen_max, int64_t total_blocks, int64_t max_blocks, int type, bool pair) {
+// considered.  fixed (the no-allocation path): only shapes whose whole-DB
+// layout is cached, since no layout may be built -- the unpaired one, or for
+// packed queries the paired one of a shape with a paired kernel (packed
+// queries also run unpaired, so there any cached shape will do).
+Shape choose_shape(size_t qlen_max, int64_t total_blocks, int64_t max_blocks, int type, bool pair,
+                   bool fixed = false) {
     const Shape ov = shape_override();
     // a forced shape of 32 rows per lane runs NW with 16 (the NW kernel has no 32-row shape)
     if (ov.lanes && ov.rows) return (type == 1 && ov.rows == 32) ? Shape{ov.lanes, 16} : ov;
+    auto allowed = [&](const Shape& sh) {
+        return !fixed || layout_cached(sh.lanes, false) ||
+               (pair && score_pair_supported(sh.lanes, sh.rows) && layout_cached(sh.lanes, true));
+    };
+    if (fixed) {
+        bool any = false;
+        for (const Shape& sh : kernel_shapes(type == 0 ? 0 : 1)) any |= allowed(sh);
+        if (!any)
+            throw FixedRegionError{-3, "no stream layout of the DB is resident; run sw_align or nw_align once first"};
+        // packed queries may run unpaired here: the paired filter below
+        // gives way to what is cached
+        if (pair) {
+            bool any_pair = false;
+            for (const Shape& sh : kernel_shapes(0))
+                any_pair |= score_pair_supported(sh.lanes, sh.rows) && allowed(sh);
+            if (!any_pair) pair = false;
+        }
+    }
     if (type >= 0) {
         // measured: each shape's time at the nearest tuned length (log scale),
         // scaled by the passes the shape needs for this length
@@ -416,7 +447,7 @@ Shape choose_shape(size_t qlen_max, int64_t total_blocks, int64_t max_blocks, in
             Shape best{0, 0};
             double bt = 1e300;
             for (const TunedShape& t : tuned_shapes()) {
-                if (t.type != type || t.qlen != near) continue;
+                if (t.type != type || t.qlen != near || !allowed(t.sh)) continue;
                 if (pair && !score_pair_supported(t.sh.lanes, t.sh.rows)) continue;
                 const size_t pr = (size_t)t.sh.lanes * t.sh.rows;
                 const double p_q = (double)std::max<size_t>(1, (qlen_max + pr - 1) / pr);
@@ -434,6 +465,7 @@ Shape choose_shape(size_t qlen_max, int64_t total_blocks, int64_t max_blocks, in
     double best_cost = 1e300;
     for (const Shape& sh : kernel_shapes(type == 0 ? 0 : 1)) {
         if (pair && !score_pair_supported(sh.lanes, sh.rows)) continue;
+        if (!allowed(sh)) continue;
         const int64_t pr = (int64_t)sh.lanes * sh.rows;
         const int64_t passes = std::max<int64_t>(1, ((int64_t)qlen_max + pr - 1) / pr);
         // measured per-row costs: short lanes pay the fixed per-step work (32
@@ -652,6 +684,17 @@ void release_layouts() {
 
 namespace {
 
+bool layout_cached(int lanes, bool paired) {
+    std::lock_guard<std::mutex> lk(layout_mutex());
+    for (const StreamLayout& c : layout_cache())
+        if (c.lanes == lanes && c.paired == paired && c.build == g_db.build) return true;
+    return false;
+}
+
+}  // namespace
+
+namespace {
+
 // One launch of the score kernel over the pairs (q, e) for all queries q of
 // qs and the entries of ps, in three parts: prepare() builds and uploads the
 // inputs, launch() issues the kernels (no host synchronisation, so a graph
@@ -713,7 +756,8 @@ void tier_prepare(Exec& ex, TierRun& R) {
             total_blocks += nb;
             max_blocks = std::max(max_blocks, nb);
         }
-        R.sh = choose_shape(qmax, total_blocks, max_blocks, R.type == kSW ? 0 : 1, R.type == kSW && nq > 1);
+        R.sh = choose_shape(qmax, total_blocks, max_blocks, R.type == kSW ? 0 : 1, R.type == kSW && nq > 1,
+                            ex.fixed());
     }
     const Shape sh = R.sh;
     if (!score_shape_supported(sh.lanes, sh.rows, R.type == kSW ? 0 : 1)) fatal("libssa_amd: unsupported kernel shape");
@@ -731,6 +775,9 @@ void tier_prepare(Exec& ex, TierRun& R) {
     // shape has a paired kernel
     R.packed = R.type == kSW && nq > 1;
     R.paired = R.packed && score_pair_supported(sh.lanes, sh.rows);
+    // no layout may be built on the no-allocation path: pairing only where
+    // the paired layout is already there, else unpaired packed queries
+    if (ex.fixed() && R.paired && !layout_cached(sh.lanes, true)) R.paired = false;
     for (size_t qi = 0; qi < nq && !R.packed; ++qi) {
         const QueryBuf& q = R.qs[qi];
         npass[qi] = (int)((q.len + PR - 1) / PR);
@@ -785,12 +832,16 @@ void tier_prepare(Exec& ex, TierRun& R) {
         for (StreamLayout& c : layout_cache())
             if (c.lanes == sh.lanes && c.paired == R.paired && c.build == db.build) R.L = &c;
         if (!R.L) {
+            if (ex.fixed())
+                throw FixedRegionError{-3, "the DB's stream layout for this kernel shape is not resident; run "
+                                           "sw_align or nw_align once first"};
             layout_cache().emplace_back();
             R.L = &layout_cache().back();
             build_layout(ex, *R.L, sh.lanes, ps, *R.sorted_all, R.paired);
             R.L->build = db.build;
         }
     } else {
+        if (ex.fixed()) throw FixedRegionError{-3, "a search over part of the DB builds a stream layout"};
         std::vector<uint32_t> order(n);
         std::iota(order.begin(), order.end(), 0u);
         std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) {
@@ -988,6 +1039,20 @@ void tier_prepare(Exec& ex, TierRun& R) {
 
     // ---- device buffers ----
     auto name = [&](const char* s) { return R.pfx + s; };
+    if (ex.fixed() && bnd_per_wave > 0) {
+        // the persistent grid keeps one slice of pass boundaries per resident
+        // workgroup: on the no-allocation path as many workgroups run as the
+        // scratch left after this launch's other buffers holds slices
+        // (fewer workgroups take more tasks each; the results are the same)
+        const size_t other = 256 * 8 + R.tasks.size() * sizeof(ScoreTask) + R.qd.size() * sizeof(QueryDesc) +
+                             R.prof.size() * sizeof(int16_t) + R.lbv.size() * sizeof(uint32_t) +
+                             R.segs.size() * sizeof(SegLane) + R.nout * sizeof(PairResult);
+        const size_t per_wg = (size_t)kWavesPerWG * (size_t)bnd_per_wave * sizeof(uint4);
+        const size_t left = ex.region_left() > other ? ex.region_left() - other : 0;
+        const int fit = (int)std::min<size_t>(left / per_wg, (size_t)INT32_MAX);
+        if (fit < 1) throw FixedRegionError{-4, "the query's pass boundaries do not fit the no-allocation scratch"};
+        R.grid = std::min(R.grid, fit);
+    }
     auto* d_tasks = (ScoreTask*)ex.arena(name("tasks"), std::max<size_t>(R.tasks.size(), 1) * sizeof(ScoreTask));
     auto* d_qd = (QueryDesc*)ex.arena(name("qdesc"), R.qd.size() * sizeof(QueryDesc));
     auto* d_prof = (int16_t*)ex.arena(name("prof"), std::max<size_t>(R.prof.size(), 8) * sizeof(int16_t));
@@ -1844,10 +1909,13 @@ void run_batch(Exec& ex, const std::vector<SearchRequest>& reqs, std::vector<Sea
         job_queries(J);
     }
 
+    // the no-allocation path (a context with a fixed region) builds no device
+    // state: the DB and a stream layout must be resident
+    const bool fixed = ex.fixed();
     DeviceDB* dbp;
     {
         std::lock_guard<std::mutex> lk(db_mutex());
-        dbp = &device_db();
+        dbp = &device_db(!fixed);
     }
     DeviceDB& db = *dbp;
 
@@ -1858,7 +1926,13 @@ void run_batch(Exec& ex, const std::vector<SearchRequest>& reqs, std::vector<Sea
     auto at = [N](size_t q, size_t e) { return q * N + e; };
     PairSet all;
     all.ext = &dc.all;
-    const bool full_db = N == db.entries.size();
+    bool full_db = N == db.entries.size();
+    if (fixed && !full_db) {
+        // the chunks stop before the last entry (a chunk of empty records):
+        // score every entry on the cached whole-DB layout and read the first N
+        all.ext = &db_all_entries(db);
+        full_db = true;
+    }
     const int64_t total_blocks = dc.total_blocks, max_blocks = dc.max_blocks;
     const uint64_t total_residues = dc.total_residues;
 
@@ -1890,7 +1964,10 @@ void run_batch(Exec& ex, const std::vector<SearchRequest>& reqs, std::vector<Sea
         if (J.first_bits == 0 || N == 0 || nq == 0 || J.skip8) continue;
         size_t qmax = 0;
         for (const QueryBuf& q : J.qs) qmax = std::max(qmax, q.len);
-        J.shape = choose_shape(qmax, total_blocks, max_blocks, J.rq.type == kSW ? 0 : 1, false);
+        // (SW with several query buffers runs packed: the group's choice below
+        // replaces this one)
+        J.shape = choose_shape(qmax, total_blocks, max_blocks, J.rq.type == kSW ? 0 : 1,
+                               J.rq.type == kSW && J.nrun > 1, fixed);
         // SW searches of one tier share one launch as a packed query whatever
         // their lengths; NW searches share a launch per kernel shape
         int gi = -1;
@@ -1910,10 +1987,11 @@ void run_batch(Exec& ex, const std::vector<SearchRequest>& reqs, std::vector<Sea
         size_t nqg = 0, qsum = 0;
         for (size_t j : G.jobs)
             for (size_t q = 0; q < jobs[j].nrun; ++q) ++nqg, qsum += jobs[j].qs[q].len;
-        if (nqg > 1) G.sh = choose_shape(qsum, total_blocks, max_blocks, 0, true);
+        if (nqg > 1) G.sh = choose_shape(qsum, total_blocks, max_blocks, 0, true, fixed);
         for (size_t j : G.jobs) jobs[j].shape = G.sh;
     }
     std::vector<std::unique_ptr<TierRun>> runs(groups.size());
+    std::vector<std::unique_ptr<PairSet>> run_sets(groups.size());
     for (size_t g = 0; g < groups.size(); ++g) {
         runs[g].reset(new TierRun());
         TierRun& R = *runs[g];
@@ -1926,9 +2004,9 @@ void run_batch(Exec& ex, const std::vector<SearchRequest>& reqs, std::vector<Sea
             jobs[j].q_off = R.qs.size();
             R.qs.insert(R.qs.end(), jobs[j].qs.begin(), jobs[j].qs.begin() + jobs[j].nrun);
         }
-        PairSet* ps = new PairSet(all);
-        ps->nq = R.qs.size();
-        R.ps = ps;
+        run_sets[g].reset(new PairSet(all));
+        run_sets[g]->nq = R.qs.size();
+        R.ps = run_sets[g].get();
         tier_prepare(ex, R);
     }
     double ms_first = 0;
@@ -1974,6 +2052,7 @@ void run_batch(Exec& ex, const std::vector<SearchRequest>& reqs, std::vector<Sea
             ms_first = tm.ms();
         }
     }
+    ex.recycle();  // the first tiers' results are on the host
 
     // ---- decode the first tiers; re-queued and exact tiers per search ----
     std::vector<double> ms_re_job(jobs.size(), 0.0);
@@ -2075,9 +2154,10 @@ void run_batch(Exec& ex, const std::vector<SearchRequest>& reqs, std::vector<Sea
                     n16 += n;
                 });
                 J.t16.reset(nq * N);
-                if (n16 * 2 >= N && N > 0) {
-                    // most entries overflowed: re-run the whole DB on its cached
-                    // streams together with the batch's other such searches
+                if ((n16 * 2 >= N || (fixed && n16 > 0)) && N > 0) {
+                    // most entries overflowed (or no layout of a subset may be
+                    // built): re-run the whole DB on its cached streams
+                    // together with the batch's other such searches
                     J.requeue_full = true;
                 } else if (n16 > 0) {
                     J.requeue_sub = true;
@@ -2117,6 +2197,7 @@ void run_batch(Exec& ex, const std::vector<SearchRequest>& reqs, std::vector<Sea
         R2.qs = J.qs;
         R2.ps = &sub;
         R2.pfx = "re_";
+        ex.recycle();
         gpu_score_tier(ex, R2, &ms_re_job[j]);
         computed_job[j] += R2.computed;
         for (size_t q = 0; q < nq; ++q)
@@ -2136,7 +2217,8 @@ void run_batch(Exec& ex, const std::vector<SearchRequest>& reqs, std::vector<Sea
             if (!J.requeue_full) continue;
             size_t qmax = 0;
             for (const QueryBuf& q : J.qs) qmax = std::max(qmax, q.len);
-            const Shape sh = choose_shape(qmax, total_blocks, max_blocks, J.rq.type == kSW ? 0 : 1, false);
+            const Shape sh = choose_shape(qmax, total_blocks, max_blocks, J.rq.type == kSW ? 0 : 1,
+                                          J.rq.type == kSW && J.qs.size() > 1, fixed);
             int gi = -1;
             for (size_t g = 0; g < rgroups.size(); ++g)
                 if (rgroups[g].type == J.rq.type &&
@@ -2154,7 +2236,7 @@ void run_batch(Exec& ex, const std::vector<SearchRequest>& reqs, std::vector<Sea
             size_t nqg = 0, qsum = 0;
             for (size_t j : G.jobs)
                 for (const QueryBuf& q : jobs[j].qs) ++nqg, qsum += q.len;
-            if (nqg > 1) G.sh = choose_shape(qsum, total_blocks, max_blocks, 0, true);
+            if (nqg > 1) G.sh = choose_shape(qsum, total_blocks, max_blocks, 0, true, fixed);
         }
         for (size_t g = 0; g < rgroups.size(); ++g) {
             TierRun R;
@@ -2171,6 +2253,7 @@ void run_batch(Exec& ex, const std::vector<SearchRequest>& reqs, std::vector<Sea
             ps.nq = R.qs.size();
             R.ps = &ps;
             double ms = 0;
+            ex.recycle();
             gpu_score_tier(ex, R, &ms);
             for (size_t j : rgroups[g].jobs) {
                 Job& J = jobs[j];
@@ -2253,16 +2336,14 @@ void run_batch(Exec& ex, const std::vector<SearchRequest>& reqs, std::vector<Sea
             p64.reserve(J.x_entries.size() * nq);
             for (uint32_t e : J.x_entries)
                 for (size_t q = 0; q < nq; ++q) p64.push_back({(uint32_t)q, e});
+            ex.recycle();
             gpu_exact(ex, J.rq.type, J.qs, p64, J.x64, &ms_re, &computed);
         }
         J.st.ms_gpu_requeue = ms_re;
         J.st.cells_computed = computed;
     }
-    for (auto& R : runs) {
-        delete R->ps;
-        R->ps = nullptr;
-    }
     runs.clear();
+    run_sets.clear();
 
     // ---- replay: searches in parallel, or chunks in parallel for few searches ----
     if (jobs.size() >= 4 && nthreads > 1) {
@@ -2274,6 +2355,24 @@ void run_batch(Exec& ex, const std::vector<SearchRequest>& reqs, std::vector<Sea
         if (J.st.overflow_8_bit_count || J.st.overflow_16_bit_count)
             print_info("Overflow occurred: %ld sequences were re-aligned with 16 bit, and %ld sequences with 64 bit\n",
                        (long)J.st.overflow_8_bit_count, (long)J.st.overflow_16_bit_count);
+
+    // ---- the no-allocation path: the hits themselves, no lists ----
+    if (fixed) {
+        for (size_t j = 0; j < jobs.size(); ++j) {
+            Job& J = jobs[j];
+            std::vector<ssa_amd_hit_t>& out = results[j].hits;
+            for (const Elem& h : J.hits) {
+                const QueryBuf& q = J.qs[h.query_id];
+                // the fields alignment_t carries, with the reference's swap of
+                // the entry's strand and frame (src/algo/aligner.c:76-77)
+                out.push_back(ssa_amd_hit_t{(size_t)h.db_id, (int)h.db_frame, (int)h.db_strand, q.strand, q.frame,
+                                            (long)h.score});
+            }
+            J.st.ms_total = (now_us() - t_start) / 1000.0;
+            results[j].stats = J.st;
+        }
+        return;
+    }
 
     // ---- alignment lists; one traceback launch for every hit of the batch ----
     std::vector<p_alignment> al;
@@ -2437,6 +2536,28 @@ int replay_selfcheck(uint64_t seed, int cases) {
         bad += !same;
     }
     return bad;
+}
+
+int run_search_noalloc(SearchType type, p_query query, size_t hitcount, int bit_width,
+                       std::vector<ssa_amd_hit_t>& hits) {
+    static std::mutex mu;  // calls share the device's scratch array
+    std::lock_guard<std::mutex> lk(mu);
+    Exec& ex = Gpu::get().fixed_exec();
+    ex.begin_call();
+    std::vector<SearchRequest> rq(1, SearchRequest{type, query, hitcount, bit_width, COMPUTE_SCORE});
+    std::vector<SearchResult> res;
+    try {
+        run_batch(ex, rq, res, false);
+    } catch (const FixedRegionError& e) {
+        (void)hipStreamSynchronize(ex.stream());  // work already queued on the scratch ends first
+        ex.begin_call();
+        print_error("ssa_amd_align_noalloc: %s", e.what);
+        return e.code;
+    }
+    ex.begin_call();
+    last_stats() = res[0].stats;
+    hits.swap(res[0].hits);
+    return 0;
 }
 
 p_alignment_list run_search(SearchType type, p_query query, size_t hitcount, int bit_width, int align_type) {

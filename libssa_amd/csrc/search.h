@@ -52,7 +52,15 @@ struct SearchRequest {
 struct SearchResult {
     p_alignment_list list = nullptr;
     ssa_amd_stats_t stats{};
+    std::vector<ssa_amd_hit_t> hits;  // the no-allocation path: the hits instead of a list
 };
+
+// sw_align / nw_align with COMPUTE_SCORE on the no-allocation context: every
+// per-call device buffer is a piece of the device's scratch array.  Returns
+// 0, -3 (the DB or a stream layout is not resident) or -4 (the search needs
+// more than the scratch holds).
+int run_search_noalloc(SearchType type, p_query query, size_t hitcount, int bit_width,
+                       std::vector<ssa_amd_hit_t>& hits);
 
 struct AutotuneResult {
     size_t qlen;
