@@ -7,10 +7,14 @@ cap 35,000; 5 % planted query fragments at 60-95 % identity), queries of 50,
 start with the 16/64-bit cascade, scores only.  One step = the three queries
 searched through sw_align with the DB resident on the GPU.
 
-    python bench.py [--steps K] [--warmup W] [--workload c2|c1|c3|c4|c5] [--no-cpu-baseline]
+    python bench.py [--steps K] [--warmup W] [--workload all|c1|c2|c3|c4|c5|dbload] [--no-cpu-baseline]
 
 Prints one JSON line.  value = algorithmic cells (sum over queries and DB
-entries of |q| * |d|) per second of the whole step, in GCUPS.
+entries of |q| * |d|) per second of the whole step, in GCUPS.  The default
+(all) adds, beside that headline, the NW DNA workload (nw_dna), the batched
+workload (batched: batched vs one query per launch) and the served mix
+(served: p50/p99 and GCUPS at a sustainable rate and under overload, one
+request per launch vs continuous batching), each at its full size.
 
 c4 (batched): 256 queries (log-uniform 30-2000) against 500,000 sequences,
 sw_align(q, 20, BIT_WIDTH_8, COMPUTE_SCORE) each, run one query per call and
@@ -69,8 +73,40 @@ def workload(name):
     raise SystemExit("unknown workload " + name)
 
 
-def cpu_baseline(w, sample_frac):
-    """The reference's own AVX2 8-bit kernels (oracle/_ref) on the host cores."""
+def host_cpus():
+    """The CPUs this process may use (its affinity set, at most 16: the host's
+    CPU share per GPU), the physical cores among them, and the CPU model."""
+    allowed = sorted(os.sched_getaffinity(0))
+    logical = allowed[:16]
+    model, core_of, cur = "unknown", {}, {}
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f.read().split("\n") + [""]:
+                if not line.strip():
+                    if "processor" in cur:
+                        core_of[int(cur["processor"])] = (cur.get("physical id", "0"), cur.get("core id",
+                                                                                            cur["processor"]))
+                    cur = {}
+                    continue
+                k, _, v = line.partition(":")
+                cur[k.strip()] = v.strip()
+                if k.strip() == "model name":
+                    model = v.strip()
+    except OSError:
+        pass
+    physical = len({core_of.get(c, c) for c in logical})
+    return len(logical), max(1, physical), model
+
+
+def cpu_baseline(w, runs=5):
+    """The reference's own AVX2 8-bit kernels with its 16/64-bit cascade
+    (oracle/_ref, compiled from the reference's sources) on the host cores,
+    driven as src/algo/manager.c:120-185 drives them: T threads pulling
+    1,000-sequence chunks, per-thread heaps merged in thread order.  T = the
+    logical CPUs of this process (the reference's default, get_nprocs(),
+    src/util/thread_pool.c:41-43, within the host's CPU share), the physical
+    cores among them, and 1; each the median of `runs` runs over a sample of
+    the DB sized to about a second per run (the whole DB at T = logical)."""
     so = os.path.join(ROOT, "oracle", "_ref", "libssaref.so")
     if not os.path.exists(so) or w["kind"] != "sw" or w["bw"] != 8:
         return None, None
@@ -87,29 +123,49 @@ def cpu_baseline(w, sample_frac):
         if ch != "-":
             lut[ord(ch)] = i
             lut[ord(ch.lower())] = i
-    n = max(1, int(len(w["db"]) * sample_frac))
-    sample = w["db"][:n]
-    lens = np.array([len(s) for s in sample], np.int64)
-    offs = np.concatenate([[0], np.cumsum(lens)[:-1]]).astype(np.int64)
-    codes = lut[np.frombuffer(b"".join(sample), np.uint8)]
-    threads = min(16, os.cpu_count() or 1)
+    db = w["db"]
+    lens_all = np.array([len(s) for s in db], np.int64)
+    offs_all = np.concatenate([[0], np.cumsum(lens_all)[:-1]]).astype(np.int64)
+    codes = lut[np.frombuffer(b"".join(db), np.uint8)]
+    qcodes = [lut[np.frombuffer(q, np.uint8)].tobytes() for q in w["queries"]]
+    logical, physical, model = host_cpus()
     k = w["k"]
-    cells = 0
-    secs = 0.0
-    tops = []
-    for q in w["queries"]:
-        qc = lut[np.frombuffer(q, np.uint8)].tobytes()
-        ids = np.zeros(k, np.int64)
-        sc = np.zeros(k, np.int64)
-        cnt = C.c_size_t()
-        secs += ref.ref_bench_sw8(codes.ctypes.data, offs.ctypes.data, lens.ctypes.data, n, qc, len(qc), k, 1000,
-                                  threads, ids.ctypes.data, sc.ctypes.data, C.byref(cnt))
-        cells += len(q) * int(lens.sum())
-        tops.append(list(zip(sc[:cnt.value].tolist(), ids[:cnt.value].tolist())))
-    return dict(value=round(cells / secs / 1e9, 2), unit="GCUPS", cores=threads, kind="reference",
-                sample="%d of %d DB sequences (the first), all %d queries, reference AVX2 8-bit kernels with "
-                       "16/64-bit cascade, %d threads, chunk 1000; %.2f s" % (n, len(w["db"]), len(w["queries"]),
-                                                                             threads, secs)), (n, tops)
+
+    def run(threads, n):
+        """one pass of every query over the first n sequences: (GCUPS, seconds, top-k per query)"""
+        cells, secs, tops = 0, 0.0, []
+        for qc in qcodes:
+            ids = np.zeros(k, np.int64)
+            sc = np.zeros(k, np.int64)
+            cnt = C.c_size_t()
+            secs += ref.ref_bench_sw8(codes.ctypes.data, offs_all.ctypes.data, lens_all.ctypes.data, n, qc,
+                                      len(qc), k, 1000, threads, ids.ctypes.data, sc.ctypes.data, C.byref(cnt))
+            cells += len(qc) * int(lens_all[:n].sum())
+            tops.append(list(zip(sc[:cnt.value].tolist(), ids[:cnt.value].tolist())))
+        return cells / secs / 1e9, secs, tops
+
+    table, check = {}, None
+    for threads in sorted({logical, physical, 1}, reverse=True):
+        n = len(db) if threads == logical else max(1000, len(db) * threads // max(logical, 16))
+        n = min(n, len(db))
+        vals, secs = [], []
+        for r in range(runs):
+            g, t, tops = run(threads, n)
+            vals.append(g)
+            secs.append(t)
+            if threads == logical and r == 0:
+                check = (n, tops)
+        table["T%d" % threads] = {"threads": threads, "gcups_median": round(float(np.median(vals)), 2),
+                                  "gcups_runs": [round(v, 2) for v in vals], "sequences": n,
+                                  "seconds_per_run": round(float(np.median(secs)), 3)}
+    best = table["T%d" % logical]
+    return dict(value=best["gcups_median"], unit="GCUPS", cores=logical, kind="reference",
+                threads=logical, physical_cores=physical, cpu_model=model,
+                sample="reference AVX2 8-bit kernels with the 16/64-bit cascade (oracle/_ref), chunk 1000, per-thread "
+                       "heaps merged in thread order; all %d queries; median of %d runs per thread count; T=%d over "
+                       "the whole DB, fewer threads over the first sequences (about a second per run)"
+                       % (len(qcodes), runs, logical),
+                by_threads=table), check
 
 
 def copy_probe(mib):
@@ -132,6 +188,7 @@ def copy_probe(mib):
 def setup_protein(db):
     import libssa_amd as ssa
     from libssa_amd import synthetic as S
+    ssa.reset_settings()
     ssa.set_output_mode(ssa.OUTPUT_ERROR)
     ssa.init_symbol_translation(ssa.AMINOACID, ssa.FORWARD_STRAND, 1, 1)
     ssa.init_score_matrix(ssa.MATRIX_BUILDIN, ssa.BLOSUM62)
@@ -305,14 +362,127 @@ def bench_dbload(args):
     }
 
 
+def bench_search(args, w, steps, warmup, cpu=False):
+    """One search workload (c1, c2, c3): `steps` timed steps of every query
+    through sw_align / nw_align with the DB resident.  Returns the JSON record."""
+    import libssa_amd as ssa
+    from libssa_amd import synthetic as S
+    ssa.reset_settings()
+    ssa.set_output_mode(ssa.OUTPUT_ERROR)
+    ssa.init_symbol_translation(w["symtype"], w["strands"], 1, 1)
+    if "const" in w:
+        ssa.init_constant_scores(*w["const"])
+    else:
+        ssa.init_score_matrix(ssa.MATRIX_BUILDIN, w["matrix"])
+    ssa.init_gap_penalties(*w["gaps"])
+    tmp = tempfile.mkdtemp(prefix="libssa_bench_")
+    path = os.path.join(tmp, "db.fasta")
+    S.write_fasta(path, w["db"])
+    ssa.init_db(path)
+    qh = [ssa.init_sequence_fasta(ssa.READ_FROM_STRING, q.decode()) for q in w["queries"]]
+    align = ssa.COMPUTE_ALIGNMENT if w.get("align") else ssa.COMPUTE_SCORE
+    search = ssa.sw_align if w["kind"] == "sw" else ssa.nw_align
+
+    def step():
+        st = []
+        tops = []
+        for h in qh:
+            hits = search(h, w["k"], w["bw"], align)
+            st.append(ssa.last_stats())
+            tops.append([(a.score, a.db_id) for a in hits])
+        return st, tops
+
+    for _ in range(warmup):
+        step()
+    t0 = time.perf_counter()
+    stats = []
+    tops = []
+    for _ in range(steps):
+        st, tops = step()
+        stats.extend(st)
+    t1 = time.perf_counter()
+    for h in qh:
+        ssa.free_sequence(h)
+    os.remove(path)
+    os.rmdir(tmp)
+
+    secs = t1 - t0
+    cells_step = sum(s.cells for s in stats) / steps
+    # NT both strands: the DB holds each sequence and its reverse complement, and
+    # both query strands run against both, as in the reference; the unique work
+    # is half of that
+    dup = 2 if w.get("strands") == 3 else 1
+    gcups = cells_step / dup * steps / secs / 1e9
+    kernel_ms = sum(s.ms_gpu_first for s in stats)
+    # the roofline counts the cells the first-tier kernels scored: a
+    # both-strand search reads its reverse strand from the forward strand's
+    # reverse-complement twins where that is exact, and scores half
+    scored_step = sum(s.cells_scored for s in stats) / steps
+    ops = sum(s.cells_scored for s in stats) * OPS_PER_CELL[w["kind"]]
+    achieved = ops / (kernel_ms / 1e3) / 1e12
+    traffic = measured_traffic(args.workload if args.workload != "all" else "c2") if cpu else None
+    out = {
+        "metric": "SW/NW GCUPS on 1 MI355X",
+        "value": round(gcups, 2),
+        "unit": "GCUPS",
+        "n_gpus": 1,
+        "steps": steps,
+        "warmup": warmup,
+        "ms_per_step": round(secs / steps * 1e3, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "int16",
+        "data": "synthetic",
+        "config": {"workload": w["desc"], "db_sequences": len(w["db"]), "cells_per_step": int(cells_step / dup),
+                   "reference_cells_per_step": int(cells_step), "scored_cells_per_step": int(scored_step),
+                   "query_lengths": [len(q) for q in w["queries"]], "hits": w["k"], "bit_width": w["bw"]},
+        "roofline": {"bound": "valu", "achieved": round(achieved, 3), "peak": round(PEAK_VALU_TOPS, 1),
+                     "unit": "TOP/s", "frac": round(achieved / PEAK_VALU_TOPS, 4),
+                     "traffic": traffic,
+                     "traffic_source": "committed rocprofv3 FETCH_SIZE + WRITE_SIZE of this workload "
+                                       "(profiles/score_kernel_hbm_traffic.txt), bytes per step" if traffic else None,
+                     "kernel": "score_kernel, the first-tier launch of each search (HIP events on the library's stream)",
+                     "ops_per_cell": OPS_PER_CELL[w["kind"]],
+                     "kernel_ms_per_step": round(kernel_ms / steps, 3),
+                     "launches_per_step": len(w["queries"])},
+        "breakdown_ms_per_step": {
+            "gpu_first_tier": round(kernel_ms / steps, 3),
+            "gpu_requeue": round(sum(s.ms_gpu_requeue for s in stats) / steps, 3),
+            "host_replay": round(sum(s.ms_replay for s in stats) / steps, 3),
+            "traceback": round(sum(s.ms_traceback for s in stats) / steps, 3)},
+        "cpu_baseline": None,
+    }
+    if cpu and not args.no_cpu_baseline:
+        cb, check = cpu_baseline(w)
+        if cb is not None and check[0] == len(w["db"]):
+            # the reference ran on the whole DB: its top-k per query against
+            # this run's (scores as a multiset; the hits above the k-th score
+            # by ID; at the k-th score the reference's threads pick among ties
+            # in an order that depends on their timing)
+            same_scores, same_above = True, True
+            for mine, ref in zip(tops, check[1]):
+                same_scores &= sorted(x[0] for x in mine) == sorted(x[0] for x in ref)
+                kth = min(x[0] for x in ref) if ref else None
+                same_above &= sorted(x for x in mine if x[0] > kth) == sorted(x for x in ref if x[0] > kth)
+            cb["agrees_with_gpu"] = {"queries": len(tops), "top_k_scores_equal": same_scores,
+                                     "hits_above_kth_score_equal": same_above}
+        out["cpu_baseline"] = cb
+    return out
+
+
+def summary(rec, keys):
+    return {k: rec[k] for k in keys if k in rec}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="c2")
+    ap.add_argument("--workload", default="all",
+                    help="all (default: c2 headline plus c3, c4, c5), or one of c1 c2 c3 c4 c5 dbload")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-sample", type=float, default=1.0)
     ap.add_argument("--scale", type=float, default=1.0, help="c4/c5: DB size factor")
     ap.add_argument("--rates", default="5,10,20,30", help="c5: arrival rates per second")
     ap.add_argument("--no-autotune", action="store_true", help="c4: keep the cost model's shapes")
@@ -330,106 +500,31 @@ def main():
         fn = {"c4": bench_c4, "c5": bench_c5, "dbload": bench_dbload}[args.workload]
         print(json.dumps(fn(args)))
         return
+    if args.workload in ("c1", "c2", "c3"):
+        out = bench_search(args, workload(args.workload), args.steps, args.warmup, cpu=args.workload == "c2")
+        if args.copy_probe:
+            copy_probe(args.copy_probe)
+        print(json.dumps(out))
+        return
+    if args.workload != "all":
+        raise SystemExit("unknown workload " + args.workload)
 
-    import libssa_amd as ssa
-    w = workload(args.workload)
-    ssa.set_output_mode(ssa.OUTPUT_ERROR)
-    ssa.init_symbol_translation(w["symtype"], w["strands"], 1, 1)
-    if "const" in w:
-        ssa.init_constant_scores(*w["const"])
-    else:
-        ssa.init_score_matrix(ssa.MATRIX_BUILDIN, w["matrix"])
-    ssa.init_gap_penalties(*w["gaps"])
-    tmp = tempfile.mkdtemp(prefix="libssa_bench_")
-    path = os.path.join(tmp, "db.fasta")
-    from libssa_amd import synthetic as S
-    S.write_fasta(path, w["db"])
-    ssa.init_db(path)
-    qh = [ssa.init_sequence_fasta(ssa.READ_FROM_STRING, q.decode()) for q in w["queries"]]
-    align = ssa.COMPUTE_ALIGNMENT if w.get("align") else ssa.COMPUTE_SCORE
-    search = ssa.sw_align if w["kind"] == "sw" else ssa.nw_align
+    # the headline: the SW 8-bit cascade workload (value, roofline, CPU baseline)
+    def note(msg):
+        print("bench: " + msg, file=sys.stderr, flush=True)
 
-    def step():
-        st = []
-        tops = []
-        for h in qh:
-            hits = search(h, w["k"], w["bw"], align)
-            st.append(ssa.last_stats())
-            tops.append([(a.score, a.db_id) for a in hits])
-        return st, tops
-
-    for _ in range(args.warmup):
-        step()
-    t0 = time.perf_counter()
-    stats = []
-    tops = []
-    for _ in range(args.steps):
-        st, tops = step()
-        stats.extend(st)
-    t1 = time.perf_counter()
-    os.remove(path)
-    os.rmdir(tmp)
-
-    secs = t1 - t0
-    cells_step = sum(s.cells for s in stats) / args.steps
-    # NT both strands: the DB holds each sequence and its reverse complement, and
-    # both query strands run against both, as in the reference; the unique work
-    # is half of that
-    dup = 2 if w.get("strands") == 3 else 1
-    gcups = cells_step / dup * args.steps / secs / 1e9
-    kernel_ms = sum(s.ms_gpu_first for s in stats)
-    # the roofline counts the cells the first-tier kernels scored: a
-    # both-strand search reads its reverse strand from the forward strand's
-    # reverse-complement twins where that is exact, and scores half
-    scored_step = sum(s.cells_scored for s in stats) / args.steps
-    ops = sum(s.cells_scored for s in stats) * OPS_PER_CELL[w["kind"]]
-    achieved = ops / (kernel_ms / 1e3) / 1e12
-    out = {
-        "metric": "SW/NW GCUPS on 1 MI355X",
-        "value": round(gcups, 2),
-        "unit": "GCUPS",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": round(secs / args.steps * 1e3, 3),
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "int16",
-        "data": "synthetic",
-        "config": {"workload": w["desc"], "db_sequences": len(w["db"]), "cells_per_step": int(cells_step / dup),
-                   "reference_cells_per_step": int(cells_step), "scored_cells_per_step": int(scored_step),
-                   "query_lengths": [len(q) for q in w["queries"]], "hits": w["k"], "bit_width": w["bw"]},
-        "roofline": {"bound": "valu", "achieved": round(achieved, 3), "peak": round(PEAK_VALU_TOPS, 1),
-                     "unit": "TOP/s", "frac": round(achieved / PEAK_VALU_TOPS, 4),
-                     "traffic": measured_traffic(args.workload),
-                     "kernel": "score_kernel, all first-tier launches of a step",
-                     "ops_per_cell": OPS_PER_CELL[w["kind"]],
-                     "kernel_ms_per_step": round(kernel_ms / args.steps, 3)},
-        "breakdown_ms_per_step": {
-            "gpu_first_tier": round(kernel_ms / args.steps, 3),
-            "gpu_requeue": round(sum(s.ms_gpu_requeue for s in stats) / args.steps, 3),
-            "host_replay": round(sum(s.ms_replay for s in stats) / args.steps, 3),
-            "traceback": round(sum(s.ms_traceback for s in stats) / args.steps, 3)},
-        "cpu_baseline": None,
-    }
-    if args.copy_probe:
-        copy_probe(args.copy_probe)
-    if not args.no_cpu_baseline:
-        cb, check = cpu_baseline(w, args.cpu_sample)
-        if cb is not None and check[0] == len(w["db"]):
-            # the reference ran on the whole DB: its top-k per query against
-            # this run's (scores as a multiset; the hits above the k-th score
-            # by ID; at the k-th score the reference's threads pick among ties
-            # in an order that depends on their timing)
-            same_scores, same_above = True, True
-            for mine, ref in zip(tops, check[1]):
-                same_scores &= sorted(x[0] for x in mine) == sorted(x[0] for x in ref)
-                kth = min(x[0] for x in ref) if ref else None
-                same_above &= sorted(x for x in mine if x[0] > kth) == sorted(x for x in ref if x[0] > kth)
-            cb["agrees_with_gpu"] = {"queries": len(tops), "top_k_scores_equal": same_scores,
-                                     "hits_above_kth_score_equal": same_above}
-        out["cpu_baseline"] = cb
+    note("c2 headline: SW 8-bit cascade, 200k DB, q = 50/500/5000")
+    out = bench_search(args, workload("c2"), args.steps, args.warmup, cpu=True)
+    keys = ("value", "unit", "ms_per_step", "config", "roofline", "breakdown_ms_per_step")
+    # the metric's other parts, each on its own workload at full size
+    note("c3: NW DNA both strands with CIGARs, 1M DB")
+    out["nw_dna"] = summary(bench_search(args, workload("c3"), 10, 2), keys)
+    note("c4: 256 queries batched vs one query per launch, 500k DB")
+    out["batched"] = bench_c4(args)
+    note("c5: served SW/NW mix, 1M DB, one request per launch vs continuous batching")
+    c5_args = argparse.Namespace(**vars(args))
+    c5_args.rates = "10,30"
+    out["served"] = bench_c5(c5_args)
     print(json.dumps(out))
 
 
