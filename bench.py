@@ -451,6 +451,12 @@ def bench_search(args, w, steps, warmup, cpu=False):
             "gpu_requeue": round(sum(s.ms_gpu_requeue for s in stats) / steps, 3),
             "host_replay": round(sum(s.ms_replay for s in stats) / steps, 3),
             "traceback": round(sum(s.ms_traceback for s in stats) / steps, 3)},
+        "host_transfer": {"d2h_bytes_per_step": int(sum(s.d2h_bytes for s in stats) / steps),
+                          # what the host would copy if it decoded every pair: 12 bytes per
+                          # (scored query buffer, entry) pair, one scored buffer per search here
+                          "all_pairs_bytes_per_step": int(sum(12 * s.seq_count for s in stats) / steps),
+                          "replayed_chunks_per_step": round(sum(s.replayed_chunks for s in stats) / steps, 1),
+                          "chunks_per_search": stats[0].chunk_count if stats else 0},
         "cpu_baseline": None,
     }
     if cpu and not args.no_cpu_baseline:

@@ -38,6 +38,11 @@ typedef struct {
     double ms_replay;         /* host replay of the reference's lane schedule and heap */
     double ms_traceback;      /* traceback kernel, including its copies */
     double ms_total;          /* whole call, host clock */
+    uint64_t d2h_bytes;       /* tier results copied to the host: 12 bytes per (query buffer,
+                                 entry) pair where the host decodes every pair, else the
+                                 per-chunk largest inserts the GPU selects (chunk_top_kernel)
+                                 and the pairs of the chunks whose lane schedules the replay runs */
+    uint64_t replayed_chunks; /* chunks whose lane schedules the host replay ran */
 } ssa_amd_stats_t;
 
 /* Counters of the last search. */

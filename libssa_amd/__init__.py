@@ -84,7 +84,8 @@ class Stats(_C.Structure):
                 ("lanes_per_stream", _C.c_int), ("rows_per_lane", _C.c_int),
                 ("overflow_tracking", _C.c_int),
                 ("ms_gpu_first", _C.c_double), ("ms_gpu_requeue", _C.c_double), ("ms_replay", _C.c_double),
-                ("ms_traceback", _C.c_double), ("ms_total", _C.c_double)]
+                ("ms_traceback", _C.c_double), ("ms_total", _C.c_double), ("d2h_bytes", _C.c_uint64),
+                ("replayed_chunks", _C.c_uint64)]
 
 
 @_dataclass

@@ -151,6 +151,11 @@ hipError_t launch_traceback(int ntasks, hipStream_t st, const TraceTask* tasks, 
 hipError_t launch_pack(int nentries, hipStream_t st, const PackEntry* entries, const uint8_t* codes,
                        uint32_t* res, int64_t res_words);
 
+// select_kernels.hip: per-chunk top inserts (chunks: entry ranges [x, y)),
+// and the copy of pair-result ranges into a compact buffer
+hipError_t launch_chunk_top(int nchunks, hipStream_t st, const TopSource& src, const uint2* chunks, ChunkTop* out);
+hipError_t launch_gather(int nsegs, hipStream_t st, const GatherSeg* segs, PairResult* out);
+
 // The device scratch array of the no-allocation entry point (a file-scope
 // __device__ array in score_kernels.hip, one copy per device): resolves the
 // current device's copy once (hipGetSymbolAddress) and returns its address

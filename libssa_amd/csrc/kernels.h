@@ -132,6 +132,43 @@ struct ExactPair {
 };
 
 // ---------------------------------------------------------------------------
+// Per-chunk top inserts (select_kernels.hip): the order-free first pass of
+// the host's replay of the reference's heap, on the GPU.  One wave per chunk
+// of DB entries decodes every (query buffer, entry) pair's tier results the
+// way the host does, counts the cascade's overflows and keeps the chunk's m
+// largest inserts, duplicates counted (an entry that overflowed 8 bit for m8
+// query buffers is inserted m8 times at 16 bit).
+// ---------------------------------------------------------------------------
+constexpr int kChunkTopMax = 16;
+constexpr int32_t kTopMode16 = 0;        // 16-bit first tier: res16[q] holds it
+constexpr int32_t kTopModeSwSingle = 1;  // SW from one int16 pass: the 8-bit tier from flag_a, 16-bit from flag_b
+constexpr int32_t kTopModeAll8 = 2;      // every pair overflowed 8 bit in its first block; res16 = the 16-bit re-run
+
+struct TopSource {
+    const PairResult* res16[6];  // per query buffer: entry e's result at [e ^ (flip >> q & 1)]
+    uint32_t flip;               // bit q: query q reads the entry's reverse-complement twin (e ^ 1)
+    int32_t nq;
+    int32_t mode;                // kTopMode*
+    int32_t type;                // 0 SW, 1 NW
+    int32_t m;                   // inserts kept per chunk (<= kChunkTopMax)
+};
+
+struct ChunkTop {
+    int64_t ov8, ov16;     // the chunk's overflow counts (src/libssa_datatypes.h:40-47)
+    int32_t ntop;          // inserts kept
+    int32_t need_exact;    // some pair that reaches the 16-bit tier overflowed it (the exact tier runs)
+    int32_t top[kChunkTopMax];  // the largest inserts, descending
+};
+
+// one range of pair results to copy into a compact buffer (gather_kernel)
+struct GatherSeg {
+    const PairResult* src;
+    int64_t dst;  // index into the compact buffer
+    int32_t n;
+    int32_t pad;
+};
+
+// ---------------------------------------------------------------------------
 // Traceback (traceback_kernels.hip): one task per hit.
 // ---------------------------------------------------------------------------
 struct TraceTask {

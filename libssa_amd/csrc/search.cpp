@@ -1459,6 +1459,16 @@ std::mutex& db_mutex() {
     return m;
 }
 
+// Each chunk's m largest inserts (descending; m = min(k + 1, kChunkTop)) and
+// its overflow counts: the order-free first pass of the replay, computed on
+// the host from the decoded tier results or on the GPU (chunk_top_kernel).
+constexpr size_t kChunkTop = 16;
+struct ChunkFirst {
+    std::vector<int64_t> top;  // kChunkTop per chunk
+    std::vector<uint8_t> ntop;
+    std::vector<size_t> ov8, ov16;
+};
+
 // per-request state of a batch
 struct Job {
     SearchRequest rq;
@@ -1484,7 +1494,46 @@ struct Job {
     int rgroup = -1;
     std::vector<Elem> hits;
     p_alignment_list list = nullptr;
+    size_t st_replayed_chunks = 0;  // chunks whose lane schedules the replay ran
+    // device top-k (chunk_top_kernel): the first pass runs on the GPU and the
+    // host reads tier results only for the chunks it simulates
+    bool dev_top = false;
+    int top_mode = 0;  // kTopMode*
+    const PairResult* dev16[6] = {};  // device results the 16-bit decode reads, per query buffer
+    uint32_t dev_flip = 0;            // bit q: query q reads entry e ^ 1 (strand twins)
+    ChunkFirst first;                 // the first pass as the GPU computed it
 };
+
+// Decodes the tier results of entries [e0, e1) of a search whose first pass
+// ran on the GPU (kTopMode*), as the host path decodes them: res(q, e) is the
+// PairResult the 16-bit decode reads for query buffer q and entry e.
+template <class Res>
+void decode_top_range(Job& J, int mode, const Res& res, const int32_t* nbk, uint32_t N, uint32_t e0, uint32_t e1) {
+    const SearchType type = J.rq.type;
+    const size_t nq = J.qs.size();
+    for (size_t q = 0; q < nq; ++q) {
+        TierOut* t16 = J.t16.data() + q * N;
+        TierOut* t8 = mode != kTopMode16 ? J.t8.data() + q * N : nullptr;
+        for (uint32_t e = e0; e < e1; ++e) {
+            const PairResult r = res(q, e);
+            if (mode == kTopModeSwSingle) {
+                const TierOut o8 = decode(type, kTier16, r, nbk[e], true);
+                t8[e] = o8;
+                if (!o8.accept()) J.in16[e] = 1;
+            } else if (mode == kTopModeAll8) {
+                t8[e] = TierOut{-1, 0};
+                J.in16[e] = 1;
+            }
+            const TierOut o = decode(type, kTier16, r, nbk[e], false);
+            t16[e] = o;
+            if (mode != kTopModeSwSingle && !o.accept()) J.need64[e] = 1;
+        }
+    }
+    if (mode == kTopModeSwSingle)
+        for (size_t q = 0; q < nq; ++q)
+            for (uint32_t e = e0; e < e1; ++e)
+                if (J.in16[e] && !J.t16[q * N + e].accept()) J.need64[e] = 1;
+}
 
 // Requests that share the search type, the first tier and the kernel shape
 // share one first-tier launch.
@@ -1539,8 +1588,12 @@ int64_t score64(const Job& J, size_t q, uint32_t e, uint32_t N) {
 // inserts above it, and feeds the heap only those inserts, in the reference's
 // order.  filter = false simulates every chunk in full (the self-check's
 // reference).
+// given: the first pass, already computed (then the tier results are read
+// only for the chunks the replay simulates); prepare(work) runs before the
+// simulation with the chunks it will read, in ascending order.
 void job_replay(Job& J, const std::vector<DbEntry>& entries, const std::vector<std::pair<uint32_t, uint32_t>>& chunks,
-                uint32_t N, bool threads, bool filter = true) {
+                uint32_t N, bool threads, bool filter = true, const ChunkFirst* given = nullptr,
+                const std::function<void(const std::vector<uint32_t>&)>& prepare = nullptr) {
     const int64_t t_replay = now_us();
     const int W8 = lanes8(), W16 = lanes16();
     const size_t nq = J.qs.size();
@@ -1603,30 +1656,38 @@ void job_replay(Job& J, const std::vector<DbEntry>& entries, const std::vector<s
     };
 
     // each chunk's m largest inserts (descending) and its overflow counts
-    constexpr size_t kTop = 16;
+    constexpr size_t kTop = kChunkTop;
     const size_t m = std::min<size_t>(k + 1, kTop);
-    std::vector<int64_t> top(nc * kTop);
-    std::vector<uint8_t> ntop(nc, 0);
-    std::vector<size_t> ov8(nc, 0), ov16(nc, 0);
-    over_chunks(nc, [&](size_t c) {
-        int64_t tv[kTop];
-        size_t tn = 0, n8 = 0, n16 = 0;
-        inserts(c, n8, n16, [&](size_t, uint32_t, int64_t score, size_t copies) {
-            for (size_t r = 0; r < copies; ++r) {
-                if (tn == m && !(score > tv[m - 1])) break;
-                size_t i = tn < m ? tn++ : m - 1;
-                while (i > 0 && tv[i - 1] < score) {
-                    tv[i] = tv[i - 1];
-                    --i;
+    ChunkFirst own;
+    if (!given) {
+        own.top.assign(nc * kTop, 0);
+        own.ntop.assign(nc, 0);
+        own.ov8.assign(nc, 0);
+        own.ov16.assign(nc, 0);
+        over_chunks(nc, [&](size_t c) {
+            int64_t tv[kTop];
+            size_t tn = 0, n8 = 0, n16 = 0;
+            inserts(c, n8, n16, [&](size_t, uint32_t, int64_t score, size_t copies) {
+                for (size_t r = 0; r < copies; ++r) {
+                    if (tn == m && !(score > tv[m - 1])) break;
+                    size_t i = tn < m ? tn++ : m - 1;
+                    while (i > 0 && tv[i - 1] < score) {
+                        tv[i] = tv[i - 1];
+                        --i;
+                    }
+                    tv[i] = score;
                 }
-                tv[i] = score;
-            }
+            });
+            std::copy(tv, tv + tn, own.top.begin() + c * kTop);
+            own.ntop[c] = (uint8_t)tn;
+            own.ov8[c] = n8;
+            own.ov16[c] = n16;
         });
-        std::copy(tv, tv + tn, top.begin() + c * kTop);
-        ntop[c] = (uint8_t)tn;
-        ov8[c] = n8;
-        ov16[c] = n16;
-    });
+    }
+    const ChunkFirst& F = given ? *given : own;
+    const std::vector<int64_t>& top = F.top;
+    const std::vector<uint8_t>& ntop = F.ntop;
+    const std::vector<size_t>&ov8 = F.ov8, &ov16 = F.ov16;
     for (size_t c = 0; c < nc; ++c) {
         J.st.overflow_8_bit_count += ov8[c];
         J.st.overflow_16_bit_count += ov16[c];
@@ -1669,6 +1730,8 @@ void job_replay(Job& J, const std::vector<DbEntry>& entries, const std::vector<s
     std::vector<uint32_t> work;
     for (size_t c = 0; c < nc; ++c)
         if (!full[c] || above[c] > 0) work.push_back((uint32_t)c);
+    J.st_replayed_chunks = work.size();
+    if (prepare) prepare(work);
     over_chunks(work.size(), [&](size_t w) {
         const size_t c = work[w];
         std::vector<Ins>& out = per_chunk[c];
@@ -1731,6 +1794,7 @@ void job_replay(Job& J, const std::vector<DbEntry>& entries, const std::vector<s
             heap.add(Elem{e.id, e.frame, e.strand, x.q, x.score});
         }
     J.hits = heap.sorted();
+    J.st.replayed_chunks = J.st_replayed_chunks;
     J.st.ms_replay = (now_us() - t_replay) / 1000.0;
 }
 
@@ -1961,6 +2025,15 @@ void run_batch(Exec& ex, const std::vector<SearchRequest>& reqs, std::vector<Sea
                   std::all_of(J.qs.begin(), J.qs.end(), [](const QueryBuf& q) { return nw8_first_block_flags(q); });
         if (J.first_bits != 0 && !J.skip8)
             for (size_t q = 0; q < J.nrun; ++q) J.st.cells_scored += (uint64_t)J.qs[q].len * total_residues;
+        // the replay's first pass on the GPU (chunk_top_kernel) where every
+        // insert follows from one 16-bit result per pair: 16-bit searches, SW
+        // 8-bit searches from one int16 pass, NW 8-bit searches whose every
+        // pair overflows in its first block (the 16-bit re-run's results)
+        if (!fixed && J.rq.hitcount > 0 && N > 0 && nq > 0 && nq <= 6) {
+            if (J.rq.bit_width == BIT_WIDTH_16) J.dev_top = true, J.top_mode = kTopMode16;
+            else if (J.rq.bit_width == BIT_WIDTH_8 && J.sw_single) J.dev_top = true, J.top_mode = kTopModeSwSingle;
+            else if (J.rq.bit_width == BIT_WIDTH_8 && J.skip8) J.dev_top = true, J.top_mode = kTopModeAll8;
+        }
         if (J.first_bits == 0 || N == 0 || nq == 0 || J.skip8) continue;
         size_t qmax = 0;
         for (const QueryBuf& q : J.qs) qmax = std::max(qmax, q.len);
@@ -2009,6 +2082,19 @@ void run_batch(Exec& ex, const std::vector<SearchRequest>& reqs, std::vector<Sea
         R.ps = run_sets[g].get();
         tier_prepare(ex, R);
     }
+    // the first tiers' results the host decodes: every search's but those
+    // whose replay takes its first pass from the GPU
+    auto download_first = [&](size_t g) {
+        TierRun& R = *runs[g];
+        if (R.empty()) return;
+        for (size_t j : groups[g].jobs) {
+            Job& J = jobs[j];
+            if (J.dev_top) continue;
+            const size_t a = R.ps->index(J.q_off, 0), b = R.ps->index(J.q_off + J.nrun, 0);
+            ex.download(R.out + a, R.d_out + a, (b - a) * sizeof(PairResult));
+            J.st.d2h_bytes += (b - a) * sizeof(PairResult);
+        }
+    };
     double ms_first = 0;
     if (!groups.empty()) {
         GpuTimer tm(ex.stream());
@@ -2036,7 +2122,7 @@ void run_batch(Exec& ex, const std::vector<SearchRequest>& reqs, std::vector<Sea
             tm.start();
             SSA_HIP(hipGraphLaunch(gexec, st));
             tm.stop();
-            for (size_t g = 0; g < groups.size(); ++g) tier_finish(ex, *runs[g]);
+            for (size_t g = 0; g < groups.size(); ++g) download_first(g);
             ex.sync();
             ms_first = tm.ms();
             SSA_HIP(hipGraphExecDestroy(gexec));
@@ -2047,7 +2133,7 @@ void run_batch(Exec& ex, const std::vector<SearchRequest>& reqs, std::vector<Sea
             tm.start();
             for (size_t g = 0; g < groups.size(); ++g) tier_launch(ex.stream(), *runs[g]);
             tm.stop();
-            for (size_t g = 0; g < groups.size(); ++g) tier_finish(ex, *runs[g]);
+            for (size_t g = 0; g < groups.size(); ++g) download_first(g);
             ex.sync();
             ms_first = tm.ms();
         }
@@ -2089,6 +2175,23 @@ void run_batch(Exec& ex, const std::vector<SearchRequest>& reqs, std::vector<Sea
             J.st.lanes_per_stream = R.sh.lanes;
             J.st.rows_per_lane = R.sh.rows;
             J.st.overflow_tracking = R.mode() != 2;
+        }
+        if (J.dev_top) {
+            // decoded later, only for the chunks the replay simulates; here the
+            // device results each query buffer's 16-bit decode reads
+            if (J.top_mode != kTopModeAll8) {
+                const TierRun& R = *runs[J.group];
+                for (size_t q = 0; q < nq; ++q) {
+                    const bool fl = J.twin && q == 1;
+                    J.dev16[q] = R.d_out + R.ps->index(J.q_off + (fl ? 0 : q), 0);
+                    if (fl) J.dev_flip |= 1u << q;
+                }
+            } else {
+                J.requeue_full = true;  // the 16-bit pass the cascade runs next, over the whole DB
+            }
+            if (J.rq.bit_width == BIT_WIDTH_8) J.t8.reset(nq * N);
+            J.t16.reset(nq * N);
+            return;
         }
         // the first tier's results of query buffer q: entry e at base[e ^ flip]
         // (a twin search reads its reverse strand from the forward strand's
@@ -2200,6 +2303,7 @@ void run_batch(Exec& ex, const std::vector<SearchRequest>& reqs, std::vector<Sea
         ex.recycle();
         gpu_score_tier(ex, R2, &ms_re_job[j]);
         computed_job[j] += R2.computed;
+        J.st.d2h_bytes += R2.nout * sizeof(PairResult);
         for (size_t q = 0; q < nq; ++q)
             for (size_t k = 0; k < sub.entries.size(); ++k) {
                 const uint32_t e = sub.entries[k];
@@ -2243,7 +2347,8 @@ void run_batch(Exec& ex, const std::vector<SearchRequest>& reqs, std::vector<Sea
             R.type = rgroups[g].type;
             R.T = kTier16;
             R.sh = rgroups[g].sh;
-            R.pfx = "rg_";
+            // a group's own arenas: the device top-k reads them after every group ran
+            R.pfx = "rg" + std::to_string(g) + "_";
             R.sorted_all = full_db ? &db.by_length : nullptr;
             for (size_t j : rgroups[g].jobs) {
                 jobs[j].rq_off = R.qs.size();
@@ -2254,12 +2359,33 @@ void run_batch(Exec& ex, const std::vector<SearchRequest>& reqs, std::vector<Sea
             R.ps = &ps;
             double ms = 0;
             ex.recycle();
-            gpu_score_tier(ex, R, &ms);
+            tier_prepare(ex, R);
+            {
+                GpuTimer tm(ex.stream());
+                tm.start();
+                tier_launch(ex.stream(), R);
+                tm.stop();
+                if (!R.empty())
+                    for (size_t j : rgroups[g].jobs) {
+                        Job& J = jobs[j];
+                        if (J.dev_top) continue;
+                        const size_t a = ps.index(J.rq_off, 0), b = ps.index(J.rq_off + J.qs.size(), 0);
+                        ex.download(R.out + a, R.d_out + a, (b - a) * sizeof(PairResult));
+                        J.st.d2h_bytes += (b - a) * sizeof(PairResult);
+                    }
+                ex.sync();
+                ms += tm.ms();
+            }
             for (size_t j : rgroups[g].jobs) {
                 Job& J = jobs[j];
                 const size_t nq = J.qs.size();
                 ms_re_job[j] += ms;
                 computed_job[j] += R.computed * nq / std::max<size_t>(R.qs.size(), 1);
+                if (J.dev_top) {
+                    for (size_t q = 0; q < nq; ++q) J.dev16[q] = R.d_out + ps.index(J.rq_off + q, 0);
+                    J.dev_flip = 0;
+                    continue;
+                }
                 for (size_t q = 0; q < nq; ++q)
                     for (uint32_t e = 0; e < N; ++e)
                         if (J.in16[e]) {
@@ -2268,6 +2394,68 @@ void run_batch(Exec& ex, const std::vector<SearchRequest>& reqs, std::vector<Sea
                             J.t16[at(q, e)] = o;
                             if (!o.accept()) J.need64[e] = 1;
                         }
+            }
+        }
+    }
+
+    // ---- the replay's first pass on the GPU: per chunk, its largest inserts ----
+    {
+        std::vector<size_t> dj;
+        for (size_t j = 0; j < jobs.size(); ++j)
+            if (jobs[j].dev_top) dj.push_back(j);
+        const size_t nc = chunks.size();
+        if (!dj.empty() && nc > 0) {
+            std::vector<uint2> hc(nc);
+            for (size_t c = 0; c < nc; ++c) hc[c] = make_uint2(chunks[c].first, chunks[c].second);
+            auto* d_chunks = (uint2*)ex.arena("top_chunks", nc * sizeof(uint2));
+            auto* d_top = (ChunkTop*)ex.arena("top_out", dj.size() * nc * sizeof(ChunkTop));
+            auto* h_top = (ChunkTop*)ex.pinned("top_out", dj.size() * nc * sizeof(ChunkTop));
+            ex.upload(d_chunks, hc.data(), nc * sizeof(uint2));
+            for (size_t i = 0; i < dj.size(); ++i) {
+                const Job& J = jobs[dj[i]];
+                TopSource S{};
+                for (size_t q = 0; q < J.qs.size(); ++q) S.res16[q] = J.dev16[q];
+                S.flip = J.dev_flip;
+                S.nq = (int32_t)J.qs.size();
+                S.mode = J.top_mode;
+                S.type = J.rq.type == kSW ? 0 : 1;
+                S.m = (int32_t)std::min<size_t>(J.rq.hitcount + 1, kChunkTop);
+                SSA_HIP(launch_chunk_top((int)nc, ex.stream(), S, d_chunks, d_top + i * nc));
+            }
+            ex.download(h_top, d_top, dj.size() * nc * sizeof(ChunkTop));
+            ex.sync();
+            for (size_t i = 0; i < dj.size(); ++i) {
+                Job& J = jobs[dj[i]];
+                const ChunkTop* t = h_top + i * nc;
+                J.st.d2h_bytes += nc * sizeof(ChunkTop);
+                bool exact = false;
+                for (size_t c = 0; c < nc && !exact; ++c) exact = t[c].need_exact != 0;
+                if (exact) {
+                    // some pair overflowed 16 bit: the exact tier's scores enter
+                    // the inserts, so this search takes the host path (all of
+                    // its results decoded on the host, then the exact tier)
+                    const size_t nq = J.qs.size();
+                    std::vector<std::vector<PairResult>> h(nq, std::vector<PairResult>(N));
+                    for (size_t q = 0; q < nq; ++q) ex.download(h[q].data(), J.dev16[q], N * sizeof(PairResult));
+                    ex.sync();
+                    J.st.d2h_bytes += nq * N * sizeof(PairResult);
+                    decode_top_range(
+                        J, J.top_mode, [&](size_t q, uint32_t e) { return h[q][e ^ ((J.dev_flip >> q) & 1u)]; }, nbk,
+                        N, 0, N);
+                    J.dev_top = false;
+                    continue;
+                }
+                ChunkFirst& F = J.first;
+                F.top.assign(nc * kChunkTop, 0);
+                F.ntop.assign(nc, 0);
+                F.ov8.assign(nc, 0);
+                F.ov16.assign(nc, 0);
+                for (size_t c = 0; c < nc; ++c) {
+                    F.ntop[c] = (uint8_t)t[c].ntop;
+                    for (int x = 0; x < t[c].ntop; ++x) F.top[c * kChunkTop + x] = t[c].top[x];
+                    F.ov8[c] = (size_t)t[c].ov8;
+                    F.ov16[c] = (size_t)t[c].ov16;
+                }
             }
         }
     }
@@ -2346,10 +2534,68 @@ void run_batch(Exec& ex, const std::vector<SearchRequest>& reqs, std::vector<Sea
     run_sets.clear();
 
     // ---- replay: searches in parallel, or chunks in parallel for few searches ----
+    // A search whose first pass ran on the GPU reads the tier results of the
+    // chunks it simulates only: gathered into one compact buffer on the
+    // device, downloaded and decoded (one context, so one search at a time).
+    std::mutex gather_mu;
+    auto replay = [&](Job& J, bool threads) {
+        if (!J.dev_top) {
+            job_replay(J, db.entries, chunks, N, threads);
+            return;
+        }
+        job_replay(J, db.entries, chunks, N, threads, true, &J.first, [&](const std::vector<uint32_t>& work) {
+            if (work.empty()) return;
+            const size_t nq = J.qs.size();
+            // query buffers reading the same device results (strand twins)
+            // share one gathered range
+            size_t src_of[6];
+            for (size_t q = 0; q < nq; ++q) {
+                src_of[q] = q;
+                for (size_t p = 0; p < q; ++p)
+                    if (J.dev16[p] == J.dev16[q]) {
+                        src_of[q] = src_of[p];
+                        break;
+                    }
+            }
+            std::vector<GatherSeg> segs;
+            std::vector<int64_t> at(work.size() * nq);
+            segs.reserve(work.size() * nq);
+            int64_t total = 0;
+            for (size_t w = 0; w < work.size(); ++w)
+                for (size_t q = 0; q < nq; ++q) {
+                    const uint32_t e0 = chunks[work[w]].first, e1 = chunks[work[w]].second;
+                    if (src_of[q] != q) {
+                        at[w * nq + q] = at[w * nq + src_of[q]];
+                        continue;
+                    }
+                    at[w * nq + q] = total;
+                    segs.push_back(GatherSeg{J.dev16[q] + e0, total, (int32_t)(e1 - e0), 0});
+                    total += e1 - e0;
+                }
+            std::lock_guard<std::mutex> lk(gather_mu);
+            auto* d_segs = (GatherSeg*)ex.arena("gather_segs", segs.size() * sizeof(GatherSeg));
+            auto* d_out = (PairResult*)ex.arena("gather_out", (size_t)total * sizeof(PairResult));
+            auto* h_out = (PairResult*)ex.pinned("gather_out", (size_t)total * sizeof(PairResult));
+            ex.upload(d_segs, segs.data(), segs.size() * sizeof(GatherSeg));
+            SSA_HIP(launch_gather((int)segs.size(), ex.stream(), d_segs, d_out));
+            ex.download(h_out, d_out, (size_t)total * sizeof(PairResult));
+            ex.sync();
+            J.st.d2h_bytes += (uint64_t)total * sizeof(PairResult);
+            for (size_t w = 0; w < work.size(); ++w) {
+                const uint32_t e0 = chunks[work[w]].first, e1 = chunks[work[w]].second;
+                const PairResult* base[6];
+                for (size_t q = 0; q < nq; ++q) base[q] = h_out + at[w * nq + q];
+                decode_top_range(
+                    J, J.top_mode,
+                    [&](size_t q, uint32_t e) { return base[q][(e ^ ((J.dev_flip >> q) & 1u)) - e0]; }, nbk, N, e0,
+                    e1);
+            }
+        });
+    };
     if (jobs.size() >= 4 && nthreads > 1) {
-        parallel_for(jobs.size(), [&](size_t j) { job_replay(jobs[j], db.entries, chunks, N, false); });
+        parallel_for(jobs.size(), [&](size_t j) { replay(jobs[j], false); });
     } else {
-        for (Job& J : jobs) job_replay(J, db.entries, chunks, N, true);
+        for (Job& J : jobs) replay(J, true);
     }
     for (Job& J : jobs)
         if (J.st.overflow_8_bit_count || J.st.overflow_16_bit_count)
