@@ -2405,12 +2405,25 @@ void run_batch(Exec& ex, const std::vector<SearchRequest>& reqs, std::vector<Sea
             if (jobs[j].dev_top) dj.push_back(j);
         const size_t nc = chunks.size();
         if (!dj.empty() && nc > 0) {
+            // The selection runs on a side stream after the tiers' work: on
+            // the score kernels' own stream, this kernel was measured to slow
+            // the next searches' score kernels by ~20 % whenever consecutive
+            // searches differ (query length, kernel variant), an effect that
+            // no kernel of the same grid, and none of this one's results,
+            // shows from a second stream.
+            const hipStream_t side = ex.aux(0);
+            hipEvent_t ready;
+            SSA_HIP(hipEventCreateWithFlags(&ready, hipEventDisableTiming));
+            SSA_HIP(hipEventRecord(ready, ex.stream()));
+            SSA_HIP(hipStreamWaitEvent(side, ready, 0));
             std::vector<uint2> hc(nc);
             for (size_t c = 0; c < nc; ++c) hc[c] = make_uint2(chunks[c].first, chunks[c].second);
             auto* d_chunks = (uint2*)ex.arena("top_chunks", nc * sizeof(uint2));
             auto* d_top = (ChunkTop*)ex.arena("top_out", dj.size() * nc * sizeof(ChunkTop));
             auto* h_top = (ChunkTop*)ex.pinned("top_out", dj.size() * nc * sizeof(ChunkTop));
-            ex.upload(d_chunks, hc.data(), nc * sizeof(uint2));
+            auto* h_chunks = (uint2*)ex.pinned("top_chunks", nc * sizeof(uint2));
+            memcpy(h_chunks, hc.data(), nc * sizeof(uint2));
+            SSA_HIP(hipMemcpyAsync(d_chunks, h_chunks, nc * sizeof(uint2), hipMemcpyHostToDevice, side));
             for (size_t i = 0; i < dj.size(); ++i) {
                 const Job& J = jobs[dj[i]];
                 TopSource S{};
@@ -2420,10 +2433,11 @@ void run_batch(Exec& ex, const std::vector<SearchRequest>& reqs, std::vector<Sea
                 S.mode = J.top_mode;
                 S.type = J.rq.type == kSW ? 0 : 1;
                 S.m = (int32_t)std::min<size_t>(J.rq.hitcount + 1, kChunkTop);
-                SSA_HIP(launch_chunk_top((int)nc, ex.stream(), S, d_chunks, d_top + i * nc));
+                SSA_HIP(launch_chunk_top((int)nc, side, S, d_chunks, d_top + i * nc));
             }
-            ex.download(h_top, d_top, dj.size() * nc * sizeof(ChunkTop));
-            ex.sync();
+            SSA_HIP(hipMemcpyAsync(h_top, d_top, dj.size() * nc * sizeof(ChunkTop), hipMemcpyDeviceToHost, side));
+            SSA_HIP(hipStreamSynchronize(side));
+            SSA_HIP(hipEventDestroy(ready));
             for (size_t i = 0; i < dj.size(); ++i) {
                 Job& J = jobs[dj[i]];
                 const ChunkTop* t = h_top + i * nc;
@@ -2572,23 +2586,37 @@ void run_batch(Exec& ex, const std::vector<SearchRequest>& reqs, std::vector<Sea
                     segs.push_back(GatherSeg{J.dev16[q] + e0, total, (int32_t)(e1 - e0), 0});
                     total += e1 - e0;
                 }
-            std::lock_guard<std::mutex> lk(gather_mu);
-            auto* d_segs = (GatherSeg*)ex.arena("gather_segs", segs.size() * sizeof(GatherSeg));
-            auto* d_out = (PairResult*)ex.arena("gather_out", (size_t)total * sizeof(PairResult));
-            auto* h_out = (PairResult*)ex.pinned("gather_out", (size_t)total * sizeof(PairResult));
-            ex.upload(d_segs, segs.data(), segs.size() * sizeof(GatherSeg));
-            SSA_HIP(launch_gather((int)segs.size(), ex.stream(), d_segs, d_out));
-            ex.download(h_out, d_out, (size_t)total * sizeof(PairResult));
-            ex.sync();
+            // the context's stream and staging buffers serve one search at a
+            // time; the decode runs outside the lock on the search's own copy
+            HostArray<PairResult> got;
+            got.reset((size_t)total);
+            {
+                std::lock_guard<std::mutex> lk(gather_mu);
+                auto* h_segs = (GatherSeg*)ex.pinned("gather_segs", segs.size() * sizeof(GatherSeg));
+                auto* d_segs = (GatherSeg*)ex.arena("gather_segs", segs.size() * sizeof(GatherSeg));
+                auto* d_out = (PairResult*)ex.arena("gather_out", (size_t)total * sizeof(PairResult));
+                auto* h_out = (PairResult*)ex.pinned("gather_out", (size_t)total * sizeof(PairResult));
+                memcpy(h_segs, segs.data(), segs.size() * sizeof(GatherSeg));
+                ex.upload(d_segs, h_segs, segs.size() * sizeof(GatherSeg));
+                SSA_HIP(launch_gather((int)segs.size(), ex.stream(), d_segs, d_out));
+                ex.download(h_out, d_out, (size_t)total * sizeof(PairResult));
+                ex.sync();
+                memcpy(got.data(), h_out, (size_t)total * sizeof(PairResult));
+            }
             J.st.d2h_bytes += (uint64_t)total * sizeof(PairResult);
-            for (size_t w = 0; w < work.size(); ++w) {
+            auto decode_chunk = [&](size_t w) {
                 const uint32_t e0 = chunks[work[w]].first, e1 = chunks[work[w]].second;
                 const PairResult* base[6];
-                for (size_t q = 0; q < nq; ++q) base[q] = h_out + at[w * nq + q];
+                for (size_t q = 0; q < nq; ++q) base[q] = got.data() + at[w * nq + q];
                 decode_top_range(
                     J, J.top_mode,
                     [&](size_t q, uint32_t e) { return base[q][(e ^ ((J.dev_flip >> q) & 1u)) - e0]; }, nbk, N, e0,
                     e1);
+            };
+            if (threads && work.size() > 8) {
+                parallel_for(work.size(), decode_chunk);
+            } else {
+                for (size_t w = 0; w < work.size(); ++w) decode_chunk(w);
             }
         });
     };
