@@ -138,9 +138,11 @@ hipError_t launch_score_pair(int lanes, int rows, hipStream_t st, const ScoreLau
 hipError_t launch_score(int lanes, int rows, int mode, hipStream_t st, const ScoreLaunch& L, const ScoreParams& P,
                         int max_wg);
 hipError_t launch_init_results(hipStream_t st, PairResult* r, int64_t n, int32_t score);
-hipError_t launch_exact(int npairs, hipStream_t st, const ExactPair* pairs, const QueryDesc* queries,
-                        const int32_t* qp32, const uint8_t* codes, int mode, int go, int ge, int32_t* scratch,
-                        int64_t scratch_stride, int64_t* out);
+// exact tier: one wave per pair, int64 (wide) or int32; bnd holds each pair's
+// pass boundaries (elements of that width, at ExactPair::bnd_off)
+hipError_t launch_exact(bool wide, int npairs, hipStream_t st, const ExactPair* pairs, const QueryDesc* queries,
+                        const uint8_t* qcodes, const int64_t* mat, const uint8_t* codes, int mode, int go, int ge,
+                        void* bnd, int64_t* out);
 hipError_t launch_traceback_wave(int ntasks, hipStream_t st, const TraceTask* tasks, const uint8_t* codes,
                                  const int64_t* mat, int go, int ge, uint8_t* dirs, int64_t* scratch, int32_t* ops,
                                  TraceResult* out);

@@ -122,11 +122,15 @@ struct PackEntry {
     int32_t pad;
 };
 
-// exact 32-bit tier (score_kernels.hip): one thread per pair
+// exact tier (score_kernels.hip): one wave per pair, int32 or int64
+constexpr int kExactR = 8;                    // query rows per lane
+constexpr int kExactRows = 64 * kExactR;      // rows per pass of a wave
 struct ExactPair {
-    int64_t res_off;  // byte offset of the entry's codes (plain, unpadded layout)
+    int64_t res_off;  // byte offset of the entry's codes in the DB code buffer
+    int64_t bnd_off;  // element offset of the pair's pass boundaries (2 per DB column; queries of
+                      // more than kExactRows rows only)
     int32_t dlen;
-    int32_t query;    // index into QueryDesc (uses the exact profile)
+    int32_t query;    // index into QueryDesc (qp_off: byte offset of the query's codes)
     int32_t out;      // index into the int64 score output
     int32_t pad;
 };

@@ -707,73 +707,133 @@ __global__ void init_results_kernel(PairResult* __restrict__ r, int64_t n, int32
 }
 
 // ---------------------------------------------------------------------------
-// Exact tier: int32 Gotoh, one thread per pair.  Restates full_sw / full_nw
-// (src/algo/64/smith_waterman_63.c:32-98, needleman_wunsch_64.c:32-98); the
-// host checks that every value of the pair fits in 32 bits.
+// Exact tier: the reference's full_sw / full_nw (src/algo/64/smith_waterman_63.c:32-98,
+// needleman_wunsch_64.c:32-98): textbook Gotoh with the same recurrence, border
+// and result, in T = int32 where the host has shown that every value of the
+// pair fits, else int64 as the reference computes.  One wave per pair, as a
+// systolic pipeline: lane g owns kExactR consecutive query rows of a pass of
+// kExactRows rows and walks the DB columns one column behind lane g - 1,
+// which hands it, per column, the vertical gap value F after its last row
+// and that row's H of the previous column (the next row's diagonal); the
+// bottom lane stores both per column for the next pass.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(64) void exact_kernel(const ExactPair* __restrict__ pairs, int npairs,
-                                                   const QueryDesc* __restrict__ queries,
-                                                   const int32_t* __restrict__ qp32,
-                                                   const uint8_t* __restrict__ codes, int mode, int go, int ge,
-                                                   int32_t* __restrict__ scratch, int64_t scratch_stride,
-                                                   int64_t* __restrict__ out) {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= npairs) return;
-    const ExactPair p = pairs[t];
-    const QueryDesc Q = queries[p.query];
-    const int32_t* prof = qp32 + Q.qp_off;  // [qlen][32]
-    const uint8_t* d = codes + p.res_off;
-    int32_t* hh = scratch + (int64_t)t * scratch_stride;
-    int32_t* ee = hh + Q.qlen;
-    const int qlen = Q.qlen;
-    if (mode == 0) {
-        for (int i = 0; i < qlen; ++i) {
-            hh[i] = 0;
-            ee[i] = 0;
-        }
-        int32_t best = 0;
-        for (int j = 0; j < p.dlen; ++j) {
-            const int dc = d[j];
-            int32_t f = 0, diag = 0;
-            for (int i = 0; i < qlen; ++i) {
-                const int32_t up_left = hh[i];
-                int32_t e = ee[i];
-                int32_t h = diag + prof[i * 32 + dc];
-                h = max(h, e);
-                h = max(h, f);
-                h = max(h, 0);
-                best = max(best, h);
-                hh[i] = h;
-                const int32_t open = h + go + ge;
-                ee[i] = max(e + ge, open);
-                f = max(f + ge, open);
-                diag = up_left;
-            }
-        }
-        out[p.out] = best;
+template <class T>
+__device__ __forceinline__ T wave_shr1(T v) {
+    if constexpr (sizeof(T) == 4) {
+        return (T)__builtin_amdgcn_update_dpp(0, (int)v, 0x138, 0xf, 0xf, false);  // wave_shr:1
     } else {
-        for (int i = 0; i < qlen; ++i) {
-            hh[i] = go + (i + 1) * ge;
-            ee[i] = 2 * go + (i + 2) * ge;
+        const int lo = __builtin_amdgcn_update_dpp(0, (int)(uint32_t)(uint64_t)v, 0x138, 0xf, 0xf, false);
+        const int hi = __builtin_amdgcn_update_dpp(0, (int)((uint64_t)v >> 32), 0x138, 0xf, 0xf, false);
+        return (T)(((uint64_t)(uint32_t)hi << 32) | (uint64_t)(uint32_t)lo);
+    }
+}
+
+template <class T>
+__global__ __launch_bounds__(64) void exact_wave_kernel(const ExactPair* __restrict__ pairs,
+                                                        const QueryDesc* __restrict__ queries,
+                                                        const uint8_t* __restrict__ qcodes,
+                                                        const int64_t* __restrict__ mat,
+                                                        const uint8_t* __restrict__ codes, int mode, int go_, int ge_,
+                                                        T* __restrict__ bnd_all, int64_t* __restrict__ out) {
+    __shared__ T smat[32 * 32];  // [db code * 32 + query code]
+    for (int i = threadIdx.x; i < 32 * 32; i += 64) smat[i] = (T)mat[i];
+    __syncthreads();
+    const ExactPair P = pairs[blockIdx.x];
+    const QueryDesc Q = queries[P.query];
+    const uint8_t* qc = qcodes + Q.qp_off;
+    const uint8_t* dc = codes + P.res_off;
+    T* bnd = bnd_all + P.bnd_off;
+    const int lane = threadIdx.x;
+    const int qlen = Q.qlen, dlen = P.dlen;
+    const T go = (T)go_, ge = (T)ge_;
+    const int npass = (qlen + kExactRows - 1) / kExactRows;
+    T best = 0, last = 0;
+    bool has_last = false;
+    for (int p = 0; p < npass; ++p) {
+        const int r0 = p * kExactRows + lane * kExactR;
+        const int nrow = max(0, min(kExactR, qlen - r0));
+        T hh[kExactR], ee[kExactR];
+        int qrow[kExactR];
+#pragma unroll
+        for (int k = 0; k < kExactR; ++k) {
+            const T i = (T)(r0 + k);
+            qrow[k] = k < nrow ? (int)qc[r0 + k] : 0;
+            hh[k] = mode == 0 ? (T)0 : go + (i + 1) * ge;
+            ee[k] = mode == 0 ? (T)0 : 2 * go + (i + 2) * ge;
         }
-        for (int j = 0; j < p.dlen; ++j) {
-            const int dc = d[j];
-            int32_t f = 2 * go + (j + 2) * ge;
-            int32_t diag = (j == 0) ? 0 : go + j * ge;
-            for (int i = 0; i < qlen; ++i) {
-                const int32_t up_left = hh[i];
-                int32_t e = ee[i];
-                int32_t h = diag + prof[i * 32 + dc];
-                h = max(h, f);
-                h = max(h, e);
-                hh[i] = h;
-                const int32_t open = h + go + ge;
-                ee[i] = max(e + ge, open);
-                f = max(f + ge, open);
-                diag = up_left;
+        T of = 0, od = 0;  // out of this lane's last row at its last column: F, and H of the column before
+        for (int t = 0; t < dlen + 63; ++t) {
+            const int j = t - lane;
+            T f = wave_shr1(of), diag = wave_shr1(od);
+            const bool valid = j >= 0 && j < dlen;
+            if (lane == 0 && valid) {
+                if (p == 0) {
+                    f = mode == 0 ? (T)0 : 2 * go + (T)(j + 2) * ge;
+                    diag = mode == 0 ? (T)0 : (j == 0 ? (T)0 : go + (T)j * ge);
+                } else {
+                    f = bnd[2 * j];
+                    diag = bnd[2 * j + 1];
+                }
+            }
+            if (valid) {
+                const T* srow = smat + 32 * (int)dc[j];
+#pragma unroll
+                for (int k = 0; k < kExactR; ++k) {
+                    if (k < nrow) {
+                        const T up_left = hh[k];
+                        const T e = ee[k];
+                        T h = diag + srow[qrow[k]];
+                        if (mode == 0) {
+                            h = max(h, e);
+                            h = max(h, f);
+                            h = max(h, (T)0);
+                            best = max(best, h);
+                        } else {
+                            h = max(h, f);
+                            h = max(h, e);
+                        }
+                        hh[k] = h;
+                        const T open = h + go + ge;
+                        ee[k] = max(e + ge, open);
+                        f = max(f + ge, open);
+                        diag = up_left;
+                    }
+                }
+                of = f;
+                od = diag;
+                if (lane == 63 && p + 1 < npass) {
+                    bnd[2 * j] = f;
+                    bnd[2 * j + 1] = od;
+                }
             }
         }
-        out[p.out] = qlen ? hh[qlen - 1] : 0;
+        // NW: the last query row's H after the last column
+        if (mode != 0 && p == npass - 1) {
+#pragma unroll
+            for (int k = 0; k < kExactR; ++k)
+                if (r0 + k == qlen - 1) {
+                    last = hh[k];
+                    has_last = true;
+                }
+        }
+        __threadfence_block();
+        __builtin_amdgcn_wave_barrier();
+    }
+    if (mode == 0) {
+        for (int o = 32; o > 0; o >>= 1) {
+            if constexpr (sizeof(T) == 4) {
+                best = max(best, (T)__shfl_xor((int)best, o, 64));
+            } else {
+                const int lo = __shfl_xor((int)(uint32_t)(uint64_t)best, o, 64);
+                const int hi = __shfl_xor((int)((uint64_t)best >> 32), o, 64);
+                best = max(best, (T)(((uint64_t)(uint32_t)hi << 32) | (uint64_t)(uint32_t)lo));
+            }
+        }
+        if (lane == 0) out[P.out] = (int64_t)best;
+    } else if (has_last) {
+        out[P.out] = (int64_t)last;
+    } else if (qlen == 0 && lane == 0) {
+        out[P.out] = 0;
     }
 }
 
@@ -973,12 +1033,16 @@ void* scratch_setup(size_t* bytes) {
     return ptr;
 }
 
-hipError_t launch_exact(int npairs, hipStream_t st, const ExactPair* pairs, const QueryDesc* queries,
-                        const int32_t* qp32, const uint8_t* codes, int mode, int go, int ge, int32_t* scratch,
-                        int64_t scratch_stride, int64_t* out) {
+hipError_t launch_exact(bool wide, int npairs, hipStream_t st, const ExactPair* pairs, const QueryDesc* queries,
+                        const uint8_t* qcodes, const int64_t* mat, const uint8_t* codes, int mode, int go, int ge,
+                        void* bnd, int64_t* out) {
     if (npairs <= 0) return hipSuccess;
-    hipLaunchKernelGGL(exact_kernel, dim3((npairs + 63) / 64), dim3(64), 0, st, pairs, npairs, queries, qp32, codes,
-                       mode, go, ge, scratch, scratch_stride, out);
+    if (wide)
+        hipLaunchKernelGGL(exact_wave_kernel<int64_t>, dim3(npairs), dim3(64), 0, st, pairs, queries, qcodes, mat,
+                           codes, mode, go, ge, (int64_t*)bnd, out);
+    else
+        hipLaunchKernelGGL(exact_wave_kernel<int32_t>, dim3(npairs), dim3(64), 0, st, pairs, queries, qcodes, mat,
+                           codes, mode, go, ge, (int32_t*)bnd, out);
     return hipGetLastError();
 }
 

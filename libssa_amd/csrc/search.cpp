@@ -1109,7 +1109,9 @@ void gpu_score_tier(Exec& ex, TierRun& R, double* ms) {
     *ms += tm.ms();
 }
 
-// exact 32-bit scores of the pairs (q, e) listed
+// Exact scores of the pairs (q, e) listed (the reference's 64-bit tier,
+// full_sw / full_nw): one wave per pair, in int32 where every value of the
+// pair provably fits, else in int64 as the reference computes.
 void gpu_exact(Exec& gpu, SearchType type, const std::vector<QueryBuf>& qs,
                const std::vector<std::pair<uint32_t, uint32_t>>& pairs, std::vector<int64_t>& out, double* ms,
                uint64_t* cells_computed) {
@@ -1117,48 +1119,55 @@ void gpu_exact(Exec& gpu, SearchType type, const std::vector<QueryBuf>& qs,
     if (pairs.empty()) return;
     DeviceDB& db = g_db;
     const Config& cf = config();
-    const Matrix& M = matrix();
     const MatrixInfo mi = matrix_info();
-    if (!mi.fits32) fatal("libssa_amd: score matrix entries exceed the exact 32-bit tier");
     std::vector<QueryDesc> qd(qs.size());
-    std::vector<int32_t> prof;
-    size_t qmax = 0;
+    std::vector<uint8_t> qcodes;
     for (size_t qi = 0; qi < qs.size(); ++qi) {
+        qd[qi] = QueryDesc{};
         qd[qi].qlen = (int32_t)qs[qi].len;
-        qd[qi].qp_off = (int32_t)prof.size();
-        qd[qi].lb_off = 0;
-        qmax = std::max(qmax, qs[qi].len);
-        for (size_t i = 0; i < qs[qi].len; ++i)
-            for (int d = 0; d < kDim; ++d) prof.push_back((int32_t)M.at(d, qs[qi].seq[i]));
+        qd[qi].qp_off = (int32_t)qcodes.size();
+        qcodes.insert(qcodes.end(), qs[qi].seq, qs[qi].seq + qs[qi].len);
     }
-    // every value of a pair stays within +-(|go|+|ge|)*(qlen+dlen+4) + max|s|*min(qlen,dlen)
+    // every value of a pair stays within +-(|go|+|ge|)*(qlen+dlen+4)*3 + max|s|*min(qlen,dlen)
     const int64_t gap = std::abs((int64_t)cf.gap_open) + std::abs((int64_t)cf.gap_extend);
     const int64_t smax = std::max(std::abs(mi.maxv), std::abs(mi.minv));
-    std::vector<ExactPair> ep(pairs.size());
+    std::vector<ExactPair> ep[2];  // int32, int64
+    int64_t bnd_elems[2] = {0, 0};
     for (size_t k = 0; k < pairs.size(); ++k) {
         const DbEntry& e = db.entries[pairs[k].second];
         const size_t ql = qs[pairs[k].first].len;
         const int64_t bound = gap * (int64_t)(ql + e.len + 4) * 3 + smax * (int64_t)std::min<size_t>(ql, e.len);
-        if (bound >= INT32_MAX / 2) fatal("libssa_amd: alignment too long for the exact 32-bit tier");
-        ep[k].res_off = e.code_off;
-        ep[k].dlen = (int32_t)e.len;
-        ep[k].query = (int32_t)pairs[k].first;
-        ep[k].out = (int32_t)k;
-        ep[k].pad = 0;
+        const int w = (mi.fits32 && bound < INT32_MAX / 2) ? 0 : 1;
+        ExactPair x{};
+        x.res_off = e.code_off;
+        x.dlen = (int32_t)e.len;
+        x.query = (int32_t)pairs[k].first;
+        x.out = (int32_t)k;
+        x.bnd_off = bnd_elems[w];
+        if (ql > (size_t)kExactRows) bnd_elems[w] += 2 * (int64_t)e.len;
+        ep[w].push_back(x);
         *cells_computed += (uint64_t)ql * e.len;
     }
-    auto* d_ep = (ExactPair*)gpu.arena("exact_pairs", ep.size() * sizeof(ExactPair));
     auto* d_qd = (QueryDesc*)gpu.arena("exact_qdesc", qd.size() * sizeof(QueryDesc));
-    auto* d_prof = (int32_t*)gpu.arena("exact_prof", std::max<size_t>(prof.size(), 1) * sizeof(int32_t));
-    auto* d_scr = (int32_t*)gpu.arena("exact_scratch", ep.size() * 2 * std::max<size_t>(qmax, 1) * sizeof(int32_t));
-    auto* d_out = (int64_t*)gpu.arena("exact_out", ep.size() * sizeof(int64_t));
-    gpu.upload(d_ep, ep.data(), ep.size() * sizeof(ExactPair));
+    auto* d_qc = (uint8_t*)gpu.arena("exact_qcodes", std::max<size_t>(qcodes.size(), 1));
+    auto* d_mat = (int64_t*)gpu.arena("exact_mat", kDim * kDim * sizeof(int64_t));
+    auto* d_out = (int64_t*)gpu.arena("exact_out", pairs.size() * sizeof(int64_t));
     gpu.upload(d_qd, qd.data(), qd.size() * sizeof(QueryDesc));
-    gpu.upload(d_prof, prof.data(), prof.size() * sizeof(int32_t));
+    gpu.upload(d_qc, qcodes.data(), qcodes.size());
+    gpu.upload(d_mat, matrix().s, kDim * kDim * sizeof(int64_t));
+    ExactPair* d_ep[2];
+    void* d_bnd[2];
+    for (int w = 0; w < 2; ++w) {
+        const std::string tag = w ? "exact64_" : "exact32_";
+        d_ep[w] = (ExactPair*)gpu.arena(tag + "pairs", std::max<size_t>(ep[w].size(), 1) * sizeof(ExactPair));
+        d_bnd[w] = gpu.arena(tag + "bnd", (size_t)std::max<int64_t>(bnd_elems[w], 1) * (w ? 8 : 4));
+        gpu.upload(d_ep[w], ep[w].data(), ep[w].size() * sizeof(ExactPair));
+    }
     GpuTimer tm(gpu.stream());
     tm.start();
-    SSA_HIP(launch_exact((int)ep.size(), gpu.stream(), d_ep, d_qd, d_prof, db.d_codes, type == kSW ? 0 : 1,
-                         cf.gap_open, cf.gap_extend, d_scr, (int64_t)(2 * std::max<size_t>(qmax, 1)), d_out));
+    for (int w = 0; w < 2; ++w)
+        SSA_HIP(launch_exact(w == 1, (int)ep[w].size(), gpu.stream(), d_ep[w], d_qd, d_qc, d_mat, db.d_codes,
+                             type == kSW ? 0 : 1, cf.gap_open, cf.gap_extend, d_bnd[w], d_out));
     tm.stop();
     gpu.download(out.data(), d_out, out.size() * sizeof(int64_t));
     gpu.sync();
