@@ -234,7 +234,10 @@ def bench_c4(args):
                                "sw_align(q, 20, 8, scores), BLOSUM62 10/1", "db_sequences": len(db),
                    "queries": len(qs), "query_residues": sum(len(q) for q in qs)},
         "batched": {"seconds": round(t2 - t1, 3), "gcups": round(cells / (t2 - t1) / 1e9, 2),
-                    "gpu_first_tier_ms": round(stats[0].ms_gpu_first, 3)},
+                    "gpu_first_tier_ms": round(stats[0].ms_gpu_first, 3),
+                    "graph_setup_ms": round(stats[0].ms_graph, 3), "graph_reused": bool(stats[0].graph_reused),
+                    "d2h_bytes": int(sum(st.d2h_bytes for st in stats)),
+                    "all_pairs_bytes": int(sum(12 * st.seq_count for st in stats))},
         "one_query_per_launch": {"seconds": round(t1 - t0, 3), "gcups": round(cells / (t1 - t0) / 1e9, 2)},
         "speedup": round((t1 - t0) / (t2 - t1), 3), "results_equal": equal, "cells": cells,
         "autotune": {"seconds": round(t_tune, 2),
@@ -303,7 +306,7 @@ def bench_c5(args):
                          "p99_ms": round(float(np.percentile(lat, 99)), 1),
                          "mean_queue_ms": round(float(np.mean([o[1] for o in out])), 1),
                          "throughput_per_s": round(len(out) / wall, 2), "gcups": round(cells / wall / 1e9, 1),
-                         "batches": c.batches, "max_batch": c.max_batch_seen}
+                         "batches": c.batches, "max_batch": c.max_batch_seen, "deferrals": c.deferrals}
         row["results_equal"] = results["one_request_per_launch"] == results["continuous_batching"]
         rows.append(row)
         for h in handles:

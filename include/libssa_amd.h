@@ -43,6 +43,10 @@ typedef struct {
                                  per-chunk largest inserts the GPU selects (chunk_top_kernel)
                                  and the pairs of the chunks whose lane schedules the replay runs */
     uint64_t replayed_chunks; /* chunks whose lane schedules the host replay ran */
+    double ms_graph;          /* batches: capturing the first tiers' graph and updating (or
+                                 instantiating) its executable, host clock */
+    int graph_reused;         /* 1 when the batch ran its context's previous executable graph,
+                                 updated to the batch's arguments */
 } ssa_amd_stats_t;
 
 /* Counters of the last search. */
@@ -117,17 +121,23 @@ typedef struct {
     size_t max_batch_seen;  /* largest batch */
     size_t queued;          /* requests waiting now */
     size_t device_budget;   /* device bytes the admission works with */
+    size_t deferrals;       /* times a free worker waited instead of starting a batch */
+    size_t device_in_use;   /* device bytes of the batches on the GPU now */
 } ssa_amd_server_counters_t;
 
 /* Starts a request loop with `slots` workers (1-8), each with its own device
- * stream and buffers; a free worker takes waiting requests (see
- * ssa_amd_server_tune for the order), at most `max_batch` of them, as one
- * batch, admitting requests while the batch's estimated device footprint fits
- * its share of `device_bytes` (0 = half the free device memory).  A worker
- * joins another one already on the GPU only once the oldest waiting request
- * has waited 50 ms, so below saturation requests run alone.  Uses the settings current
- * at the time (scoring, gaps, symbol type, DB); they must not change while the
- * server runs.  Returns NULL if the arguments or the settings are invalid. */
+ * stream and buffers; a free worker takes waiting requests as one batch, by
+ * the policy of ssa_amd_admission_plan: at most `max_batch` of them, within
+ * the cell budget (ssa_amd_server_tune), while their device footprint fits
+ * what `device_bytes` (0 = half the free device memory) leaves beside the
+ * batches on the GPU, and beside running batches only if the batch's
+ * first-tier workgroups fit the resident slots they leave (or once the
+ * oldest waiting request has waited 50 ms), so below saturation requests
+ * run alone.  Each request's footprint comes from the kernel shape its search
+ * would choose over the resident DB.  Makes the DB resident; uses the
+ * settings current at the time (scoring, gaps, symbol type, DB), which must
+ * not change while the server runs.  Returns NULL if the arguments or the
+ * settings are invalid. */
 ssa_amd_server* ssa_amd_server_start(int slots, size_t max_batch, size_t device_bytes);
 
 /* Batch formation: a free worker takes the oldest waiting request first once
@@ -144,7 +154,8 @@ int64_t ssa_amd_server_submit(ssa_amd_server* server, const ssa_amd_request_t* r
 /* Waits for a ticket's result: the list sw_align / nw_align returns for the
  * request alone (the caller frees it with free_alignment).  stats, latency_ms
  * (submit to result) and queue_ms (submit to batch start) may be NULL.
- * Returns NULL for an unknown ticket. */
+ * Returns NULL for an unknown ticket, and for a ticket another thread is
+ * already waiting for (one waiter takes each result). */
 p_alignment_list ssa_amd_server_wait(ssa_amd_server* server, int64_t ticket, ssa_amd_stats_t* stats,
                                      double* latency_ms, double* queue_ms);
 
@@ -152,6 +163,40 @@ void ssa_amd_server_counters(ssa_amd_server* server, ssa_amd_server_counters_t* 
 
 /* Finishes every queued request, then stops the workers; results not taken are freed. */
 void ssa_amd_server_stop(ssa_amd_server* server);
+
+/* The server's admission policy as a pure function, for tests and for
+ * callers with their own request loop.  No counterpart in the reference. */
+typedef struct {
+    double wait_ms;       /* time the request has waited */
+    double cells;         /* its GPU work: first-tier cells */
+    size_t device_bytes;  /* device memory its buffers hold while it runs */
+    double occupancy;     /* share (0-1) of the chip's resident workgroup slots its first-tier
+                             grid fills: the kernel shape's LDS and VGPR use set how many
+                             workgroups a CU holds */
+} ssa_amd_admission_request_t;
+
+typedef struct {
+    size_t max_batch;
+    double batch_cells;       /* cell budget of a batch */
+    double aging_ms;          /* the oldest request goes first once it waited longer */
+    double assist_ms;         /* a batch may queue behind running ones once the oldest waited longer */
+    size_t device_budget;     /* device bytes all batches may hold together */
+    size_t device_in_use;     /* device bytes of the batches on the GPU now */
+    double occupancy_in_use;  /* their summed occupancy */
+    int running;              /* batches on the GPU now */
+} ssa_amd_admission_state_t;
+
+/* Which of the n waiting requests (queue[0] the oldest) a free worker takes
+ * as its next batch: candidates are the oldest first once it has waited past
+ * aging_ms, then the cheapest by cells; each joins while the batch stays
+ * within max_batch and batch_cells and its device bytes fit device_budget -
+ * device_in_use (a request larger than the whole budget runs alone when
+ * nothing runs); with batches running, the batch starts only if its summed
+ * occupancy fits 1 - occupancy_in_use or the oldest request has waited
+ * assist_ms.  Writes the chosen indices to chosen (room for n) and returns
+ * how many; 0 means wait. */
+size_t ssa_amd_admission_plan(const ssa_amd_admission_request_t* queue, size_t n,
+                              const ssa_amd_admission_state_t* state, size_t* chosen);
 
 /* ---- a search that allocates no device memory ---- */
 

@@ -70,10 +70,22 @@ class Hit(_C.Structure):
                 ("query_strand", _C.c_int), ("query_frame", _C.c_int), ("score", _C.c_long)]
 
 
+class AdmissionRequest(_C.Structure):
+    _fields_ = [("wait_ms", _C.c_double), ("cells", _C.c_double), ("device_bytes", _C.c_size_t),
+                ("occupancy", _C.c_double)]
+
+
+class AdmissionState(_C.Structure):
+    _fields_ = [("max_batch", _C.c_size_t), ("batch_cells", _C.c_double), ("aging_ms", _C.c_double),
+                ("assist_ms", _C.c_double), ("device_budget", _C.c_size_t), ("device_in_use", _C.c_size_t),
+                ("occupancy_in_use", _C.c_double), ("running", _C.c_int)]
+
+
 class ServerCounters(_C.Structure):
     """ssa_amd_server_counters_t (include/libssa_amd.h)."""
     _fields_ = [("batches", _C.c_size_t), ("requests", _C.c_size_t), ("max_batch_seen", _C.c_size_t),
-                ("queued", _C.c_size_t), ("device_budget", _C.c_size_t)]
+                ("queued", _C.c_size_t), ("device_budget", _C.c_size_t), ("deferrals", _C.c_size_t),
+                ("device_in_use", _C.c_size_t)]
 
 
 class Stats(_C.Structure):
@@ -85,7 +97,7 @@ class Stats(_C.Structure):
                 ("overflow_tracking", _C.c_int),
                 ("ms_gpu_first", _C.c_double), ("ms_gpu_requeue", _C.c_double), ("ms_replay", _C.c_double),
                 ("ms_traceback", _C.c_double), ("ms_total", _C.c_double), ("d2h_bytes", _C.c_uint64),
-                ("replayed_chunks", _C.c_uint64)]
+                ("replayed_chunks", _C.c_uint64), ("ms_graph", _C.c_double), ("graph_reused", _C.c_int)]
 
 
 @_dataclass
@@ -138,6 +150,8 @@ def _load():
         "ssa_amd_server_tune": (None, [P, _C.c_double, _C.c_double]),
         "ssa_amd_server_stop": (None, [P]),
         "ssa_amd_align_noalloc": (I, [I, P, S, I, _C.POINTER(Hit), _C.POINTER(_C.c_size_t)]),
+        "ssa_amd_admission_plan": (S, [_C.POINTER(AdmissionRequest), S, _C.POINTER(AdmissionState),
+                                       _C.POINTER(_C.c_size_t)]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -343,6 +357,19 @@ def align_noalloc(search_type, query, hitcount, bit_width):
     n = _C.c_size_t()
     rc = lib.ssa_amd_align_noalloc(search_type, query, hitcount, bit_width, hits, _C.byref(n))
     return rc, [hits[i] for i in range(n.value)]
+
+
+def admission_plan(queue, state):
+    """The server's admission policy (ssa_amd_admission_plan): queue is a list
+    of (wait_ms, cells, device_bytes, occupancy), oldest first; state a dict of
+    AdmissionState fields.  Returns the indices of the requests chosen (empty:
+    wait)."""
+    n = len(queue)
+    arr = (AdmissionRequest * max(n, 1))(*[AdmissionRequest(*r) for r in queue])
+    st = AdmissionState(**state)
+    out = (_C.c_size_t * max(n, 1))()
+    k = lib.ssa_amd_admission_plan(arr, n, _C.byref(st), out)
+    return [out[i] for i in range(k)]
 
 
 def selfcheck_replay(seed, cases):

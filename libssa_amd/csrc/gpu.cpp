@@ -111,6 +111,7 @@ Exec::Exec() { SSA_HIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking))
 
 Exec::~Exec() {
     release_all();
+    if (graph_exec_) (void)hipGraphExecDestroy(graph_exec_);
     for (hipStream_t& a : aux_)
         if (a) (void)hipStreamDestroy(a);
     if (stream_) (void)hipStreamDestroy(stream_);

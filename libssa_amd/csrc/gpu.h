@@ -64,6 +64,11 @@ public:
     void sync();
     void release_all();
 
+    // the instantiated graph of this context's last batch (reused for the
+    // next batch through hipGraphExecUpdate when its topology is the same)
+    hipGraphExec_t graph_exec() const { return graph_exec_; }
+    void set_graph_exec(hipGraphExec_t g) { graph_exec_ = g; }
+
     void use_region(void* base, size_t bytes);  // from now on, arena() carves this region
     bool fixed() const { return region_ != nullptr; }
     void begin_call();                          // fixed: the region is free again
@@ -77,6 +82,7 @@ public:
 private:
     hipStream_t stream_ = nullptr;
     hipStream_t aux_[4] = {nullptr, nullptr, nullptr, nullptr};
+    hipGraphExec_t graph_exec_ = nullptr;
     std::unordered_map<std::string, DeviceArena> arenas_;
     std::unordered_map<std::string, PinnedArena> pinned_;
     // fixed region: base, size, bytes handed out this call, pieces by name

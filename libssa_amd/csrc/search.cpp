@@ -2104,7 +2104,8 @@ void run_batch(Exec& ex, const std::vector<SearchRequest>& reqs, std::vector<Sea
             J.st.d2h_bytes += (b - a) * sizeof(PairResult);
         }
     };
-    double ms_first = 0;
+    double ms_first = 0, ms_graph = 0;
+    bool graph_reused = false;
     if (!groups.empty()) {
         GpuTimer tm(ex.stream());
         if (one_graph) {
@@ -2113,6 +2114,7 @@ void run_batch(Exec& ex, const std::vector<SearchRequest>& reqs, std::vector<Sea
             hipGraph_t graph;
             hipGraphExec_t gexec;
             const hipStream_t st = ex.stream();
+            const int64_t t_graph = now_us();
             SSA_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
             hipEvent_t fork;
             SSA_HIP(hipEventCreateWithFlags(&fork, hipEventDisableTiming));
@@ -2127,14 +2129,31 @@ void run_batch(Exec& ex, const std::vector<SearchRequest>& reqs, std::vector<Sea
                 SSA_HIP(hipStreamWaitEvent(st, joins[g], 0));
             }
             SSA_HIP(hipStreamEndCapture(st, &graph));
-            SSA_HIP(hipGraphInstantiate(&gexec, graph, nullptr, nullptr, 0));
+            // the context's last instantiated graph, updated to this batch's
+            // kernel arguments when the topology is the same, else a new one
+            gexec = ex.graph_exec();
+            graph_reused = false;
+            if (gexec) {
+                hipGraphNode_t bad = nullptr;
+                hipGraphExecUpdateResult ur;
+                graph_reused = hipGraphExecUpdate(gexec, graph, &bad, &ur) == hipSuccess && ur == hipGraphExecUpdateSuccess;
+                if (!graph_reused) {
+                    (void)hipGetLastError();
+                    SSA_HIP(hipGraphExecDestroy(gexec));
+                    ex.set_graph_exec(nullptr);
+                }
+            }
+            if (!graph_reused) {
+                SSA_HIP(hipGraphInstantiate(&gexec, graph, nullptr, nullptr, 0));
+                ex.set_graph_exec(gexec);
+            }
+            ms_graph = (now_us() - t_graph) / 1000.0;
             tm.start();
             SSA_HIP(hipGraphLaunch(gexec, st));
             tm.stop();
             for (size_t g = 0; g < groups.size(); ++g) download_first(g);
             ex.sync();
             ms_first = tm.ms();
-            SSA_HIP(hipGraphExecDestroy(gexec));
             SSA_HIP(hipGraphDestroy(graph));
             SSA_HIP(hipEventDestroy(fork));
             for (hipEvent_t e : joins) SSA_HIP(hipEventDestroy(e));
@@ -2177,6 +2196,8 @@ void run_batch(Exec& ex, const std::vector<SearchRequest>& reqs, std::vector<Sea
         const SearchType type = J.rq.type;
         const size_t nq = J.qs.size();
         J.st.ms_gpu_first = J.group >= 0 ? ms_first : 0.0;
+        J.st.ms_graph = ms_graph;
+        J.st.graph_reused = graph_reused ? 1 : 0;
         uint64_t& computed = computed_job[j];
         if (J.group >= 0) {
             TierRun& R = *runs[J.group];
@@ -2819,6 +2840,73 @@ int replay_selfcheck(uint64_t seed, int cases) {
         bad += !same;
     }
     return bad;
+}
+
+// The device DB for the current settings, built if needed (the server makes
+// it resident once, at start, before its workers run).
+void ensure_device_db() {
+    std::lock_guard<std::mutex> lk(db_mutex());
+    (void)device_db();
+}
+
+// What one request will take on the device, from the kernel shape the search
+// would choose over the resident DB: its first-tier cells, the HBM its
+// buffers hold while it runs (pair results of the first tier and of a 16-bit
+// re-run, pass boundaries of a multi-pass query, traceback matrices), and
+// the share of the chip's resident workgroup slots its persistent first-tier
+// grid fills (the shape's LDS and VGPR use set how many workgroups a CU holds).
+RequestPlan plan_request(const SearchRequest& rq, const std::vector<size_t>& qlens) {
+    RequestPlan pl;
+    DeviceDB* dbp;
+    {
+        std::lock_guard<std::mutex> lk(db_mutex());
+        dbp = &device_db(false);
+    }
+    const DeviceDB& db = *dbp;
+    const Config& cf = config();
+    const DbChunks& dc = db_chunks(db, cf.chunk_size ? cf.chunk_size : kDefaultChunkSize);
+    const int type = rq.type == kSW ? 0 : 1;
+    size_t qmax = 0, qsum = 0;
+    for (size_t l : qlens) qmax = std::max(qmax, l), qsum += l;
+    const bool packed = type == 0 && qlens.size() > 1;
+    const Shape sh = choose_shape(packed ? qsum : qmax, dc.total_blocks, dc.max_blocks, type, packed);
+    pl.lanes = sh.lanes;
+    pl.rows = sh.rows;
+    const double nw8 = (rq.type == kNW && rq.bit_width == BIT_WIDTH_8) ? 2.0 : 1.0;  // its 16-bit re-run
+    pl.cells = (double)qsum * (double)dc.total_residues * nw8;
+    const size_t nq = qlens.size(), N = dc.N;
+    size_t bytes = (size_t)(nq * N * sizeof(PairResult) * nw8);
+    // waves of the shape's stream layout (cached, else the layout's target)
+    int nwaves = 0, max_nb = 0;
+    {
+        std::lock_guard<std::mutex> lk(layout_mutex());
+        for (const StreamLayout& c : layout_cache())
+            if (c.lanes == sh.lanes && c.build == db.build && !c.wave_nblocks.empty()) {
+                nwaves = c.nwaves;
+                max_nb = *std::max_element(c.wave_nblocks.begin(), c.wave_nblocks.end());
+            }
+    }
+    if (nwaves == 0) {
+        nwaves = std::max(kWavesPerWG, target_waves());
+        max_nb = (int)std::max<int64_t>(dc.max_blocks, 16 * (int64_t)sh.lanes);
+    }
+    const int PR = sh.lanes * sh.rows;
+    const int passes = (int)((std::max<size_t>(packed ? qsum : qmax, 1) + PR - 1) / PR);
+    const int cap = std::max(1, score_resident_workgroups(sh.lanes, type == 0 ? sh.rows : std::min(sh.rows, 16),
+                                                          type == 0 ? 0 : 1));
+    const int tasks = std::max(1, nwaves / kWavesPerWG) * (packed ? 1 : (int)nq);
+    const int grid = std::min(tasks, cap);
+    pl.occupancy = (double)grid / (double)cap;
+    if (passes > 1)
+        bytes += (size_t)grid * kWavesPerWG * (size_t)(max_nb + 2) * (size_t)(64 / sh.lanes) * 2 * sizeof(uint4);
+    if (rq.align_type != COMPUTE_SCORE) {
+        int64_t dmax = 0;
+        for (uint32_t e = 0; e < N; ++e) dmax = std::max<int64_t>(dmax, db.entries[e].len);
+        bytes += (size_t)std::min<int64_t>((int64_t)rq.hitcount * (int64_t)((qmax + 7) & ~(size_t)7) * dmax,
+                                           kTraceDirBudget);
+    }
+    pl.hbm_bytes = bytes;
+    return pl;
 }
 
 int run_search_noalloc(SearchType type, p_query query, size_t hitcount, int bit_width,

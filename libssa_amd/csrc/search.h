@@ -100,6 +100,21 @@ struct HostDb {
 };
 void map_provider_db(int symtype, int strands, int gencode, HostDb& out);
 
+// The device DB for the current settings, built if needed.
+void ensure_device_db();
+
+// What one request takes on the device (the resident DB required): its
+// first-tier cells, the HBM its buffers hold while it runs, the share of the
+// chip's resident workgroup slots its first-tier grid fills (set by the
+// kernel shape's LDS and VGPR use) and that shape.
+struct RequestPlan {
+    double cells = 0;
+    size_t hbm_bytes = 0;
+    double occupancy = 0;
+    int lanes = 0, rows = 0;
+};
+RequestPlan plan_request(const SearchRequest& rq, const std::vector<size_t>& query_lengths);
+
 ssa_amd_stats_t& last_stats();
 void set_shape_override(int lanes, int rows);  // 0, 0 = chosen per query length
 void set_target_waves(int waves);             // 0 = default

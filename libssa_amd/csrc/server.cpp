@@ -42,8 +42,7 @@ double now_ms() {
 struct Ticket {
     SearchRequest rq;
     SearchResult res;
-    size_t footprint = 0;
-    double cells = 0;  // estimated GPU work
+    RequestPlan plan;  // cells, device bytes, occupancy of its first tier
     double t_submit = 0, t_start = 0, t_done = 0;
     bool done = false;
     bool claimed = false;  // a thread waits for the result
@@ -51,13 +50,52 @@ struct Ticket {
 
 }  // namespace
 
+// The admission policy (ssa_amd_admission_plan): which waiting requests a free
+// worker takes as its next batch.  Candidates in order: the oldest request
+// first once it has waited past the aging limit, then the cheapest (cells),
+// shortest first.  A candidate joins while the batch stays within the batch
+// size and cell budget and its device bytes fit what the budget leaves beside
+// the batches on the GPU (a request larger than the whole budget runs alone
+// on an idle device).  Beside running batches, a batch starts only if its
+// workgroups fit into the resident slots theirs leave free (occupancy, set
+// by the kernel shapes' LDS and VGPR use), or once the oldest request has
+// waited past the assist limit; otherwise the worker waits.
+std::vector<size_t> plan_batch(const ssa_amd_admission_request_t* q, size_t n, const ssa_amd_admission_state_t& st) {
+    std::vector<size_t> out;
+    if (n == 0) return out;
+    std::vector<size_t> order(n);
+    for (size_t i = 0; i < n; ++i) order[i] = i;
+    const bool aged = q[0].wait_ms > st.aging_ms;
+    std::stable_sort(order.begin() + (aged ? 1 : 0), order.end(),
+                     [&](size_t a, size_t b) { return q[a].cells < q[b].cells; });
+    const size_t free_bytes = st.device_budget > st.device_in_use ? st.device_budget - st.device_in_use : 0;
+    size_t bytes = 0;
+    double cells = 0, occ = 0;
+    for (size_t i : order) {
+        if (out.size() >= std::max<size_t>(st.max_batch, 1)) break;
+        const ssa_amd_admission_request_t& r = q[i];
+        if (!out.empty() && cells + r.cells > st.batch_cells) break;
+        if (bytes + r.device_bytes > free_bytes) {
+            if (out.empty() && st.running == 0) out.push_back(i);  // larger than the budget: alone, on an idle device
+            break;
+        }
+        bytes += r.device_bytes;
+        cells += r.cells;
+        occ += r.occupancy;
+        out.push_back(i);
+    }
+    if (out.empty()) return out;
+    if (st.running > 0 && st.occupancy_in_use + std::min(occ, 1.0) > 1.0 + 1e-9 && q[0].wait_ms < st.assist_ms)
+        out.clear();  // its workgroups would queue behind the running ones': wait
+    return out;
+}
+
 class Server {
 public:
     Server(int slots, size_t max_batch, size_t device_bytes) : max_batch_(std::max<size_t>(1, max_batch)) {
         size_t free_b = 0, total_b = 0;
         SSA_HIP(hipMemGetInfo(&free_b, &total_b));
         budget_ = device_bytes ? device_bytes : free_b / 2;
-        slot_budget_ = budget_ / (size_t)slots;
         for (int i = 0; i < slots; ++i) workers_.emplace_back([this] { work(); });
     }
 
@@ -78,13 +116,12 @@ public:
         if (aging_ms > 0) aging_ms_ = aging_ms;
     }
 
-    int64_t submit(const SearchRequest& rq, size_t footprint, double cells) {
+    int64_t submit(const SearchRequest& rq, const RequestPlan& plan) {
         std::lock_guard<std::mutex> lk(mu_);
         const int64_t id = next_id_++;
         Ticket& t = tickets_[id];
         t.rq = rq;
-        t.footprint = footprint;
-        t.cells = cells;
+        t.plan = plan;
         t.t_submit = now_ms();
         queue_.push_back(id);
         cv_work_.notify_one();
@@ -115,6 +152,8 @@ public:
         c->max_batch_seen = max_batch_seen_;
         c->queued = queue_.size();
         c->device_budget = budget_;
+        c->deferrals = deferrals_;
+        c->device_in_use = in_use_;
     }
 
 private:
@@ -122,43 +161,46 @@ private:
         Exec ex;
         for (;;) {
             std::vector<int64_t> batch;
+            size_t batch_bytes = 0;
+            double batch_occ = 0;
             {
                 std::unique_lock<std::mutex> lk(mu_);
                 for (;;) {
                     cv_work_.wait(lk, [&] { return stopping_ || !queue_.empty(); });
                     if (queue_.empty()) return;  // stopping with nothing left
-                    if (busy_ == 0 || stopping_) break;
-                    // another worker is on the GPU: join it once the oldest
-                    // request has waited past the assist limit (a worker
-                    // that finishes wakes the others)
-                    const double waited = now_ms() - tickets_[queue_.front()].t_submit;
-                    if (waited >= assist_ms_) break;
-                    cv_work_.wait_for(lk, std::chrono::duration<double, std::milli>(assist_ms_ - waited));
+                    const double now = now_ms();
+                    std::vector<ssa_amd_admission_request_t> q;
+                    q.reserve(queue_.size());
+                    for (int64_t id : queue_) {
+                        const Ticket& t = tickets_[id];
+                        q.push_back(ssa_amd_admission_request_t{now - t.t_submit, t.plan.cells, t.plan.hbm_bytes,
+                                                                t.plan.occupancy});
+                    }
+                    ssa_amd_admission_state_t st{max_batch_, batch_cells_, aging_ms_, assist_ms_, budget_,
+                                                 in_use_,    occ_in_use_, busy_};
+                    const std::vector<size_t> pick = plan_batch(q.data(), q.size(), st);
+                    if (!pick.empty()) {
+                        for (size_t i : pick) batch.push_back(queue_[i]);
+                        break;
+                    }
+                    // deferred: wait for a running batch to finish, or until the
+                    // oldest request reaches the assist limit
+                    ++deferrals_;
+                    const double left = std::max(1.0, assist_ms_ - q[0].wait_ms);
+                    cv_work_.wait_for(lk, std::chrono::duration<double, std::milli>(left));
                 }
-                // the oldest request first once it has waited past the aging
-                // limit, then the cheapest ones
-                size_t bytes = 0;
-                double cells = 0;
-                const double now = now_ms();
-                std::vector<int64_t> order(queue_.begin(), queue_.end());
-                const Ticket& oldest = tickets_[order.front()];
-                const bool aged = now - oldest.t_submit > aging_ms_;
-                std::stable_sort(order.begin() + (aged ? 1 : 0), order.end(),
-                                 [&](int64_t a, int64_t b) { return tickets_[a].cells < tickets_[b].cells; });
-                for (int64_t id : order) {
-                    if (batch.size() >= max_batch_) break;
-                    const Ticket& t = tickets_[id];
-                    // a request larger than the budgets runs alone
-                    if (!batch.empty() && (bytes + t.footprint > slot_budget_ || cells + t.cells > batch_cells_)) break;
-                    bytes += t.footprint;
-                    cells += t.cells;
-                    batch.push_back(id);
+                for (int64_t id : batch) {
+                    queue_.erase(std::find(queue_.begin(), queue_.end(), id));
+                    batch_bytes += tickets_[id].plan.hbm_bytes;
+                    batch_occ += tickets_[id].plan.occupancy;
                 }
-                for (int64_t id : batch) queue_.erase(std::find(queue_.begin(), queue_.end(), id));
+                batch_occ = std::min(batch_occ, 1.0);
                 const double t0 = now_ms();
                 for (int64_t id : batch) tickets_[id].t_start = t0;
                 ++batches_;
                 ++busy_;
+                in_use_ += batch_bytes;
+                occ_in_use_ += batch_occ;
                 requests_ += batch.size();
                 max_batch_seen_ = std::max<size_t>(max_batch_seen_, batch.size());
                 if (!queue_.empty()) cv_work_.notify_one();
@@ -180,6 +222,8 @@ private:
                     t.done = true;
                 }
                 --busy_;
+                in_use_ -= batch_bytes;
+                occ_in_use_ = std::max(0.0, occ_in_use_ - batch_occ);
             }
             cv_done_.notify_all();
             cv_work_.notify_all();
@@ -192,7 +236,10 @@ private:
     std::unordered_map<int64_t, Ticket> tickets_;
     std::vector<std::thread> workers_;
     size_t max_batch_;
-    size_t budget_ = 0, slot_budget_ = 0;
+    size_t budget_ = 0;       // device bytes the batches may hold together
+    size_t in_use_ = 0;       // device bytes of the batches on the GPU
+    double occ_in_use_ = 0;   // their share of the resident workgroup slots
+    size_t deferrals_ = 0;    // times a free worker waited instead of starting a batch
     double batch_cells_ = 2.5e11;  // GPU work per batch (about one average request, ~45 ms)
     double aging_ms_ = 750.0;    // a request waiting longer goes first
     double assist_ms_ = 50.0;    // a request waiting longer brings another worker onto the GPU
@@ -220,63 +267,6 @@ std::vector<size_t> query_lengths(p_query q) {
     return v;
 }
 
-// search entries per DB sequence, and their residues per DB residue
-// (src/db_adapter.c:124-138): 2 and 2 for nucleotides of both strands, 6 or 3
-// frames of a third of the length for translated DBs, else 1 and 1
-std::pair<double, double> entry_factors() {
-    const Config& cf = config();
-    if (cf.symtype == NUCLEOTIDE && (cf.strands & 2)) return {2.0, 2.0};
-    if (db_is_translated(cf.symtype)) return cf.strands == BOTH_STRANDS ? std::make_pair(6.0, 2.0) : std::make_pair(3.0, 1.0);
-    return {1.0, 1.0};
-}
-
-// GPU work of one request: cells of its first tier, twice for NW starting at
-// 8 bit (its 16-bit pass re-runs nearly every entry); residues = residues of
-// the DB's search entries
-double request_cells(const SearchRequest& rq, double residues) {
-    double qlen = 0;
-    for (size_t l : query_lengths(rq.query)) qlen += (double)l;
-    return qlen * residues * ((rq.type == kNW && rq.bit_width == BIT_WIDTH_8) ? 2.0 : 1.0);
-}
-
-// residues of the DB's search entries, and the longest DB sequence
-struct DbSize {
-    double residues = 0;
-    size_t max_len = 0;
-};
-
-DbSize db_size() {
-    DbSize d;
-    const size_t n = ssa_db_get_sequence_count();
-    for (size_t i = 0; i < n; ++i) {
-        p_seqinfo s = ssa_db_get_sequence(i);
-        if (!s) break;
-        d.residues += (double)s->seqlen;
-        d.max_len = std::max<size_t>(d.max_len, s->seqlen);
-    }
-    d.residues *= entry_factors().second;
-    return d;
-}
-
-// device bytes one request needs in a batch: pair results and pass boundaries
-// of its first tier, and the direction matrices of its hits' traceback (hits
-// against the DB's longest sequence, at most one traceback launch's budget:
-// the hits of a batch run in launches that fit it)
-size_t request_footprint(const SearchRequest& rq, size_t db_max_len) {
-    const size_t n = ssa_db_get_sequence_count();
-    const size_t entries = (size_t)((double)n * entry_factors().first);
-    const std::vector<size_t> ql = query_lengths(rq.query);
-    const size_t nq = ql.size();
-    size_t qlen = 0;
-    for (size_t l : ql) qlen = std::max(qlen, l);
-    const size_t mean_len = 400;  // a planning figure, not a limit
-    size_t bytes = nq * entries * sizeof(PairResult) + nq * entries * (mean_len / 4) * 16;
-    if (rq.align_type != COMPUTE_SCORE)
-        bytes += std::min<size_t>(rq.hitcount * ((qlen + 7) & ~(size_t)7) * std::max<size_t>(db_max_len, 1),
-                                  (size_t)1 << 30);
-    return bytes;
-}
-
 }  // namespace ssa
 
 using namespace ssa;
@@ -285,7 +275,6 @@ extern "C" {
 
 struct ssa_amd_server {
     Server* s;
-    DbSize db;  // of the DB at start
 };
 
 ssa_amd_server* ssa_amd_server_start(int slots, size_t max_batch, size_t device_bytes) {
@@ -293,8 +282,8 @@ ssa_amd_server* ssa_amd_server_start(int slots, size_t max_batch, size_t device_
     if (!matrix().valid) return nullptr;
     const Config& c = config();
     if (!matrix().constant && c.symtype == NUCLEOTIDE) return nullptr;
+    ensure_device_db();  // resident before any request is planned against it
     ssa_amd_server* h = new ssa_amd_server;
-    h->db = db_size();
     h->s = new Server(slots, max_batch, device_bytes);
     return h;
 }
@@ -307,7 +296,7 @@ int64_t ssa_amd_server_submit(ssa_amd_server* h, const ssa_amd_request_t* r) {
     if (r->align_type != COMPUTE_SCORE && r->align_type != COMPUTE_ALIGNMENT) return -1;
     if (r->bit_width == BIT_WIDTH_8 && config().simd_mode < COMPUTE_ON_SSE41) return -1;
     const SearchRequest rq{r->search_type == 0 ? kSW : kNW, r->query, r->hitcount, r->bit_width, r->align_type};
-    return h->s->submit(rq, request_footprint(rq, h->db.max_len), request_cells(rq, h->db.residues));
+    return h->s->submit(rq, plan_request(rq, query_lengths(rq.query)));
 }
 
 p_alignment_list ssa_amd_server_wait(ssa_amd_server* h, int64_t ticket, ssa_amd_stats_t* stats, double* latency_ms,
@@ -325,6 +314,14 @@ void ssa_amd_server_tune(ssa_amd_server* h, double batch_cells, double aging_ms)
 
 void ssa_amd_server_counters(ssa_amd_server* h, ssa_amd_server_counters_t* out) {
     if (h && out) h->s->counters(out);
+}
+
+size_t ssa_amd_admission_plan(const ssa_amd_admission_request_t* queue, size_t n,
+                              const ssa_amd_admission_state_t* state, size_t* chosen) {
+    if (!queue || !state || !chosen || n == 0) return 0;
+    const std::vector<size_t> pick = plan_batch(queue, n, *state);
+    std::copy(pick.begin(), pick.end(), chosen);
+    return pick.size();
 }
 
 void ssa_amd_server_stop(ssa_amd_server* h) {
