@@ -170,7 +170,9 @@ def cpu_baseline(w, runs=5):
 
 def copy_probe(mib):
     """One hipMemcpy device-to-device of a known size: the reference point that
-    turns the FETCH_SIZE / WRITE_SIZE counters of a profile into bytes."""
+    turns the FETCH_SIZE / WRITE_SIZE counters of a profile into bytes.  Run the
+    profile with HSA_ENABLE_SDMA=0, so that the copy is a kernel the counters
+    see rather than a DMA-engine transfer."""
     hip = C.CDLL("libamdhip64.so")
     n = C.c_size_t(mib << 20)
     a, b = C.c_void_p(), C.c_void_p()
