@@ -18,7 +18,9 @@ import os as _os
 from dataclasses import dataclass as _dataclass
 
 _HERE = _os.path.dirname(_os.path.abspath(__file__))
-LIB_PATH = _os.path.join(_HERE, "libssa_amd.so")
+# SSA_AMD_LIB names another build of the library to load (the sanitizer
+# build of the host code, for instance)
+LIB_PATH = _os.environ.get("SSA_AMD_LIB") or _os.path.join(_HERE, "libssa_amd.so")
 
 # ---- constants (src/libssa.h:29-72) ----
 BLOSUM45, BLOSUM50, BLOSUM62, BLOSUM80, BLOSUM90 = "blosum45", "blosum50", "blosum62", "blosum80", "blosum90"

@@ -60,7 +60,7 @@ def test_constant_scores_first_argument_is_match():
 @pytest.mark.parametrize("name,count", [("one_seq.fas", 1), ("AF091148.fas", 1403), ("test.fas", 5)])
 def test_fasta_provider_counts_and_ids(name, count):
     # tests/test_libssa_extern_db.c:24-58
-    lib = ssa.lib
+    lib = C.CDLL(ssa.LIB_PATH)  # its own function objects: the package's keep their types
     lib.ssa_db_get_sequence.restype = C.c_void_p
     lib.ssa_db_get_sequence.argtypes = [C.c_size_t]
     assert lib.ssa_db_init(O.data(name).encode()) == 0
