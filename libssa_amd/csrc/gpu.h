@@ -152,6 +152,13 @@ hipError_t launch_exact(bool wide, int npairs, hipStream_t st, const ExactPair* 
 hipError_t launch_traceback_wave(int ntasks, hipStream_t st, const TraceTask* tasks, const uint8_t* codes,
                                  const int64_t* mat, int go, int ge, uint8_t* dirs, int64_t* scratch, int32_t* ops,
                                  TraceResult* out);
+// workgroup traceback: a hit's passes of 64 * rows_per_lane query rows (1, 2,
+// 4 or 8) spread over `waves` waves (1-16); at most traceback_wg_max_passes()
+// passes per dynamic programme
+hipError_t launch_traceback_wg(int rows_per_lane, int waves, int ntasks, hipStream_t st, const TraceTask* tasks,
+                               const uint8_t* codes, const int64_t* mat, int go, int ge, uint8_t* dirs,
+                               int64_t* scratch, int32_t* ops, TraceResult* out);
+int traceback_wg_max_passes();
 hipError_t launch_traceback(int ntasks, hipStream_t st, const TraceTask* tasks, const uint8_t* codes,
                             const int64_t* mat, int go, int ge, uint8_t* dirs, int64_t* scratch, int32_t* ops,
                             TraceResult* out);

@@ -1333,10 +1333,62 @@ std::vector<uint8_t> prepared_db_sequence(p_seqinfo s, int strand_field, int fra
 // hits a batch asks for.
 constexpr int64_t kTraceDirBudget = (int64_t)1 << 30;
 
+// Shape of the workgroup traceback for a hit of qlen query rows and dlen DB
+// columns: rows per lane (1, 2, 4, 8) and waves (1-16), the one of least
+// modelled time -- phases of the pass schedule (passes start kTgL phases
+// apart, a wave takes its next pass once it has finished the one before) times
+// a step's cost (a fixed part for the carries and loads, a part per row).
+struct TbShape {
+    int rows = 8, waves = 1;
+};
+
+TbShape traceback_shape(int qlen, int dlen) {
+    constexpr int kS = 16, kL = 5;  // as the kernel's phase width and pass lag
+    const int nblk = (dlen + 63 + kS - 1) / kS;
+    TbShape best;
+    double best_cost = 1e300;
+    std::vector<int> start;
+    for (int R : {8, 4, 2, 1}) {
+        const int P = std::max(1, (qlen + 64 * R - 1) / (64 * R));
+        if (P > traceback_wg_max_passes()) continue;
+        const int W = std::min(16, P);
+        start.assign(P, 0);
+        for (int p = 1; p < P; ++p) {
+            start[p] = start[p - 1] + kL;
+            if (p >= W) start[p] = std::max(start[p], start[p - W] + nblk);
+        }
+        const double cost = (double)(start[P - 1] + nblk) * kS * (12.0 + 10.0 * R);
+        if (cost < best_cost) {
+            best_cost = cost;
+            best = TbShape{R, W};
+        }
+    }
+    return best;
+}
+
 // Regions and CIGARs of the hits listed (types[k] = the search type of al[k]).
-void gpu_traceback(Exec& gpu, const std::vector<SearchType>& types, std::vector<p_alignment>& al, double* ms) {
-    if (al.empty()) return;
+void gpu_traceback(Exec& gpu, const std::vector<SearchType>& types, std::vector<p_alignment>& al_in, double* ms) {
+    if (al_in.empty()) return;
     const Config& cf = config();
+    // hits in order of their workgroup shape, so that each run of one shape is one launch
+    std::vector<TbShape> shape_of(al_in.size());
+    std::vector<size_t> order(al_in.size());
+    for (size_t k = 0; k < al_in.size(); ++k) {
+        shape_of[k] = traceback_shape((int)al_in[k]->query.len, (int)al_in[k]->db_seq.len);
+        order[k] = k;
+    }
+    std::stable_sort(order.begin(), order.end(), [&](size_t x, size_t y) {
+        return shape_of[x].rows != shape_of[y].rows ? shape_of[x].rows > shape_of[y].rows
+                                                     : shape_of[x].waves > shape_of[y].waves;
+    });
+    std::vector<p_alignment> al(al_in.size());
+    std::vector<TbShape> shapes(al_in.size());
+    std::vector<SearchType> ty(al_in.size());
+    for (size_t k = 0; k < order.size(); ++k) {
+        al[k] = al_in[order[k]];
+        shapes[k] = shape_of[order[k]];
+        ty[k] = types[order[k]];
+    }
     std::vector<uint8_t> codes;
     std::vector<TraceTask> tasks(al.size());
     std::vector<int64_t> dir_bytes(al.size());
@@ -1367,7 +1419,7 @@ void gpu_traceback(Exec& gpu, const std::vector<SearchType>& types, std::vector<
         t.ops_off = ops_total;
         t.ops_cap = t.qlen + t.dlen + 1;
         ops_total += t.ops_cap;
-        t.type = types[k] == kSW ? 0 : 1;
+        t.type = ty[k] == kSW ? 0 : 1;
     }
     // launches: consecutive hits whose direction matrices fit the budget
     std::vector<std::pair<size_t, size_t>> launches;  // [first, last) hit
@@ -1395,15 +1447,21 @@ void gpu_traceback(Exec& gpu, const std::vector<SearchType>& types, std::vector<
     gpu.upload(d_codes, codes.data(), codes.size());
     gpu.upload(d_tasks, tasks.data(), tasks.size() * sizeof(TraceTask));
     gpu.upload(d_mat, matrix().s, kDim * kDim * sizeof(int64_t));
-    // launches in stream order share the direction arena
+    // launches in stream order share the direction arena; within one, a
+    // workgroup launch per run of hits of one shape
     for (const auto& lr : launches) {
-        const int n = (int)(lr.second - lr.first);
-        if (wave_ok)
-            SSA_HIP(launch_traceback_wave(n, gpu.stream(), d_tasks + lr.first, d_codes, d_mat, cf.gap_open,
-                                          cf.gap_extend, d_dirs, d_hh, d_ops, d_res + lr.first));
-        else
-            SSA_HIP(launch_traceback(n, gpu.stream(), d_tasks + lr.first, d_codes, d_mat, cf.gap_open,
-                                     cf.gap_extend, d_dirs, d_hh, d_ops, d_res + lr.first));
+        if (!wave_ok) {
+            SSA_HIP(launch_traceback((int)(lr.second - lr.first), gpu.stream(), d_tasks + lr.first, d_codes, d_mat,
+                                     cf.gap_open, cf.gap_extend, d_dirs, d_hh, d_ops, d_res + lr.first));
+            continue;
+        }
+        for (size_t k = lr.first; k < lr.second;) {
+            size_t e = k;
+            while (e < lr.second && shapes[e].rows == shapes[k].rows && shapes[e].waves == shapes[k].waves) ++e;
+            SSA_HIP(launch_traceback_wg(shapes[k].rows, shapes[k].waves, (int)(e - k), gpu.stream(), d_tasks + k,
+                                        d_codes, d_mat, cf.gap_open, cf.gap_extend, d_dirs, d_hh, d_ops, d_res + k));
+            k = e;
+        }
     }
     std::vector<TraceResult> res(tasks.size());
     std::vector<int32_t> ops((size_t)ops_total * 2);
