@@ -47,6 +47,8 @@ typedef struct {
                                  instantiating) its executable, host clock */
     int graph_reused;         /* 1 when the batch ran its context's previous executable graph,
                                  updated to the batch's arguments */
+    int columns_per_step;     /* first tier's column tile: 4 or 8 DB columns per step */
+    int profile_placement;    /* first tier's query profile: 1 = LDS, 2 = VGPRs */
 } ssa_amd_stats_t;
 
 /* Counters of the last search. */
@@ -65,6 +67,15 @@ void ssa_amd_reset_settings(void);
  * rows); NW searches run a 32-row shape with 16 rows; 0, 0 chooses from the
  * query length. */
 void ssa_amd_set_kernel_shape(int lanes_per_stream, int rows_per_lane);
+
+/* Variant of the score kernel: the column tile -- DB columns a lane scores
+ * per step, 4 (one block of the reference's 4-column unit) or 8 (two blocks
+ * per step, shapes of at most 16 rows per lane in groups of 8 or 16 lanes) --
+ * and where the query profile sits: 1 = LDS, 2 = VGPRs (nucleotide
+ * databases of A/C/G/T only, 16 or fewer rows per lane).  0 leaves the choice
+ * to the cost model or the autotuner; a forced variant a shape lacks is left
+ * at its default for that shape.  Results do not depend on the variant. */
+void ssa_amd_set_kernel_variant(int columns_per_step, int profile_placement);
 
 /* Number of waves the DB's pair streams are cut into (0 = the default, 10240).
  * Streams never get shorter than 16 blocks per lane of a group. */

@@ -99,7 +99,8 @@ class Stats(_C.Structure):
                 ("overflow_tracking", _C.c_int),
                 ("ms_gpu_first", _C.c_double), ("ms_gpu_requeue", _C.c_double), ("ms_replay", _C.c_double),
                 ("ms_traceback", _C.c_double), ("ms_total", _C.c_double), ("d2h_bytes", _C.c_uint64),
-                ("replayed_chunks", _C.c_uint64), ("ms_graph", _C.c_double), ("graph_reused", _C.c_int)]
+                ("replayed_chunks", _C.c_uint64), ("ms_graph", _C.c_double), ("graph_reused", _C.c_int),
+                ("columns_per_step", _C.c_int), ("profile_placement", _C.c_int)]
 
 
 @_dataclass
@@ -137,6 +138,7 @@ def _load():
         "ssa_db_get_sequence": (_C.POINTER(_SeqInfo), [S]), "ssa_amd_db_pack": (I, [cp]),
         "ssa_amd_last_stats": (None, [_C.POINTER(Stats)]), "ssa_amd_reset_settings": (None, []),
         "ssa_amd_set_kernel_shape": (None, [I, I]), "ssa_amd_set_stream_waves": (None, [I]),
+        "ssa_amd_set_kernel_variant": (None, [I, I]),
         "ssa_amd_get_matrix": (None, [P]), "ssa_amd_device_arch": (cp, []),
         "ssa_amd_batch_align": (I, [_C.POINTER(_Request), S, _C.POINTER(_C.POINTER(_AlignmentList)),
                                     _C.POINTER(Stats)]),
@@ -242,6 +244,12 @@ def reset_settings():
 
 
 def set_kernel_shape(lanes_per_stream, rows_per_lane): lib.ssa_amd_set_kernel_shape(lanes_per_stream, rows_per_lane)
+
+
+def set_kernel_variant(columns_per_step=0, profile_placement=0):
+    """Column tile (4 or 8 DB columns per step) and profile placement (1 LDS,
+    2 VGPRs) of the score kernel; 0 = chosen per search."""
+    lib.ssa_amd_set_kernel_variant(columns_per_step, profile_placement)
 
 
 def set_stream_waves(waves): lib.ssa_amd_set_stream_waves(waves)

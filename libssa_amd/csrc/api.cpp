@@ -155,6 +155,15 @@ void ssa_amd_set_kernel_shape(int lanes_per_stream, int rows_per_lane) {
     set_shape_override(lanes_per_stream, rows_per_lane);
 }
 
+void ssa_amd_set_kernel_variant(int columns_per_step, int profile_placement) {
+    if (columns_per_step != 0 && columns_per_step != 4 && columns_per_step != 8)
+        fatal("libssa_amd: columns per step must be 0, 4 or 8 (got %d)", columns_per_step);
+    if (profile_placement < 0 || profile_placement > 2)
+        fatal("libssa_amd: profile placement must be 0, 1 (LDS) or 2 (VGPRs) (got %d)", profile_placement);
+    set_cols_override(columns_per_step);
+    set_placement_override(profile_placement);
+}
+
 void ssa_amd_set_stream_waves(int waves) {
     if (waves < 0) fatal("libssa_amd: stream waves must be >= 0 (got %d)", waves);
     set_target_waves(waves);

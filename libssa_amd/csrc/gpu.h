@@ -131,8 +131,8 @@ private:
 };
 
 // ---- kernel launchers (score_kernels.hip, traceback_kernels.hip, pack_kernels.hip) ----
-bool score_shape_supported(int lanes, int rows, int mode);
-int score_resident_workgroups(int lanes, int rows, int mode);  // persistent grid size of a shape
+bool score_shape_supported(int lanes, int rows, int mode, int cols);
+int score_resident_workgroups(int lanes, int rows, int mode, int cols);  // persistent grid size of a shape
 // NW shapes with an untracked variant (mode 2: no value of any pair can reach
 // either end of the int16 range)
 bool score_nw_untracked_supported(int lanes, int rows);
@@ -141,8 +141,8 @@ bool score_pair_supported(int lanes, int rows);
 int score_pair_resident_workgroups(int lanes, int rows);
 hipError_t launch_score_pair(int lanes, int rows, hipStream_t st, const ScoreLaunch& L, const ScoreParams& P,
                              int max_wg);
-hipError_t launch_score(int lanes, int rows, int mode, hipStream_t st, const ScoreLaunch& L, const ScoreParams& P,
-                        int max_wg);
+hipError_t launch_score(int lanes, int rows, int mode, int cols, hipStream_t st, const ScoreLaunch& L,
+                        const ScoreParams& P, int max_wg);
 hipError_t launch_init_results(hipStream_t st, PairResult* r, int64_t n, int32_t score);
 // exact tier: one wave per pair, int64 (wide) or int32; bnd holds each pair's
 // pass boundaries (elements of that width, at ExactPair::bnd_off)

@@ -178,7 +178,21 @@ __device__ __forceinline__ void finish_pair(PairResult* results, int pair, bool 
 // packed queries; the staged profile holds the two queries' rows interleaved,
 // one dword (lo, hi) per row, so no v_perm is needed, and each half has its
 // own segment table.
-template <int R, int G, int MODE, bool MULTI, bool SEG, bool PAIR, class Stage>
+//
+// NB (the column tile): blocks per step, 1 or 2 (4 or 8 columns).  A step
+// runs its NB blocks one after the other with everything that belongs to a
+// block -- pair starts, the overflow predicates and first flagged block
+// (still per 4-column block, the reference's unit), pass boundaries, pair
+// ends -- and pays the per-step work (residue and boundary loads of the next
+// step, pass bookkeeping) once; the lane above hands down the bottom row of
+// all 4 * NB columns.  Pass lengths are whole numbers of steps.
+template <int NB>
+struct StepIn {
+    uint2 w[NB];      // residue words of the step's blocks
+    uint4 t[2 * NB];  // top lane: the previous pass's bottom row over them
+};
+
+template <int R, int G, int MODE, int NB, bool MULTI, bool SEG, bool PAIR, class Stage>
 __device__ __forceinline__ void run_task(const ScoreParams& P, uint32_t gbase0, const char* sp,
                                          const uint32_t* slbh0, const uint32_t* slbe0,
                                          const uint2* __restrict__ wres, uint4* __restrict__ wbnd,
@@ -201,7 +215,7 @@ __device__ __forceinline__ void run_task(const ScoreParams& P, uint32_t gbase0, 
 
     const bool top_lane = g == 0;
     // the lane's pass and block within it (negative before the lane's first block)
-    int pass = 0, b = -g;
+    int pass = 0, b = -NB * g;
     // SEG: the lane's segment in its pass, per half (a plain query: one
     // segment per pass); seg_top holds the halves that start a query there
     int po_lo = pair_off, po_hi = pair_off;
@@ -246,279 +260,299 @@ __device__ __forceinline__ void run_task(const ScoreParams& P, uint32_t gbase0, 
         hb[c] = as_v(P.hb_init[c]);
         fb[c] = as_v(P.fb_init[c]);
     }
-    uint32_t obh[4] = {0, 0, 0, 0}, obf[4] = {0, 0, 0, 0};  // bottom row H, F of the last block
+    uint32_t obh[4 * NB], obf[4 * NB];  // bottom row H, F of the last step's blocks
+#pragma unroll
+    for (int c = 0; c < 4 * NB; ++c) {
+        obh[c] = 0u;
+        obf[c] = 0u;
+    }
     uint32_t TA = TA_off, TB = TB_off;
-    uint32_t scout = as_u(Z);  // SW: max over this lane and the lanes above for the pair it finished last
+    // SW: max over this lane and the lanes above for the pair it finished
+    // last in block j of a step (the lane below reads entry j one step later)
+    uint32_t scout[NB];
+#pragma unroll
+    for (int j = 0; j < NB; ++j) scout[j] = as_u(Z);
     int kp_lo = 0, kp_hi = 0, pair_lo = -1, pair_hi = -1, blk_lo = 0, blk_hi = 0;
     int fa_lo = kNoFlag, fa_hi = kNoFlag, fb_lo = kNoFlag, fb_hi = kNoFlag;
 
     // FIRST: the top lane is in the first pass and takes the matrix border
-    auto step = [&](auto first_tag, const uint2 w, const uint4 t0, const uint4 t1, uint2& wn, uint4& tn0,
-                    uint4& tn1) {
+    auto step = [&](auto first_tag, const StepIn<NB>& in, StepIn<NB>& nx) {
         constexpr bool FIRST = decltype(first_tag)::value;
         // profile dwords of the current column (lo, hi halves); column 0's now
         // (PAIR: pl holds one dword per row, both halves' scores)
         uint32_t pl[PAIR ? R : R / 2], ph[PAIR ? 1 : R / 2];
         if constexpr (PAIR) {
 #pragma unroll
-            for (int q = 0; q < (R >= 8 ? R / 8 : 1); ++q) read_part_pair<R>(sp + colofs(w.x, 0), pl, q);
+            for (int q = 0; q < (R >= 8 ? R / 8 : 1); ++q) read_part_pair<R>(sp + colofs(in.w[0].x, 0), pl, q);
         } else {
-            read_rows<R>(sp + colofs(w.x, 0), pl);
-            read_rows<R>(sp + colofs(w.y, 0), ph);
+            read_rows<R>(sp + colofs(in.w[0].x, 0), pl);
+            read_rows<R>(sp + colofs(in.w[0].y, 0), ph);
         }
         const bool lane_last = pass == npass - 1;
-        const bool valid = pass < npass && (unsigned)b < (unsigned)nbw;
         const bool bot_out = g == G - 1 && pass < npass - 1;  // bottom row to the next pass
         const bool score_lo = MODE == 0 ? bot_lo : (lane_last && g == last_g);
         const bool score_hi = MODE == 0 ? bot_hi : (lane_last && g == last_g);
-        // the lane's next block, in this pass or (MULTI) at the start of the next
-        const bool wrap = MULTI && b + 1 == nbp;
-        const int b_n = wrap ? 0 : b + 1, pass_n = wrap ? pass + 1 : pass;
-        if ((!MULTI || pass_n < npass) && (unsigned)b_n < (unsigned)nbw) {
-            const uint32_t o = (uint32_t)(b_n * NG + k);
-            wn = wres[o];
-            if (MULTI && top_lane && pass_n > 0) {
-                tn0 = wbnd[2 * o];
-                tn1 = wbnd[2 * o + 1];
-            }
-        } else {
-            wn = make_uint2(0, 0);
-        }
-
-        // the wave-level tests read both halves' flags with one or; the
-        // per-half flags are only taken inside the rare branches
-        if (__any(((w.x | w.y) & kResStart) != 0)) {
-            const bool st_lo = (w.x & kResStart) != 0, st_hi = (w.y & kResStart) != 0;
-            const uint32_t m = (st_lo ? 0x0000ffffu : 0u) | (st_hi ? 0xffff0000u : 0u);
-            if (st_lo) {
-                pair_lo = none_lo ? -1 : po_lo + slot_entries[sb_lo + kp_lo];
-                kp_lo++;
-                blk_lo = 0;
-                fa_lo = kNoFlag;
-                fb_lo = kNoFlag;
-            }
-            if (st_hi) {
-                pair_hi = none_hi ? -1 : po_hi + slot_entries[sb_hi + kp_hi];
-                kp_hi++;
-                blk_hi = 0;
-                fa_hi = kNoFlag;
-                fb_hi = kNoFlag;
-            }
-            TA = (m & TA_on) | (~m & TA);
-            TB = (m & TB_on) | (~m & TB);
-            if (MODE == 0) {
+        // the lane's next step, in this pass or (MULTI) at the start of the next
+        const bool wrap = MULTI && b + NB == nbp;
+        const int b_n = wrap ? 0 : b + NB, pass_n = wrap ? pass + 1 : pass;
 #pragma unroll
-                for (int r = 0; r < R; ++r) {
-                    Hl[r] = sel(m, Z, Hl[r]);
-                    El[r] = sel(m, Z, El[r]);
+        for (int j = 0; j < NB; ++j) {
+            if ((!MULTI || pass_n < npass) && (unsigned)(b_n + j) < (unsigned)nbw) {
+                const uint32_t o = (uint32_t)((b_n + j) * NG + k);
+                nx.w[j] = wres[o];
+                if (MULTI && top_lane && pass_n > 0) {
+                    nx.t[2 * j] = wbnd[2 * o];
+                    nx.t[2 * j + 1] = wbnd[2 * o + 1];
                 }
-                S = sel(m, Z, S);
-                htop = sel(m, Z, htop);
             } else {
-                const int r0 = g * R;
-#pragma unroll
-                for (int r = 0; r < R; ++r) {
-                    Hl[r] = sel(m, as_v(slbh[1 + r0 + r]), Hl[r]);
-                    El[r] = sel(m, as_v(slbe[r0 + r]), El[r]);
-                }
-                htop = sel(m, as_v(slbh[r0]), htop);  // row -1 of the first pass: the corner, 0
-                if (FIRST) {
-#pragma unroll
-                    for (int c = 0; c < 4; ++c) {
-                        hb[c] = sel(m, as_v(P.hb_init[c]), hb[c]);
-                        fb[c] = sel(m, as_v(P.fb_init[c]), fb[c]);
-                    }
-                }
+                nx.w[j] = make_uint2(0, 0);
             }
         }
 
-        // top boundary of this block: the lane above, one step earlier; the
-        // group's top lane takes the previous pass's bottom row or the matrix
-        // border.  tbh[3] also carries the next block's diagonal.
-        uint32_t toph[4], topf[4];
-        if (!FIRST) {
-            toph[0] = t0.x;
-            topf[0] = t0.y;
-            toph[1] = t0.z;
-            topf[1] = t0.w;
-            toph[2] = t1.x;
-            topf[2] = t1.y;
-            toph[3] = t1.z;
-            topf[3] = t1.w;
-        } else if (MODE == 0) {
 #pragma unroll
-            for (int c = 0; c < 4; ++c) {
-                toph[c] = as_u(Z);
-                topf[c] = as_u(Z);
-            }
-        } else {
-            toph[0] = as_u(hb[1]);
-            toph[1] = as_u(hb[2]);
-            toph[2] = as_u(hb[3]);
-            toph[3] = as_u(adds(hb[3], RR));
-#pragma unroll
-            for (int c = 0; c < 4; ++c) topf[c] = as_u(adds(fb[c], QR));
-        }
-        uint32_t tbh[4], tbf[4];
-#pragma unroll
-        for (int c = 0; c < 4; ++c) {
-            tbh[c] = shift_down<G>(toph[c], obh[c], top_lane);
-            tbf[c] = shift_down<G>(topf[c], obf[c], top_lane);
-            if (SEG) {  // a packed query's first row: the SW matrix border
-                tbh[c] = (seg_top & as_u(Z)) | (~seg_top & tbh[c]);
-                tbf[c] = (seg_top & as_u(Z)) | (~seg_top & tbf[c]);
-            }
-        }
+        for (int j = 0; j < NB; ++j) {
+            const uint2 w = in.w[j];
+            const uint4 t0 = in.t[2 * j], t1 = in.t[2 * j + 1];
+            const bool valid = pass < npass && (unsigned)(b + j) < (unsigned)nbw;
 
-        s16x2 hmn = as_v(0u), hmx = as_v(0u);  // NW trackers, reset value of the reference
-        const bool any_last = __any(((w.x | w.y) & kResLast) != 0);
-        const int z_lo = (int)((w.x >> kResZShift) & 3u), z_hi = (int)((w.y >> kResZShift) & 3u);
-        // the block's four columns; CAP also records the NW score cell (the
-        // last query row at the pair's last column), a select per row on a
-        // wave-uniform condition, so the loop stays straight-line code
-        auto columns = [&](auto cap_tag) {
-            constexpr bool CAP = decltype(cap_tag)::value;
-            uint32_t cm_lo = 0u, cm_hi = 0u;
-            if (CAP && g == last_g && lane_last) {
-                cm_lo = (w.x & kResLast) ? 0x0000ffffu : 0u;
-                cm_hi = (w.y & kResLast) ? 0xffff0000u : 0u;
-            }
-            // The cell statements are written so that every dependent packed
-            // op has an independent one in front of it (gfx950 needs a wait
-            // state between a packed op and a packed op that reads its
-            // result).  The order is not pinned with scheduling barriers:
-            // instruction selection already reorders pure ops, and left free
-            // the post-RA scheduler removes most of the s_nop padding (what
-            // it leaves is hidden by the other waves of the SIMD).
+            // the wave-level tests read both halves' flags with one or; the
+            // per-half flags are only taken inside the rare branches
+            if (__any(((w.x | w.y) & kResStart) != 0)) {
+                const bool st_lo = (w.x & kResStart) != 0, st_hi = (w.y & kResStart) != 0;
+                const uint32_t m = (st_lo ? 0x0000ffffu : 0u) | (st_hi ? 0xffff0000u : 0u);
+                if (st_lo) {
+                    pair_lo = none_lo ? -1 : po_lo + slot_entries[sb_lo + kp_lo];
+                    kp_lo++;
+                    blk_lo = 0;
+                    fa_lo = kNoFlag;
+                    fb_lo = kNoFlag;
+                }
+                if (st_hi) {
+                    pair_hi = none_hi ? -1 : po_hi + slot_entries[sb_hi + kp_hi];
+                    kp_hi++;
+                    blk_hi = 0;
+                    fa_hi = kNoFlag;
+                    fb_hi = kNoFlag;
+                }
+                TA = (m & TA_on) | (~m & TA);
+                TB = (m & TB_on) | (~m & TB);
+                if (MODE == 0) {
 #pragma unroll
-            for (int c = 0; c < 4; ++c) {
-                s16x2 f = as_v(tbf[c]);
-                s16x2 fR = adds(f, RR);
-                s16x2 v;
-                if constexpr (PAIR) v = as_v(pl[0]);
-                else v = pair_row(pl, ph, 0);
-                s16x2 h1 = adds(htop, v);  // row 0: diagonal H from the row above
-                htop = as_v(tbh[c]);
-                s16x2 hq = Z;
-#pragma unroll
-                for (int r = 0; r < R; ++r) {
-                    const bool more = r + 1 < R;
-                    if (more) {
-                        if constexpr (PAIR) v = as_v(pl[r + 1 < R ? r + 1 : 0]);
-                        else v = pair_row(pl, ph, r + 1);
+                    for (int r = 0; r < R; ++r) {
+                        Hl[r] = sel(m, Z, Hl[r]);
+                        El[r] = sel(m, Z, El[r]);
                     }
-                    // the next column's profile rows, once this column's dwords are consumed
-                    if (c < 3 && ((R == 4 && r == 3) || (R >= 8 && r % 8 == 7))) {
-                        if constexpr (PAIR) {
-                            read_part_pair<R>(sp + colofs(w.x, c + 1), pl, r / 8);
-                        } else {
-                            read_part<R>(sp + colofs(w.x, c + 1), pl, r / 8);
-                            read_part<R>(sp + colofs(w.y, c + 1), ph, r / 8);
+                    S = sel(m, Z, S);
+                    htop = sel(m, Z, htop);
+                } else {
+                    const int r0 = g * R;
+#pragma unroll
+                    for (int r = 0; r < R; ++r) {
+                        Hl[r] = sel(m, as_v(slbh[1 + r0 + r]), Hl[r]);
+                        El[r] = sel(m, as_v(slbe[r0 + r]), El[r]);
+                    }
+                    htop = sel(m, as_v(slbh[r0]), htop);  // row -1 of the first pass: the corner, 0
+                    if (FIRST) {
+#pragma unroll
+                        for (int c = 0; c < 4; ++c) {
+                            hb[c] = sel(m, as_v(P.hb_init[c]), hb[c]);
+                            fb[c] = sel(m, as_v(P.fb_init[c]), fb[c]);
                         }
                     }
-                    s16x2 h = vmax(h1, f);
-                    const s16x2 eR = adds(El[r], RR);
-                    h = vmax(h, El[r]);
-                    if (more) h1 = adds(Hl[r], v);  // next row's diagonal term
-                    const s16x2 tt = adds(h, QR);
-                    if (MODE == 0) {
-                        S = vmax(S, h);
-                    } else if (MODE == 1) {
-                        hmn = vmin(hmn, h);
-                        hmx = vmax(hmx, h);
-                    }
-                    f = vmax(fR, tt);
-                    El[r] = vmax(eR, tt);
-                    if (more) fR = adds(f, RR);
-                    Hl[r] = h;
-                    if (CAP && r == last_r) hq = h;
-                }
-                obh[c] = as_u(Hl[R - 1]);
-                obf[c] = as_u(f);
-                if (CAP) {
-                    const uint32_t cm = (z_lo == c ? cm_lo : 0u) | (z_hi == c ? cm_hi : 0u);
-                    cap = sel(cm, hq, cap);
                 }
             }
-        };
-        // only the lane holding the last query row records a score cell
-        if (MODE >= 1 && __any(lane_last && g == last_g && ((w.x | w.y) & kResLast) != 0)) columns(Tag<true>{});
-        else columns(Tag<false>{});
-        // overflow predicates at the end of the block: a packed test here (in
-        // the cell loop's block, so the running reductions stay in the loop),
-        // the exact per-half bookkeeping below only where a predicate fired
-        bool fire;
-        if (MODE == 0) {
-            fire = as_u(vmax(S, as_v(TA))) != TA || as_u(vmax(S, as_v(TB))) != TB;
-        } else if (MODE == 1) {
-            fire = as_u(vmin(hmn, as_v(TA))) != TA || as_u(vmax(hmx, as_v(TB))) != TB;
-        } else {
-            fire = false;
-        }
-        if (__any(fire)) {
-            if (MODE == 0) {
-                if (pair_lo >= 0) {
-                    const int v = lo16(S);
-                    if (v >= P.thr_a && fa_lo == kNoFlag) fa_lo = blk_lo;
-                    if (v >= P.thr_b && fb_lo == kNoFlag) fb_lo = blk_lo;
-                }
-                if (pair_hi >= 0) {
-                    const int v = hi16(S);
-                    if (v >= P.thr_a && fa_hi == kNoFlag) fa_hi = blk_hi;
-                    if (v >= P.thr_b && fb_hi == kNoFlag) fb_hi = blk_hi;
+
+            // top boundary of this block: the lane above, one step earlier; the
+            // group's top lane takes the previous pass's bottom row or the matrix
+            // border.  tbh[3] also carries the next block's diagonal.
+            uint32_t toph[4], topf[4];
+            if (!FIRST) {
+                toph[0] = t0.x;
+                topf[0] = t0.y;
+                toph[1] = t0.z;
+                topf[1] = t0.w;
+                toph[2] = t1.x;
+                topf[2] = t1.y;
+                toph[3] = t1.z;
+                topf[3] = t1.w;
+            } else if (MODE == 0) {
+#pragma unroll
+                for (int c = 0; c < 4; ++c) {
+                    toph[c] = as_u(Z);
+                    topf[c] = as_u(Z);
                 }
             } else {
-                if (pair_lo >= 0 && fa_lo == kNoFlag && (lo16(hmn) < P.thr_a || lo16(hmx) >= P.thr_b))
-                    fa_lo = blk_lo;
-                if (pair_hi >= 0 && fa_hi == kNoFlag && (hi16(hmn) < P.thr_a || hi16(hmx) >= P.thr_b))
-                    fa_hi = blk_hi;
+                toph[0] = as_u(hb[1]);
+                toph[1] = as_u(hb[2]);
+                toph[2] = as_u(hb[3]);
+                toph[3] = as_u(adds(hb[3], RR));
+#pragma unroll
+                for (int c = 0; c < 4; ++c) topf[c] = as_u(adds(fb[c], QR));
             }
-            const uint32_t offa = (fa_lo != kNoFlag ? 0x0000ffffu : 0u) | (fa_hi != kNoFlag ? 0xffff0000u : 0u);
-            const uint32_t offb =
-                MODE == 0 ? ((fb_lo != kNoFlag ? 0x0000ffffu : 0u) | (fb_hi != kNoFlag ? 0xffff0000u : 0u)) : offa;
-            TA = (offa & TA_off) | (~offa & TA);
-            TB = (offb & TB_off) | (~offb & TB);
-        }
-        if (MULTI && bot_out && valid) {
-            const uint32_t o = (uint32_t)(b * NG + k) * 2;
-            wbnd[o] = make_uint4(obh[0], obf[0], obh[1], obf[1]);
-            wbnd[o + 1] = make_uint4(obh[2], obf[2], obh[3], obf[3]);
-        }
+            uint32_t tbh[4], tbf[4];
+#pragma unroll
+            for (int c = 0; c < 4; ++c) {
+                tbh[c] = shift_down<G>(toph[c], obh[4 * j + c], top_lane);
+                tbf[c] = shift_down<G>(topf[c], obf[4 * j + c], top_lane);
+                if (SEG) {  // a packed query's first row: the SW matrix border
+                    tbh[c] = (seg_top & as_u(Z)) | (~seg_top & tbh[c]);
+                    tbf[c] = (seg_top & as_u(Z)) | (~seg_top & tbf[c]);
+                }
+            }
 
-        if (MODE >= 1 && FIRST) {  // the reference advances the top border by four columns per block
-            fb[0] = adds(fb[3], RR);
-            fb[1] = adds(fb[0], RR);
-            fb[2] = adds(fb[1], RR);
-            fb[3] = adds(fb[2], RR);
-            hb[0] = adds(hb[3], RR);
-            hb[1] = adds(hb[0], RR);
-            hb[2] = adds(hb[1], RR);
-            hb[3] = adds(hb[2], RR);
+            s16x2 hmn = as_v(0u), hmx = as_v(0u);  // NW trackers, reset value of the reference
+            const bool any_last = __any(((w.x | w.y) & kResLast) != 0);
+            const int z_lo = (int)((w.x >> kResZShift) & 3u), z_hi = (int)((w.y >> kResZShift) & 3u);
+            // the block's four columns; CAP also records the NW score cell (the
+            // last query row at the pair's last column), a select per row on a
+            // wave-uniform condition, so the loop stays straight-line code
+            auto columns = [&](auto cap_tag) {
+                constexpr bool CAP = decltype(cap_tag)::value;
+                uint32_t cm_lo = 0u, cm_hi = 0u;
+                if (CAP && g == last_g && lane_last) {
+                    cm_lo = (w.x & kResLast) ? 0x0000ffffu : 0u;
+                    cm_hi = (w.y & kResLast) ? 0xffff0000u : 0u;
+                }
+                // The cell statements are written so that every dependent packed
+                // op has an independent one in front of it (gfx950 needs a wait
+                // state between a packed op and a packed op that reads its
+                // result).  The order is not pinned with scheduling barriers:
+                // instruction selection already reorders pure ops, and left free
+                // the post-RA scheduler removes most of the s_nop padding (what
+                // it leaves is hidden by the other waves of the SIMD).
+#pragma unroll
+                for (int c = 0; c < 4; ++c) {
+                    s16x2 f = as_v(tbf[c]);
+                    s16x2 fR = adds(f, RR);
+                    s16x2 v;
+                    if constexpr (PAIR) v = as_v(pl[0]);
+                    else v = pair_row(pl, ph, 0);
+                    s16x2 h1 = adds(htop, v);  // row 0: diagonal H from the row above
+                    htop = as_v(tbh[c]);
+                    s16x2 hq = Z;
+#pragma unroll
+                    for (int r = 0; r < R; ++r) {
+                        const bool more = r + 1 < R;
+                        if (more) {
+                            if constexpr (PAIR) v = as_v(pl[r + 1 < R ? r + 1 : 0]);
+                            else v = pair_row(pl, ph, r + 1);
+                        }
+                        // the next column's profile rows (the step's next block
+                        // after column 3), once this column's dwords are consumed
+                        if ((c < 3 || j + 1 < NB) && ((R == 4 && r == 3) || (R >= 8 && r % 8 == 7))) {
+                            const uint2 wc = c < 3 ? w : in.w[j + 1 < NB ? j + 1 : j];
+                            const int cn = c < 3 ? c + 1 : 0;
+                            if constexpr (PAIR) {
+                                read_part_pair<R>(sp + colofs(wc.x, cn), pl, r / 8);
+                            } else {
+                                read_part<R>(sp + colofs(wc.x, cn), pl, r / 8);
+                                read_part<R>(sp + colofs(wc.y, cn), ph, r / 8);
+                            }
+                        }
+                        s16x2 h = vmax(h1, f);
+                        const s16x2 eR = adds(El[r], RR);
+                        h = vmax(h, El[r]);
+                        if (more) h1 = adds(Hl[r], v);  // next row's diagonal term
+                        const s16x2 tt = adds(h, QR);
+                        if (MODE == 0) {
+                            S = vmax(S, h);
+                        } else if (MODE == 1) {
+                            hmn = vmin(hmn, h);
+                            hmx = vmax(hmx, h);
+                        }
+                        f = vmax(fR, tt);
+                        El[r] = vmax(eR, tt);
+                        if (more) fR = adds(f, RR);
+                        Hl[r] = h;
+                        if (CAP && r == last_r) hq = h;
+                    }
+                    obh[4 * j + c] = as_u(Hl[R - 1]);
+                    obf[4 * j + c] = as_u(f);
+                    if (CAP) {
+                        const uint32_t cm = (z_lo == c ? cm_lo : 0u) | (z_hi == c ? cm_hi : 0u);
+                        cap = sel(cm, hq, cap);
+                    }
+                }
+            };
+            // only the lane holding the last query row records a score cell
+            if (MODE >= 1 && __any(lane_last && g == last_g && ((w.x | w.y) & kResLast) != 0)) columns(Tag<true>{});
+            else columns(Tag<false>{});
+            // overflow predicates at the end of the block: a packed test here (in
+            // the cell loop's block, so the running reductions stay in the loop),
+            // the exact per-half bookkeeping below only where a predicate fired
+            bool fire;
+            if (MODE == 0) {
+                fire = as_u(vmax(S, as_v(TA))) != TA || as_u(vmax(S, as_v(TB))) != TB;
+            } else if (MODE == 1) {
+                fire = as_u(vmin(hmn, as_v(TA))) != TA || as_u(vmax(hmx, as_v(TB))) != TB;
+            } else {
+                fire = false;
+            }
+            if (__any(fire)) {
+                if (MODE == 0) {
+                    if (pair_lo >= 0) {
+                        const int v = lo16(S);
+                        if (v >= P.thr_a && fa_lo == kNoFlag) fa_lo = blk_lo;
+                        if (v >= P.thr_b && fb_lo == kNoFlag) fb_lo = blk_lo;
+                    }
+                    if (pair_hi >= 0) {
+                        const int v = hi16(S);
+                        if (v >= P.thr_a && fa_hi == kNoFlag) fa_hi = blk_hi;
+                        if (v >= P.thr_b && fb_hi == kNoFlag) fb_hi = blk_hi;
+                    }
+                } else {
+                    if (pair_lo >= 0 && fa_lo == kNoFlag && (lo16(hmn) < P.thr_a || lo16(hmx) >= P.thr_b))
+                        fa_lo = blk_lo;
+                    if (pair_hi >= 0 && fa_hi == kNoFlag && (hi16(hmn) < P.thr_a || hi16(hmx) >= P.thr_b))
+                        fa_hi = blk_hi;
+                }
+                const uint32_t offa = (fa_lo != kNoFlag ? 0x0000ffffu : 0u) | (fa_hi != kNoFlag ? 0xffff0000u : 0u);
+                const uint32_t offb =
+                    MODE == 0 ? ((fb_lo != kNoFlag ? 0x0000ffffu : 0u) | (fb_hi != kNoFlag ? 0xffff0000u : 0u)) : offa;
+                TA = (offa & TA_off) | (~offa & TA);
+                TB = (offb & TB_off) | (~offb & TB);
+            }
+            if (MULTI && bot_out && valid) {
+                const uint32_t o = (uint32_t)((b + j) * NG + k) * 2;
+                wbnd[o] = make_uint4(obh[4 * j], obf[4 * j], obh[4 * j + 1], obf[4 * j + 1]);
+                wbnd[o + 1] = make_uint4(obh[4 * j + 2], obf[4 * j + 2], obh[4 * j + 3], obf[4 * j + 3]);
+            }
+
+            if (MODE >= 1 && FIRST) {  // the reference advances the top border by four columns per block
+                fb[0] = adds(fb[3], RR);
+                fb[1] = adds(fb[0], RR);
+                fb[2] = adds(fb[1], RR);
+                fb[3] = adds(fb[2], RR);
+                hb[0] = adds(hb[3], RR);
+                hb[1] = adds(hb[0], RR);
+                hb[2] = adds(hb[1], RR);
+                hb[3] = adds(hb[2], RR);
+            }
+            if (any_last) {
+                const bool ls_lo = (w.x & kResLast) != 0, ls_hi = (w.y & kResLast) != 0;
+                if (MODE == 0) {  // the lane above finished the same pair one step earlier
+                    uint32_t scin = shift_down<G>(as_u(Z), scout[j], top_lane);
+                    if (SEG) scin = (seg_top & as_u(Z)) | (~seg_top & scin);
+                    scout[j] = as_u(vmax(S, as_v(scin)));
+                }
+                const s16x2 fin = MODE == 0 ? as_v(scout[j]) : cap;
+                uint32_t m = 0u;
+                if (ls_lo && pair_lo >= 0) {
+                    finish_pair<MODE>(results, pair_lo, score_lo, lo16(fin), fa_lo, fb_lo);
+                    pair_lo = -1;
+                    m |= 0x0000ffffu;
+                }
+                if (ls_hi && pair_hi >= 0) {
+                    finish_pair<MODE>(results, pair_hi, score_hi, hi16(fin), fa_hi, fb_hi);
+                    pair_hi = -1;
+                    m |= 0xffff0000u;
+                }
+                TA = (m & TA_off) | (~m & TA);
+                TB = (m & TB_off) | (~m & TB);
+            }
+            blk_lo++;
+            blk_hi++;
         }
-        if (any_last) {
-            const bool ls_lo = (w.x & kResLast) != 0, ls_hi = (w.y & kResLast) != 0;
-            if (MODE == 0) {  // the lane above finished the same pair one step earlier
-                uint32_t scin = shift_down<G>(as_u(Z), scout, top_lane);
-                if (SEG) scin = (seg_top & as_u(Z)) | (~seg_top & scin);
-                scout = as_u(vmax(S, as_v(scin)));
-            }
-            const s16x2 fin = MODE == 0 ? as_v(scout) : cap;
-            uint32_t m = 0u;
-            if (ls_lo && pair_lo >= 0) {
-                finish_pair<MODE>(results, pair_lo, score_lo, lo16(fin), fa_lo, fb_lo);
-                pair_lo = -1;
-                m |= 0x0000ffffu;
-            }
-            if (ls_hi && pair_hi >= 0) {
-                finish_pair<MODE>(results, pair_hi, score_hi, hi16(fin), fa_hi, fb_hi);
-                pair_hi = -1;
-                m |= 0xffff0000u;
-            }
-            TA = (m & TA_off) | (~m & TA);
-            TB = (m & TB_off) | (~m & TB);
-        }
-        blk_lo++;
-        blk_hi++;
         b = b_n;
         if (MULTI && __any(wrap)) {  // lanes entering their next pass: its entry list and staged buffers
             if (wrap) {
@@ -534,31 +568,36 @@ __device__ __forceinline__ void run_task(const ScoreParams& P, uint32_t gbase0, 
         }
     };
 
-    // block -g for step 0: only the top lanes start with a real block
-    uint2 wa = make_uint2(0, 0), wb = make_uint2(0, 0);
-    const uint4 zero4 = make_uint4(0, 0, 0, 0);
-    uint4 ta0 = zero4, ta1 = zero4, tb0 = zero4, tb1 = zero4;
-    if (nbw > 0 && g == 0) wa = wres[k];
-    // steps with the top lane in the first pass (nbp is even)
-    for (int t = 0; t < nbp; t += 2) {
-        step(Tag<true>{}, wa, ta0, ta1, wb, tb0, tb1);
-        step(Tag<true>{}, wb, tb0, tb1, wa, ta0, ta1);
+    // the blocks of step 0 (-NB g ..): only the top lanes start with real blocks
+    StepIn<NB> sa, sb;
+#pragma unroll
+    for (int j = 0; j < NB; ++j) {
+        sa.w[j] = sb.w[j] = make_uint2(0, 0);
+        sa.t[2 * j] = sa.t[2 * j + 1] = sb.t[2 * j] = sb.t[2 * j + 1] = make_uint4(0, 0, 0, 0);
+        if (g == 0 && j < nbw) sa.w[j] = wres[j * NG + k];
+    }
+    // steps with the top lane in the first pass (an even number: nbp is a
+    // multiple of 2 * NB)
+    const int spp = nbp / NB;  // steps per pass
+    for (int t = 0; t < spp; t += 2) {
+        step(Tag<true>{}, sa, sb);
+        step(Tag<true>{}, sb, sa);
     }
     // the rest: pass p + 2 is staged once lane G - 1 has left pass p
-    const int nsteps = (npass * nbp + G) & ~1;
-    int next_stage = ((nbp + G - 1) + 1) & ~1, staged = 2;
-    for (int t = nbp; t < nsteps; t += 2) {
+    const int nsteps = (npass * spp + G) & ~1;
+    int next_stage = ((spp + G - 1) + 1) & ~1, staged = 2;
+    for (int t = spp; t < nsteps; t += 2) {
         if (MULTI && t == next_stage && staged < npass) {  // the same for the whole workgroup
             stage(staged);
             ++staged;
-            next_stage += nbp;
+            next_stage += spp;
         }
-        step(Tag<false>{}, wa, ta0, ta1, wb, tb0, tb1);
-        step(Tag<false>{}, wb, tb0, tb1, wa, ta0, ta1);
+        step(Tag<false>{}, sa, sb);
+        step(Tag<false>{}, sb, sa);
     }
 }
 
-template <int R, int G, int MODE>
+template <int R, int G, int MODE, int NB>
 __global__ __launch_bounds__(64 * kWavesPerWG) void score_kernel(const ScoreLaunch L, const ScoreParams P) {
     constexpr int PR = G * R;  // rows per pass
     static_assert((1 << Log2<PR>::value) == PR, "rows per pass must be a power of two");
@@ -585,13 +624,13 @@ __global__ __launch_bounds__(64 * kWavesPerWG) void score_kernel(const ScoreLaun
         const QueryDesc Q = L.queries[T.query];
         const int npass = (Q.qlen + PR - 1) / PR;
         // the workgroup's waves share the pass length: the longest of them,
-        // at least G + 2 blocks (so that a pass's bottom rows are written
-        // before the next pass reads them) and even
+        // at least G + 2 steps (so that a pass's bottom rows are written
+        // before the next pass reads them) and an even number of steps
         int nbmax = 0;
 #pragma unroll
         for (int i = 0; i < kWavesPerWG; ++i) nbmax = max(nbmax, L.waves[T.wave0 + i].nblocks);
         if (nbmax == 0) continue;
-        const int nbp = npass > 1 ? (max(nbmax, G + 2) + 1) & ~1 : (nbmax + 1) & ~1;
+        const int nbp = ((npass > 1 ? max(nbmax, NB * (G + 2)) : nbmax) + 2 * NB - 1) / (2 * NB) * (2 * NB);
         const uint2* __restrict__ wres = L.res + Wv.res_off;
         const int sb_lo = L.slot_first[Wv.slot_base + 2 * k];
         const int sb_hi = L.slot_first[Wv.slot_base + 2 * k + 1];
@@ -624,15 +663,15 @@ __global__ __launch_bounds__(64 * kWavesPerWG) void score_kernel(const ScoreLaun
             __syncthreads();
         };
         if (MODE == 0 && Q.seg_off >= 0)
-            run_task<R, G, MODE, true, MODE == 0, false>(P, gbase, sp, slbh, slbe, wres, wbnd, L.slot_entries,
+            run_task<R, G, MODE, NB, true, MODE == 0, false>(P, gbase, sp, slbh, slbe, wres, wbnd, L.slot_entries,
                                                          Q.pair_off, L.segs + Q.seg_off, nullptr, L.results, sb_lo,
                                                          sb_hi, k, g, Wv.nblocks, nbp, npass, last_g, last_r, stage);
         else if (npass > 1)
-            run_task<R, G, MODE, true, false, false>(P, gbase, sp, slbh, slbe, wres, wbnd, L.slot_entries,
+            run_task<R, G, MODE, NB, true, false, false>(P, gbase, sp, slbh, slbe, wres, wbnd, L.slot_entries,
                                                      Q.pair_off, nullptr, nullptr, L.results, sb_lo, sb_hi, k, g,
                                                      Wv.nblocks, nbp, npass, last_g, last_r, stage);
         else
-            run_task<R, G, MODE, false, false, false>(P, gbase, sp, slbh, slbe, wres, wbnd, L.slot_entries,
+            run_task<R, G, MODE, NB, false, false, false>(P, gbase, sp, slbh, slbe, wres, wbnd, L.slot_entries,
                                                       Q.pair_off, nullptr, nullptr, L.results, sb_lo, sb_hi, k, g,
                                                       Wv.nblocks, nbp, npass, last_g, last_r, stage);
     }
@@ -689,7 +728,7 @@ __global__ __launch_bounds__(64 * kWavesPerWG) void score_kernel_pair(const Scor
             load_pass(pass);
             __syncthreads();
         };
-        run_task<R, G, 0, true, true, true>(P, gbase, sp, slb_unused, slb_unused, wres, wbnd, L.slot_entries,
+        run_task<R, G, 0, 1, true, true, true>(P, gbase, sp, slb_unused, slb_unused, wres, wbnd, L.slot_entries,
                                             Q.pair_off, L.segs + Q.seg_off, L.segs + Q.seg_off_hi, L.results, sb_lo,
                                             sb_hi, k, g, Wv.nblocks, nbp, npass, 0, 0, stage);
     }
@@ -858,35 +897,28 @@ __global__ __launch_bounds__(64) void pack_kernel(const PackEntry* __restrict__ 
 // ---------------------------------------------------------------------------
 // launch wrappers (host side)
 // ---------------------------------------------------------------------------
-template <int R, int G, int MODE>
+template <int R, int G, int MODE, int NB>
 static hipError_t launch_score_t(hipStream_t st, const ScoreLaunch& L, const ScoreParams& P, int max_wg) {
     const int nwg = std::max(1, std::min(L.ntasks, max_wg));
     hipLaunchKernelGGL(fill_u32_kernel, dim3(1), dim3(64), 0, st, reinterpret_cast<uint32_t*>(L.counter),
                        (int64_t)1, 0u);
-    hipLaunchKernelGGL((score_kernel<R, G, MODE>), dim3(nwg), dim3(64 * kWavesPerWG), 0, st, L, P);
+    hipLaunchKernelGGL((score_kernel<R, G, MODE, NB>), dim3(nwg), dim3(64 * kWavesPerWG), 0, st, L, P);
     return hipGetLastError();
 }
 
-template <int R, int G, int MODE>
+template <int R, int G, int MODE, int NB>
 static int resident_wg_t() {
     static int cap = 0;
     if (!cap) {
         int per_cu = 0, dev = 0;
         hipDeviceProp_t prop;
-        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, score_kernel<R, G, MODE>, 64 * kWavesPerWG, 0) !=
-                hipSuccess ||
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, score_kernel<R, G, MODE, NB>, 64 * kWavesPerWG,
+                                                         0) != hipSuccess ||
             hipGetDevice(&dev) != hipSuccess || hipGetDeviceProperties(&prop, dev) != hipSuccess)
             return 0;
         cap = std::max(1, per_cu) * prop.multiProcessorCount;
     }
     return cap;
-}
-
-// Passes of at most 512 rows (two staged profiles of 32 KiB); 32 rows per
-// lane for SW only (NW holds more state per row and would run one wave per SIMD)
-bool score_shape_supported(int lanes, int rows, int mode) {
-    return (lanes == 8 || lanes == 16 || lanes == 32 || lanes == 64) &&
-           (rows == 4 || rows == 8 || rows == 16 || (rows == 32 && mode == 0)) && lanes * rows <= 512;
 }
 
 // NW without trackers (MODE 2) exists for the common NW shapes only
@@ -895,49 +927,81 @@ constexpr bool nw_untracked_shape() {
     return (G == 8 && (R == 8 || R == 16)) || (G == 16 && (R == 8 || R == 16)) || (G == 32 && R == 8);
 }
 
+// 8-column tiles (two blocks per step) for lanes of at most 16 rows in groups
+// of 8 or 16 lanes: the step's second block holds 4 more bottom-row H/F pairs
+// and residue/boundary loads, which 32-row lanes have no registers for
+template <int R, int G>
+constexpr bool tile8_shape() {
+    return R <= 16 && (G == 8 || G == 16);
+}
+
+// Passes of at most 512 rows (two staged profiles of 32 KiB); 32 rows per
+// lane for SW only (NW holds more state per row and would run one wave per SIMD)
+bool score_shape_supported(int lanes, int rows, int mode, int cols) {
+    return (lanes == 8 || lanes == 16 || lanes == 32 || lanes == 64) &&
+           (rows == 4 || rows == 8 || rows == 16 || (rows == 32 && mode == 0)) && lanes * rows <= 512 &&
+           (cols == 4 || (cols == 8 && rows <= 16 && (lanes == 8 || lanes == 16)));
+}
+
 bool score_nw_untracked_supported(int lanes, int rows) {
     return (lanes == 8 && (rows == 8 || rows == 16)) || (lanes == 16 && (rows == 8 || rows == 16)) ||
            (lanes == 32 && rows == 8);
 }
 
-template <int R, int G, int MODE>
-static int resident_wg_shape() {
-    if constexpr (R == 32 && MODE >= 1) return 0;
-    else if constexpr (MODE == 2 && !nw_untracked_shape<R, G>()) return 0;
-    else return resident_wg_t<R, G, MODE>();
+template <int R, int G, int MODE, int NB>
+static constexpr bool shape_built() {
+    return !(R == 32 && MODE >= 1) && !(MODE == 2 && !nw_untracked_shape<R, G>()) && (NB == 1 || tile8_shape<R, G>());
 }
 
-template <int R, int G, int MODE>
+template <int R, int G, int MODE, int NB>
+static int resident_wg_shape() {
+    if constexpr (!shape_built<R, G, MODE, NB>()) return 0;
+    else return resident_wg_t<R, G, MODE, NB>();
+}
+
+template <int R, int G, int MODE, int NB>
 static hipError_t launch_shape(hipStream_t st, const ScoreLaunch& L, const ScoreParams& P, int max_wg) {
-    if constexpr (R == 32 && MODE >= 1) return hipErrorInvalidValue;
-    else if constexpr (MODE == 2 && !nw_untracked_shape<R, G>()) return hipErrorInvalidValue;
-    else return launch_score_t<R, G, MODE>(st, L, P, max_wg);
+    if constexpr (!shape_built<R, G, MODE, NB>()) return hipErrorInvalidValue;
+    else return launch_score_t<R, G, MODE, NB>(st, L, P, max_wg);
 }
 
 #define SSA_SHAPES(X)                                                                                   \
     X(8, 4) X(8, 8) X(8, 16) X(8, 32) X(16, 4) X(16, 8) X(16, 16) X(16, 32) X(32, 4) X(32, 8) X(32, 16) \
     X(64, 4) X(64, 8)
 
-int score_resident_workgroups(int lanes, int rows, int mode) {
+template <int NB>
+static int resident_nb(int lanes, int rows, int mode) {
 #define SSA_CASE(GG, RR)                                                                              \
     if (lanes == GG && rows == RR)                                                                  \
-        return mode == 0 ? resident_wg_shape<RR, GG, 0>()                                           \
-                         : (mode == 1 ? resident_wg_shape<RR, GG, 1>() : resident_wg_shape<RR, GG, 2>());
+        return mode == 0 ? resident_wg_shape<RR, GG, 0, NB>()                                       \
+                         : (mode == 1 ? resident_wg_shape<RR, GG, 1, NB>() : resident_wg_shape<RR, GG, 2, NB>());
     SSA_SHAPES(SSA_CASE)
 #undef SSA_CASE
     return 0;
 }
 
-hipError_t launch_score(int lanes, int rows, int mode, hipStream_t st, const ScoreLaunch& L, const ScoreParams& P,
-                        int max_wg) {
-    if (L.ntasks <= 0) return hipSuccess;
+int score_resident_workgroups(int lanes, int rows, int mode, int cols) {
+    return cols == 8 ? resident_nb<2>(lanes, rows, mode) : resident_nb<1>(lanes, rows, mode);
+}
+
+template <int NB>
+static hipError_t launch_nb(int lanes, int rows, int mode, hipStream_t st, const ScoreLaunch& L, const ScoreParams& P,
+                            int max_wg) {
 #define SSA_CASE(GG, RR)                                                                               \
     if (lanes == GG && rows == RR)                                                                   \
-        return mode == 0 ? launch_shape<RR, GG, 0>(st, L, P, max_wg)                                 \
-                         : (mode == 1 ? launch_shape<RR, GG, 1>(st, L, P, max_wg)                    \
-                                      : launch_shape<RR, GG, 2>(st, L, P, max_wg));
+        return mode == 0 ? launch_shape<RR, GG, 0, NB>(st, L, P, max_wg)                             \
+                         : (mode == 1 ? launch_shape<RR, GG, 1, NB>(st, L, P, max_wg)                \
+                                      : launch_shape<RR, GG, 2, NB>(st, L, P, max_wg));
     SSA_SHAPES(SSA_CASE)
 #undef SSA_CASE
+    return hipErrorInvalidValue;
+}
+
+hipError_t launch_score(int lanes, int rows, int mode, int cols, hipStream_t st, const ScoreLaunch& L,
+                        const ScoreParams& P, int max_wg) {
+    if (L.ntasks <= 0) return hipSuccess;
+    if (cols == 8) return launch_nb<2>(lanes, rows, mode, st, L, P, max_wg);
+    if (cols == 4) return launch_nb<1>(lanes, rows, mode, st, L, P, max_wg);
     return hipErrorInvalidValue;
 }
 

@@ -343,13 +343,15 @@ struct TierOut {
 namespace {
 
 struct Shape {
-    int lanes;  // G: lanes per stream pair
-    int rows;   // R: query rows per lane
+    int lanes;     // G: lanes per stream pair
+    int rows;      // R: query rows per lane
+    int cols = 4;  // columns per step (the column tile): 4 or 8
 };
 
 }  // namespace
 
 Shape& shape_override_ref();
+int& cols_override_ref();
 void release_layouts();
 
 namespace {
@@ -406,8 +408,7 @@ bool layout_cached(int lanes, bool paired);
 // layout is cached, since no layout may be built -- the unpaired one, or for
 // packed queries the paired one of a shape with a paired kernel (packed
 // queries also run unpaired, so there any cached shape will do).
-Shape choose_shape(size_t qlen_max, int64_t total_blocks, int64_t max_blocks, int type, bool pair,
-                   bool fixed = false) {
+Shape choose_shape_base(size_t qlen_max, int64_t total_blocks, int64_t max_blocks, int type, bool pair, bool fixed) {
     const Shape ov = shape_override();
     // a forced shape of 32 rows per lane runs NW with 16 (the NW kernel has no 32-row shape)
     if (ov.lanes && ov.rows) return (type == 1 && ov.rows == 32) ? Shape{ov.lanes, 16} : ov;
@@ -490,6 +491,16 @@ Shape choose_shape(size_t qlen_max, int64_t total_blocks, int64_t max_blocks, in
         }
     }
     return best;
+}
+
+// The shape with the column tile forced by ssa_amd_set_kernel_variant, where
+// the shape has it.
+Shape choose_shape(size_t qlen_max, int64_t total_blocks, int64_t max_blocks, int type, bool pair,
+                   bool fixed = false) {
+    Shape sh = choose_shape_base(qlen_max, total_blocks, max_blocks, type, pair, fixed);
+    const int cols = cols_override_ref();
+    if (cols && score_shape_supported(sh.lanes, sh.rows, type == 0 ? 0 : 1, cols)) sh.cols = cols;
+    return sh;
 }
 
 struct PairSet {
@@ -760,7 +771,8 @@ void tier_prepare(Exec& ex, TierRun& R) {
                             ex.fixed());
     }
     const Shape sh = R.sh;
-    if (!score_shape_supported(sh.lanes, sh.rows, R.type == kSW ? 0 : 1)) fatal("libssa_amd: unsupported kernel shape");
+    if (!score_shape_supported(sh.lanes, sh.rows, R.type == kSW ? 0 : 1, sh.cols))
+        fatal("libssa_amd: unsupported kernel shape");
     const int PR = sh.lanes * sh.rows, ng = 64 / sh.lanes, spw = 2 * ng;
 
     // ---- profiles [pass][code][row], left boundaries, per-query descriptors ----
@@ -774,7 +786,7 @@ void tier_prepare(Exec& ex, TierRun& R) {
     // SW with several queries: packed queries (below), two by two where the
     // shape has a paired kernel
     R.packed = R.type == kSW && nq > 1;
-    R.paired = R.packed && score_pair_supported(sh.lanes, sh.rows);
+    R.paired = R.packed && sh.cols == 4 && score_pair_supported(sh.lanes, sh.rows);
     // no layout may be built on the no-allocation path: pairing only where
     // the paired layout is already there, else unpaired packed queries
     if (ex.fixed() && R.paired && !layout_cached(sh.lanes, true)) R.paired = false;
@@ -864,7 +876,7 @@ void tier_prepare(Exec& ex, TierRun& R) {
             R.untracked = int16_exact(R.qs[qi].len, 4 * (int64_t)L->max_entry_blocks, mi);
     }
     const int cap = R.paired ? score_pair_resident_workgroups(sh.lanes, sh.rows)
-                             : score_resident_workgroups(sh.lanes, sh.rows, R.mode());
+                             : score_resident_workgroups(sh.lanes, sh.rows, R.mode(), sh.cols);
 
     // ---- packed queries (SW, several queries) ----
     // Each query starts at a lane boundary and is padded to whole lanes (the
@@ -1033,8 +1045,11 @@ void tier_prepare(Exec& ex, TierRun& R) {
     R.grid = std::max(1, std::min<int>((int)R.tasks.size(), cap > 0 ? cap : (int)R.tasks.size()));
     int32_t max_nb = 0;
     for (int w = 0; w < nw; ++w) max_nb = std::max(max_nb, L->wave_nblocks[w]);
-    // the kernel's pass length: the longest wave of a task, at least G + 2, even
-    const int64_t max_nbp = (std::max<int64_t>(max_nb, sh.lanes + 2) + 1) & ~(int64_t)1;
+    // the kernel's pass length: the longest wave of a task, at least G + 2
+    // steps, an even number of steps
+    const int64_t nb_step = sh.cols / 4;
+    const int64_t max_nbp =
+        (std::max<int64_t>(max_nb, nb_step * (sh.lanes + 2)) + 2 * nb_step - 1) / (2 * nb_step) * (2 * nb_step);
     const int64_t bnd_per_wave = multipass ? max_nbp * ng * 2 : 0;
 
     // ---- device buffers ----
@@ -1089,7 +1104,7 @@ void tier_launch(hipStream_t st, TierRun& R) {
     if (R.empty()) return;
     SSA_HIP(launch_init_results(st, R.d_out, (int64_t)R.nout, -32768));
     if (R.paired) SSA_HIP(launch_score_pair(R.sh.lanes, R.sh.rows, st, R.K, R.P, R.grid));
-    else SSA_HIP(launch_score(R.sh.lanes, R.sh.rows, R.mode(), st, R.K, R.P, R.grid));
+    else SSA_HIP(launch_score(R.sh.lanes, R.sh.rows, R.mode(), R.sh.cols, st, R.K, R.P, R.grid));
 }
 
 void tier_finish(Exec& ex, TierRun& R) {
@@ -1513,6 +1528,20 @@ Shape& shape_override_ref() {
 }
 
 void set_shape_override(int lanes, int rows) { shape_override_ref() = Shape{lanes, rows}; }
+
+int& cols_override_ref() {
+    static int c = 0;  // 0: chosen per search
+    return c;
+}
+
+void set_cols_override(int cols) { cols_override_ref() = cols; }
+
+int& placement_override_ref() {
+    static int p = 0;  // 0: chosen per search
+    return p;
+}
+
+void set_placement_override(int placement) { placement_override_ref() = placement; }
 
 void set_target_waves(int waves) {
     target_waves() = waves > 0 ? waves : 10240;
@@ -2262,6 +2291,8 @@ void run_batch(Exec& ex, const std::vector<SearchRequest>& reqs, std::vector<Sea
             computed += R.computed * J.nrun / std::max<size_t>(R.qs.size(), 1);
             J.st.lanes_per_stream = R.sh.lanes;
             J.st.rows_per_lane = R.sh.rows;
+            J.st.columns_per_step = R.sh.cols;
+            J.st.profile_placement = 1;
             J.st.overflow_tracking = R.mode() != 2;
         }
         if (J.dev_top) {
@@ -2812,13 +2843,17 @@ int autotune(int type, const std::vector<size_t>& lengths, int copies, std::vect
                 x ^= x << 17;
                 q[i] = alphabet[x % alphabet.size()];
             }
-        AutotuneResult best{L, 0, 0, 1e300};
+        AutotuneResult best{L, 0, 0, 4, 1e300};
         std::vector<TunedShape> meas;
-        for (const Shape& sh : kShapes) {
-            const double ms = run(sh, qs) / copies;
-            meas.push_back(TunedShape{type, L, sh, ms});
-            if (ms < best.ms) best = AutotuneResult{L, sh.lanes, sh.rows, ms};
-        }
+        for (const Shape& base : kShapes)
+            for (int cols : {4, 8}) {
+                if (!score_shape_supported(base.lanes, base.rows, type, cols)) continue;
+                Shape sh = base;
+                sh.cols = cols;
+                const double ms = run(sh, qs) / copies;
+                meas.push_back(TunedShape{type, L, sh, ms});
+                if (ms < best.ms) best = AutotuneResult{L, sh.lanes, sh.rows, sh.cols, ms};
+            }
         out.push_back(best);
         std::lock_guard<std::mutex> lk(tuned_mutex());
         auto& t = tuned_shapes();
@@ -2951,7 +2986,7 @@ RequestPlan plan_request(const SearchRequest& rq, const std::vector<size_t>& qle
     const int PR = sh.lanes * sh.rows;
     const int passes = (int)((std::max<size_t>(packed ? qsum : qmax, 1) + PR - 1) / PR);
     const int cap = std::max(1, score_resident_workgroups(sh.lanes, type == 0 ? sh.rows : std::min(sh.rows, 16),
-                                                          type == 0 ? 0 : 1));
+                                                          type == 0 ? 0 : 1, sh.cols));
     const int tasks = std::max(1, nwaves / kWavesPerWG) * (packed ? 1 : (int)nq);
     const int grid = std::min(tasks, cap);
     pl.occupancy = (double)grid / (double)cap;

@@ -64,7 +64,7 @@ int run_search_noalloc(SearchType type, p_query query, size_t hitcount, int bit_
 
 struct AutotuneResult {
     size_t qlen;
-    int lanes, rows;
+    int lanes, rows, cols;
     double ms;
 };
 // shape measurements on the loaded DB; the fastest shape per (type, length)
@@ -117,6 +117,8 @@ RequestPlan plan_request(const SearchRequest& rq, const std::vector<size_t>& que
 
 ssa_amd_stats_t& last_stats();
 void set_shape_override(int lanes, int rows);  // 0, 0 = chosen per query length
+void set_cols_override(int cols);             // 0 = chosen per search, else 4 or 8
+void set_placement_override(int placement);   // 0 = chosen per search, 1 LDS, 2 VGPRs
 void set_target_waves(int waves);             // 0 = default
 
 }  // namespace ssa

@@ -229,10 +229,13 @@ def test_random_parity(seed):
     shapes = [(8, 4), (16, 4), (32, 4), (64, 4), (8, 8), (16, 8), (32, 8), (64, 8), (8, 16), (16, 16), (32, 16),
               (8, 32), (16, 32)]
     ssa.set_kernel_shape(*(rnd.choice(shapes) if seed >= 24 else (0, 0)))
+    # and a forced column tile (shapes without 8-column tiles keep 4)
+    ssa.set_kernel_variant(rnd.choice([0, 4, 8]) if seed >= 24 else 0, 0)
     try:
         check(t, bw, symtype, strands, qraw, db, k, matrix, gaps, isa=isa, chunk=chunk, align=(seed % 4 == 1))
     finally:
         ssa.set_kernel_shape(0, 0)
+        ssa.set_kernel_variant(0, 0)
 
 
 @pytest.mark.parametrize("t", [O.SW, O.NW])
@@ -295,6 +298,35 @@ def test_32_row_shapes_over_passes(shape):
         check(O.SW, 8, O.AMINOACID, 1, qraw, db, 30, ("const", (5, -4)), (-3, -1))
     finally:
         ssa.set_kernel_shape(0, 0)
+
+
+@pytest.mark.parametrize("shape", [(8, 4), (8, 8), (8, 16), (16, 4), (16, 8), (16, 16)])
+def test_column_tile_of_8(shape):
+    """Two 4-column blocks per step: pair starts, ends and the first flagged
+    block fall on either block of a step, passes of a 1,100-residue query
+    hand their bottom rows on per block, one-block entries start and end
+    inside one step; SW, NW with and without the overflow trackers."""
+    rnd = random.Random(808 + shape[0] * shape[1])
+    qraw = "".join(rnd.choice("ARNDCQEGHILKMFPSTWYV") for _ in range(1100))
+    db = rand_db(rnd, O.AMINOACID, 300, 1, 500, homolog_of=qraw, frac=0.15)
+    db += ["".join(rnd.choice("ARNDCQEGHILKMFPSTWYV") for _ in range(rnd.randint(1, 9))) for _ in range(40)]
+    short = qraw[:150]
+    ssa.set_kernel_shape(*shape)
+    ssa.set_kernel_variant(8, 0)
+    try:
+        for t, bw in [(O.SW, 8), (O.SW, 16), (O.NW, 16), (O.NW, 8)]:
+            check(t, bw, O.AMINOACID, 1, qraw, db, 25, ("builtin", "blosum62"), (-10, -1))
+            st = ssa.last_stats()
+            if (t, bw) != (O.NW, 8):  # (NW 8-bit skips its first tier here: every pair overflows at once)
+                assert (st.lanes_per_stream, st.rows_per_lane, st.columns_per_step) == shape + (8,)
+            check(t, bw, O.AMINOACID, 1, short, db, 25, ("builtin", "blosum62"), (-10, -1), align=(bw == 16))
+        check(O.SW, 8, O.AMINOACID, 1, qraw, db, 30, ("const", (5, -4)), (-3, -1))
+        check(O.NW, 16, O.NUCLEOTIDE, 2, "".join(rnd.choice("ACGT") for _ in range(200)),
+              ["".join(rnd.choice("ACGT") for _ in range(rnd.randint(1, 300))) for _ in range(300)], 40,
+              ("const", (2, -4)), (-20, -2))
+    finally:
+        ssa.set_kernel_shape(0, 0)
+        ssa.set_kernel_variant(0, 0)
 
 
 @pytest.mark.parametrize("shape,waves", [((8, 4), 4), ((16, 8), 37), ((64, 4), 9), ((32, 8), 1), ((8, 16), 16)])
