@@ -49,6 +49,9 @@ typedef struct {
                                  updated to the batch's arguments */
     int columns_per_step;     /* first tier's column tile: 4 or 8 DB columns per step */
     int profile_placement;    /* first tier's query profile: 1 = LDS, 2 = VGPRs */
+    int cell_arithmetic;      /* first tier's cell arithmetic: 1 = packed saturating int16,
+                                 2 = packed fp16 windows (SW; exact, see
+                                 ssa_amd_set_cell_arithmetic) */
 } ssa_amd_stats_t;
 
 /* Counters of the last search. */
@@ -76,6 +79,17 @@ void ssa_amd_set_kernel_shape(int lanes_per_stream, int rows_per_lane);
  * to the cost model or the autotuner; a forced variant a shape lacks is left
  * at its default for that shape.  Results do not depend on the variant. */
 void ssa_amd_set_kernel_variant(int columns_per_step, int profile_placement);
+
+/* Cell arithmetic of the score kernel's SW first tier (the tier that records
+ * the 8-bit and 16-bit overflow blocks and scores below 65,535).  0 (the
+ * default): packed fp16, each lane's values held relative to an offset of
+ * its own (a window kept within +-2047, where fp16 integers are exact) --
+ * the three-input packed maximum takes a SW cell from 9 packed operations to
+ * 7.5 -- wherever the matrix and gaps keep a lane's values within the window
+ * (BLOSUM62 with gaps 10/1 does), packed saturating int16 elsewhere.  1:
+ * packed int16 always.  NW and the other tiers compute in packed int16.
+ * Results do not depend on it. */
+void ssa_amd_set_cell_arithmetic(int arithmetic);
 
 /* Number of waves the DB's pair streams are cut into (0 = the default, 10240).
  * Streams never get shorter than 16 blocks per lane of a group. */

@@ -100,7 +100,8 @@ class Stats(_C.Structure):
                 ("ms_gpu_first", _C.c_double), ("ms_gpu_requeue", _C.c_double), ("ms_replay", _C.c_double),
                 ("ms_traceback", _C.c_double), ("ms_total", _C.c_double), ("d2h_bytes", _C.c_uint64),
                 ("replayed_chunks", _C.c_uint64), ("ms_graph", _C.c_double), ("graph_reused", _C.c_int),
-                ("columns_per_step", _C.c_int), ("profile_placement", _C.c_int)]
+                ("columns_per_step", _C.c_int), ("profile_placement", _C.c_int),
+                ("cell_arithmetic", _C.c_int)]
 
 
 @_dataclass
@@ -139,6 +140,7 @@ def _load():
         "ssa_amd_last_stats": (None, [_C.POINTER(Stats)]), "ssa_amd_reset_settings": (None, []),
         "ssa_amd_set_kernel_shape": (None, [I, I]), "ssa_amd_set_stream_waves": (None, [I]),
         "ssa_amd_set_kernel_variant": (None, [I, I]),
+        "ssa_amd_set_cell_arithmetic": (None, [I]),
         "ssa_amd_get_matrix": (None, [P]), "ssa_amd_device_arch": (cp, []),
         "ssa_amd_batch_align": (I, [_C.POINTER(_Request), S, _C.POINTER(_C.POINTER(_AlignmentList)),
                                     _C.POINTER(Stats)]),
@@ -253,6 +255,12 @@ def set_kernel_variant(columns_per_step=0, profile_placement=0):
 
 
 def set_stream_waves(waves): lib.ssa_amd_set_stream_waves(waves)
+
+
+def set_cell_arithmetic(arithmetic):
+    """0: the SW first tier in fp16 windows where the matrix and gaps allow
+    (the default), 1: packed int16 everywhere (ssa_amd_set_cell_arithmetic)."""
+    lib.ssa_amd_set_cell_arithmetic(arithmetic)
 
 
 def get_matrix():

@@ -155,6 +155,12 @@ void ssa_amd_set_kernel_shape(int lanes_per_stream, int rows_per_lane) {
     set_shape_override(lanes_per_stream, rows_per_lane);
 }
 
+void ssa_amd_set_cell_arithmetic(int arithmetic) {
+    if (arithmetic < 0 || arithmetic > 1)
+        fatal("libssa_amd: cell arithmetic must be 0 (chosen) or 1 (int16) (got %d)", arithmetic);
+    set_arith_override(arithmetic);
+}
+
 void ssa_amd_set_kernel_variant(int columns_per_step, int profile_placement) {
     if (columns_per_step != 0 && columns_per_step != 4 && columns_per_step != 8)
         fatal("libssa_amd: columns per step must be 0, 4 or 8 (got %d)", columns_per_step);

@@ -131,6 +131,8 @@ private:
 };
 
 // ---- kernel launchers (score_kernels.hip, traceback_kernels.hip, pack_kernels.hip) ----
+// modes: 0 SW (int16), 1 NW, 2 NW without overflow trackers, 3 SW in fp16
+// windows
 bool score_shape_supported(int lanes, int rows, int mode, int cols);
 int score_resident_workgroups(int lanes, int rows, int mode, int cols);  // persistent grid size of a shape
 // NW shapes with an untracked variant (mode 2: no value of any pair can reach
@@ -138,8 +140,8 @@ int score_resident_workgroups(int lanes, int rows, int mode, int cols);  // pers
 bool score_nw_untracked_supported(int lanes, int rows);
 // paired packed SW queries (score_kernel_pair): shapes, grid size, launch
 bool score_pair_supported(int lanes, int rows);
-int score_pair_resident_workgroups(int lanes, int rows);
-hipError_t launch_score_pair(int lanes, int rows, hipStream_t st, const ScoreLaunch& L, const ScoreParams& P,
+int score_pair_resident_workgroups(int lanes, int rows, bool f16);
+hipError_t launch_score_pair(int lanes, int rows, bool f16, hipStream_t st, const ScoreLaunch& L, const ScoreParams& P,
                              int max_wg);
 hipError_t launch_score(int lanes, int rows, int mode, int cols, hipStream_t st, const ScoreLaunch& L,
                         const ScoreParams& P, int max_wg);

@@ -1,738 +1,11 @@
-// score_kernels.hip -- gfx950 kernels of the database search.
-//
-// score_kernel<R, G, MODE>: SW (MODE 0) / NW (MODE 1) over packed int16 lanes;
-// MODE 2 is NW without the overflow trackers, for 16-bit launches where the
-// host has shown that no value of any pair can reach either end of the int16
-// range (then no block can be flagged and the trackers are dead work).
-// Restates the per-lane arithmetic of the reference's width kernels
-// (src/algo/simd/search_simd_sw.c:172-277, search_simd_nw.c:180-312):
-// saturating adds, the H/E/F recurrence and the overflow predicates, but not
-// their schedule: the host replays the reference's lane schedule from the
-// per-pair results.
-//
-// Work shape: a lane group of G lanes scores two streams of pairs (the lo and
-// hi halves of the packed registers).  The query is cut into passes of G*R
-// rows; lane g of the group owns R consecutive rows of the pass, keeps their H
-// and E in VGPRs and walks the streams one 4-column block behind lane g-1,
-// which hands it the H and F of its bottom row through a DPP wave shift.  The
-// top lane of a group takes its top boundary from the matrix border (first
-// pass) or from the bottom lane of the previous pass through HBM.  A block
-// costs each lane 4*R cells per half, so the cross-lane traffic is 9 DPP moves
-// per 8*R cells.
-//
-// The pass's score profile is staged in LDS as [code][row] int16; a lane reads
-// the R scores of its rows for one DB code with one 16-byte (R = 8) read per
-// half and interleaves the two halves with v_perm.  A lane runs one block per
-// step; lane g is idle for the first g steps of a pass (and the last G-1-g),
-// so a pass takes nblocks + G - 1 steps.
-//
-// Per-pair results combine over lanes and passes: SW takes the maximum cell
-// (atomic max) and the first flagged block (atomic min); a block is flagged
-// when any row of it is.  The NW score is the last real row's cell at the
-// pair's last column, written by the lane holding that row.
-//
-// The host pads the query to whole passes.  Padding rows sit below the last
-// query row, so no real cell depends on them, and their profile values keep
-// them from changing any result: SW pads with the lane minimum (a padded cell
-// never exceeds the real maximum), NW pads with 0 and resets padded E and H
-// to 0 at each pair start (a padded cell stays between min(0, earlier real
-// cells) and max(0, earlier real cells), so it raises no overflow flag the
-// real cells do not).
-#include <hip/hip_runtime.h>
-
-#include <algorithm>
-
-#include "common.h"
-#include "kernels.h"
+// score_kernels.hip -- gfx950 kernels of the database search other than the
+// score kernel's instantiations (score_kernel.h holds its device code):
+// the launchers that dispatch a shape to them, fills, the exact tier, residue
+// packing and the no-allocation scratch array.
+#include "score_kernel.h"
 
 namespace ssa {
 
-typedef short s16x2 __attribute__((ext_vector_type(2)));
-
-__device__ __forceinline__ s16x2 as_v(uint32_t u) { return __builtin_bit_cast(s16x2, u); }
-__device__ __forceinline__ uint32_t as_u(s16x2 v) { return __builtin_bit_cast(uint32_t, v); }
-__device__ __forceinline__ s16x2 adds(s16x2 a, s16x2 b) { return __builtin_elementwise_add_sat(a, b); }
-__device__ __forceinline__ s16x2 vmax(s16x2 a, s16x2 b) { return __builtin_elementwise_max(a, b); }
-__device__ __forceinline__ s16x2 vmin(s16x2 a, s16x2 b) { return __builtin_elementwise_min(a, b); }
-// per-half select: halves whose mask bits are set take a, the others b
-__device__ __forceinline__ s16x2 sel(uint32_t m, s16x2 a, s16x2 b) {
-    return as_v((m & as_u(a)) | (~m & as_u(b)));
-}
-__host__ __device__ __forceinline__ uint32_t pack2d(int v) { return ((uint32_t)(uint16_t)v << 16) | (uint16_t)v; }
-__device__ __forceinline__ int lo16(s16x2 v) { return (int)(short)(as_u(v) & 0xffffu); }
-__device__ __forceinline__ int hi16(s16x2 v) { return (int)(short)(as_u(v) >> 16); }
-// Moves each lane's value one lane down within its group of G lanes: lane g
-// receives lane g-1's value and the group's top lane receives `top`.  G = 64
-// uses the wave shift and G = 16 the row shift, whose vacated lane takes the
-// old value; G = 8 and 32 shift across group boundaries and fix the top lanes.
-template <int G>
-__device__ __forceinline__ uint32_t shift_down(uint32_t top, uint32_t v, bool top_lane) {
-    if constexpr (G == 64) {
-        return (uint32_t)__builtin_amdgcn_update_dpp((int)top, (int)v, 0x138, 0xf, 0xf, false);  // wave_shr:1
-    } else if constexpr (G == 16) {
-        return (uint32_t)__builtin_amdgcn_update_dpp((int)top, (int)v, 0x111, 0xf, 0xf, false);  // row_shr:1
-    } else if constexpr (G == 32) {
-        const uint32_t s = (uint32_t)__builtin_amdgcn_update_dpp((int)top, (int)v, 0x138, 0xf, 0xf, false);
-        return top_lane ? top : s;
-    } else {
-        const uint32_t s = (uint32_t)__builtin_amdgcn_update_dpp((int)top, (int)v, 0x111, 0xf, 0xf, false);
-        return top_lane ? top : s;
-    }
-}
-
-template <bool B>
-struct Tag {
-    static constexpr bool value = B;
-};
-
-template <int N>
-struct Log2 {
-    static constexpr int value = 1 + Log2<N / 2>::value;
-};
-template <>
-struct Log2<1> {
-    static constexpr int value = 0;
-};
-
-// Part q of the profile scores of one lane for one DB code: rows 8q .. 8q+7
-// (R >= 8) or rows 0 .. 3 (R = 4), two rows per dword.  q is a constant once
-// the callers' loops are unrolled.
-template <int R>
-__device__ __forceinline__ void read_part(const char* p, uint32_t* d, int q) {
-    if constexpr (R == 4) {
-        const uint2 v = *reinterpret_cast<const uint2*>(p);
-        d[0] = v.x;
-        d[1] = v.y;
-    } else {
-        const uint4 v = *reinterpret_cast<const uint4*>(p + 16 * q);
-        d[4 * q + 0] = v.x;
-        d[4 * q + 1] = v.y;
-        d[4 * q + 2] = v.z;
-        d[4 * q + 3] = v.w;
-    }
-}
-
-template <int R>
-__device__ __forceinline__ void read_rows(const char* p, uint32_t* d) {
-#pragma unroll
-    for (int q = 0; q < (R >= 8 ? R / 8 : 1); ++q) read_part<R>(p, d, q);
-}
-
-// Part q of a paired profile (lo, hi int16 per row, one dword per row): rows
-// 8q .. 8q+7 (R >= 8) or rows 0 .. 3 (R = 4).
-template <int R>
-__device__ __forceinline__ void read_part_pair(const char* p, uint32_t* d, int q) {
-    if constexpr (R == 4) {
-        const uint4 v = *reinterpret_cast<const uint4*>(p);
-        d[0] = v.x;
-        d[1] = v.y;
-        d[2] = v.z;
-        d[3] = v.w;
-    } else {
-        const uint4 a = *reinterpret_cast<const uint4*>(p + 32 * q);
-        const uint4 b = *reinterpret_cast<const uint4*>(p + 32 * q + 16);
-        d[8 * q + 0] = a.x;
-        d[8 * q + 1] = a.y;
-        d[8 * q + 2] = a.z;
-        d[8 * q + 3] = a.w;
-        d[8 * q + 4] = b.x;
-        d[8 * q + 5] = b.y;
-        d[8 * q + 6] = b.z;
-        d[8 * q + 7] = b.w;
-    }
-}
-
-// the packed (lo, hi) score pair of row r from the two halves' profile dwords
-__device__ __forceinline__ s16x2 pair_row(const uint32_t* lo, const uint32_t* hi, int r) {
-    return as_v(__builtin_amdgcn_perm(hi[r / 2], lo[r / 2], (r & 1) ? 0x07060302u : 0x05040100u));
-}
-
-
-// SW: the group's bottom lane adds the maximum over the group's lanes (handed
-// down with the pair's last block); NW: the lane holding the last query row
-// writes the score.
-template <int MODE>
-__device__ __forceinline__ void finish_pair(PairResult* results, int pair, bool score_lane, int score, int fa,
-                                            int fb) {
-    if (score_lane) {
-        if (MODE == 0) atomicMax(&results[pair].score, score);
-        else results[pair].score = score;
-    }
-    if (fa != kNoFlag) atomicMin(&results[pair].flag_a, fa);
-    if (MODE == 0 && fb != kNoFlag) atomicMin(&results[pair].flag_b, fb);
-}
-
-// All passes of a wave over its streams, as one pipeline: lane g handles
-// block t - g of the passes laid end to end (nbp blocks per pass), so lane 0
-// starts pass p + 1 while the lanes below still finish pass p, and a task
-// takes npass * nbp + G - 1 steps instead of npass * (nbp + G - 1).  Lane 0
-// of pass p + 1 reads the bottom row that lane G - 1 wrote for the same block
-// in pass p, nbp - G + 1 steps earlier (nbp > G).  Each lane keeps its own
-// pass: the staged profile and NW left border are double-buffered, and
-// stage(p) loads pass p's into buffer p & 1 once every lane has left pass
-// p - 2.  Steps run two per iteration (an even count; an extra step is idle)
-// so that the one-step-ahead loads alternate between two register sets.
-//
-// PAIR (SW, packed queries): both halves of a lane group walk the same
-// stream (the layout gives the lo and hi slots the same entries) for two
-// packed queries; the staged profile holds the two queries' rows interleaved,
-// one dword (lo, hi) per row, so no v_perm is needed, and each half has its
-// own segment table.
-//
-// NB (the column tile): blocks per step, 1 or 2 (4 or 8 columns).  A step
-// runs its NB blocks one after the other with everything that belongs to a
-// block -- pair starts, the overflow predicates and first flagged block
-// (still per 4-column block, the reference's unit), pass boundaries, pair
-// ends -- and pays the per-step work (residue and boundary loads of the next
-// step, pass bookkeeping) once; the lane above hands down the bottom row of
-// all 4 * NB columns.  Pass lengths are whole numbers of steps.
-template <int NB>
-struct StepIn {
-    uint2 w[NB];      // residue words of the step's blocks
-    uint4 t[2 * NB];  // top lane: the previous pass's bottom row over them
-};
-
-template <int R, int G, int MODE, int NB, bool MULTI, bool SEG, bool PAIR, class Stage>
-__device__ __forceinline__ void run_task(const ScoreParams& P, uint32_t gbase0, const char* sp,
-                                         const uint32_t* slbh0, const uint32_t* slbe0,
-                                         const uint2* __restrict__ wres, uint4* __restrict__ wbnd,
-                                         const int32_t* __restrict__ slot_entries, int pair_off,
-                                         const SegLane* __restrict__ segs, const SegLane* __restrict__ segs_hi,
-                                         PairResult* __restrict__ results, int sb_lo, int sb_hi, int k, int g,
-                                         int nbw, int nbp, int npass, int last_g, int last_r, const Stage& stage) {
-    static_assert(!PAIR || (SEG && MULTI && MODE == 0), "paired queries are packed SW queries");
-    constexpr int NG = 64 / G;
-    constexpr int PR = G * R;
-    constexpr int LPR = Log2<PR>::value + (PAIR ? 1 : 0);  // log2 of a profile code row's int16 count
-    constexpr uint32_t kProfBytes = 32 * PR * (PAIR ? 4 : 2);  // one staged profile buffer
-    const s16x2 QR = as_v(P.qr), RR = as_v(P.r), Z = as_v(P.zero);
-    // per-half thresholds of the overflow predicates; a half whose predicate
-    // cannot fire (flag taken, or no pair running) holds the neutral value
-    const uint32_t TA_on = MODE == 0 ? pack2d(P.thr_a - 1) : pack2d(P.thr_a);
-    const uint32_t TB_on = pack2d(P.thr_b - 1 > 32767 ? 32767 : P.thr_b - 1);
-    const uint32_t TA_off = MODE == 0 ? 0x7fff7fffu : 0x80008000u;
-    const uint32_t TB_off = 0x7fff7fffu;
-
-    const bool top_lane = g == 0;
-    // the lane's pass and block within it (negative before the lane's first block)
-    int pass = 0, b = -NB * g;
-    // SEG: the lane's segment in its pass, per half (a plain query: one
-    // segment per pass); seg_top holds the halves that start a query there
-    int po_lo = pair_off, po_hi = pair_off;
-    uint32_t seg_top = 0u;
-    bool bot_lo = g == G - 1, bot_hi = g == G - 1, none_lo = false, none_hi = false;
-    auto load_seg = [&](int p) {
-        if constexpr (SEG) {
-            if (p < npass) {
-                const SegLane sl = segs[p * G + g];
-                const SegLane sh = PAIR ? segs_hi[p * G + g] : sl;
-                po_lo = sl.pair_off;
-                po_hi = sh.pair_off;
-                seg_top = ((sl.flags & kSegTop) ? 0x0000ffffu : 0u) | ((sh.flags & kSegTop) ? 0xffff0000u : 0u);
-                bot_lo = (sl.flags & kSegBottom) != 0;
-                bot_hi = (sh.flags & kSegBottom) != 0;
-                none_lo = (sl.flags & kSegNone) != 0;
-                none_hi = (sh.flags & kSegNone) != 0;
-            }
-        }
-    };
-    load_seg(0);
-    uint32_t gbase = gbase0;  // the lane's rows in the profile buffer of its pass
-    const uint32_t* slbh = slbh0;
-    const uint32_t* slbe = slbe0;
-    // byte offset of the lane's rows for column c's DB code in the staged profile
-    // (the code field lies above the lane's rows and below the buffer bit of
-    // gbase, so the two combine with an or)
-    auto colofs = [&gbase](uint32_t word, int c) { return (((word >> (8 * c)) & 0x3eu) << LPR) | gbase; };
-
-    s16x2 Hl[R], El[R];
-#pragma unroll
-    for (int r = 0; r < R; ++r) {
-        Hl[r] = Z;
-        El[r] = Z;
-    }
-    s16x2 S = Z;     // SW: running maximum of the lane's cells of the current pair
-    s16x2 htop = Z;  // H of the row above at the previous column
-    s16x2 cap = Z;   // NW: the last row's H at the pair's last column
-    s16x2 hb[4], fb[4];  // NW: the matrix's top border over the current block (top lane, first pass)
-#pragma unroll
-    for (int c = 0; c < 4; ++c) {
-        hb[c] = as_v(P.hb_init[c]);
-        fb[c] = as_v(P.fb_init[c]);
-    }
-    uint32_t obh[4 * NB], obf[4 * NB];  // bottom row H, F of the last step's blocks
-#pragma unroll
-    for (int c = 0; c < 4 * NB; ++c) {
-        obh[c] = 0u;
-        obf[c] = 0u;
-    }
-    uint32_t TA = TA_off, TB = TB_off;
-    // SW: max over this lane and the lanes above for the pair it finished
-    // last in block j of a step (the lane below reads entry j one step later)
-    uint32_t scout[NB];
-#pragma unroll
-    for (int j = 0; j < NB; ++j) scout[j] = as_u(Z);
-    int kp_lo = 0, kp_hi = 0, pair_lo = -1, pair_hi = -1, blk_lo = 0, blk_hi = 0;
-    int fa_lo = kNoFlag, fa_hi = kNoFlag, fb_lo = kNoFlag, fb_hi = kNoFlag;
-
-    // FIRST: the top lane is in the first pass and takes the matrix border
-    auto step = [&](auto first_tag, const StepIn<NB>& in, StepIn<NB>& nx) {
-        constexpr bool FIRST = decltype(first_tag)::value;
-        // profile dwords of the current column (lo, hi halves); column 0's now
-        // (PAIR: pl holds one dword per row, both halves' scores)
-        uint32_t pl[PAIR ? R : R / 2], ph[PAIR ? 1 : R / 2];
-        if constexpr (PAIR) {
-#pragma unroll
-            for (int q = 0; q < (R >= 8 ? R / 8 : 1); ++q) read_part_pair<R>(sp + colofs(in.w[0].x, 0), pl, q);
-        } else {
-            read_rows<R>(sp + colofs(in.w[0].x, 0), pl);
-            read_rows<R>(sp + colofs(in.w[0].y, 0), ph);
-        }
-        const bool lane_last = pass == npass - 1;
-        const bool bot_out = g == G - 1 && pass < npass - 1;  // bottom row to the next pass
-        const bool score_lo = MODE == 0 ? bot_lo : (lane_last && g == last_g);
-        const bool score_hi = MODE == 0 ? bot_hi : (lane_last && g == last_g);
-        // the lane's next step, in this pass or (MULTI) at the start of the next
-        const bool wrap = MULTI && b + NB == nbp;
-        const int b_n = wrap ? 0 : b + NB, pass_n = wrap ? pass + 1 : pass;
-#pragma unroll
-        for (int j = 0; j < NB; ++j) {
-            if ((!MULTI || pass_n < npass) && (unsigned)(b_n + j) < (unsigned)nbw) {
-                const uint32_t o = (uint32_t)((b_n + j) * NG + k);
-                nx.w[j] = wres[o];
-                if (MULTI && top_lane && pass_n > 0) {
-                    nx.t[2 * j] = wbnd[2 * o];
-                    nx.t[2 * j + 1] = wbnd[2 * o + 1];
-                }
-            } else {
-                nx.w[j] = make_uint2(0, 0);
-            }
-        }
-
-#pragma unroll
-        for (int j = 0; j < NB; ++j) {
-            const uint2 w = in.w[j];
-            const uint4 t0 = in.t[2 * j], t1 = in.t[2 * j + 1];
-            const bool valid = pass < npass && (unsigned)(b + j) < (unsigned)nbw;
-
-            // the wave-level tests read both halves' flags with one or; the
-            // per-half flags are only taken inside the rare branches
-            if (__any(((w.x | w.y) & kResStart) != 0)) {
-                const bool st_lo = (w.x & kResStart) != 0, st_hi = (w.y & kResStart) != 0;
-                const uint32_t m = (st_lo ? 0x0000ffffu : 0u) | (st_hi ? 0xffff0000u : 0u);
-                if (st_lo) {
-                    pair_lo = none_lo ? -1 : po_lo + slot_entries[sb_lo + kp_lo];
-                    kp_lo++;
-                    blk_lo = 0;
-                    fa_lo = kNoFlag;
-                    fb_lo = kNoFlag;
-                }
-                if (st_hi) {
-                    pair_hi = none_hi ? -1 : po_hi + slot_entries[sb_hi + kp_hi];
-                    kp_hi++;
-                    blk_hi = 0;
-                    fa_hi = kNoFlag;
-                    fb_hi = kNoFlag;
-                }
-                TA = (m & TA_on) | (~m & TA);
-                TB = (m & TB_on) | (~m & TB);
-                if (MODE == 0) {
-#pragma unroll
-                    for (int r = 0; r < R; ++r) {
-                        Hl[r] = sel(m, Z, Hl[r]);
-                        El[r] = sel(m, Z, El[r]);
-                    }
-                    S = sel(m, Z, S);
-                    htop = sel(m, Z, htop);
-                } else {
-                    const int r0 = g * R;
-#pragma unroll
-                    for (int r = 0; r < R; ++r) {
-                        Hl[r] = sel(m, as_v(slbh[1 + r0 + r]), Hl[r]);
-                        El[r] = sel(m, as_v(slbe[r0 + r]), El[r]);
-                    }
-                    htop = sel(m, as_v(slbh[r0]), htop);  // row -1 of the first pass: the corner, 0
-                    if (FIRST) {
-#pragma unroll
-                        for (int c = 0; c < 4; ++c) {
-                            hb[c] = sel(m, as_v(P.hb_init[c]), hb[c]);
-                            fb[c] = sel(m, as_v(P.fb_init[c]), fb[c]);
-                        }
-                    }
-                }
-            }
-
-            // top boundary of this block: the lane above, one step earlier; the
-            // group's top lane takes the previous pass's bottom row or the matrix
-            // border.  tbh[3] also carries the next block's diagonal.
-            uint32_t toph[4], topf[4];
-            if (!FIRST) {
-                toph[0] = t0.x;
-                topf[0] = t0.y;
-                toph[1] = t0.z;
-                topf[1] = t0.w;
-                toph[2] = t1.x;
-                topf[2] = t1.y;
-                toph[3] = t1.z;
-                topf[3] = t1.w;
-            } else if (MODE == 0) {
-#pragma unroll
-                for (int c = 0; c < 4; ++c) {
-                    toph[c] = as_u(Z);
-                    topf[c] = as_u(Z);
-                }
-            } else {
-                toph[0] = as_u(hb[1]);
-                toph[1] = as_u(hb[2]);
-                toph[2] = as_u(hb[3]);
-                toph[3] = as_u(adds(hb[3], RR));
-#pragma unroll
-                for (int c = 0; c < 4; ++c) topf[c] = as_u(adds(fb[c], QR));
-            }
-            uint32_t tbh[4], tbf[4];
-#pragma unroll
-            for (int c = 0; c < 4; ++c) {
-                tbh[c] = shift_down<G>(toph[c], obh[4 * j + c], top_lane);
-                tbf[c] = shift_down<G>(topf[c], obf[4 * j + c], top_lane);
-                if (SEG) {  // a packed query's first row: the SW matrix border
-                    tbh[c] = (seg_top & as_u(Z)) | (~seg_top & tbh[c]);
-                    tbf[c] = (seg_top & as_u(Z)) | (~seg_top & tbf[c]);
-                }
-            }
-
-            s16x2 hmn = as_v(0u), hmx = as_v(0u);  // NW trackers, reset value of the reference
-            const bool any_last = __any(((w.x | w.y) & kResLast) != 0);
-            const int z_lo = (int)((w.x >> kResZShift) & 3u), z_hi = (int)((w.y >> kResZShift) & 3u);
-            // the block's four columns; CAP also records the NW score cell (the
-            // last query row at the pair's last column), a select per row on a
-            // wave-uniform condition, so the loop stays straight-line code
-            auto columns = [&](auto cap_tag) {
-                constexpr bool CAP = decltype(cap_tag)::value;
-                uint32_t cm_lo = 0u, cm_hi = 0u;
-                if (CAP && g == last_g && lane_last) {
-                    cm_lo = (w.x & kResLast) ? 0x0000ffffu : 0u;
-                    cm_hi = (w.y & kResLast) ? 0xffff0000u : 0u;
-                }
-                // The cell statements are written so that every dependent packed
-                // op has an independent one in front of it (gfx950 needs a wait
-                // state between a packed op and a packed op that reads its
-                // result).  The order is not pinned with scheduling barriers:
-                // instruction selection already reorders pure ops, and left free
-                // the post-RA scheduler removes most of the s_nop padding (what
-                // it leaves is hidden by the other waves of the SIMD).
-#pragma unroll
-                for (int c = 0; c < 4; ++c) {
-                    s16x2 f = as_v(tbf[c]);
-                    s16x2 fR = adds(f, RR);
-                    s16x2 v;
-                    if constexpr (PAIR) v = as_v(pl[0]);
-                    else v = pair_row(pl, ph, 0);
-                    s16x2 h1 = adds(htop, v);  // row 0: diagonal H from the row above
-                    htop = as_v(tbh[c]);
-                    s16x2 hq = Z;
-#pragma unroll
-                    for (int r = 0; r < R; ++r) {
-                        const bool more = r + 1 < R;
-                        if (more) {
-                            if constexpr (PAIR) v = as_v(pl[r + 1 < R ? r + 1 : 0]);
-                            else v = pair_row(pl, ph, r + 1);
-                        }
-                        // the next column's profile rows (the step's next block
-                        // after column 3), once this column's dwords are consumed
-                        if ((c < 3 || j + 1 < NB) && ((R == 4 && r == 3) || (R >= 8 && r % 8 == 7))) {
-                            const uint2 wc = c < 3 ? w : in.w[j + 1 < NB ? j + 1 : j];
-                            const int cn = c < 3 ? c + 1 : 0;
-                            if constexpr (PAIR) {
-                                read_part_pair<R>(sp + colofs(wc.x, cn), pl, r / 8);
-                            } else {
-                                read_part<R>(sp + colofs(wc.x, cn), pl, r / 8);
-                                read_part<R>(sp + colofs(wc.y, cn), ph, r / 8);
-                            }
-                        }
-                        s16x2 h = vmax(h1, f);
-                        const s16x2 eR = adds(El[r], RR);
-                        h = vmax(h, El[r]);
-                        if (more) h1 = adds(Hl[r], v);  // next row's diagonal term
-                        const s16x2 tt = adds(h, QR);
-                        if (MODE == 0) {
-                            S = vmax(S, h);
-                        } else if (MODE == 1) {
-                            hmn = vmin(hmn, h);
-                            hmx = vmax(hmx, h);
-                        }
-                        f = vmax(fR, tt);
-                        El[r] = vmax(eR, tt);
-                        if (more) fR = adds(f, RR);
-                        Hl[r] = h;
-                        if (CAP && r == last_r) hq = h;
-                    }
-                    obh[4 * j + c] = as_u(Hl[R - 1]);
-                    obf[4 * j + c] = as_u(f);
-                    if (CAP) {
-                        const uint32_t cm = (z_lo == c ? cm_lo : 0u) | (z_hi == c ? cm_hi : 0u);
-                        cap = sel(cm, hq, cap);
-                    }
-                }
-            };
-            // only the lane holding the last query row records a score cell
-            if (MODE >= 1 && __any(lane_last && g == last_g && ((w.x | w.y) & kResLast) != 0)) columns(Tag<true>{});
-            else columns(Tag<false>{});
-            // overflow predicates at the end of the block: a packed test here (in
-            // the cell loop's block, so the running reductions stay in the loop),
-            // the exact per-half bookkeeping below only where a predicate fired
-            bool fire;
-            if (MODE == 0) {
-                fire = as_u(vmax(S, as_v(TA))) != TA || as_u(vmax(S, as_v(TB))) != TB;
-            } else if (MODE == 1) {
-                fire = as_u(vmin(hmn, as_v(TA))) != TA || as_u(vmax(hmx, as_v(TB))) != TB;
-            } else {
-                fire = false;
-            }
-            if (__any(fire)) {
-                if (MODE == 0) {
-                    if (pair_lo >= 0) {
-                        const int v = lo16(S);
-                        if (v >= P.thr_a && fa_lo == kNoFlag) fa_lo = blk_lo;
-                        if (v >= P.thr_b && fb_lo == kNoFlag) fb_lo = blk_lo;
-                    }
-                    if (pair_hi >= 0) {
-                        const int v = hi16(S);
-                        if (v >= P.thr_a && fa_hi == kNoFlag) fa_hi = blk_hi;
-                        if (v >= P.thr_b && fb_hi == kNoFlag) fb_hi = blk_hi;
-                    }
-                } else {
-                    if (pair_lo >= 0 && fa_lo == kNoFlag && (lo16(hmn) < P.thr_a || lo16(hmx) >= P.thr_b))
-                        fa_lo = blk_lo;
-                    if (pair_hi >= 0 && fa_hi == kNoFlag && (hi16(hmn) < P.thr_a || hi16(hmx) >= P.thr_b))
-                        fa_hi = blk_hi;
-                }
-                const uint32_t offa = (fa_lo != kNoFlag ? 0x0000ffffu : 0u) | (fa_hi != kNoFlag ? 0xffff0000u : 0u);
-                const uint32_t offb =
-                    MODE == 0 ? ((fb_lo != kNoFlag ? 0x0000ffffu : 0u) | (fb_hi != kNoFlag ? 0xffff0000u : 0u)) : offa;
-                TA = (offa & TA_off) | (~offa & TA);
-                TB = (offb & TB_off) | (~offb & TB);
-            }
-            if (MULTI && bot_out && valid) {
-                const uint32_t o = (uint32_t)((b + j) * NG + k) * 2;
-                wbnd[o] = make_uint4(obh[4 * j], obf[4 * j], obh[4 * j + 1], obf[4 * j + 1]);
-                wbnd[o + 1] = make_uint4(obh[4 * j + 2], obf[4 * j + 2], obh[4 * j + 3], obf[4 * j + 3]);
-            }
-
-            if (MODE >= 1 && FIRST) {  // the reference advances the top border by four columns per block
-                fb[0] = adds(fb[3], RR);
-                fb[1] = adds(fb[0], RR);
-                fb[2] = adds(fb[1], RR);
-                fb[3] = adds(fb[2], RR);
-                hb[0] = adds(hb[3], RR);
-                hb[1] = adds(hb[0], RR);
-                hb[2] = adds(hb[1], RR);
-                hb[3] = adds(hb[2], RR);
-            }
-            if (any_last) {
-                const bool ls_lo = (w.x & kResLast) != 0, ls_hi = (w.y & kResLast) != 0;
-                if (MODE == 0) {  // the lane above finished the same pair one step earlier
-                    uint32_t scin = shift_down<G>(as_u(Z), scout[j], top_lane);
-                    if (SEG) scin = (seg_top & as_u(Z)) | (~seg_top & scin);
-                    scout[j] = as_u(vmax(S, as_v(scin)));
-                }
-                const s16x2 fin = MODE == 0 ? as_v(scout[j]) : cap;
-                uint32_t m = 0u;
-                if (ls_lo && pair_lo >= 0) {
-                    finish_pair<MODE>(results, pair_lo, score_lo, lo16(fin), fa_lo, fb_lo);
-                    pair_lo = -1;
-                    m |= 0x0000ffffu;
-                }
-                if (ls_hi && pair_hi >= 0) {
-                    finish_pair<MODE>(results, pair_hi, score_hi, hi16(fin), fa_hi, fb_hi);
-                    pair_hi = -1;
-                    m |= 0xffff0000u;
-                }
-                TA = (m & TA_off) | (~m & TA);
-                TB = (m & TB_off) | (~m & TB);
-            }
-            blk_lo++;
-            blk_hi++;
-        }
-        b = b_n;
-        if (MULTI && __any(wrap)) {  // lanes entering their next pass: its entry list and staged buffers
-            if (wrap) {
-                pass = pass_n;
-                kp_lo = 0;
-                kp_hi = 0;
-                gbase ^= kProfBytes;
-                load_seg(pass);
-                const bool odd = pass & 1;
-                slbh = odd ? slbh0 + (PR + 1) : slbh0;
-                slbe = odd ? slbe0 + PR : slbe0;
-            }
-        }
-    };
-
-    // the blocks of step 0 (-NB g ..): only the top lanes start with real blocks
-    StepIn<NB> sa, sb;
-#pragma unroll
-    for (int j = 0; j < NB; ++j) {
-        sa.w[j] = sb.w[j] = make_uint2(0, 0);
-        sa.t[2 * j] = sa.t[2 * j + 1] = sb.t[2 * j] = sb.t[2 * j + 1] = make_uint4(0, 0, 0, 0);
-        if (g == 0 && j < nbw) sa.w[j] = wres[j * NG + k];
-    }
-    // steps with the top lane in the first pass (an even number: nbp is a
-    // multiple of 2 * NB)
-    const int spp = nbp / NB;  // steps per pass
-    for (int t = 0; t < spp; t += 2) {
-        step(Tag<true>{}, sa, sb);
-        step(Tag<true>{}, sb, sa);
-    }
-    // the rest: pass p + 2 is staged once lane G - 1 has left pass p
-    const int nsteps = (npass * spp + G) & ~1;
-    int next_stage = ((spp + G - 1) + 1) & ~1, staged = 2;
-    for (int t = spp; t < nsteps; t += 2) {
-        if (MULTI && t == next_stage && staged < npass) {  // the same for the whole workgroup
-            stage(staged);
-            ++staged;
-            next_stage += spp;
-        }
-        step(Tag<false>{}, sa, sb);
-        step(Tag<false>{}, sb, sa);
-    }
-}
-
-template <int R, int G, int MODE, int NB>
-__global__ __launch_bounds__(64 * kWavesPerWG) void score_kernel(const ScoreLaunch L, const ScoreParams P) {
-    constexpr int PR = G * R;  // rows per pass
-    static_assert((1 << Log2<PR>::value) == PR, "rows per pass must be a power of two");
-    static_assert(PR <= 512, "two staged profiles of 32 codes x PR rows must fit the LDS budget");
-    __shared__ __align__(16) int16_t sprof[2 * 32 * PR];      // profiles of passes p (even, odd)
-    __shared__ uint32_t slbh[MODE >= 1 ? 2 * (PR + 1) : 1];  // NW left boundary H of rows base-1 .. base+PR-1
-    __shared__ uint32_t slbe[MODE >= 1 ? 2 * PR : 1];        // NW left boundary E of rows base .. base+PR-1
-    __shared__ int s_task;
-
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    const int k = lane / G, g = lane % G;
-    const uint32_t gbase = (uint32_t)(g * R * 2);  // byte offset of the lane's rows in a profile code row
-    const char* sp = reinterpret_cast<const char*>(sprof);
-    uint4* __restrict__ wbnd = L.bnd + ((int64_t)blockIdx.x * kWavesPerWG + wv) * L.bnd_per_wave;
-
-    for (;;) {
-        __syncthreads();  // the previous task is done with s_task and the staged profiles
-        if (threadIdx.x == 0) s_task = atomicAdd(L.counter, 1);
-        __syncthreads();
-        const int task = s_task;
-        if (task >= L.ntasks) break;  // the same for the whole workgroup
-        const ScoreTask T = L.tasks[task];
-        const ScoreWave Wv = L.waves[T.wave0 + wv];
-        const QueryDesc Q = L.queries[T.query];
-        const int npass = (Q.qlen + PR - 1) / PR;
-        // the workgroup's waves share the pass length: the longest of them,
-        // at least G + 2 steps (so that a pass's bottom rows are written
-        // before the next pass reads them) and an even number of steps
-        int nbmax = 0;
-#pragma unroll
-        for (int i = 0; i < kWavesPerWG; ++i) nbmax = max(nbmax, L.waves[T.wave0 + i].nblocks);
-        if (nbmax == 0) continue;
-        const int nbp = ((npass > 1 ? max(nbmax, NB * (G + 2)) : nbmax) + 2 * NB - 1) / (2 * NB) * (2 * NB);
-        const uint2* __restrict__ wres = L.res + Wv.res_off;
-        const int sb_lo = L.slot_first[Wv.slot_base + 2 * k];
-        const int sb_hi = L.slot_first[Wv.slot_base + 2 * k + 1];
-        const int last_lr = Q.qlen - 1 - (npass - 1) * PR;  // last real row within the last pass
-        const int last_g = last_lr / R, last_r = last_lr % R;
-
-        // pass p's profile (and NW left border) into buffer p & 1
-        auto load_pass = [&](int pass) {
-            const int buf = pass & 1;
-            const uint4* src = reinterpret_cast<const uint4*>(L.qp + Q.qp_off + (int64_t)pass * 32 * PR);
-            uint4* dst = reinterpret_cast<uint4*>(sprof + buf * 32 * PR);
-            for (int i = threadIdx.x; i < 32 * PR / 8; i += 64 * kWavesPerWG) dst[i] = src[i];
-            if (MODE >= 1) {
-                const int base = pass * PR;
-                uint32_t* bh = slbh + buf * (PR + 1);
-                uint32_t* be = slbe + buf * PR;
-                for (int i = threadIdx.x; i < PR; i += 64 * kWavesPerWG) {
-                    bh[1 + i] = L.lb[Q.lb_off + 2 * (base + i)];
-                    be[i] = L.lb[Q.lb_off + 2 * (base + i) + 1];
-                }
-                if (threadIdx.x == 0) bh[0] = base > 0 ? L.lb[Q.lb_off + 2 * (base - 1)] : 0u;
-            }
-        };
-        load_pass(0);
-        if (npass > 1) load_pass(1);
-        __syncthreads();
-        auto stage = [&](int pass) {
-            __syncthreads();  // every lane of the workgroup has left pass - 2
-            load_pass(pass);
-            __syncthreads();
-        };
-        if (MODE == 0 && Q.seg_off >= 0)
-            run_task<R, G, MODE, NB, true, MODE == 0, false>(P, gbase, sp, slbh, slbe, wres, wbnd, L.slot_entries,
-                                                         Q.pair_off, L.segs + Q.seg_off, nullptr, L.results, sb_lo,
-                                                         sb_hi, k, g, Wv.nblocks, nbp, npass, last_g, last_r, stage);
-        else if (npass > 1)
-            run_task<R, G, MODE, NB, true, false, false>(P, gbase, sp, slbh, slbe, wres, wbnd, L.slot_entries,
-                                                     Q.pair_off, nullptr, nullptr, L.results, sb_lo, sb_hi, k, g,
-                                                     Wv.nblocks, nbp, npass, last_g, last_r, stage);
-        else
-            run_task<R, G, MODE, NB, false, false, false>(P, gbase, sp, slbh, slbe, wres, wbnd, L.slot_entries,
-                                                      Q.pair_off, nullptr, nullptr, L.results, sb_lo, sb_hi, k, g,
-                                                      Wv.nblocks, nbp, npass, last_g, last_r, stage);
-    }
-}
-
-// Paired packed SW queries (run_task's PAIR): two packed queries share every
-// stream, one per half.  The staged profile interleaves their rows (one
-// dword per row), twice the bytes per pass of score_kernel, so a pass holds at
-// most 256 rows.
-template <int R, int G>
-__global__ __launch_bounds__(64 * kWavesPerWG) void score_kernel_pair(const ScoreLaunch L, const ScoreParams P) {
-    constexpr int PR = G * R;
-    static_assert((1 << Log2<PR>::value) == PR, "rows per pass must be a power of two");
-    static_assert(PR <= 256, "two staged paired profiles of 32 codes x PR rows must fit the LDS budget");
-    __shared__ __align__(16) int16_t sprof[2 * 2 * 32 * PR];  // paired profiles of passes p (even, odd)
-    __shared__ uint32_t slb_unused[1];                         // (the NW left border of score_kernel)
-    __shared__ int s_task;
-
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    const int k = lane / G, g = lane % G;
-    const uint32_t gbase = (uint32_t)(g * R * 4);  // byte offset of the lane's rows in a paired profile code row
-    const char* sp = reinterpret_cast<const char*>(sprof);
-    uint4* __restrict__ wbnd = L.bnd + ((int64_t)blockIdx.x * kWavesPerWG + wv) * L.bnd_per_wave;
-
-    for (;;) {
-        __syncthreads();  // the previous task is done with s_task and the staged profiles
-        if (threadIdx.x == 0) s_task = atomicAdd(L.counter, 1);
-        __syncthreads();
-        const int task = s_task;
-        if (task >= L.ntasks) break;  // the same for the whole workgroup
-        const ScoreTask T = L.tasks[task];
-        const ScoreWave Wv = L.waves[T.wave0 + wv];
-        const QueryDesc Q = L.queries[T.query];
-        const int npass = (Q.qlen + PR - 1) / PR;
-        int nbmax = 0;
-#pragma unroll
-        for (int i = 0; i < kWavesPerWG; ++i) nbmax = max(nbmax, L.waves[T.wave0 + i].nblocks);
-        if (nbmax == 0) continue;
-        const int nbp = npass > 1 ? (max(nbmax, G + 2) + 1) & ~1 : (nbmax + 1) & ~1;
-        const uint2* __restrict__ wres = L.res + Wv.res_off;
-        const int sb_lo = L.slot_first[Wv.slot_base + 2 * k];
-        const int sb_hi = L.slot_first[Wv.slot_base + 2 * k + 1];
-        auto load_pass = [&](int pass) {
-            const int buf = pass & 1;
-            const uint4* src = reinterpret_cast<const uint4*>(L.qp + Q.qp_off + (int64_t)pass * 2 * 32 * PR);
-            uint4* dst = reinterpret_cast<uint4*>(sprof + buf * 2 * 32 * PR);
-            for (int i = threadIdx.x; i < 2 * 32 * PR / 8; i += 64 * kWavesPerWG) dst[i] = src[i];
-        };
-        load_pass(0);
-        if (npass > 1) load_pass(1);
-        __syncthreads();
-        auto stage = [&](int pass) {
-            __syncthreads();  // every lane of the workgroup has left pass - 2
-            load_pass(pass);
-            __syncthreads();
-        };
-        run_task<R, G, 0, 1, true, true, true>(P, gbase, sp, slb_unused, slb_unused, wres, wbnd, L.slot_entries,
-                                            Q.pair_off, L.segs + Q.seg_off, L.segs + Q.seg_off_hi, L.results, sb_lo,
-                                            sb_hi, k, g, Wv.nblocks, nbp, npass, 0, 0, stage);
-    }
-}
 
 __global__ void fill_u32_kernel(uint32_t* __restrict__ p, int64_t n, uint32_t v) {
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
@@ -895,51 +168,21 @@ __global__ __launch_bounds__(64) void pack_kernel(const PackEntry* __restrict__ 
 
 
 // ---------------------------------------------------------------------------
-// launch wrappers (host side)
+// launch wrappers (host side): dispatch to the per-mode instantiations
 // ---------------------------------------------------------------------------
-template <int R, int G, int MODE, int NB>
-static hipError_t launch_score_t(hipStream_t st, const ScoreLaunch& L, const ScoreParams& P, int max_wg) {
-    const int nwg = std::max(1, std::min(L.ntasks, max_wg));
-    hipLaunchKernelGGL(fill_u32_kernel, dim3(1), dim3(64), 0, st, reinterpret_cast<uint32_t*>(L.counter),
-                       (int64_t)1, 0u);
-    hipLaunchKernelGGL((score_kernel<R, G, MODE, NB>), dim3(nwg), dim3(64 * kWavesPerWG), 0, st, L, P);
+hipError_t launch_fill_u32(hipStream_t st, uint32_t* p, int64_t n, uint32_t v) {
+    if (n <= 0) return hipSuccess;
+    const int64_t blocks = std::min<int64_t>((n + 255) / 256, 65536);
+    hipLaunchKernelGGL(fill_u32_kernel, dim3((unsigned)blocks), dim3(256), 0, st, p, n, v);
     return hipGetLastError();
-}
-
-template <int R, int G, int MODE, int NB>
-static int resident_wg_t() {
-    static int cap = 0;
-    if (!cap) {
-        int per_cu = 0, dev = 0;
-        hipDeviceProp_t prop;
-        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, score_kernel<R, G, MODE, NB>, 64 * kWavesPerWG,
-                                                         0) != hipSuccess ||
-            hipGetDevice(&dev) != hipSuccess || hipGetDeviceProperties(&prop, dev) != hipSuccess)
-            return 0;
-        cap = std::max(1, per_cu) * prop.multiProcessorCount;
-    }
-    return cap;
-}
-
-// NW without trackers (MODE 2) exists for the common NW shapes only
-template <int R, int G>
-constexpr bool nw_untracked_shape() {
-    return (G == 8 && (R == 8 || R == 16)) || (G == 16 && (R == 8 || R == 16)) || (G == 32 && R == 8);
-}
-
-// 8-column tiles (two blocks per step) for lanes of at most 16 rows in groups
-// of 8 or 16 lanes: the step's second block holds 4 more bottom-row H/F pairs
-// and residue/boundary loads, which 32-row lanes have no registers for
-template <int R, int G>
-constexpr bool tile8_shape() {
-    return R <= 16 && (G == 8 || G == 16);
 }
 
 // Passes of at most 512 rows (two staged profiles of 32 KiB); 32 rows per
 // lane for SW only (NW holds more state per row and would run one wave per SIMD)
 bool score_shape_supported(int lanes, int rows, int mode, int cols) {
+    const bool sw = mode == 0 || mode == 3;
     return (lanes == 8 || lanes == 16 || lanes == 32 || lanes == 64) &&
-           (rows == 4 || rows == 8 || rows == 16 || (rows == 32 && mode == 0)) && lanes * rows <= 512 &&
+           (rows == 4 || rows == 8 || rows == 16 || (rows == 32 && sw)) && lanes * rows <= 512 &&
            (cols == 4 || (cols == 8 && rows <= 16 && (lanes == 8 || lanes == 16)));
 }
 
@@ -948,89 +191,27 @@ bool score_nw_untracked_supported(int lanes, int rows) {
            (lanes == 32 && rows == 8);
 }
 
-template <int R, int G, int MODE, int NB>
-static constexpr bool shape_built() {
-    return !(R == 32 && MODE >= 1) && !(MODE == 2 && !nw_untracked_shape<R, G>()) && (NB == 1 || tile8_shape<R, G>());
-}
-
-template <int R, int G, int MODE, int NB>
-static int resident_wg_shape() {
-    if constexpr (!shape_built<R, G, MODE, NB>()) return 0;
-    else return resident_wg_t<R, G, MODE, NB>();
-}
-
-template <int R, int G, int MODE, int NB>
-static hipError_t launch_shape(hipStream_t st, const ScoreLaunch& L, const ScoreParams& P, int max_wg) {
-    if constexpr (!shape_built<R, G, MODE, NB>()) return hipErrorInvalidValue;
-    else return launch_score_t<R, G, MODE, NB>(st, L, P, max_wg);
-}
-
-#define SSA_SHAPES(X)                                                                                   \
-    X(8, 4) X(8, 8) X(8, 16) X(8, 32) X(16, 4) X(16, 8) X(16, 16) X(16, 32) X(32, 4) X(32, 8) X(32, 16) \
-    X(64, 4) X(64, 8)
-
-template <int NB>
-static int resident_nb(int lanes, int rows, int mode) {
-#define SSA_CASE(GG, RR)                                                                              \
-    if (lanes == GG && rows == RR)                                                                  \
-        return mode == 0 ? resident_wg_shape<RR, GG, 0, NB>()                                       \
-                         : (mode == 1 ? resident_wg_shape<RR, GG, 1, NB>() : resident_wg_shape<RR, GG, 2, NB>());
-    SSA_SHAPES(SSA_CASE)
-#undef SSA_CASE
-    return 0;
-}
-
 int score_resident_workgroups(int lanes, int rows, int mode, int cols) {
-    return cols == 8 ? resident_nb<2>(lanes, rows, mode) : resident_nb<1>(lanes, rows, mode);
-}
-
-template <int NB>
-static hipError_t launch_nb(int lanes, int rows, int mode, hipStream_t st, const ScoreLaunch& L, const ScoreParams& P,
-                            int max_wg) {
-#define SSA_CASE(GG, RR)                                                                               \
-    if (lanes == GG && rows == RR)                                                                   \
-        return mode == 0 ? launch_shape<RR, GG, 0, NB>(st, L, P, max_wg)                             \
-                         : (mode == 1 ? launch_shape<RR, GG, 1, NB>(st, L, P, max_wg)                \
-                                      : launch_shape<RR, GG, 2, NB>(st, L, P, max_wg));
-    SSA_SHAPES(SSA_CASE)
-#undef SSA_CASE
-    return hipErrorInvalidValue;
+    switch (mode) {
+    case 0: return resident_mode<0>(lanes, rows, cols);
+    case 1: return resident_mode<1>(lanes, rows, cols);
+    case 2: return resident_mode<2>(lanes, rows, cols);
+    case 3: return resident_mode<3>(lanes, rows, cols);
+    default: return 0;
+    }
 }
 
 hipError_t launch_score(int lanes, int rows, int mode, int cols, hipStream_t st, const ScoreLaunch& L,
                         const ScoreParams& P, int max_wg) {
     if (L.ntasks <= 0) return hipSuccess;
-    if (cols == 8) return launch_nb<2>(lanes, rows, mode, st, L, P, max_wg);
-    if (cols == 4) return launch_nb<1>(lanes, rows, mode, st, L, P, max_wg);
-    return hipErrorInvalidValue;
-}
-
-template <int R, int G>
-static hipError_t launch_pair_t(hipStream_t st, const ScoreLaunch& L, const ScoreParams& P, int max_wg) {
-    const int nwg = std::max(1, std::min(L.ntasks, max_wg));
-    hipLaunchKernelGGL(fill_u32_kernel, dim3(1), dim3(64), 0, st, reinterpret_cast<uint32_t*>(L.counter),
-                       (int64_t)1, 0u);
-    hipLaunchKernelGGL((score_kernel_pair<R, G>), dim3(nwg), dim3(64 * kWavesPerWG), 0, st, L, P);
-    return hipGetLastError();
-}
-
-template <int R, int G>
-static int resident_pair_t() {
-    static int cap = 0;
-    if (!cap) {
-        int per_cu = 0, dev = 0;
-        hipDeviceProp_t prop;
-        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, score_kernel_pair<R, G>, 64 * kWavesPerWG, 0) !=
-                hipSuccess ||
-            hipGetDevice(&dev) != hipSuccess || hipGetDeviceProperties(&prop, dev) != hipSuccess)
-            return 0;
-        cap = std::max(1, per_cu) * prop.multiProcessorCount;
+    switch (mode) {
+    case 0: return launch_mode<0>(lanes, rows, cols, st, L, P, max_wg);
+    case 1: return launch_mode<1>(lanes, rows, cols, st, L, P, max_wg);
+    case 2: return launch_mode<2>(lanes, rows, cols, st, L, P, max_wg);
+    case 3: return launch_mode<3>(lanes, rows, cols, st, L, P, max_wg);
+    default: return hipErrorInvalidValue;
     }
-    return cap;
 }
-
-// shapes of the paired kernel (lanes per stream pair, rows per lane)
-#define SSA_PAIR_SHAPES(X) X(8, 16) X(8, 32) X(16, 8) X(16, 16) X(32, 8)
 
 bool score_pair_supported(int lanes, int rows) {
 #define SSA_CASE(GG, RR) \
@@ -1040,22 +221,15 @@ bool score_pair_supported(int lanes, int rows) {
     return false;
 }
 
-int score_pair_resident_workgroups(int lanes, int rows) {
-#define SSA_CASE(GG, RR) \
-    if (lanes == GG && rows == RR) return resident_pair_t<RR, GG>();
-    SSA_PAIR_SHAPES(SSA_CASE)
-#undef SSA_CASE
-    return 0;
+int score_pair_resident_workgroups(int lanes, int rows, bool f16) {
+    return f16 ? resident_pair_mode<true>(lanes, rows) : resident_pair_mode<false>(lanes, rows);
 }
 
-hipError_t launch_score_pair(int lanes, int rows, hipStream_t st, const ScoreLaunch& L, const ScoreParams& P,
+hipError_t launch_score_pair(int lanes, int rows, bool f16, hipStream_t st, const ScoreLaunch& L, const ScoreParams& P,
                              int max_wg) {
     if (L.ntasks <= 0) return hipSuccess;
-#define SSA_CASE(GG, RR) \
-    if (lanes == GG && rows == RR) return launch_pair_t<RR, GG>(st, L, P, max_wg);
-    SSA_PAIR_SHAPES(SSA_CASE)
-#undef SSA_CASE
-    return hipErrorInvalidValue;
+    return f16 ? launch_pair_mode<true>(lanes, rows, st, L, P, max_wg)
+               : launch_pair_mode<false>(lanes, rows, st, L, P, max_wg);
 }
 
 hipError_t launch_init_results(hipStream_t st, PairResult* r, int64_t n, int32_t score) {
@@ -1066,12 +240,8 @@ hipError_t launch_init_results(hipStream_t st, PairResult* r, int64_t n, int32_t
 
 hipError_t launch_pack(int nentries, hipStream_t st, const PackEntry* entries, const uint8_t* codes,
                        uint32_t* res, int64_t res_words) {
-    if (res_words > 0) {
-        const int64_t blocks = std::min<int64_t>((res_words + 255) / 256, 65536);
-        hipLaunchKernelGGL(fill_u32_kernel, dim3((unsigned)blocks), dim3(256), 0, st, res, res_words, 0u);
-        const hipError_t e = hipGetLastError();
-        if (e != hipSuccess) return e;
-    }
+    const hipError_t e = launch_fill_u32(st, res, res_words, 0u);
+    if (e != hipSuccess) return e;
     if (nentries <= 0) return hipSuccess;
     hipLaunchKernelGGL(pack_kernel, dim3(nentries), dim3(64), 0, st, entries, codes, res);
     return hipGetLastError();

@@ -1,0 +1,9 @@
+// score_nw1.hip -- score_kernel instantiations of mode 1 (NW with the
+// overflow trackers, packed int16).
+#define SSA_SCORE_INSTANTIATE
+#include "score_kernel.h"
+
+namespace ssa {
+template hipError_t launch_mode<1>(int, int, int, hipStream_t, const ScoreLaunch&, const ScoreParams&, int);
+template int resident_mode<1>(int, int, int);
+}  // namespace ssa

@@ -1,0 +1,8 @@
+// score_sw.hip -- score_kernel instantiations of mode 0 (SW, packed int16).
+#define SSA_SCORE_INSTANTIATE
+#include "score_kernel.h"
+
+namespace ssa {
+template hipError_t launch_mode<0>(int, int, int, hipStream_t, const ScoreLaunch&, const ScoreParams&, int);
+template int resident_mode<0>(int, int, int);
+}  // namespace ssa

@@ -286,6 +286,19 @@ const Tier kTier8{8, 8}, kTier16{16, 0};
 
 uint32_t pack2(int32_t v) { return ((uint32_t)(uint16_t)v << 16) | (uint32_t)(uint16_t)v; }
 
+// IEEE binary16 bits of an integer that binary16 holds exactly (at most 11
+// significant bits, magnitude below 2^16)
+int32_t f16_bits(int64_t v) {
+    if (v == 0) return 0;
+    const uint32_t sign = v < 0 ? 0x8000u : 0u;
+    const uint64_t a = (uint64_t)(v < 0 ? -v : v);
+    int e = 0;
+    while ((a >> (e + 1)) != 0) ++e;  // a = 1.m * 2^e
+    if (e > 15 || (e > 10 && (a & ((1ull << (e - 10)) - 1)) != 0)) fatal("libssa_amd: %lld is not an fp16 integer", (long long)v);
+    const uint32_t mant = (uint32_t)(e <= 10 ? (a << (10 - e)) : (a >> (e - 10))) & 0x3ffu;
+    return (int32_t)(sign | ((uint32_t)(e + 15) << 10) | mant);
+}
+
 int64_t now_us() {
     return std::chrono::duration_cast<std::chrono::microseconds>(
                std::chrono::steady_clock::now().time_since_epoch())
@@ -309,6 +322,28 @@ MatrixInfo matrix_info() {
         if (v < INT32_MIN / 4 || v > INT32_MAX / 4) mi.fits32 = false;
     }
     return mi;
+}
+
+// Mode 3 of the score kernel (score_kernel.h) keeps each lane's values in an
+// fp16 window around an offset of its own, re-centred (maximum near 1024)
+// once the window's maximum passes 1536 and every 128 columns while the
+// offset is above 0.  True when the matrix and gaps keep every value a lane
+// of `rows` query rows holds within +-2047 of its offset: a cell differs
+// from its neighbours by at most max score + |gap open + extend| (step), so
+// a lane's values lie within (rows - 1) steps (E and F up to |open + extend|
+// more) below its maximum, a block of four columns raises the maximum by at
+// most four steps (one more for the values handed down), and 128 columns
+// lower it by at most |open + extend| + 127 |extend|.
+bool f16_window_ok(int rows, int64_t qr, int64_t r) {
+    const MatrixInfo mi = matrix_info();
+    if (mi.maxv > 1024 || mi.minv < -1024) return false;
+    const int64_t smax = std::max<int64_t>(0, mi.maxv);
+    const int64_t aqr = qr < 0 ? -qr : qr, ar = r < 0 ? -r : r;
+    const int64_t step = smax + aqr;
+    const int64_t spread = (int64_t)(rows - 1) * step + aqr;
+    const int64_t grow = 5 * step;
+    const int64_t fall = aqr + 127 * ar;
+    return 1536 + grow <= 2047 && 1024 - spread - fall >= -2047;
 }
 
 // True when no H, E or F value of an alignment of a query of qlen residues
@@ -352,6 +387,7 @@ struct Shape {
 
 Shape& shape_override_ref();
 int& cols_override_ref();
+int& arith_override_ref();
 void release_layouts();
 
 namespace {
@@ -738,7 +774,8 @@ struct TierRun {
     bool packed = false;  // SW with several queries: scored as packed queries
     bool paired = false;  // packed queries two by two, one per half of every lane group (score_kernel_pair)
     bool untracked = false;  // NW 16-bit where no pair can overflow: the kernel without trackers
-    int mode() const { return type == kSW ? 0 : (untracked ? 2 : 1); }
+    bool f16 = false;        // SW first tier in windowed packed fp16 (kernel mode 3)
+    int mode() const { return type == kSW ? (f16 ? 3 : 0) : (untracked ? 2 : 1); }
 
     ~TierRun() { local.release(); }
     bool empty() const { return ps == nullptr || ps->list().empty() || qs.empty(); }
@@ -773,6 +810,8 @@ void tier_prepare(Exec& ex, TierRun& R) {
     const Shape sh = R.sh;
     if (!score_shape_supported(sh.lanes, sh.rows, R.type == kSW ? 0 : 1, sh.cols))
         fatal("libssa_amd: unsupported kernel shape");
+    R.f16 = R.type == kSW && T.bits == 16 && arith_override_ref() != 1 &&
+            f16_window_ok(sh.rows, T.wrap(cf.gap_open + cf.gap_extend), T.wrap(cf.gap_extend));
     const int PR = sh.lanes * sh.rows, ng = 64 / sh.lanes, spw = 2 * ng;
 
     // ---- profiles [pass][code][row], left boundaries, per-query descriptors ----
@@ -834,6 +873,13 @@ void tier_prepare(Exec& ex, TierRun& R) {
         P.thr_a = T.repr(T.wrap(T.imin() - go - ge - 1));
         P.thr_b = T.repr(T.imax());
     }
+    if (R.f16) {  // real scores in fp16 windows: real 0 at offset 0, thresholds as real maxima
+        P.qr = pack2(f16_bits(QRw));
+        P.r = pack2(f16_bits(Rw));
+        P.zero = 0;
+        P.thr_a = 255;    // the 8-bit tier's overflow
+        P.thr_b = 65535;  // the 16-bit tier's overflow
+    }
     P.hb_init[0] = 0;
     for (int k = 1; k < 4; ++k) P.hb_init[k] = pack2(T.repr(T.wrap(go + k * ge)));
     for (int k = 0; k < 4; ++k) P.fb_init[k] = pack2(T.repr(T.wrap(go + (k + 1) * ge)));
@@ -875,7 +921,7 @@ void tier_prepare(Exec& ex, TierRun& R) {
         for (size_t qi = 0; qi < nq && R.untracked; ++qi)
             R.untracked = int16_exact(R.qs[qi].len, 4 * (int64_t)L->max_entry_blocks, mi);
     }
-    const int cap = R.paired ? score_pair_resident_workgroups(sh.lanes, sh.rows)
+    const int cap = R.paired ? score_pair_resident_workgroups(sh.lanes, sh.rows, R.f16)
                              : score_resident_workgroups(sh.lanes, sh.rows, R.mode(), sh.cols);
 
     // ---- packed queries (SW, several queries) ----
@@ -1050,7 +1096,8 @@ void tier_prepare(Exec& ex, TierRun& R) {
     const int64_t nb_step = sh.cols / 4;
     const int64_t max_nbp =
         (std::max<int64_t>(max_nb, nb_step * (sh.lanes + 2)) + 2 * nb_step - 1) / (2 * nb_step) * (2 * nb_step);
-    const int64_t bnd_per_wave = multipass ? max_nbp * ng * 2 : 0;
+    // (mode 3 stores each block's offsets beside its bottom row: 3 uint4)
+    const int64_t bnd_per_wave = multipass ? max_nbp * ng * (R.f16 ? 3 : 2) : 0;
 
     // ---- device buffers ----
     auto name = [&](const char* s) { return R.pfx + s; };
@@ -1079,6 +1126,10 @@ void tier_prepare(Exec& ex, TierRun& R) {
     R.d_out = (PairResult*)ex.arena(name("results"), R.nout * sizeof(PairResult));
     ex.upload(d_tasks, R.tasks.data(), R.tasks.size() * sizeof(ScoreTask));
     ex.upload(d_qd, R.qd.data(), R.qd.size() * sizeof(QueryDesc));
+    // the fp16 kernel's profile: the 16-bit tier's scores as fp16 values (the
+    // padding rows' -32768 as well)
+    if (R.f16)
+        for (int16_t& x : R.prof) x = (int16_t)f16_bits(x);
     ex.upload(d_prof, R.prof.data(), R.prof.size() * sizeof(int16_t));
     ex.upload(d_lb, R.lbv.data(), R.lbv.size() * sizeof(uint32_t));
     ex.upload(d_segs, R.segs.data(), R.segs.size() * sizeof(SegLane));
@@ -1103,7 +1154,7 @@ void tier_prepare(Exec& ex, TierRun& R) {
 void tier_launch(hipStream_t st, TierRun& R) {
     if (R.empty()) return;
     SSA_HIP(launch_init_results(st, R.d_out, (int64_t)R.nout, -32768));
-    if (R.paired) SSA_HIP(launch_score_pair(R.sh.lanes, R.sh.rows, st, R.K, R.P, R.grid));
+    if (R.paired) SSA_HIP(launch_score_pair(R.sh.lanes, R.sh.rows, R.f16, st, R.K, R.P, R.grid));
     else SSA_HIP(launch_score(R.sh.lanes, R.sh.rows, R.mode(), R.sh.cols, st, R.K, R.P, R.grid));
 }
 
@@ -1535,6 +1586,13 @@ int& cols_override_ref() {
 }
 
 void set_cols_override(int cols) { cols_override_ref() = cols; }
+
+int& arith_override_ref() {
+    static int a = 0;  // 0: fp16 windows for the SW first tier where they hold; 1: int16 everywhere
+    return a;
+}
+
+void set_arith_override(int arith) { arith_override_ref() = arith; }
 
 int& placement_override_ref() {
     static int p = 0;  // 0: chosen per search
@@ -2294,6 +2352,7 @@ void run_batch(Exec& ex, const std::vector<SearchRequest>& reqs, std::vector<Sea
             J.st.columns_per_step = R.sh.cols;
             J.st.profile_placement = 1;
             J.st.overflow_tracking = R.mode() != 2;
+            J.st.cell_arithmetic = R.f16 ? 2 : 1;
         }
         if (J.dev_top) {
             // decoded later, only for the chunks the replay simulates; here the
@@ -2991,7 +3050,8 @@ RequestPlan plan_request(const SearchRequest& rq, const std::vector<size_t>& qle
     const int grid = std::min(tasks, cap);
     pl.occupancy = (double)grid / (double)cap;
     if (passes > 1)
-        bytes += (size_t)grid * kWavesPerWG * (size_t)(max_nb + 2) * (size_t)(64 / sh.lanes) * 2 * sizeof(uint4);
+        bytes += (size_t)grid * kWavesPerWG * (size_t)(max_nb + 2) * (size_t)(64 / sh.lanes) * (type == 0 ? 3 : 2) *
+                 sizeof(uint4);
     if (rq.align_type != COMPUTE_SCORE) {
         int64_t dmax = 0;
         for (uint32_t e = 0; e < N; ++e) dmax = std::max<int64_t>(dmax, db.entries[e].len);

@@ -119,6 +119,7 @@ ssa_amd_stats_t& last_stats();
 void set_shape_override(int lanes, int rows);  // 0, 0 = chosen per query length
 void set_cols_override(int cols);             // 0 = chosen per search, else 4 or 8
 void set_placement_override(int placement);   // 0 = chosen per search, 1 LDS, 2 VGPRs
+void set_arith_override(int arith);           // 0 = fp16 windows for the SW first tier, 1 = int16 everywhere
 void set_target_waves(int waves);             // 0 = default
 
 }  // namespace ssa

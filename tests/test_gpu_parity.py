@@ -231,11 +231,15 @@ def test_random_parity(seed):
     ssa.set_kernel_shape(*(rnd.choice(shapes) if seed >= 24 else (0, 0)))
     # and a forced column tile (shapes without 8-column tiles keep 4)
     ssa.set_kernel_variant(rnd.choice([0, 4, 8]) if seed >= 24 else 0, 0)
+    # the SW first tier in int16 instead of fp16 windows: three of the default
+    # cases, a random half past them
+    ssa.set_cell_arithmetic((seed % 8 == 6) if seed < 24 else rnd.randrange(2))
     try:
         check(t, bw, symtype, strands, qraw, db, k, matrix, gaps, isa=isa, chunk=chunk, align=(seed % 4 == 1))
     finally:
         ssa.set_kernel_shape(0, 0)
         ssa.set_kernel_variant(0, 0)
+        ssa.set_cell_arithmetic(0)
 
 
 @pytest.mark.parametrize("t", [O.SW, O.NW])
