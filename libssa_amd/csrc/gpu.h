@@ -132,7 +132,7 @@ private:
 
 // ---- kernel launchers (score_kernels.hip, traceback_kernels.hip, pack_kernels.hip) ----
 // modes: 0 SW (int16), 1 NW, 2 NW without overflow trackers, 3 SW in fp16
-// windows
+// windows, 4 NW without overflow trackers in fp16
 bool score_shape_supported(int lanes, int rows, int mode, int cols);
 int score_resident_workgroups(int lanes, int rows, int mode, int cols);  // persistent grid size of a shape
 // NW shapes with an untracked variant (mode 2: no value of any pair can reach

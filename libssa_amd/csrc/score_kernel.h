@@ -6,8 +6,8 @@
 // MODE 2 is NW without the overflow trackers, for 16-bit launches where the
 // host has shown that no value of any pair can reach either end of the int16
 // range (then no block can be flagged and the trackers are dead work).
-// MODE 3 is the SW first tier in packed fp16 windows (see cell arithmetic
-// below).
+// MODE 3 is the SW first tier in packed fp16 windows and MODE 4 NW without
+// the trackers in packed fp16 (see cell arithmetic below).
 // Restates the per-lane arithmetic of the reference's width kernels
 // (src/algo/simd/search_simd_sw.c:172-277, search_simd_nw.c:180-312):
 // saturating adds, the H/E/F recurrence and the overflow predicates, but not
@@ -83,7 +83,12 @@ __device__ __forceinline__ s16x2 sel(uint32_t m, s16x2 a, s16x2 b) {
 // the window against the thresholds less the offset, and the pair's running
 // maximum is folded into int32 at each re-centring.  The host enables the
 // mode where the matrix and gaps keep every value within +-2047 of the
-// window (BLOSUM62 with gaps 10/1: the window spans at most 700).
+// window (BLOSUM62 with gaps 10/1: the window spans at most 700).  Mode 4 is
+// the NW kernel without overflow trackers (mode 2) in plain packed fp16: the
+// host enables it where every value of every pair lies within +-2047 (an NW
+// cell is at least the two-gap path to it, 2 gap open + (i + j) gap extend,
+// and at most max score * min(i, j)), and the three-input maximum takes the
+// NW cell from 8 packed operations to 7.
 typedef _Float16 f16x2 __attribute__((ext_vector_type(2)));
 __device__ __forceinline__ f16x2 as_h(s16x2 v) { return __builtin_bit_cast(f16x2, v); }
 __device__ __forceinline__ s16x2 of_h(f16x2 v) { return __builtin_bit_cast(s16x2, v); }
@@ -262,7 +267,8 @@ __device__ __forceinline__ void run_task(const ScoreParams& P, uint32_t gbase0, 
                                          PairResult* __restrict__ results, int sb_lo, int sb_hi, int k, int g,
                                          int nbw, int nbp, int npass, int last_g, int last_r, const Stage& stage) {
     constexpr bool SW = MODE == 0 || MODE == 3;  // SW semantics (mode 3: in fp16)
-    constexpr bool F16 = MODE == 3;
+    constexpr bool F16 = MODE == 3;               // SW in fp16 windows
+    constexpr bool FP = MODE == 3 || MODE == 4;   // fp16 cell arithmetic
     static_assert(!PAIR || (SEG && MULTI && SW), "paired queries are packed SW queries");
     constexpr int NG = 64 / G;
     constexpr int PR = G * R;
@@ -485,9 +491,9 @@ __device__ __forceinline__ void run_task(const ScoreParams& P, uint32_t gbase0, 
                 toph[0] = as_u(hb[1]);
                 toph[1] = as_u(hb[2]);
                 toph[2] = as_u(hb[3]);
-                toph[3] = as_u(adds(hb[3], RR));
+                toph[3] = as_u(c_add<FP>(hb[3], RR));
 #pragma unroll
-                for (int c = 0; c < 4; ++c) topf[c] = as_u(adds(fb[c], QR));
+                for (int c = 0; c < 4; ++c) topf[c] = as_u(c_add<FP>(fb[c], QR));
             }
             uint32_t tbh[4], tbf[4];
 #pragma unroll
@@ -542,11 +548,11 @@ __device__ __forceinline__ void run_task(const ScoreParams& P, uint32_t gbase0, 
 #pragma unroll
                 for (int c = 0; c < 4; ++c) {
                     s16x2 f = as_v(tbf[c]);
-                    s16x2 fR = c_add<F16>(f, RR);
+                    s16x2 fR = c_add<FP>(f, RR);
                     s16x2 v;
                     if constexpr (PAIR) v = as_v(pl[0]);
                     else v = pair_row(pl, ph, 0);
-                    s16x2 h1 = c_add<F16>(htop, v);  // row 0: diagonal H from the row above
+                    s16x2 h1 = c_add<FP>(htop, v);  // row 0: diagonal H from the row above
                     htop = as_v(tbh[c]);
                     s16x2 hq = Z;
                     s16x2 hp = Z;  // mode 3: the previous row's H (S takes two rows per maximum)
@@ -583,6 +589,16 @@ __device__ __forceinline__ void run_task(const ScoreParams& P, uint32_t gbase0, 
                             El[r] = c_max3<true>(eR, tt, as_v(flr));
                             if (more) fR = c_add<true>(f, RR);
                             Hl[r] = h;
+                        } else if constexpr (MODE == 4) {  // NW in fp16: no floor, no trackers
+                            const s16x2 h = c_max3<true>(h1, f, El[r]);
+                            const s16x2 eR = c_add<true>(El[r], RR);
+                            if (more) h1 = c_add<true>(Hl[r], v);  // next row's diagonal term
+                            const s16x2 tt = c_add<true>(h, QR);
+                            f = c_max<true>(fR, tt);
+                            El[r] = c_max<true>(eR, tt);
+                            if (more) fR = c_add<true>(f, RR);
+                            Hl[r] = h;
+                            if (CAP && r == last_r) hq = h;
                         } else {
                             s16x2 h = vmax(h1, f);
                             const s16x2 eR = adds(El[r], RR);
@@ -720,14 +736,14 @@ __device__ __forceinline__ void run_task(const ScoreParams& P, uint32_t gbase0, 
             }
 
             if (!SW && FIRST) {  // the reference advances the top border by four columns per block
-                fb[0] = adds(fb[3], RR);
-                fb[1] = adds(fb[0], RR);
-                fb[2] = adds(fb[1], RR);
-                fb[3] = adds(fb[2], RR);
-                hb[0] = adds(hb[3], RR);
-                hb[1] = adds(hb[0], RR);
-                hb[2] = adds(hb[1], RR);
-                hb[3] = adds(hb[2], RR);
+                fb[0] = c_add<FP>(fb[3], RR);
+                fb[1] = c_add<FP>(fb[0], RR);
+                fb[2] = c_add<FP>(fb[1], RR);
+                fb[3] = c_add<FP>(fb[2], RR);
+                hb[0] = c_add<FP>(hb[3], RR);
+                hb[1] = c_add<FP>(hb[0], RR);
+                hb[2] = c_add<FP>(hb[1], RR);
+                hb[3] = c_add<FP>(hb[2], RR);
             }
             if (any_last) {
                 const bool ls_lo = (w.x & kResLast) != 0, ls_hi = (w.y & kResLast) != 0;
@@ -755,8 +771,8 @@ __device__ __forceinline__ void run_task(const ScoreParams& P, uint32_t gbase0, 
                         scout[j] = as_u(vmax(S, as_v(scin)));
                     }
                     const s16x2 fin = SW ? as_v(scout[j]) : cap;
-                    sc_lo = lo16(fin);
-                    sc_hi = hi16(fin);
+                    sc_lo = MODE == 4 ? f16_int(as_u(fin) & 0xffffu) : lo16(fin);
+                    sc_hi = MODE == 4 ? f16_int(as_u(fin) >> 16) : hi16(fin);
                 }
                 uint32_t m = 0u;
                 if (ls_lo && pair_lo >= 0) {
@@ -826,7 +842,7 @@ __global__ __launch_bounds__(64 * kWavesPerWG) void score_kernel(const ScoreLaun
     constexpr int PR = G * R;  // rows per pass
     static_assert((1 << Log2<PR>::value) == PR, "rows per pass must be a power of two");
     static_assert(PR <= 512, "two staged profiles of 32 codes x PR rows must fit the LDS budget");
-    constexpr bool NW = MODE == 1 || MODE == 2;
+    constexpr bool NW = MODE == 1 || MODE == 2 || MODE == 4;
     __shared__ __align__(16) int16_t sprof[2 * 32 * PR];  // profiles of passes p (even, odd)
     __shared__ uint32_t slbh[NW ? 2 * (PR + 1) : 1];     // NW left boundary H of rows base-1 .. base+PR-1
     __shared__ uint32_t slbe[NW ? 2 * PR : 1];           // NW left boundary E of rows base .. base+PR-1
@@ -961,7 +977,8 @@ __global__ __launch_bounds__(64 * kWavesPerWG) void score_kernel_pair(const Scor
 
 // ---------------------------------------------------------------------------
 // Instantiation: one translation unit per mode (score_sw.hip: 0, score_nw1.hip:
-// 1, score_nw2.hip: 2, score_swf.hip: 3, score_pair.hip: the paired kernel)
+// 1, score_nw2.hip: 2, score_swf.hip: 3, score_nwf.hip: 4, score_pair.hip:
+// the paired kernel)
 // defines SSA_SCORE_INSTANTIATE and instantiates launch_mode / resident_mode
 // for its mode; the dispatch (score_kernels.hip) sees the declarations only.
 // ---------------------------------------------------------------------------
@@ -1000,7 +1017,8 @@ constexpr bool tile8_shape() {
 // wave per SIMD)
 template <int R, int G, int MODE, int NB>
 constexpr bool shape_built() {
-    return !(R == 32 && (MODE == 1 || MODE == 2)) && !(MODE == 2 && !nw_untracked_shape<R, G>()) &&
+    return !(R == 32 && (MODE == 1 || MODE == 2 || MODE == 4)) &&
+           !((MODE == 2 || MODE == 4) && !nw_untracked_shape<R, G>()) &&
            (NB == 1 || tile8_shape<R, G>());
 }
 

@@ -197,6 +197,7 @@ int score_resident_workgroups(int lanes, int rows, int mode, int cols) {
     case 1: return resident_mode<1>(lanes, rows, cols);
     case 2: return resident_mode<2>(lanes, rows, cols);
     case 3: return resident_mode<3>(lanes, rows, cols);
+    case 4: return resident_mode<4>(lanes, rows, cols);
     default: return 0;
     }
 }
@@ -209,6 +210,7 @@ hipError_t launch_score(int lanes, int rows, int mode, int cols, hipStream_t st,
     case 1: return launch_mode<1>(lanes, rows, cols, st, L, P, max_wg);
     case 2: return launch_mode<2>(lanes, rows, cols, st, L, P, max_wg);
     case 3: return launch_mode<3>(lanes, rows, cols, st, L, P, max_wg);
+    case 4: return launch_mode<4>(lanes, rows, cols, st, L, P, max_wg);
     default: return hipErrorInvalidValue;
     }
 }

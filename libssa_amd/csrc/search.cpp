@@ -360,6 +360,20 @@ bool int16_exact(size_t qlen, int64_t dmax, const MatrixInfo& mi) {
     return lowest > -32768 + gap + 1 && up < 32767;
 }
 
+// True when every H, E and F value of an NW alignment of a query of qlen
+// residues with a DB entry of at most dmax residues lies within +-2047, so
+// that fp16 holds it exactly: a cell (i, j) is at least the path of two gaps
+// to it, 2 gap open + (i + j) gap extend, E and F one more gap open below
+// that (with the padding columns of a block and the borders within four
+// more extends), and at most max score * min(i, j).
+bool nw_f16_exact(size_t qlen, int64_t dmax, const MatrixInfo& mi) {
+    const Config& cf = config();
+    const int64_t go = std::abs((int64_t)cf.gap_open), ge = std::abs((int64_t)cf.gap_extend);
+    const int64_t lowest = -(3 * go + ((int64_t)qlen + dmax + 4) * ge) - std::max<int64_t>(-mi.minv, 0);
+    const int64_t up = std::max<int64_t>(mi.maxv, 0) * std::min<int64_t>((int64_t)qlen, dmax);
+    return mi.maxv <= 1024 && mi.minv >= -1024 && lowest >= -2047 && up <= 2047;
+}
+
 // result of one pair at one tier, decoded: d > 0 is a heap insert of score
 // after d blocks in a reference lane, d < 0 an overflow after -d blocks
 // (blocks >= 1 either way)
@@ -774,8 +788,9 @@ struct TierRun {
     bool packed = false;  // SW with several queries: scored as packed queries
     bool paired = false;  // packed queries two by two, one per half of every lane group (score_kernel_pair)
     bool untracked = false;  // NW 16-bit where no pair can overflow: the kernel without trackers
-    bool f16 = false;        // SW first tier in windowed packed fp16 (kernel mode 3)
-    int mode() const { return type == kSW ? (f16 ? 3 : 0) : (untracked ? 2 : 1); }
+    bool f16 = false;        // SW first tier in windowed packed fp16 (kernel mode 3), or the
+                             // untracked NW kernel in packed fp16 (mode 4)
+    int mode() const { return type == kSW ? (f16 ? 3 : 0) : (untracked ? (f16 ? 4 : 2) : 1); }
 
     ~TierRun() { local.release(); }
     bool empty() const { return ps == nullptr || ps->list().empty() || qs.empty(); }
@@ -920,6 +935,23 @@ void tier_prepare(Exec& ex, TierRun& R) {
         R.untracked = R.type == kNW && T.bits == 16 && score_nw_untracked_supported(sh.lanes, sh.rows);
         for (size_t qi = 0; qi < nq && R.untracked; ++qi)
             R.untracked = int16_exact(R.qs[qi].len, 4 * (int64_t)L->max_entry_blocks, mi);
+        // and in fp16 where every value lies within +-2047
+        if (R.untracked && arith_override_ref() != 1) {
+            R.f16 = true;
+            for (size_t qi = 0; qi < nq && R.f16; ++qi)
+                R.f16 = nw_f16_exact(R.qs[qi].len, 4 * (int64_t)L->max_entry_blocks, mi);
+        }
+        if (R.type == kNW && R.f16) {  // the NW parameters and borders as fp16 values
+            auto cv = [](uint32_t w) { return pack2(f16_bits((int16_t)(w & 0xffffu))); };
+            P.qr = cv(P.qr);
+            P.r = cv(P.r);
+            P.zero = 0;
+            for (int c = 0; c < 4; ++c) {
+                P.hb_init[c] = cv(P.hb_init[c]);
+                P.fb_init[c] = cv(P.fb_init[c]);
+            }
+            for (uint32_t& w : R.lbv) w = cv(w);
+        }
     }
     const int cap = R.paired ? score_pair_resident_workgroups(sh.lanes, sh.rows, R.f16)
                              : score_resident_workgroups(sh.lanes, sh.rows, R.mode(), sh.cols);
@@ -1126,8 +1158,8 @@ void tier_prepare(Exec& ex, TierRun& R) {
     R.d_out = (PairResult*)ex.arena(name("results"), R.nout * sizeof(PairResult));
     ex.upload(d_tasks, R.tasks.data(), R.tasks.size() * sizeof(ScoreTask));
     ex.upload(d_qd, R.qd.data(), R.qd.size() * sizeof(QueryDesc));
-    // the fp16 kernel's profile: the 16-bit tier's scores as fp16 values (the
-    // padding rows' -32768 as well)
+    // the fp16 kernels' profile: the 16-bit tier's scores as fp16 values (the
+    // padding rows' -32768 (SW) and 0 (NW) as well)
     if (R.f16)
         for (int16_t& x : R.prof) x = (int16_t)f16_bits(x);
     ex.upload(d_prof, R.prof.data(), R.prof.size() * sizeof(int16_t));
@@ -2351,7 +2383,7 @@ void run_batch(Exec& ex, const std::vector<SearchRequest>& reqs, std::vector<Sea
             J.st.rows_per_lane = R.sh.rows;
             J.st.columns_per_step = R.sh.cols;
             J.st.profile_placement = 1;
-            J.st.overflow_tracking = R.mode() != 2;
+            J.st.overflow_tracking = !R.untracked;
             J.st.cell_arithmetic = R.f16 ? 2 : 1;
         }
         if (J.dev_top) {

@@ -174,3 +174,46 @@ def test_fp16_packed_and_paired_queries():
             ssa.set_cell_arithmetic(0)
             for q in qs:
                 ssa.free_sequence(q)
+
+
+@pytest.mark.parametrize("strands", [1, 3])
+def test_fp16_nw_dna(strands):
+    """NW without overflow trackers in fp16 (mode 4): DNA, constant +2/-4 with
+    gaps 20/2 as the DNA workload, entries of 100-300 (every value within
+    +-2047), one and both strands, scores and CIGARs."""
+    rnd = random.Random(61 + strands)
+    qraw = "".join(rnd.choice("ACGT") for _ in range(250))
+    db = ["".join(rnd.choice("ACGT") for _ in range(rnd.randint(100, 300))) for _ in range(400)]
+    comp = str.maketrans("ACGT", "TGCA")
+    for c in range(8):
+        frag = "".join(x if rnd.random() < 0.9 else rnd.choice("ACGT") for x in qraw)
+        db[rnd.randrange(400)] = frag.translate(comp)[::-1] if c % 2 else frag
+    # the shapes with an untracked NW kernel (the cost model picks <32, 4> for
+    # a DB this small; the DNA workload runs <16, 16>)
+    try:
+        for shape in ((16, 16), (8, 16), (16, 8), (8, 8), (32, 8)):
+            ssa.set_kernel_shape(*shape)
+            seen = run_both(lambda: check(O.NW, 16, O.NUCLEOTIDE, strands, qraw, db, 40, ("const", (2, -4)),
+                                          (-20, -2), align=shape == (16, 16)))
+            assert seen == [2, 1]
+    finally:
+        ssa.set_kernel_shape(0, 0)
+
+
+def test_fp16_nw_protein_and_fallback():
+    """NW protein with short entries runs in fp16; entries long enough that a
+    two-gap path passes -2047 run in int16; both against the oracle."""
+    rnd = random.Random(12)
+    qraw = "".join(rnd.choice(AA) for _ in range(120))
+    short = ["".join(rnd.choice(AA) for _ in range(rnd.randint(1, 200))) for _ in range(300)]
+    short[5] = qraw[10:110]
+    ssa.set_kernel_shape(16, 16)
+    try:
+        seen = run_both(lambda: check(O.NW, 16, O.AMINOACID, 1, qraw, short, 20, ("builtin", "blosum62"), (-10, -1)))
+        assert seen == [2, 1]
+        long_db = short[:200] + ["".join(rnd.choice(AA) for _ in range(2500))]
+        seen = run_both(lambda: check(O.NW, 16, O.AMINOACID, 1, qraw, long_db, 20, ("builtin", "blosum62"),
+                                      (-10, -1)))
+        assert seen == [1, 1]
+    finally:
+        ssa.set_kernel_shape(0, 0)
