@@ -106,6 +106,8 @@ struct ScoreLaunch {
     const uint2* res;
     uint4* bnd;
     int64_t bnd_per_wave;
+    int64_t bnd_rows;  // uint4 of a wave's slice that hold bottom rows; the SW fp16 windows'
+                       // offsets (one uint32 per block and lane group) follow them
     const int32_t* slot_first;
     const int32_t* slot_entries;
     PairResult* results;

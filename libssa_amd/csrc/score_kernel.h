@@ -262,6 +262,7 @@ template <int R, int G, int MODE, int NB, bool MULTI, bool SEG, bool PAIR, class
 __device__ __forceinline__ void run_task(const ScoreParams& P, uint32_t gbase0, const char* sp,
                                          const uint32_t* slbh0, const uint32_t* slbe0,
                                          const uint2* __restrict__ wres, uint4* __restrict__ wbnd,
+                                         uint32_t* __restrict__ wboff,
                                          const int32_t* __restrict__ slot_entries, int pair_off,
                                          const SegLane* __restrict__ segs, const SegLane* __restrict__ segs_hi,
                                          PairResult* __restrict__ results, int sb_lo, int sb_hi, int k, int g,
@@ -398,10 +399,9 @@ __device__ __forceinline__ void run_task(const ScoreParams& P, uint32_t gbase0, 
                 const uint32_t o = (uint32_t)((b_n + j) * NG + k);
                 nx.w[j] = wres[o];
                 if (MULTI && top_lane && pass_n > 0) {
-                    constexpr int BW = F16 ? 3 : 2;  // uint4 per block and lane group
-                    nx.t[2 * j] = wbnd[BW * o];
-                    nx.t[2 * j + 1] = wbnd[BW * o + 1];
-                    if (F16) nx.toff[j] = wbnd[BW * o + 2].x;
+                    nx.t[2 * j] = wbnd[2 * o];
+                    nx.t[2 * j + 1] = wbnd[2 * o + 1];
+                    if (F16) nx.toff[j] = wboff[o];
                 }
             } else {
                 nx.w[j] = make_uint2(0, 0);
@@ -728,11 +728,10 @@ __device__ __forceinline__ void run_task(const ScoreParams& P, uint32_t gbase0, 
             }
             if (F16) ob_off[j] = off;
             if (MULTI && bot_out && valid) {
-                constexpr int BW = F16 ? 3 : 2;  // uint4 per block and lane group
-                const uint32_t o = (uint32_t)((b + j) * NG + k) * BW;
-                wbnd[o] = make_uint4(obh[4 * j], obf[4 * j], obh[4 * j + 1], obf[4 * j + 1]);
-                wbnd[o + 1] = make_uint4(obh[4 * j + 2], obf[4 * j + 2], obh[4 * j + 3], obf[4 * j + 3]);
-                if (F16) wbnd[o + 2] = make_uint4(off, 0u, 0u, 0u);
+                const uint32_t o = (uint32_t)((b + j) * NG + k);
+                wbnd[2 * o] = make_uint4(obh[4 * j], obf[4 * j], obh[4 * j + 1], obf[4 * j + 1]);
+                wbnd[2 * o + 1] = make_uint4(obh[4 * j + 2], obf[4 * j + 2], obh[4 * j + 3], obf[4 * j + 3]);
+                if (F16) wboff[o] = off;
             }
 
             if (!SW && FIRST) {  // the reference advances the top border by four columns per block
@@ -853,6 +852,7 @@ __global__ __launch_bounds__(64 * kWavesPerWG) void score_kernel(const ScoreLaun
     const uint32_t gbase = (uint32_t)(g * R * 2);  // byte offset of the lane's rows in a profile code row
     const char* sp = reinterpret_cast<const char*>(sprof);
     uint4* __restrict__ wbnd = L.bnd + ((int64_t)blockIdx.x * kWavesPerWG + wv) * L.bnd_per_wave;
+    uint32_t* __restrict__ wboff = reinterpret_cast<uint32_t*>(wbnd + L.bnd_rows);
 
     for (;;) {
         __syncthreads();  // the previous task is done with s_task and the staged profiles
@@ -904,15 +904,15 @@ __global__ __launch_bounds__(64 * kWavesPerWG) void score_kernel(const ScoreLaun
             __syncthreads();
         };
         if (!NW && Q.seg_off >= 0)
-            run_task<R, G, MODE, NB, true, !NW, false>(P, gbase, sp, slbh, slbe, wres, wbnd, L.slot_entries,
+            run_task<R, G, MODE, NB, true, !NW, false>(P, gbase, sp, slbh, slbe, wres, wbnd, wboff, L.slot_entries,
                                                          Q.pair_off, L.segs + Q.seg_off, nullptr, L.results, sb_lo,
                                                          sb_hi, k, g, Wv.nblocks, nbp, npass, last_g, last_r, stage);
         else if (npass > 1)
-            run_task<R, G, MODE, NB, true, false, false>(P, gbase, sp, slbh, slbe, wres, wbnd, L.slot_entries,
+            run_task<R, G, MODE, NB, true, false, false>(P, gbase, sp, slbh, slbe, wres, wbnd, wboff, L.slot_entries,
                                                      Q.pair_off, nullptr, nullptr, L.results, sb_lo, sb_hi, k, g,
                                                      Wv.nblocks, nbp, npass, last_g, last_r, stage);
         else
-            run_task<R, G, MODE, NB, false, false, false>(P, gbase, sp, slbh, slbe, wres, wbnd, L.slot_entries,
+            run_task<R, G, MODE, NB, false, false, false>(P, gbase, sp, slbh, slbe, wres, wbnd, wboff, L.slot_entries,
                                                       Q.pair_off, nullptr, nullptr, L.results, sb_lo, sb_hi, k, g,
                                                       Wv.nblocks, nbp, npass, last_g, last_r, stage);
     }
@@ -936,6 +936,7 @@ __global__ __launch_bounds__(64 * kWavesPerWG) void score_kernel_pair(const Scor
     const uint32_t gbase = (uint32_t)(g * R * 4);  // byte offset of the lane's rows in a paired profile code row
     const char* sp = reinterpret_cast<const char*>(sprof);
     uint4* __restrict__ wbnd = L.bnd + ((int64_t)blockIdx.x * kWavesPerWG + wv) * L.bnd_per_wave;
+    uint32_t* __restrict__ wboff = reinterpret_cast<uint32_t*>(wbnd + L.bnd_rows);
 
     for (;;) {
         __syncthreads();  // the previous task is done with s_task and the staged profiles
@@ -969,7 +970,8 @@ __global__ __launch_bounds__(64 * kWavesPerWG) void score_kernel_pair(const Scor
             load_pass(pass);
             __syncthreads();
         };
-        run_task<R, G, F16 ? 3 : 0, 1, true, true, true>(P, gbase, sp, slb_unused, slb_unused, wres, wbnd, L.slot_entries,
+        run_task<R, G, F16 ? 3 : 0, 1, true, true, true>(P, gbase, sp, slb_unused, slb_unused, wres, wbnd, wboff,
+                                                         L.slot_entries,
                                             Q.pair_off, L.segs + Q.seg_off, L.segs + Q.seg_off_hi, L.results, sb_lo,
                                             sb_hi, k, g, Wv.nblocks, nbp, npass, 0, 0, stage);
     }

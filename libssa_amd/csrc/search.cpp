@@ -1128,8 +1128,10 @@ void tier_prepare(Exec& ex, TierRun& R) {
     const int64_t nb_step = sh.cols / 4;
     const int64_t max_nbp =
         (std::max<int64_t>(max_nb, nb_step * (sh.lanes + 2)) + 2 * nb_step - 1) / (2 * nb_step) * (2 * nb_step);
-    // (mode 3 stores each block's offsets beside its bottom row: 3 uint4)
-    const int64_t bnd_per_wave = multipass ? max_nbp * ng * (R.f16 ? 3 : 2) : 0;
+    // (mode 3 keeps each block's window offsets, one uint32 per lane group,
+    // after the bottom rows)
+    const int64_t bnd_rows = multipass ? max_nbp * ng * 2 : 0;
+    const int64_t bnd_per_wave = bnd_rows + (multipass && R.type == kSW && R.f16 ? (max_nbp * ng + 3) / 4 : 0);
 
     // ---- device buffers ----
     auto name = [&](const char* s) { return R.pfx + s; };
@@ -1177,6 +1179,7 @@ void tier_prepare(Exec& ex, TierRun& R) {
     K.res = L->d_res;
     K.bnd = d_bnd;
     K.bnd_per_wave = bnd_per_wave;
+    K.bnd_rows = bnd_rows;
     K.slot_first = L->d_slot_first;
     K.slot_entries = L->d_slot_entries;
     K.results = R.d_out;
@@ -3082,8 +3085,8 @@ RequestPlan plan_request(const SearchRequest& rq, const std::vector<size_t>& qle
     const int grid = std::min(tasks, cap);
     pl.occupancy = (double)grid / (double)cap;
     if (passes > 1)
-        bytes += (size_t)grid * kWavesPerWG * (size_t)(max_nb + 2) * (size_t)(64 / sh.lanes) * (type == 0 ? 3 : 2) *
-                 sizeof(uint4);
+        bytes += (size_t)grid * kWavesPerWG * (size_t)(max_nb + 2) * (size_t)(64 / sh.lanes) *
+                 (2 * sizeof(uint4) + (type == 0 ? sizeof(uint32_t) : 0));
     if (rq.align_type != COMPUTE_SCORE) {
         int64_t dmax = 0;
         for (uint32_t e = 0; e < N; ++e) dmax = std::max<int64_t>(dmax, db.entries[e].len);
