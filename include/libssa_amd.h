@@ -74,10 +74,13 @@ void ssa_amd_set_kernel_shape(int lanes_per_stream, int rows_per_lane);
 /* Variant of the score kernel: the column tile -- DB columns a lane scores
  * per step, 4 (one block of the reference's 4-column unit) or 8 (two blocks
  * per step, shapes of at most 16 rows per lane in groups of 8 or 16 lanes) --
- * and where the query profile sits: 1 = LDS, 2 = VGPRs (nucleotide
- * databases of A/C/G/T only, 16 or fewer rows per lane).  0 leaves the choice
- * to the cost model or the autotuner; a forced variant a shape lacks is left
- * at its default for that shape.  Results do not depend on the variant. */
+ * and where the query profile sits: 1 = LDS (each lane reads its rows'
+ * scores for a column's DB codes), 2 = VGPRs (each lane holds its rows'
+ * scores for A, C, G and T and picks a column's pair with v_perm: nucleotide
+ * databases of A, C, G and T only, 4-column tiles of at most 16 rows per
+ * lane, SW and NW without overflow trackers).  0 leaves the choice to the
+ * cost model (LDS) or the autotuner; a forced variant a search or shape lacks
+ * is left at its default.  Results do not depend on the variant. */
 void ssa_amd_set_kernel_variant(int columns_per_step, int profile_placement);
 
 /* Cell arithmetic of the score kernel's SW first tier (the tier that records
@@ -109,6 +112,14 @@ void ssa_amd_set_stream_waves(int waves);
  * scoring is not initialised.  Results do not depend on the shape. */
 int ssa_amd_autotune(int search_type, const size_t* lengths, size_t n, int copies, int* lanes_out, int* rows_out,
                      double* ms_out);
+
+/* The choice of the last ssa_amd_autotune for its i-th length: lanes per
+ * stream pair, rows per lane, columns per step (4 or 8), where the query
+ * profile sits (1 = LDS, 2 = VGPRs: measured for nucleotide DBs of A, C, G
+ * and T only) and the time per query (NULL allowed for each).  Returns 0, or
+ * -1 for i past the last tuned length. */
+int ssa_amd_autotune_choice(size_t i, int* lanes_per_stream, int* rows_per_lane, int* columns_per_step,
+                            int* profile_placement, double* ms);
 
 /* Forgets the measured shapes (the cost model chooses again). */
 void ssa_amd_autotune_reset(void);

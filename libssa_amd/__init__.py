@@ -141,6 +141,8 @@ def _load():
         "ssa_amd_set_kernel_shape": (None, [I, I]), "ssa_amd_set_stream_waves": (None, [I]),
         "ssa_amd_set_kernel_variant": (None, [I, I]),
         "ssa_amd_set_cell_arithmetic": (None, [I]),
+        "ssa_amd_autotune_choice": (I, [_C.c_size_t, _C.POINTER(I), _C.POINTER(I), _C.POINTER(I), _C.POINTER(I),
+                                        _C.POINTER(_C.c_double)]),
         "ssa_amd_get_matrix": (None, [P]), "ssa_amd_device_arch": (cp, []),
         "ssa_amd_batch_align": (I, [_C.POINTER(_Request), S, _C.POINTER(_C.POINTER(_AlignmentList)),
                                     _C.POINTER(Stats)]),
@@ -363,6 +365,19 @@ def autotune(search_type, lengths, copies=1):
     if k < 0:
         raise ValueError("ssa_amd_autotune failed (code %d)" % k)
     return [(lengths[i], lanes[i], rows[i], ms[i]) for i in range(k)]
+
+
+def autotune_choices():
+    """The choices of the last autotune (ssa_amd_autotune_choice): [(lanes
+    per stream, rows per lane, columns per step, profile placement, ms)]."""
+    out, i = [], 0
+    while True:
+        v = [_C.c_int() for _ in range(4)]
+        ms = _C.c_double()
+        if lib.ssa_amd_autotune_choice(i, *[_C.byref(x) for x in v], _C.byref(ms)) != 0:
+            return out
+        out.append(tuple(x.value for x in v) + (ms.value,))
+        i += 1
 
 
 def autotune_reset(): lib.ssa_amd_autotune_reset()

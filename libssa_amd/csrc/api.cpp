@@ -20,6 +20,12 @@ p_query query_from_file(const char* path);
 
 using namespace ssa;
 
+// the choices of the last ssa_amd_autotune (ssa_amd_autotune_choice)
+static std::vector<AutotuneResult>& last_autotune() {
+    static std::vector<AutotuneResult> r;
+    return r;
+}
+
 extern "C" {
 
 // src/libssa.c:41-43
@@ -235,7 +241,20 @@ int ssa_amd_autotune(int search_type, const size_t* lengths, size_t n, int copie
         if (rows_out) rows_out[i] = res[i].rows;
         if (ms_out) ms_out[i] = res[i].ms;
     }
+    last_autotune() = res;
     return k;
+}
+
+int ssa_amd_autotune_choice(size_t i, int* lanes_per_stream, int* rows_per_lane, int* columns_per_step,
+                            int* profile_placement, double* ms) {
+    const std::vector<AutotuneResult>& r = last_autotune();
+    if (i >= r.size()) return -1;
+    if (lanes_per_stream) *lanes_per_stream = r[i].lanes;
+    if (rows_per_lane) *rows_per_lane = r[i].rows;
+    if (columns_per_step) *columns_per_step = r[i].cols;
+    if (profile_placement) *profile_placement = r[i].place;
+    if (ms) *ms = r[i].ms;
+    return 0;
 }
 
 void ssa_amd_autotune_reset(void) { autotune_reset(); }

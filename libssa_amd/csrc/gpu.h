@@ -134,7 +134,9 @@ private:
 // modes: 0 SW (int16), 1 NW, 2 NW without overflow trackers, 3 SW in fp16
 // windows, 4 NW without overflow trackers in fp16
 bool score_shape_supported(int lanes, int rows, int mode, int cols);
-int score_resident_workgroups(int lanes, int rows, int mode, int cols);  // persistent grid size of a shape
+// the query profile in VGPRs (nucleotide DBs of A, C, G and T only)
+bool score_vp_supported(int lanes, int rows, int mode, int cols);
+int score_resident_workgroups(int lanes, int rows, int mode, int cols, bool vp);  // persistent grid size of a shape
 // NW shapes with an untracked variant (mode 2: no value of any pair can reach
 // either end of the int16 range)
 bool score_nw_untracked_supported(int lanes, int rows);
@@ -143,7 +145,7 @@ bool score_pair_supported(int lanes, int rows);
 int score_pair_resident_workgroups(int lanes, int rows, bool f16);
 hipError_t launch_score_pair(int lanes, int rows, bool f16, hipStream_t st, const ScoreLaunch& L, const ScoreParams& P,
                              int max_wg);
-hipError_t launch_score(int lanes, int rows, int mode, int cols, hipStream_t st, const ScoreLaunch& L,
+hipError_t launch_score(int lanes, int rows, int mode, int cols, bool vp, hipStream_t st, const ScoreLaunch& L,
                         const ScoreParams& P, int max_wg);
 hipError_t launch_init_results(hipStream_t st, PairResult* r, int64_t n, int32_t score);
 // exact tier: one wave per pair, int64 (wide) or int32; bnd holds each pair's

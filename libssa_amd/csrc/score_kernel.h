@@ -26,7 +26,13 @@
 //
 // The pass's score profile is staged in LDS as [code][row] int16; a lane reads
 // the R scores of its rows for one DB code with one 16-byte (R = 8) read per
-// half and interleaves the two halves with v_perm.  A lane runs one block per
+// half and interleaves the two halves with v_perm.  With the profile in VGPRs
+// (VP: nucleotide DBs of A, C, G and T only) a lane holds its rows' scores
+// for the four codes, two dwords per row, and one LDS read per column fetches
+// the v_perm selector of the column's two codes from a table of 32 x 32
+// selectors; padding columns (code 0) score 0 there, which changes no result
+// of the modes that run it (SW: a padding cell never exceeds a real one; NW
+// without trackers: the score cell is the last real column).  A lane runs one block per
 // step; lane g is idle for the first g steps of a pass (and the last G-1-g),
 // so a pass takes nblocks + G - 1 steps.
 //
@@ -258,8 +264,9 @@ struct StepIn {
     uint32_t toff[NB];  // mode 3: the offsets of that bottom row (offset / 64, per half)
 };
 
-template <int R, int G, int MODE, int NB, bool MULTI, bool SEG, bool PAIR, class Stage>
+template <int R, int G, int MODE, int NB, bool MULTI, bool SEG, bool PAIR, bool VP, class Stage>
 __device__ __forceinline__ void run_task(const ScoreParams& P, uint32_t gbase0, const char* sp,
+                                         const uint32_t* __restrict__ slut,
                                          const uint32_t* slbh0, const uint32_t* slbe0,
                                          const uint2* __restrict__ wres, uint4* __restrict__ wbnd,
                                          uint32_t* __restrict__ wboff,
@@ -271,6 +278,7 @@ __device__ __forceinline__ void run_task(const ScoreParams& P, uint32_t gbase0, 
     constexpr bool F16 = MODE == 3;               // SW in fp16 windows
     constexpr bool FP = MODE == 3 || MODE == 4;   // fp16 cell arithmetic
     static_assert(!PAIR || (SEG && MULTI && SW), "paired queries are packed SW queries");
+    static_assert(!VP || (!PAIR && NB == 1 && R <= 16 && MODE != 1), "the VGPR profile's shapes and modes");
     constexpr int NG = 64 / G;
     constexpr int PR = G * R;
     constexpr int LPR = Log2<PR>::value + (PAIR ? 1 : 0);  // log2 of a profile code row's int16 count
@@ -321,6 +329,23 @@ __device__ __forceinline__ void run_task(const ScoreParams& P, uint32_t gbase0, 
         Hl[r] = Z;
         El[r] = Z;
     }
+    // VP: the lane's rows' scores for A, C (dword 2r) and G, T (dword 2r + 1)
+    // in the pass it is in, from the staged rows of buffer pass & 1
+    uint32_t vp[VP ? 2 * R : 1];
+    auto load_vp = [&](int p) {
+        if constexpr (VP) {
+            const uint4* src = reinterpret_cast<const uint4*>(sp) + ((p & 1) * PR + g * R) / 2;
+#pragma unroll
+            for (int q = 0; q < R / 2; ++q) {
+                const uint4 t = src[q];
+                vp[4 * q] = t.x;
+                vp[4 * q + 1] = t.y;
+                vp[4 * q + 2] = t.z;
+                vp[4 * q + 3] = t.w;
+            }
+        }
+    };
+    load_vp(0);
     s16x2 S = Z;     // SW: running maximum of the lane's cells of the current pair
     s16x2 htop = Z;  // H of the row above at the previous column
     s16x2 cap = Z;   // NW: the last row's H at the pair's last column
@@ -377,11 +402,11 @@ __device__ __forceinline__ void run_task(const ScoreParams& P, uint32_t gbase0, 
         constexpr bool FIRST = decltype(first_tag)::value;
         // profile dwords of the current column (lo, hi halves); column 0's now
         // (PAIR: pl holds one dword per row, both halves' scores)
-        uint32_t pl[PAIR ? R : R / 2], ph[PAIR ? 1 : R / 2];
+        uint32_t pl[PAIR ? R : (VP ? 1 : R / 2)], ph[PAIR || VP ? 1 : R / 2];
         if constexpr (PAIR) {
 #pragma unroll
             for (int q = 0; q < (R >= 8 ? R / 8 : 1); ++q) read_part_pair<R>(sp + colofs(in.w[0].x, 0), pl, q);
-        } else {
+        } else if constexpr (!VP) {
             read_rows<R>(sp + colofs(in.w[0].x, 0), pl);
             read_rows<R>(sp + colofs(in.w[0].y, 0), ph);
         }
@@ -528,6 +553,14 @@ __device__ __forceinline__ void run_task(const ScoreParams& P, uint32_t gbase0, 
             s16x2 hmn = as_v(0u), hmx = as_v(0u);  // NW trackers, reset value of the reference
             const bool any_last = __any(((w.x | w.y) & kResLast) != 0);
             const int z_lo = (int)((w.x >> kResZShift) & 3u), z_hi = (int)((w.y >> kResZShift) & 3u);
+            // VP: the selectors of the block's four columns (codes of the lo
+            // and hi streams: bits 1-5 of each byte)
+            uint32_t vsel[4];
+            if constexpr (VP) {
+#pragma unroll
+                for (int c = 0; c < 4; ++c)
+                    vsel[c] = slut[(__builtin_amdgcn_ubfe(w.x, 8 * c + 1, 5) << 5) | __builtin_amdgcn_ubfe(w.y, 8 * c + 1, 5)];
+            }
             // the block's four columns; CAP also records the NW score cell (the
             // last query row at the pair's last column), a select per row on a
             // wave-uniform condition, so the loop stays straight-line code
@@ -551,6 +584,7 @@ __device__ __forceinline__ void run_task(const ScoreParams& P, uint32_t gbase0, 
                     s16x2 fR = c_add<FP>(f, RR);
                     s16x2 v;
                     if constexpr (PAIR) v = as_v(pl[0]);
+                    else if constexpr (VP) v = as_v(__builtin_amdgcn_perm(vp[1], vp[0], vsel[c]));
                     else v = pair_row(pl, ph, 0);
                     s16x2 h1 = c_add<FP>(htop, v);  // row 0: diagonal H from the row above
                     htop = as_v(tbh[c]);
@@ -561,11 +595,14 @@ __device__ __forceinline__ void run_task(const ScoreParams& P, uint32_t gbase0, 
                         const bool more = r + 1 < R;
                         if (more) {
                             if constexpr (PAIR) v = as_v(pl[r + 1 < R ? r + 1 : 0]);
+                            else if constexpr (VP)
+                                v = as_v(__builtin_amdgcn_perm(vp[2 * (r + 1 < R ? r + 1 : 0) + 1],
+                                                               vp[2 * (r + 1 < R ? r + 1 : 0)], vsel[c]));
                             else v = pair_row(pl, ph, r + 1);
                         }
                         // the next column's profile rows (the step's next block
                         // after column 3), once this column's dwords are consumed
-                        if ((c < 3 || j + 1 < NB) && ((R == 4 && r == 3) || (R >= 8 && r % 8 == 7))) {
+                        if (!VP && (c < 3 || j + 1 < NB) && ((R == 4 && r == 3) || (R >= 8 && r % 8 == 7))) {
                             const uint2 wc = c < 3 ? w : in.w[j + 1 < NB ? j + 1 : j];
                             const int cn = c < 3 ? c + 1 : 0;
                             if constexpr (PAIR) {
@@ -803,6 +840,9 @@ __device__ __forceinline__ void run_task(const ScoreParams& P, uint32_t gbase0, 
                 slbh = odd ? slbh0 + (PR + 1) : slbh0;
                 slbe = odd ? slbe0 + PR : slbe0;
             }
+            // (the whole wave: lanes still in the earlier pass read its rows
+            // again, which are still staged)
+            load_vp(pass);
         }
     };
 
@@ -836,13 +876,15 @@ __device__ __forceinline__ void run_task(const ScoreParams& P, uint32_t gbase0, 
     }
 }
 
-template <int R, int G, int MODE, int NB>
+template <int R, int G, int MODE, int NB, bool VP>
 __global__ __launch_bounds__(64 * kWavesPerWG) void score_kernel(const ScoreLaunch L, const ScoreParams P) {
     constexpr int PR = G * R;  // rows per pass
     static_assert((1 << Log2<PR>::value) == PR, "rows per pass must be a power of two");
     static_assert(PR <= 512, "two staged profiles of 32 codes x PR rows must fit the LDS budget");
     constexpr bool NW = MODE == 1 || MODE == 2 || MODE == 4;
-    __shared__ __align__(16) int16_t sprof[2 * 32 * PR];  // profiles of passes p (even, odd)
+    __shared__ __align__(16) int16_t sprof[VP ? 8 : 2 * 32 * PR];  // profiles of passes p (even, odd)
+    __shared__ __align__(16) uint2 svp[VP ? 2 * PR : 1];  // VP: rows' scores for A, C, G, T of passes p
+    __shared__ uint32_t slut[VP ? 32 * 32 : 1];           // VP: v_perm selector of a column's two codes
     __shared__ uint32_t slbh[NW ? 2 * (PR + 1) : 1];     // NW left boundary H of rows base-1 .. base+PR-1
     __shared__ uint32_t slbe[NW ? 2 * PR : 1];           // NW left boundary E of rows base .. base+PR-1
     __shared__ int s_task;
@@ -850,7 +892,19 @@ __global__ __launch_bounds__(64 * kWavesPerWG) void score_kernel(const ScoreLaun
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     const int k = lane / G, g = lane % G;
     const uint32_t gbase = (uint32_t)(g * R * 2);  // byte offset of the lane's rows in a profile code row
-    const char* sp = reinterpret_cast<const char*>(sprof);
+    const char* sp = VP ? reinterpret_cast<const char*>(svp) : reinterpret_cast<const char*>(sprof);
+    if constexpr (VP) {
+        // selector of (lo code, hi code): the bytes of the code's score in
+        // the row's dword pair (A 0-1, C 2-3, G 4-5, T 6-7); other codes
+        // (the padding) select 0
+        for (int i = threadIdx.x; i < 32 * 32; i += 64 * kWavesPerWG) {
+            auto half = [](int code) -> uint32_t {
+                const int slot = code == 1 ? 0 : code == 2 ? 1 : code == 4 ? 2 : code == 8 ? 3 : -1;
+                return slot < 0 ? 0x0c0cu : (uint32_t)(2 * slot) | ((uint32_t)(2 * slot + 1) << 8);
+            };
+            slut[i] = half(i >> 5) | (half(i & 31) << 16);
+        }
+    }
     uint4* __restrict__ wbnd = L.bnd + ((int64_t)blockIdx.x * kWavesPerWG + wv) * L.bnd_per_wave;
     uint32_t* __restrict__ wboff = reinterpret_cast<uint32_t*>(wbnd + L.bnd_rows);
 
@@ -881,9 +935,16 @@ __global__ __launch_bounds__(64 * kWavesPerWG) void score_kernel(const ScoreLaun
         // pass p's profile (and NW left border) into buffer p & 1
         auto load_pass = [&](int pass) {
             const int buf = pass & 1;
-            const uint4* src = reinterpret_cast<const uint4*>(L.qp + Q.qp_off + (int64_t)pass * 32 * PR);
-            uint4* dst = reinterpret_cast<uint4*>(sprof + buf * 32 * PR);
-            for (int i = threadIdx.x; i < 32 * PR / 8; i += 64 * kWavesPerWG) dst[i] = src[i];
+            if constexpr (VP) {  // each row's scores for codes 1, 2, 4 and 8 (A, C, G, T)
+                const int16_t* src = L.qp + Q.qp_off + (int64_t)pass * 32 * PR;
+                for (int i = threadIdx.x; i < PR; i += 64 * kWavesPerWG)
+                    svp[buf * PR + i] = make_uint2((uint16_t)src[PR + i] | ((uint32_t)(uint16_t)src[2 * PR + i] << 16),
+                                                   (uint16_t)src[4 * PR + i] | ((uint32_t)(uint16_t)src[8 * PR + i] << 16));
+            } else {
+                const uint4* src = reinterpret_cast<const uint4*>(L.qp + Q.qp_off + (int64_t)pass * 32 * PR);
+                uint4* dst = reinterpret_cast<uint4*>(sprof + buf * 32 * PR);
+                for (int i = threadIdx.x; i < 32 * PR / 8; i += 64 * kWavesPerWG) dst[i] = src[i];
+            }
             if (NW) {
                 const int base = pass * PR;
                 uint32_t* bh = slbh + buf * (PR + 1);
@@ -904,15 +965,15 @@ __global__ __launch_bounds__(64 * kWavesPerWG) void score_kernel(const ScoreLaun
             __syncthreads();
         };
         if (!NW && Q.seg_off >= 0)
-            run_task<R, G, MODE, NB, true, !NW, false>(P, gbase, sp, slbh, slbe, wres, wbnd, wboff, L.slot_entries,
+            run_task<R, G, MODE, NB, true, !NW, false, VP>(P, gbase, sp, slut, slbh, slbe, wres, wbnd, wboff, L.slot_entries,
                                                          Q.pair_off, L.segs + Q.seg_off, nullptr, L.results, sb_lo,
                                                          sb_hi, k, g, Wv.nblocks, nbp, npass, last_g, last_r, stage);
         else if (npass > 1)
-            run_task<R, G, MODE, NB, true, false, false>(P, gbase, sp, slbh, slbe, wres, wbnd, wboff, L.slot_entries,
+            run_task<R, G, MODE, NB, true, false, false, VP>(P, gbase, sp, slut, slbh, slbe, wres, wbnd, wboff, L.slot_entries,
                                                      Q.pair_off, nullptr, nullptr, L.results, sb_lo, sb_hi, k, g,
                                                      Wv.nblocks, nbp, npass, last_g, last_r, stage);
         else
-            run_task<R, G, MODE, NB, false, false, false>(P, gbase, sp, slbh, slbe, wres, wbnd, wboff, L.slot_entries,
+            run_task<R, G, MODE, NB, false, false, false, VP>(P, gbase, sp, slut, slbh, slbe, wres, wbnd, wboff, L.slot_entries,
                                                       Q.pair_off, nullptr, nullptr, L.results, sb_lo, sb_hi, k, g,
                                                       Wv.nblocks, nbp, npass, last_g, last_r, stage);
     }
@@ -970,7 +1031,7 @@ __global__ __launch_bounds__(64 * kWavesPerWG) void score_kernel_pair(const Scor
             load_pass(pass);
             __syncthreads();
         };
-        run_task<R, G, F16 ? 3 : 0, 1, true, true, true>(P, gbase, sp, slb_unused, slb_unused, wres, wbnd, wboff,
+        run_task<R, G, F16 ? 3 : 0, 1, true, true, true, false>(P, gbase, sp, nullptr, slb_unused, slb_unused, wres, wbnd, wboff,
                                                          L.slot_entries,
                                             Q.pair_off, L.segs + Q.seg_off, L.segs + Q.seg_off_hi, L.results, sb_lo,
                                             sb_hi, k, g, Wv.nblocks, nbp, npass, 0, 0, stage);
@@ -979,20 +1040,21 @@ __global__ __launch_bounds__(64 * kWavesPerWG) void score_kernel_pair(const Scor
 
 // ---------------------------------------------------------------------------
 // Instantiation: one translation unit per mode (score_sw.hip: 0, score_nw1.hip:
-// 1, score_nw2.hip: 2, score_swf.hip: 3, score_nwf.hip: 4, score_pair.hip:
-// the paired kernel)
-// defines SSA_SCORE_INSTANTIATE and instantiates launch_mode / resident_mode
-// for its mode; the dispatch (score_kernels.hip) sees the declarations only.
+// 1, score_nw2.hip: 2, score_swf.hip: 3, score_nwf.hip: 4; with the profile in
+// VGPRs score_swv.hip: 0 and 3, score_nwv.hip: 2 and 4; score_pair.hip: the
+// paired kernel) defines SSA_SCORE_INSTANTIATE and instantiates launch_mode /
+// resident_mode for its modes; the dispatch (score_kernels.hip) sees the
+// declarations only.
 // ---------------------------------------------------------------------------
 hipError_t launch_fill_u32(hipStream_t st, uint32_t* p, int64_t n, uint32_t v);
 
-// score_kernel<R, G, MODE, NB> for lanes = G, rows = R, cols = 4 * NB:
+// score_kernel<R, G, MODE, NB, VP> for lanes = G, rows = R, cols = 4 * NB:
 // launch on a persistent grid of at most max_wg workgroups / resident
 // workgroups of the chip (0 for a shape the mode does not build)
-template <int MODE>
+template <int MODE, bool VP>
 hipError_t launch_mode(int lanes, int rows, int cols, hipStream_t st, const ScoreLaunch& L, const ScoreParams& P,
                        int max_wg);
-template <int MODE>
+template <int MODE, bool VP>
 int resident_mode(int lanes, int rows, int cols);
 // score_kernel_pair<R, G, F16>
 template <bool F16>
@@ -1016,12 +1078,13 @@ constexpr bool tile8_shape() {
 }
 
 // 32 rows per lane for SW only (NW holds more state per row and would run one
-// wave per SIMD)
-template <int R, int G, int MODE, int NB>
+// wave per SIMD); the profile in VGPRs for 4-column tiles of at most 16 rows
+// per lane, SW and NW without trackers (2R more registers per lane)
+template <int R, int G, int MODE, int NB, bool VP>
 constexpr bool shape_built() {
     return !(R == 32 && (MODE == 1 || MODE == 2 || MODE == 4)) &&
            !((MODE == 2 || MODE == 4) && !nw_untracked_shape<R, G>()) &&
-           (NB == 1 || tile8_shape<R, G>());
+           (NB == 1 || tile8_shape<R, G>()) && !(VP && (NB != 1 || R > 16 || MODE == 1));
 }
 
 #define SSA_SHAPES(X)                                                                                   \
@@ -1032,16 +1095,16 @@ constexpr bool shape_built() {
 #define SSA_PAIR_SHAPES(X) X(8, 16) X(8, 32) X(16, 8) X(16, 16) X(32, 8)
 
 #ifdef SSA_SCORE_INSTANTIATE
-template <int R, int G, int MODE, int NB>
+template <int R, int G, int MODE, int NB, bool VP>
 static int resident_wg_t() {
-    if constexpr (!shape_built<R, G, MODE, NB>()) {
+    if constexpr (!shape_built<R, G, MODE, NB, VP>()) {
         return 0;
     } else {
         static int cap = 0;
         if (!cap) {
             int per_cu = 0, dev = 0;
             hipDeviceProp_t prop;
-            if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, score_kernel<R, G, MODE, NB>,
+            if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, score_kernel<R, G, MODE, NB, VP>,
                                                              64 * kWavesPerWG, 0) != hipSuccess ||
                 hipGetDevice(&dev) != hipSuccess || hipGetDeviceProperties(&prop, dev) != hipSuccess)
                 return 0;
@@ -1051,35 +1114,36 @@ static int resident_wg_t() {
     }
 }
 
-template <int R, int G, int MODE, int NB>
+template <int R, int G, int MODE, int NB, bool VP>
 static hipError_t launch_score_t(hipStream_t st, const ScoreLaunch& L, const ScoreParams& P, int max_wg) {
-    if constexpr (!shape_built<R, G, MODE, NB>()) {
+    if constexpr (!shape_built<R, G, MODE, NB, VP>()) {
         return hipErrorInvalidValue;
     } else {
         const int nwg = std::max(1, std::min(L.ntasks, max_wg));
         const hipError_t e = launch_fill_u32(st, reinterpret_cast<uint32_t*>(L.counter), 1, 0u);
         if (e != hipSuccess) return e;
-        hipLaunchKernelGGL((score_kernel<R, G, MODE, NB>), dim3(nwg), dim3(64 * kWavesPerWG), 0, st, L, P);
+        hipLaunchKernelGGL((score_kernel<R, G, MODE, NB, VP>), dim3(nwg), dim3(64 * kWavesPerWG), 0, st, L, P);
         return hipGetLastError();
     }
 }
 
-template <int MODE>
+template <int MODE, bool VP>
 hipError_t launch_mode(int lanes, int rows, int cols, hipStream_t st, const ScoreLaunch& L, const ScoreParams& P,
                        int max_wg) {
 #define SSA_CASE(GG, RR)                                                              \
     if (lanes == GG && rows == RR)                                                  \
-        return cols == 8 ? launch_score_t<RR, GG, MODE, 2>(st, L, P, max_wg)        \
-                         : (cols == 4 ? launch_score_t<RR, GG, MODE, 1>(st, L, P, max_wg) : hipErrorInvalidValue);
+        return cols == 8 ? launch_score_t<RR, GG, MODE, 2, VP>(st, L, P, max_wg)    \
+                         : (cols == 4 ? launch_score_t<RR, GG, MODE, 1, VP>(st, L, P, max_wg) : hipErrorInvalidValue);
     SSA_SHAPES(SSA_CASE)
 #undef SSA_CASE
     return hipErrorInvalidValue;
 }
 
-template <int MODE>
+template <int MODE, bool VP>
 int resident_mode(int lanes, int rows, int cols) {
-#define SSA_CASE(GG, RR) \
-    if (lanes == GG && rows == RR) return cols == 8 ? resident_wg_t<RR, GG, MODE, 2>() : resident_wg_t<RR, GG, MODE, 1>();
+#define SSA_CASE(GG, RR)                                                                     \
+    if (lanes == GG && rows == RR)                                                         \
+        return cols == 8 ? resident_wg_t<RR, GG, MODE, 2, VP>() : resident_wg_t<RR, GG, MODE, 1, VP>();
     SSA_SHAPES(SSA_CASE)
 #undef SSA_CASE
     return 0;

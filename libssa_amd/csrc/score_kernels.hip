@@ -191,26 +191,52 @@ bool score_nw_untracked_supported(int lanes, int rows) {
            (lanes == 32 && rows == 8);
 }
 
-int score_resident_workgroups(int lanes, int rows, int mode, int cols) {
+// the profile in VGPRs: 4-column tiles of at most 16 rows per lane, SW and
+// NW without overflow trackers
+bool score_vp_supported(int lanes, int rows, int mode, int cols) {
+    if (cols != 4 || rows > 16 || !(mode == 0 || mode == 2 || mode == 3 || mode == 4)) return false;
+    if (!score_shape_supported(lanes, rows, mode, cols)) return false;
+    return (mode == 0 || mode == 3) || score_nw_untracked_supported(lanes, rows);
+}
+
+int score_resident_workgroups(int lanes, int rows, int mode, int cols, bool vp) {
+    if (vp) {
+        switch (mode) {
+        case 0: return resident_mode<0, true>(lanes, rows, cols);
+        case 2: return resident_mode<2, true>(lanes, rows, cols);
+        case 3: return resident_mode<3, true>(lanes, rows, cols);
+        case 4: return resident_mode<4, true>(lanes, rows, cols);
+        default: return 0;
+        }
+    }
     switch (mode) {
-    case 0: return resident_mode<0>(lanes, rows, cols);
-    case 1: return resident_mode<1>(lanes, rows, cols);
-    case 2: return resident_mode<2>(lanes, rows, cols);
-    case 3: return resident_mode<3>(lanes, rows, cols);
-    case 4: return resident_mode<4>(lanes, rows, cols);
+    case 0: return resident_mode<0, false>(lanes, rows, cols);
+    case 1: return resident_mode<1, false>(lanes, rows, cols);
+    case 2: return resident_mode<2, false>(lanes, rows, cols);
+    case 3: return resident_mode<3, false>(lanes, rows, cols);
+    case 4: return resident_mode<4, false>(lanes, rows, cols);
     default: return 0;
     }
 }
 
-hipError_t launch_score(int lanes, int rows, int mode, int cols, hipStream_t st, const ScoreLaunch& L,
+hipError_t launch_score(int lanes, int rows, int mode, int cols, bool vp, hipStream_t st, const ScoreLaunch& L,
                         const ScoreParams& P, int max_wg) {
     if (L.ntasks <= 0) return hipSuccess;
+    if (vp) {
+        switch (mode) {
+        case 0: return launch_mode<0, true>(lanes, rows, cols, st, L, P, max_wg);
+        case 2: return launch_mode<2, true>(lanes, rows, cols, st, L, P, max_wg);
+        case 3: return launch_mode<3, true>(lanes, rows, cols, st, L, P, max_wg);
+        case 4: return launch_mode<4, true>(lanes, rows, cols, st, L, P, max_wg);
+        default: return hipErrorInvalidValue;
+        }
+    }
     switch (mode) {
-    case 0: return launch_mode<0>(lanes, rows, cols, st, L, P, max_wg);
-    case 1: return launch_mode<1>(lanes, rows, cols, st, L, P, max_wg);
-    case 2: return launch_mode<2>(lanes, rows, cols, st, L, P, max_wg);
-    case 3: return launch_mode<3>(lanes, rows, cols, st, L, P, max_wg);
-    case 4: return launch_mode<4>(lanes, rows, cols, st, L, P, max_wg);
+    case 0: return launch_mode<0, false>(lanes, rows, cols, st, L, P, max_wg);
+    case 1: return launch_mode<1, false>(lanes, rows, cols, st, L, P, max_wg);
+    case 2: return launch_mode<2, false>(lanes, rows, cols, st, L, P, max_wg);
+    case 3: return launch_mode<3, false>(lanes, rows, cols, st, L, P, max_wg);
+    case 4: return launch_mode<4, false>(lanes, rows, cols, st, L, P, max_wg);
     default: return hipErrorInvalidValue;
     }
 }

@@ -4,6 +4,6 @@
 #include "score_kernel.h"
 
 namespace ssa {
-template hipError_t launch_mode<2>(int, int, int, hipStream_t, const ScoreLaunch&, const ScoreParams&, int);
-template int resident_mode<2>(int, int, int);
+template hipError_t launch_mode<2, false>(int, int, int, hipStream_t, const ScoreLaunch&, const ScoreParams&, int);
+template int resident_mode<2, false>(int, int, int);
 }  // namespace ssa

@@ -52,6 +52,8 @@ struct DeviceDB {
     size_t d_bytes = 0;
     std::vector<uint32_t> by_length;    // all entries, longest first (stable)
     bool packed = false;                // loaded from a packed file
+    bool acgt = false;                  // nucleotide entries of A, C, G and T only (the query
+                                        // profile may sit in VGPRs)
 };
 
 DeviceDB g_db;
@@ -80,6 +82,17 @@ static size_t map_db_codes(const char* s, size_t n, bool nucleotide, uint8_t* ou
         }
     }
     return unknown;
+}
+
+// True when every residue of every entry is A, C, G or T (codes 1, 2, 4, 8;
+// the padding after an entry is not looked at)
+static bool entries_acgt(const uint8_t* codes, const std::vector<DbEntry>& entries) {
+    for (const DbEntry& e : entries) {
+        const uint8_t* c = codes + e.code_off;
+        for (uint32_t i = 0; i < e.len; ++i)
+            if (c[i] != 1 && c[i] != 2 && c[i] != 4 && c[i] != 8) return false;
+    }
+    return true;
 }
 
 static void warn_unknown(size_t unknown) {
@@ -251,6 +264,7 @@ static DeviceDB& device_db(bool may_build = true) {
         SSA_HIP(hipMemset(d.d_codes + pv->h->code_bytes, 0, 4));
         SSA_HIP(hipMemcpy(d.d_codes, pv->codes, pv->h->code_bytes, hipMemcpyHostToDevice));
         d.packed = true;
+        d.acgt = nuc && entries_acgt(pv->codes, d.entries);
         return d;
     }
     HostDb hd;
@@ -263,6 +277,7 @@ static DeviceDB& device_db(bool may_build = true) {
     d.d_bytes = hd.codes.size();
     SSA_HIP(hipMalloc(&d.d_codes, d.d_bytes));
     SSA_HIP(hipMemcpy(d.d_codes, hd.codes.data(), d.d_bytes, hipMemcpyHostToDevice));
+    d.acgt = nuc && entries_acgt(hd.codes.data(), d.entries);
     return d;
 }
 
@@ -392,9 +407,10 @@ struct TierOut {
 namespace {
 
 struct Shape {
-    int lanes;     // G: lanes per stream pair
-    int rows;      // R: query rows per lane
-    int cols = 4;  // columns per step (the column tile): 4 or 8
+    int lanes;      // G: lanes per stream pair
+    int rows;       // R: query rows per lane
+    int cols = 4;   // columns per step (the column tile): 4 or 8
+    int place = 1;  // query profile: 1 LDS, 2 VGPRs (nucleotide DBs of A, C, G, T)
 };
 
 }  // namespace
@@ -402,6 +418,7 @@ struct Shape {
 Shape& shape_override_ref();
 int& cols_override_ref();
 int& arith_override_ref();
+int& placement_override_ref();
 void release_layouts();
 
 namespace {
@@ -788,6 +805,7 @@ struct TierRun {
     bool packed = false;  // SW with several queries: scored as packed queries
     bool paired = false;  // packed queries two by two, one per half of every lane group (score_kernel_pair)
     bool untracked = false;  // NW 16-bit where no pair can overflow: the kernel without trackers
+    bool vp = false;         // the query profile in VGPRs
     bool f16 = false;        // SW first tier in windowed packed fp16 (kernel mode 3), or the
                              // untracked NW kernel in packed fp16 (mode 4)
     int mode() const { return type == kSW ? (f16 ? 3 : 0) : (untracked ? (f16 ? 4 : 2) : 1); }
@@ -953,8 +971,15 @@ void tier_prepare(Exec& ex, TierRun& R) {
             for (uint32_t& w : R.lbv) w = cv(w);
         }
     }
+    // the query profile in VGPRs: forced, or the autotuner's choice for the
+    // shape, where the DB is A, C, G and T only and the kernel has it
+    {
+        const int po = placement_override_ref();
+        R.vp = !R.paired && db.acgt && score_vp_supported(sh.lanes, sh.rows, R.mode(), sh.cols) &&
+               (po == 2 || (po == 0 && sh.place == 2));
+    }
     const int cap = R.paired ? score_pair_resident_workgroups(sh.lanes, sh.rows, R.f16)
-                             : score_resident_workgroups(sh.lanes, sh.rows, R.mode(), sh.cols);
+                             : score_resident_workgroups(sh.lanes, sh.rows, R.mode(), sh.cols, R.vp);
 
     // ---- packed queries (SW, several queries) ----
     // Each query starts at a lane boundary and is padded to whole lanes (the
@@ -1190,7 +1215,7 @@ void tier_launch(hipStream_t st, TierRun& R) {
     if (R.empty()) return;
     SSA_HIP(launch_init_results(st, R.d_out, (int64_t)R.nout, -32768));
     if (R.paired) SSA_HIP(launch_score_pair(R.sh.lanes, R.sh.rows, R.f16, st, R.K, R.P, R.grid));
-    else SSA_HIP(launch_score(R.sh.lanes, R.sh.rows, R.mode(), R.sh.cols, st, R.K, R.P, R.grid));
+    else SSA_HIP(launch_score(R.sh.lanes, R.sh.rows, R.mode(), R.sh.cols, R.vp, st, R.K, R.P, R.grid));
 }
 
 void tier_finish(Exec& ex, TierRun& R) {
@@ -2385,7 +2410,7 @@ void run_batch(Exec& ex, const std::vector<SearchRequest>& reqs, std::vector<Sea
             J.st.lanes_per_stream = R.sh.lanes;
             J.st.rows_per_lane = R.sh.rows;
             J.st.columns_per_step = R.sh.cols;
-            J.st.profile_placement = 1;
+            J.st.profile_placement = R.vp ? 2 : 1;
             J.st.overflow_tracking = !R.untracked;
             J.st.cell_arithmetic = R.f16 ? 2 : 1;
         }
@@ -2937,17 +2962,23 @@ int autotune(int type, const std::vector<size_t>& lengths, int copies, std::vect
                 x ^= x << 17;
                 q[i] = alphabet[x % alphabet.size()];
             }
-        AutotuneResult best{L, 0, 0, 4, 1e300};
+        AutotuneResult best{L, 0, 0, 4, 1, 1e300};
         std::vector<TunedShape> meas;
         for (const Shape& base : kShapes)
-            for (int cols : {4, 8}) {
-                if (!score_shape_supported(base.lanes, base.rows, type, cols)) continue;
-                Shape sh = base;
-                sh.cols = cols;
-                const double ms = run(sh, qs) / copies;
-                meas.push_back(TunedShape{type, L, sh, ms});
-                if (ms < best.ms) best = AutotuneResult{L, sh.lanes, sh.rows, sh.cols, ms};
-            }
+            for (int cols : {4, 8})
+                for (int place : {1, 2}) {
+                    if (!score_shape_supported(base.lanes, base.rows, type, cols)) continue;
+                    // the profile in VGPRs where the DB allows it (NW: the kernel
+                    // without overflow trackers has it, which the run decides)
+                    if (place == 2 && !(db.acgt && score_vp_supported(base.lanes, base.rows, type == 0 ? 0 : 2, cols)))
+                        continue;
+                    Shape sh = base;
+                    sh.cols = cols;
+                    sh.place = place;
+                    const double ms = run(sh, qs) / copies;
+                    meas.push_back(TunedShape{type, L, sh, ms});
+                    if (ms < best.ms) best = AutotuneResult{L, sh.lanes, sh.rows, sh.cols, sh.place, ms};
+                }
         out.push_back(best);
         std::lock_guard<std::mutex> lk(tuned_mutex());
         auto& t = tuned_shapes();
@@ -3080,7 +3111,7 @@ RequestPlan plan_request(const SearchRequest& rq, const std::vector<size_t>& qle
     const int PR = sh.lanes * sh.rows;
     const int passes = (int)((std::max<size_t>(packed ? qsum : qmax, 1) + PR - 1) / PR);
     const int cap = std::max(1, score_resident_workgroups(sh.lanes, type == 0 ? sh.rows : std::min(sh.rows, 16),
-                                                          type == 0 ? 0 : 1, sh.cols));
+                                                          type == 0 ? 0 : 1, sh.cols, false));
     const int tasks = std::max(1, nwaves / kWavesPerWG) * (packed ? 1 : (int)nq);
     const int grid = std::min(tasks, cap);
     pl.occupancy = (double)grid / (double)cap;

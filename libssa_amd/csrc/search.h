@@ -64,7 +64,7 @@ int run_search_noalloc(SearchType type, p_query query, size_t hitcount, int bit_
 
 struct AutotuneResult {
     size_t qlen;
-    int lanes, rows, cols;
+    int lanes, rows, cols, place;
     double ms;
 };
 // shape measurements on the loaded DB; the fastest shape per (type, length)
