@@ -213,11 +213,12 @@ def bench_c4(args):
     path, tmp = setup_protein(db)
     hs = [ssa.init_sequence_fasta(ssa.READ_FROM_STRING, q.decode()) for q in qs]
     reqs = [(ssa.SEARCH_SW, h, 20, 8, ssa.COMPUTE_SCORE) for h in hs]
-    tuned, t_tune = [], 0.0
+    tuned, t_tune, choices = [], 0.0, []
     if not args.no_autotune:
         ta = time.perf_counter()
         tuned = ssa.autotune(ssa.SEARCH_SW, [30, 60, 120, 250, 500, 1000, 2000], copies=4)
         t_tune = time.perf_counter() - ta
+        choices = ssa.autotune_choices()
     ssa.batch_align(reqs)  # warm-up: builds the DB's streams for every lane width used
     t0 = time.perf_counter()
     plain = [ssa.sw_align(h, 20, 8, ssa.COMPUTE_SCORE) for h in hs]
@@ -231,7 +232,7 @@ def bench_c4(args):
     return {
         "metric": "batched SW GCUPS on 1 MI355X (vs one query per launch)",
         "value": round(cells / (t2 - t1) / 1e9, 2), "unit": "GCUPS", "n_gpus": 1, "higher_is_better": True,
-        "dtype": "int16", "data": "synthetic",
+        "dtype": "fp16" if stats[0].cell_arithmetic == 2 else "int16", "data": "synthetic",
         "config": {"workload": "256 queries log-uniform 30-2000 vs 500k lognormal protein DB, 2% planted, "
                                "sw_align(q, 20, 8, scores), BLOSUM62 10/1", "db_sequences": len(db),
                    "queries": len(qs), "query_residues": sum(len(q) for q in qs)},
@@ -243,8 +244,9 @@ def bench_c4(args):
         "one_query_per_launch": {"seconds": round(t1 - t0, 3), "gcups": round(cells / (t1 - t0) / 1e9, 2)},
         "speedup": round((t1 - t0) / (t2 - t1), 3), "results_equal": equal, "cells": cells,
         "autotune": {"seconds": round(t_tune, 2),
-                     "shapes": [{"qlen": L, "lanes_per_stream": g, "rows_per_lane": r, "ms": round(ms, 3)}
-                                for L, g, r, ms in tuned]},
+                     "shapes": [{"qlen": L, "lanes_per_stream": g, "rows_per_lane": r, "columns_per_step": c[2],
+                                 "profile_placement": c[3], "ms": round(ms, 3)}
+                                for (L, g, r, ms), c in zip(tuned, choices)]},
     }
 
 
@@ -318,7 +320,7 @@ def bench_c5(args):
     os.rmdir(tmp)
     return {
         "metric": "served p50/p99 latency and GCUPS on 1 MI355X (vs one request per launch)",
-        "unit": "ms", "n_gpus": 1, "higher_is_better": False, "dtype": "int16", "data": "synthetic",
+        "unit": "ms", "n_gpus": 1, "higher_is_better": False, "dtype": "fp16/int16", "data": "synthetic",
         "config": {"workload": "1M lognormal protein DB resident; Poisson arrivals; 80% SW / 20% NW, top 32 "
                                "with CIGARs, 8-bit start; |q| 70% U(50,400) + 30% log-U(400,5000)",
                    "db_sequences": len(db), "slots": args.slots, "batch_cells": args.batch_cells or 2.5e11},
@@ -360,7 +362,7 @@ def bench_dbload(args):
     return {
         "metric": "init_db + first search latency on 1 MI355X (FASTA vs packed file)",
         "value": round(pk["init_db_ms"] + pk["first_search_ms"], 1), "unit": "ms", "n_gpus": 1,
-        "higher_is_better": False, "dtype": "int16", "data": "synthetic",
+        "higher_is_better": False, "dtype": "fp16", "data": "synthetic",
         "config": {"workload": "1M lognormal protein DB (median 300); sw_align 8-bit, |q| = 500, top 20 with CIGARs",
                    "db_sequences": len(db), "packed_file_bytes": size},
         "fasta": fasta, "packed": pk, "pack_ms": round(t_pack, 1), "results_equal": got == want,
@@ -386,6 +388,9 @@ def bench_search(args, w, steps, warmup, cpu=False):
     ssa.init_db(path)
     qh = [ssa.init_sequence_fasta(ssa.READ_FROM_STRING, q.decode()) for q in w["queries"]]
     align = ssa.COMPUTE_ALIGNMENT if w.get("align") else ssa.COMPUTE_SCORE
+    ssa.set_kernel_shape(*[int(x) for x in getattr(args, "kernel_shape", "0,0").split(",")])
+    ssa.set_kernel_variant(*[int(x) for x in getattr(args, "kernel_variant", "0,0").split(",")])
+    ssa.set_cell_arithmetic(getattr(args, "cell_arithmetic", 0))
     search = ssa.sw_align if w["kind"] == "sw" else ssa.nw_align
 
     def step():
@@ -410,6 +415,13 @@ def bench_search(args, w, steps, warmup, cpu=False):
         ssa.free_sequence(h)
     os.remove(path)
     os.rmdir(tmp)
+    ssa.set_kernel_shape(0, 0)
+    ssa.set_kernel_variant(0, 0)
+    ssa.set_cell_arithmetic(0)
+    # the first tier's kernel of each query in the last step
+    kernels = [{"lanes_per_stream": s.lanes_per_stream, "rows_per_lane": s.rows_per_lane,
+                "columns_per_step": s.columns_per_step, "profile_placement": s.profile_placement,
+                "cell_arithmetic": "fp16" if s.cell_arithmetic == 2 else "int16"} for s in stats[-len(qh):]]
 
     secs = t1 - t0
     cells_step = sum(s.cells for s in stats) / steps
@@ -437,11 +449,14 @@ def bench_search(args, w, steps, warmup, cpu=False):
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "int16",
+        # the first tier's cell arithmetic: packed fp16 holding integer scores
+        # exactly (per-lane windows for SW), or packed saturating int16
+        "dtype": "fp16" if any(k["cell_arithmetic"] == "fp16" for k in kernels) else "int16",
         "data": "synthetic",
         "config": {"workload": w["desc"], "db_sequences": len(w["db"]), "cells_per_step": int(cells_step / dup),
                    "reference_cells_per_step": int(cells_step), "scored_cells_per_step": int(scored_step),
-                   "query_lengths": [len(q) for q in w["queries"]], "hits": w["k"], "bit_width": w["bw"]},
+                   "query_lengths": [len(q) for q in w["queries"]], "hits": w["k"], "bit_width": w["bw"],
+                   "kernels": kernels},
         "roofline": {"bound": "valu", "achieved": round(achieved, 3), "peak": round(PEAK_VALU_TOPS, 1),
                      "unit": "TOP/s", "frac": round(achieved / PEAK_VALU_TOPS, 4),
                      "traffic": traffic,
@@ -500,6 +515,12 @@ def main():
     ap.add_argument("--slots", type=int, default=2, help="c5: server workers for continuous batching")
     ap.add_argument("--batch-cells", type=float, default=0.0, help="c5: cell budget per batch (0 = default)")
     ap.add_argument("--requests", type=int, default=60, help="c5: requests per rate")
+    ap.add_argument("--kernel-shape", default="0,0", metavar="G,R",
+                    help="c1-c3: force lanes per stream pair and rows per lane (0,0 = chosen per query)")
+    ap.add_argument("--kernel-variant", default="0,0", metavar="C,P",
+                    help="c1-c3: force columns per step (4, 8) and profile placement (1 LDS, 2 VGPRs); 0 = chosen")
+    ap.add_argument("--cell-arithmetic", type=int, default=0, choices=[0, 1],
+                    help="c1-c3: 0 = fp16 where it holds (default), 1 = packed int16 everywhere")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="also run one device-to-device copy of MIB MiB (calibrates HBM byte counters)")
     args = ap.parse_args()
