@@ -908,9 +908,19 @@ __global__ __launch_bounds__(64 * kWavesPerWG) void score_kernel(const ScoreLaun
     uint4* __restrict__ wbnd = L.bnd + ((int64_t)blockIdx.x * kWavesPerWG + wv) * L.bnd_per_wave;
     uint32_t* __restrict__ wboff = reinterpret_cast<uint32_t*>(wbnd + L.bnd_rows);
 
+    // Each workgroup's first task is its own index, later ones come from the
+    // counter (past the grid).  The hardware places workgroups 0 .. CUs - 1
+    // first, one per CU, and at equal priority a SIMD issues from its older
+    // wave first, so while both workgroups of a CU are busy the second runs at
+    // about a third of the first one's rate.  The tasks come costliest first,
+    // so this starts the longest ones on first workgroups; drawn from the
+    // counter, some landed on second workgroups, where one outlasted the rest
+    // of the launch (62 instead of 57 ms for the 5,000-residue SW query).
+    bool first_task = true;
     for (;;) {
         __syncthreads();  // the previous task is done with s_task and the staged profiles
-        if (threadIdx.x == 0) s_task = atomicAdd(L.counter, 1);
+        if (threadIdx.x == 0) s_task = first_task ? (int)blockIdx.x : (int)gridDim.x + atomicAdd(L.counter, 1);
+        first_task = false;
         __syncthreads();
         const int task = s_task;
         if (task >= L.ntasks) break;  // the same for the whole workgroup
@@ -999,9 +1009,11 @@ __global__ __launch_bounds__(64 * kWavesPerWG) void score_kernel_pair(const Scor
     uint4* __restrict__ wbnd = L.bnd + ((int64_t)blockIdx.x * kWavesPerWG + wv) * L.bnd_per_wave;
     uint32_t* __restrict__ wboff = reinterpret_cast<uint32_t*>(wbnd + L.bnd_rows);
 
+    bool first_task = true;  // as in score_kernel: the first task is the workgroup's index
     for (;;) {
         __syncthreads();  // the previous task is done with s_task and the staged profiles
-        if (threadIdx.x == 0) s_task = atomicAdd(L.counter, 1);
+        if (threadIdx.x == 0) s_task = first_task ? (int)blockIdx.x : (int)gridDim.x + atomicAdd(L.counter, 1);
+        first_task = false;
         __syncthreads();
         const int task = s_task;
         if (task >= L.ntasks) break;  // the same for the whole workgroup

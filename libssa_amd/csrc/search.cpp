@@ -1154,9 +1154,14 @@ void tier_prepare(Exec& ex, TierRun& R) {
     const int64_t max_nbp =
         (std::max<int64_t>(max_nb, nb_step * (sh.lanes + 2)) + 2 * nb_step - 1) / (2 * nb_step) * (2 * nb_step);
     // (mode 3 keeps each block's window offsets, one uint32 per lane group,
-    // after the bottom rows)
-    const int64_t bnd_rows = multipass ? max_nbp * ng * 2 : 0;
-    const int64_t bnd_per_wave = bnd_rows + (multipass && R.type == kSW && R.f16 ? (max_nbp * ng + 3) / 4 : 0);
+    // after the bottom rows).  Both parts are whole 128-byte lines, so that
+    // every wave's slice starts on a line and a block's bottom rows of the
+    // wave's lane groups (128 bytes for 16-lane groups) fill one line instead
+    // of straddling two.
+    auto lines = [](int64_t u4) { return (u4 + 7) / 8 * 8; };
+    const int64_t bnd_rows = multipass ? lines(max_nbp * ng * 2) : 0;
+    const int64_t bnd_per_wave =
+        bnd_rows + (multipass && R.type == kSW && R.f16 ? lines((max_nbp * ng + 3) / 4) : 0);
 
     // ---- device buffers ----
     auto name = [&](const char* s) { return R.pfx + s; };
