@@ -411,6 +411,17 @@ def bench_search(args, w, steps, warmup, cpu=False):
         st, tops = step()
         stats.extend(st)
     t1 = time.perf_counter()
+    trace = None
+    if getattr(args, "task_trace", False):
+        # one more search per query with the first tier's tasks recorded
+        trace = {}
+        ssa.set_task_trace(True)
+        try:
+            for q, h in zip(w["queries"], qh):
+                search(h, w["k"], w["bw"], align)
+                trace["q%d" % len(q)] = task_trace_summary(ssa.task_trace())
+        finally:
+            ssa.set_task_trace(False)
     for h in qh:
         ssa.free_sequence(h)
     os.remove(path)
@@ -479,6 +490,8 @@ def bench_search(args, w, steps, warmup, cpu=False):
                           "chunks_per_search": stats[0].chunk_count if stats else 0},
         "cpu_baseline": None,
     }
+    if trace is not None:
+        out["task_trace"] = trace
     if cpu and not args.no_cpu_baseline:
         cb, check = cpu_baseline(w)
         if cb is not None and check[0] == len(w["db"]):
@@ -495,6 +508,40 @@ def bench_search(args, w, steps, warmup, cpu=False):
                                      "hits_above_kth_score_equal": same_above}
         out["cpu_baseline"] = cb
     return out
+
+
+def task_trace_summary(recs):
+    """Statistics of a traced first-tier launch (ssa.task_trace records, in
+    task order).  Each workgroup runs its first task by its index, so the
+    first `grid` records are the first round, and workgroup w holds slot
+    w // CUs of its CU.  `us_per_step_by_slot` is the first round's time per
+    4-column step of a wave in each slot (all slots busy); `ms_at_slot_rates`
+    is the launch's task steps at the per-CU rate those give, with no tail."""
+    if not recs:
+        return None
+    import libssa_amd as ssa
+    a = np.array([r[:4] for r in recs], dtype=np.float64)
+    start, end, wg, steps = a.T
+    dur = end - start
+    grid = int(wg.max()) + 1
+    cus = max(1, ssa.device_cus())
+    first = (np.arange(len(recs)) < grid) & (steps > 0)
+    slot = (wg // cus).astype(np.int64)
+    us = []
+    for k in range(int(slot.max()) + 1):
+        m = first & (slot == k)
+        us.append(round(float((dur[m] / steps[m]).mean()) / 1e3, 3) if m.any() else None)
+    rate = sum(1 / u for u in us if u)  # steps per us per CU
+    wg_end = np.zeros(grid)
+    np.maximum.at(wg_end, wg.astype(np.int64), end)
+    lt = int(np.argmax(steps))
+    return {"tasks": len(recs), "workgroups": grid, "cus": cus, "launch_ms": round(float(end.max()) / 1e6, 3),
+            "task_steps": int(steps.sum()), "us_per_step_by_slot": us,
+            "ms_at_slot_rates": round(float(steps.sum()) / (cus * rate) / 1e3, 3) if rate else None,
+            "workgroup_end_ms": {p: round(float(np.percentile(wg_end, q)) / 1e6, 3)
+                                 for p, q in (("p10", 10), ("p50", 50), ("p90", 90), ("max", 100))},
+            "longest_task": {"steps": int(steps[lt]), "workgroup": int(wg[lt]),
+                             "start_ms": round(float(start[lt]) / 1e6, 3), "end_ms": round(float(end[lt]) / 1e6, 3)}}
 
 
 def summary(rec, keys):
@@ -521,6 +568,8 @@ def main():
                     help="c1-c3: force columns per step (4, 8) and profile placement (1 LDS, 2 VGPRs); 0 = chosen")
     ap.add_argument("--cell-arithmetic", type=int, default=0, choices=[0, 1],
                     help="c1-c3: 0 = fp16 where it holds (default), 1 = packed int16 everywhere")
+    ap.add_argument("--task-trace", action="store_true",
+                    help="c1-c3: also trace one more search per query and summarise its score launch's tasks")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="also run one device-to-device copy of MIB MiB (calibrates HBM byte counters)")
     args = ap.parse_args()

@@ -124,6 +124,33 @@ int ssa_amd_autotune_choice(size_t i, int* lanes_per_stream, int* rows_per_lane,
 /* Forgets the measured shapes (the cost model chooses again). */
 void ssa_amd_autotune_reset(void);
 
+/* ---- task trace (diagnostics) ---- */
+
+/* No counterpart in the reference: a timeline of the score kernels for
+ * callers tuning or diagnosing launches.  With ssa_amd_set_task_trace(1), the
+ * first-tier score launches of the following searches and batches record
+ * every task (a workgroup's 4 waves over all passes of their streams): its
+ * start and end in ns from the first task's start (the GPU's constant-rate
+ * clock), the workgroup that ran it, its 4-column steps per wave and the
+ * launch of the batch it belongs to.  Each workgroup runs its first task by
+ * its index and takes later ones from a queue, costliest first; workgroups 0
+ * .. CUs - 1 hold the first slot of each CU.  Tracing costs two stores per
+ * task. */
+typedef struct {
+    uint64_t start_ns;
+    uint64_t end_ns;
+    int32_t workgroup;
+    int32_t steps;
+    int32_t launch;
+    int32_t pad;
+} ssa_amd_task_record_t;
+
+void ssa_amd_set_task_trace(int on);
+
+/* Copies up to max records of the last traced search (or batch) to out and
+ * returns how many there are. */
+size_t ssa_amd_task_trace(ssa_amd_task_record_t* out, size_t max);
+
 /* ---- batched searches ---- */
 
 /* One search of a batch: the arguments of sw_align (search_type 0) or
@@ -287,6 +314,10 @@ void ssa_amd_get_matrix(int64_t* out);
 
 /* Name of the GPU architecture the library runs on (e.g. "gfx950:sramecc+:xnack-"). */
 const char* ssa_amd_device_arch(void);
+
+/* Compute units of the GPU the library runs on (a score launch's workgroups
+ * 0 .. CUs - 1 hold the first slot of each CU, as in the task trace). */
+int ssa_amd_device_cus(void);
 
 /* Self-check of the host replay of the reference's lane schedule and heap
  * (host only, no GPU call): `cases` random sets of per-pair tier results

@@ -104,6 +104,12 @@ class Stats(_C.Structure):
                 ("cell_arithmetic", _C.c_int)]
 
 
+class TaskRecord(_C.Structure):
+    """ssa_amd_task_record_t (include/libssa_amd.h)."""
+    _fields_ = [("start_ns", _C.c_uint64), ("end_ns", _C.c_uint64), ("workgroup", _C.c_int32),
+                ("steps", _C.c_int32), ("launch", _C.c_int32), ("pad", _C.c_int32)]
+
+
 @_dataclass
 class Alignment:
     """One entry of the returned alignment list (alignment_t, src/libssa.h:100-110)."""
@@ -141,9 +147,10 @@ def _load():
         "ssa_amd_set_kernel_shape": (None, [I, I]), "ssa_amd_set_stream_waves": (None, [I]),
         "ssa_amd_set_kernel_variant": (None, [I, I]),
         "ssa_amd_set_cell_arithmetic": (None, [I]),
+        "ssa_amd_set_task_trace": (None, [I]), "ssa_amd_task_trace": (S, [_C.POINTER(TaskRecord), S]),
         "ssa_amd_autotune_choice": (I, [_C.c_size_t, _C.POINTER(I), _C.POINTER(I), _C.POINTER(I), _C.POINTER(I),
                                         _C.POINTER(_C.c_double)]),
-        "ssa_amd_get_matrix": (None, [P]), "ssa_amd_device_arch": (cp, []),
+        "ssa_amd_get_matrix": (None, [P]), "ssa_amd_device_arch": (cp, []), "ssa_amd_device_cus": (I, []),
         "ssa_amd_batch_align": (I, [_C.POINTER(_Request), S, _C.POINTER(_C.POINTER(_AlignmentList)),
                                     _C.POINTER(Stats)]),
         "ssa_amd_autotune": (I, [I, _C.POINTER(_C.c_size_t), S, I, _C.POINTER(I), _C.POINTER(I),
@@ -271,6 +278,11 @@ def get_matrix():
     return list(buf)
 
 
+def device_cus():
+    """Compute units of the GPU (ssa_amd_device_cus)."""
+    return lib.ssa_amd_device_cus()
+
+
 def device_arch():
     return lib.ssa_amd_device_arch().decode()
 
@@ -381,6 +393,21 @@ def autotune_choices():
 
 
 def autotune_reset(): lib.ssa_amd_autotune_reset()
+
+
+def set_task_trace(on):
+    """Record the tasks of the following searches' first-tier score launches
+    (ssa_amd_set_task_trace)."""
+    lib.ssa_amd_set_task_trace(1 if on else 0)
+
+
+def task_trace():
+    """The last traced search's tasks (ssa_amd_task_trace): [(start ns, end ns,
+    workgroup, steps, launch)]."""
+    n = lib.ssa_amd_task_trace(None, 0)
+    buf = (TaskRecord * max(n, 1))()
+    n = lib.ssa_amd_task_trace(buf, n)
+    return [(r.start_ns, r.end_ns, r.workgroup, r.steps, r.launch) for r in buf[:n]]
 
 
 def align_noalloc(search_type, query, hitcount, bit_width):

@@ -245,6 +245,14 @@ int ssa_amd_autotune(int search_type, const size_t* lengths, size_t n, int copie
     return k;
 }
 
+void ssa_amd_set_task_trace(int on) { set_task_trace(on != 0); }
+
+size_t ssa_amd_task_trace(ssa_amd_task_record_t* out, size_t max) {
+    const std::vector<ssa_amd_task_record_t> r = last_task_trace();
+    if (out) std::copy(r.begin(), r.begin() + std::min(max, r.size()), out);
+    return r.size();
+}
+
 int ssa_amd_autotune_choice(size_t i, int* lanes_per_stream, int* rows_per_lane, int* columns_per_step,
                             int* profile_placement, double* ms) {
     const std::vector<AutotuneResult>& r = last_autotune();
@@ -270,5 +278,7 @@ const char* ssa_amd_device_arch(void) {
     arch = Gpu::get().arch();
     return arch.c_str();
 }
+
+int ssa_amd_device_cus(void) { return Gpu::get().cu_count(); }
 
 }  // extern "C"

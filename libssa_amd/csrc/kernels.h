@@ -94,6 +94,16 @@ struct PairResult {
 // workgroup takes tasks from *counter (zeroed before the launch) until they
 // run out, and keeps the pass boundaries of its 4 waves in its own slice of
 // bnd (bnd_per_wave uint4 per wave).
+// one task of a score launch, recorded when the caller traces the launches:
+// start and end on the GPU's constant-rate wall clock, the workgroup that ran
+// it and its 4-column steps per wave (passes x pass length)
+struct TaskTrace {
+    uint64_t start;
+    uint64_t end;
+    int32_t wg;
+    int32_t steps;
+};
+
 struct ScoreLaunch {
     const ScoreWave* waves;
     const ScoreTask* tasks;
@@ -112,6 +122,7 @@ struct ScoreLaunch {
     const int32_t* slot_entries;
     PairResult* results;
     const SegLane* segs;
+    TaskTrace* trace;  // per-task records, or null (no tracing)
 };
 
 // one (stream, entry) of the residue interleaving kernel

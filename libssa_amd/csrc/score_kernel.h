@@ -876,6 +876,18 @@ __device__ __forceinline__ void run_task(const ScoreParams& P, uint32_t gbase0, 
     }
 }
 
+// Task trace (thread 0 of a workgroup, at each fetch): the end of the
+// workgroup's previous task and the start of the one it fetched, on the GPU's
+// constant-rate wall clock.
+__device__ __forceinline__ void trace_fetch(const ScoreLaunch& L, int prev, int task) {
+    const uint64_t now = wall_clock64();
+    if (prev >= 0) L.trace[prev].end = now;
+    if (task < L.ntasks) {
+        L.trace[task].start = now;
+        L.trace[task].wg = (int32_t)blockIdx.x;
+    }
+}
+
 template <int R, int G, int MODE, int NB, bool VP>
 __global__ __launch_bounds__(64 * kWavesPerWG) void score_kernel(const ScoreLaunch L, const ScoreParams P) {
     constexpr int PR = G * R;  // rows per pass
@@ -917,12 +929,17 @@ __global__ __launch_bounds__(64 * kWavesPerWG) void score_kernel(const ScoreLaun
     // counter, some landed on second workgroups, where one outlasted the rest
     // of the launch (62 instead of 57 ms for the 5,000-residue SW query).
     bool first_task = true;
+    int prev_task = -1;
     for (;;) {
         __syncthreads();  // the previous task is done with s_task and the staged profiles
-        if (threadIdx.x == 0) s_task = first_task ? (int)blockIdx.x : (int)gridDim.x + atomicAdd(L.counter, 1);
+        if (threadIdx.x == 0) {
+            s_task = first_task ? (int)blockIdx.x : (int)gridDim.x + atomicAdd(L.counter, 1);
+            if (L.trace) trace_fetch(L, prev_task, s_task);
+        }
         first_task = false;
         __syncthreads();
         const int task = s_task;
+        prev_task = task;
         if (task >= L.ntasks) break;  // the same for the whole workgroup
         const ScoreTask T = L.tasks[task];
         const ScoreWave Wv = L.waves[T.wave0 + wv];
@@ -936,6 +953,7 @@ __global__ __launch_bounds__(64 * kWavesPerWG) void score_kernel(const ScoreLaun
         for (int i = 0; i < kWavesPerWG; ++i) nbmax = max(nbmax, L.waves[T.wave0 + i].nblocks);
         if (nbmax == 0) continue;
         const int nbp = ((npass > 1 ? max(nbmax, NB * (G + 2)) : nbmax) + 2 * NB - 1) / (2 * NB) * (2 * NB);
+        if (L.trace && threadIdx.x == 0) L.trace[task].steps = npass * (nbp / NB);
         const uint2* __restrict__ wres = L.res + Wv.res_off;
         const int sb_lo = L.slot_first[Wv.slot_base + 2 * k];
         const int sb_hi = L.slot_first[Wv.slot_base + 2 * k + 1];
@@ -1010,12 +1028,17 @@ __global__ __launch_bounds__(64 * kWavesPerWG) void score_kernel_pair(const Scor
     uint32_t* __restrict__ wboff = reinterpret_cast<uint32_t*>(wbnd + L.bnd_rows);
 
     bool first_task = true;  // as in score_kernel: the first task is the workgroup's index
+    int prev_task = -1;
     for (;;) {
         __syncthreads();  // the previous task is done with s_task and the staged profiles
-        if (threadIdx.x == 0) s_task = first_task ? (int)blockIdx.x : (int)gridDim.x + atomicAdd(L.counter, 1);
+        if (threadIdx.x == 0) {
+            s_task = first_task ? (int)blockIdx.x : (int)gridDim.x + atomicAdd(L.counter, 1);
+            if (L.trace) trace_fetch(L, prev_task, s_task);
+        }
         first_task = false;
         __syncthreads();
         const int task = s_task;
+        prev_task = task;
         if (task >= L.ntasks) break;  // the same for the whole workgroup
         const ScoreTask T = L.tasks[task];
         const ScoreWave Wv = L.waves[T.wave0 + wv];
@@ -1026,6 +1049,7 @@ __global__ __launch_bounds__(64 * kWavesPerWG) void score_kernel_pair(const Scor
         for (int i = 0; i < kWavesPerWG; ++i) nbmax = max(nbmax, L.waves[T.wave0 + i].nblocks);
         if (nbmax == 0) continue;
         const int nbp = npass > 1 ? (max(nbmax, G + 2) + 1) & ~1 : (nbmax + 1) & ~1;
+        if (L.trace && threadIdx.x == 0) L.trace[task].steps = npass * nbp;
         const uint2* __restrict__ wres = L.res + Wv.res_off;
         const int sb_lo = L.slot_first[Wv.slot_base + 2 * k];
         const int sb_hi = L.slot_first[Wv.slot_base + 2 * k + 1];
@@ -1132,7 +1156,9 @@ static hipError_t launch_score_t(hipStream_t st, const ScoreLaunch& L, const Sco
         return hipErrorInvalidValue;
     } else {
         const int nwg = std::max(1, std::min(L.ntasks, max_wg));
-        const hipError_t e = launch_fill_u32(st, reinterpret_cast<uint32_t*>(L.counter), 1, 0u);
+        hipError_t e = launch_fill_u32(st, reinterpret_cast<uint32_t*>(L.counter), 1, 0u);
+        if (e == hipSuccess && L.trace)
+            e = launch_fill_u32(st, reinterpret_cast<uint32_t*>(L.trace), (int64_t)L.ntasks * (sizeof(TaskTrace) / 4), 0u);
         if (e != hipSuccess) return e;
         hipLaunchKernelGGL((score_kernel<R, G, MODE, NB, VP>), dim3(nwg), dim3(64 * kWavesPerWG), 0, st, L, P);
         return hipGetLastError();
@@ -1164,7 +1190,9 @@ int resident_mode(int lanes, int rows, int cols) {
 template <int R, int G, bool F16>
 static hipError_t launch_pair_t(hipStream_t st, const ScoreLaunch& L, const ScoreParams& P, int max_wg) {
     const int nwg = std::max(1, std::min(L.ntasks, max_wg));
-    const hipError_t e = launch_fill_u32(st, reinterpret_cast<uint32_t*>(L.counter), 1, 0u);
+    hipError_t e = launch_fill_u32(st, reinterpret_cast<uint32_t*>(L.counter), 1, 0u);
+    if (e == hipSuccess && L.trace)
+        e = launch_fill_u32(st, reinterpret_cast<uint32_t*>(L.trace), (int64_t)L.ntasks * (sizeof(TaskTrace) / 4), 0u);
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL((score_kernel_pair<R, G, F16>), dim3(nwg), dim3(64 * kWavesPerWG), 0, st, L, P);
     return hipGetLastError();

@@ -419,6 +419,7 @@ Shape& shape_override_ref();
 int& cols_override_ref();
 int& arith_override_ref();
 int& placement_override_ref();
+bool& task_trace_ref();
 void release_layouts();
 
 namespace {
@@ -1214,6 +1215,9 @@ void tier_prepare(Exec& ex, TierRun& R) {
     K.slot_entries = L->d_slot_entries;
     K.results = R.d_out;
     K.segs = d_segs;
+    K.trace = task_trace_ref() && !R.tasks.empty()
+                  ? (TaskTrace*)ex.arena(name("trace"), R.tasks.size() * sizeof(TaskTrace))
+                  : nullptr;
 }
 
 void tier_launch(hipStream_t st, TierRun& R) {
@@ -1665,6 +1669,51 @@ int& placement_override_ref() {
 }
 
 void set_placement_override(int placement) { placement_override_ref() = placement; }
+
+bool& task_trace_ref() {
+    static bool on = false;
+    return on;
+}
+
+void set_task_trace(bool on) { task_trace_ref() = on; }
+
+namespace {
+std::mutex g_trace_mu;
+std::vector<ssa_amd_task_record_t> g_trace;  // the last traced search's (or batch's) tasks
+}  // namespace
+
+std::vector<ssa_amd_task_record_t> last_task_trace() {
+    std::lock_guard<std::mutex> lk(g_trace_mu);
+    return g_trace;
+}
+
+// The first tiers' task records of a batch, in ns from its first task start.
+static void collect_task_trace(Exec& ex, const std::vector<std::unique_ptr<TierRun>>& runs) {
+    std::vector<ssa_amd_task_record_t> recs;
+    std::vector<TaskTrace> buf;
+    uint64_t t0 = UINT64_MAX;
+    for (size_t g = 0; g < runs.size(); ++g) {
+        const TierRun& R = *runs[g];
+        if (R.empty() || !R.K.trace) continue;
+        buf.resize((size_t)R.K.ntasks);
+        ex.download(buf.data(), R.K.trace, buf.size() * sizeof(TaskTrace));
+        ex.sync();
+        for (const TaskTrace& t : buf) {
+            recs.push_back(ssa_amd_task_record_t{t.start, t.end, t.wg, t.steps, (int32_t)g, 0});
+            if (t.start) t0 = std::min(t0, t.start);
+        }
+    }
+    int dev = 0, khz = 0;
+    if (hipGetDevice(&dev) != hipSuccess || hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev) != hipSuccess)
+        khz = 0;
+    const double ns_per_tick = khz > 0 ? 1e6 / khz : 10.0;
+    for (ssa_amd_task_record_t& r : recs) {
+        r.start_ns = r.start_ns >= t0 ? (uint64_t)((double)(r.start_ns - t0) * ns_per_tick) : 0;
+        r.end_ns = r.end_ns >= t0 ? (uint64_t)((double)(r.end_ns - t0) * ns_per_tick) : 0;
+    }
+    std::lock_guard<std::mutex> lk(g_trace_mu);
+    g_trace.swap(recs);
+}
 
 void set_target_waves(int waves) {
     target_waves() = waves > 0 ? waves : 10240;
@@ -2376,6 +2425,7 @@ void run_batch(Exec& ex, const std::vector<SearchRequest>& reqs, std::vector<Sea
             ms_first = tm.ms();
         }
     }
+    if (task_trace_ref()) collect_task_trace(ex, runs);
     ex.recycle();  // the first tiers' results are on the host
 
     // ---- decode the first tiers; re-queued and exact tiers per search ----

@@ -121,5 +121,7 @@ void set_cols_override(int cols);             // 0 = chosen per search, else 4 o
 void set_placement_override(int placement);   // 0 = chosen per search, 1 LDS, 2 VGPRs
 void set_arith_override(int arith);           // 0 = fp16 windows for the SW first tier, 1 = int16 everywhere
 void set_target_waves(int waves);             // 0 = default
+void set_task_trace(bool on);                 // record the first-tier score launches' tasks
+std::vector<ssa_amd_task_record_t> last_task_trace();
 
 }  // namespace ssa
