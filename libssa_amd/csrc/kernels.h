@@ -108,7 +108,7 @@ struct ScoreLaunch {
     const ScoreWave* waves;
     const ScoreTask* tasks;
     int32_t ntasks;
-    int32_t pad;
+    int32_t prio_steps;  // wave priority unit: a task's remaining 4-column steps in these units set it (0: off)
     int32_t* counter;
     const QueryDesc* queries;
     const int16_t* qp;

@@ -876,6 +876,22 @@ __device__ __forceinline__ void run_task(const ScoreParams& P, uint32_t gbase0, 
     }
 }
 
+// Wave priority from the steps the workgroup's task has left, in units of
+// L.prio_steps (about half a typical task): 0 below one unit, then 1, 2 and 3
+// from 1, 2 and 4 units.  The two workgroups of a CU share its SIMDs, and at
+// equal priority a SIMD issues from its older wave first, so the CU's issue
+// slots go to whichever task has more work left: a long task that a second
+// workgroup takes late does not run at the leftover rate while its CU's
+// first workgroup works through short ones (the end of the launch waited on
+// such tasks).
+__device__ __forceinline__ void set_task_priority(int64_t steps_left, int32_t unit) {
+    if (unit <= 0) return;
+    if (steps_left >= 4 * (int64_t)unit) __builtin_amdgcn_s_setprio(3);
+    else if (steps_left >= 2 * (int64_t)unit) __builtin_amdgcn_s_setprio(2);
+    else if (steps_left >= unit) __builtin_amdgcn_s_setprio(1);
+    else __builtin_amdgcn_s_setprio(0);
+}
+
 // Task trace (thread 0 of a workgroup, at each fetch): the end of the
 // workgroup's previous task and the start of the one it fetched, on the GPU's
 // constant-rate wall clock.
@@ -954,6 +970,7 @@ __global__ __launch_bounds__(64 * kWavesPerWG) void score_kernel(const ScoreLaun
         if (nbmax == 0) continue;
         const int nbp = ((npass > 1 ? max(nbmax, NB * (G + 2)) : nbmax) + 2 * NB - 1) / (2 * NB) * (2 * NB);
         if (L.trace && threadIdx.x == 0) L.trace[task].steps = npass * (nbp / NB);
+        set_task_priority((int64_t)npass * (nbp / NB), L.prio_steps);
         const uint2* __restrict__ wres = L.res + Wv.res_off;
         const int sb_lo = L.slot_first[Wv.slot_base + 2 * k];
         const int sb_hi = L.slot_first[Wv.slot_base + 2 * k + 1];
@@ -991,6 +1008,8 @@ __global__ __launch_bounds__(64 * kWavesPerWG) void score_kernel(const ScoreLaun
             __syncthreads();  // every lane of the workgroup has left pass - 2
             load_pass(pass);
             __syncthreads();
+            // the lanes are in passes pass - 2 and pass - 1
+            set_task_priority((int64_t)(npass - pass + 1) * (nbp / NB), L.prio_steps);
         };
         if (!NW && Q.seg_off >= 0)
             run_task<R, G, MODE, NB, true, !NW, false, VP>(P, gbase, sp, slut, slbh, slbe, wres, wbnd, wboff, L.slot_entries,
@@ -1050,6 +1069,7 @@ __global__ __launch_bounds__(64 * kWavesPerWG) void score_kernel_pair(const Scor
         if (nbmax == 0) continue;
         const int nbp = npass > 1 ? (max(nbmax, G + 2) + 1) & ~1 : (nbmax + 1) & ~1;
         if (L.trace && threadIdx.x == 0) L.trace[task].steps = npass * nbp;
+        set_task_priority((int64_t)npass * nbp, L.prio_steps);
         const uint2* __restrict__ wres = L.res + Wv.res_off;
         const int sb_lo = L.slot_first[Wv.slot_base + 2 * k];
         const int sb_hi = L.slot_first[Wv.slot_base + 2 * k + 1];
@@ -1066,6 +1086,7 @@ __global__ __launch_bounds__(64 * kWavesPerWG) void score_kernel_pair(const Scor
             __syncthreads();  // every lane of the workgroup has left pass - 2
             load_pass(pass);
             __syncthreads();
+            set_task_priority((int64_t)(npass - pass + 1) * nbp, L.prio_steps);
         };
         run_task<R, G, F16 ? 3 : 0, 1, true, true, true, false>(P, gbase, sp, nullptr, slb_unused, slb_unused, wres, wbnd, wboff,
                                                          L.slot_entries,

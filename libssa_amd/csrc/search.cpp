@@ -663,22 +663,26 @@ void build_layout(Exec& ex, StreamLayout& L, int lanes, const PairSet& ps, const
     // lanes of a wave reset their state at a pair start in the same steps
     // (one run of G steps per group) instead of whenever any of the wave's
     // streams starts a pair; consecutive entries in length order differ by
-    // little, so the padding is small.  Groups go longest first to the least
-    // loaded wave (LPT).
+    // little, so the padding is small.  Groups go longest first to the wave
+    // least loaded for its capacity (LPT): the last fifth of the waves hold
+    // half and the last tenth a quarter of a full wave's share, so that the
+    // workgroup tasks that end a launch's queue (costliest first) are small
+    // and the launch's tail is short.
     const size_t ngroups = (n + gsz - 1) / gsz;
     std::vector<int64_t> wload(nw, 0);
     std::vector<std::vector<uint32_t>> wgroups(nw);
     {
-        typedef std::pair<int64_t, int32_t> Item;
+        auto quarters = [nw](int w) { return w >= nw - nw / 10 ? 1 : w >= nw - nw / 5 ? 2 : 4; };
+        typedef std::pair<double, int32_t> Item;  // (load per capacity, wave)
         std::priority_queue<Item, std::vector<Item>, std::greater<Item>> pq;
-        for (int w = 0; w < nw; ++w) pq.push(Item(0, w));
+        for (int w = 0; w < nw; ++w) pq.push(Item(0.0, w));
         for (size_t gi = 0; gi < ngroups; ++gi) {
             Item it = pq.top();
             pq.pop();
-            wgroups[it.second].push_back((uint32_t)gi);
-            it.first += nbk[order[gi * gsz]];  // the group's first entry is its longest
-            wload[it.second] = it.first;
-            pq.push(it);
+            const int w = it.second;
+            wgroups[w].push_back((uint32_t)gi);
+            wload[w] += nbk[order[gi * gsz]];  // the group's first entry is its longest
+            pq.push(Item((double)wload[w] / quarters(w), w));
         }
     }
     // waves longest first, so that the four waves of a workgroup task are alike
@@ -807,6 +811,7 @@ struct TierRun {
     bool paired = false;  // packed queries two by two, one per half of every lane group (score_kernel_pair)
     bool untracked = false;  // NW 16-bit where no pair can overflow: the kernel without trackers
     bool vp = false;         // the query profile in VGPRs
+    int32_t prio_steps = 0;  // the score kernels' wave priority unit (steps)
     bool f16 = false;        // SW first tier in windowed packed fp16 (kernel mode 3), or the
                              // untracked NW kernel in packed fp16 (mode 4)
     int mode() const { return type == kSW ? (f16 ? 3 : 0) : (untracked ? (f16 ? 4 : 2) : 1); }
@@ -1147,6 +1152,12 @@ void tier_prepare(Exec& ex, TierRun& R) {
         R.tasks.swap(sorted);
     }
     R.grid = std::max(1, std::min<int>((int)R.tasks.size(), cap > 0 ? cap : (int)R.tasks.size()));
+    // the wave priority unit: half the median task's steps
+    if (!cost.empty()) {
+        std::vector<int64_t> c(cost);
+        std::nth_element(c.begin(), c.begin() + c.size() / 2, c.end());
+        R.prio_steps = (int32_t)std::max<int64_t>(1, c[c.size() / 2] / (sh.cols / 4) / 2);
+    }
     int32_t max_nb = 0;
     for (int w = 0; w < nw; ++w) max_nb = std::max(max_nb, L->wave_nblocks[w]);
     // the kernel's pass length: the longest wave of a task, at least G + 2
@@ -1202,7 +1213,7 @@ void tier_prepare(Exec& ex, TierRun& R) {
     K.waves = L->d_waves;
     K.tasks = d_tasks;
     K.ntasks = (int32_t)R.tasks.size();
-    K.pad = 0;
+    K.prio_steps = R.prio_steps;
     K.counter = d_counter;
     K.queries = d_qd;
     K.qp = d_prof;
