@@ -307,11 +307,26 @@ int32_t f16_bits(int64_t v) {
     if (v == 0) return 0;
     const uint32_t sign = v < 0 ? 0x8000u : 0u;
     const uint64_t a = (uint64_t)(v < 0 ? -v : v);
-    int e = 0;
-    while ((a >> (e + 1)) != 0) ++e;  // a = 1.m * 2^e
+    const int e = 63 - __builtin_clzll(a);  // a = 1.m * 2^e
     if (e > 15 || (e > 10 && (a & ((1ull << (e - 10)) - 1)) != 0)) fatal("libssa_amd: %lld is not an fp16 integer", (long long)v);
     const uint32_t mant = (uint32_t)(e <= 10 ? (a << (10 - e)) : (a >> (e - 10))) & 0x3ffu;
     return (int32_t)(sign | ((uint32_t)(e + 15) << 10) | mant);
+}
+
+// f16_bits of a 16-bit value through a process-wide table (each value
+// converted on first use): a profile holds few distinct values but up to
+// millions of entries.
+int16_t f16_of_int16(int16_t v) {
+    static std::vector<int32_t> table(65536, -1);
+    static std::mutex mu;
+    const uint16_t k = (uint16_t)v;
+    int32_t b = __atomic_load_n(&table[k], __ATOMIC_ACQUIRE);
+    if (b < 0) {
+        std::lock_guard<std::mutex> lk(mu);
+        b = f16_bits(v);
+        __atomic_store_n(&table[k], b, __ATOMIC_RELEASE);
+    }
+    return (int16_t)b;
 }
 
 int64_t now_us() {
@@ -875,14 +890,19 @@ void tier_prepare(Exec& ex, TierRun& R) {
         R.qd[qi].qlen = (int32_t)q.len;
         R.qd[qi].qp_off = (int32_t)R.prof.size();
         R.qd[qi].lb_off = (int32_t)R.lbv.size();
+        // the tier's value of every (DB code, query code), then the rows
+        int16_t val[kDim][32];
+        for (int d = 0; d < kDim; ++d)
+            for (int c = 0; c < 32; ++c) val[d][c] = (int16_t)T.repr(T.wrap(M.at(d, c)));
+        const int16_t pad = R.type == kSW ? (int16_t)-32768 : (int16_t)0;
+        const size_t base = R.prof.size();
+        R.prof.resize(base + rows * kDim);
+        int16_t* out = R.prof.data() + base;
         for (int p = 0; p < npass[qi]; ++p)
             for (int d = 0; d < kDim; ++d)
                 for (int r = 0; r < PR; ++r) {
                     const size_t i = (size_t)p * PR + r;
-                    int32_t v;
-                    if (i < q.len) v = T.repr(T.wrap(M.at(d, q.seq[i])));
-                    else v = R.type == kSW ? -32768 : 0;
-                    R.prof.push_back((int16_t)v);
+                    *out++ = i < q.len ? val[d][q.seq[i] & 31] : pad;
                 }
         int64_t c = QRw;
         for (size_t i = 0; i < rows; ++i) {
@@ -1205,7 +1225,7 @@ void tier_prepare(Exec& ex, TierRun& R) {
     // the fp16 kernels' profile: the 16-bit tier's scores as fp16 values (the
     // padding rows' -32768 (SW) and 0 (NW) as well)
     if (R.f16)
-        for (int16_t& x : R.prof) x = (int16_t)f16_bits(x);
+        for (int16_t& x : R.prof) x = f16_of_int16(x);
     ex.upload(d_prof, R.prof.data(), R.prof.size() * sizeof(int16_t));
     ex.upload(d_lb, R.lbv.data(), R.lbv.size() * sizeof(uint32_t));
     ex.upload(d_segs, R.segs.data(), R.segs.size() * sizeof(SegLane));
