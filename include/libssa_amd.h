@@ -52,6 +52,8 @@ typedef struct {
     int cell_arithmetic;      /* first tier's cell arithmetic: 1 = packed saturating int16,
                                  2 = packed fp16 windows (SW; exact, see
                                  ssa_amd_set_cell_arithmetic) */
+    double ms_setup;          /* host work from the call to the first tier's launch (query
+                                 buffers, profiles, uploads; for a batch, the whole batch's) */
 } ssa_amd_stats_t;
 
 /* Counters of the last search. */

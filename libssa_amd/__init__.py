@@ -101,7 +101,7 @@ class Stats(_C.Structure):
                 ("ms_traceback", _C.c_double), ("ms_total", _C.c_double), ("d2h_bytes", _C.c_uint64),
                 ("replayed_chunks", _C.c_uint64), ("ms_graph", _C.c_double), ("graph_reused", _C.c_int),
                 ("columns_per_step", _C.c_int), ("profile_placement", _C.c_int),
-                ("cell_arithmetic", _C.c_int)]
+                ("cell_arithmetic", _C.c_int), ("ms_setup", _C.c_double)]
 
 
 class TaskRecord(_C.Structure):

@@ -2394,6 +2394,7 @@ void run_batch(Exec& ex, const std::vector<SearchRequest>& reqs, std::vector<Sea
             J.st.d2h_bytes += (b - a) * sizeof(PairResult);
         }
     };
+    const double ms_setup = (now_us() - t_start) / 1000.0;
     double ms_first = 0, ms_graph = 0;
     bool graph_reused = false;
     if (!groups.empty()) {
@@ -2487,6 +2488,7 @@ void run_batch(Exec& ex, const std::vector<SearchRequest>& reqs, std::vector<Sea
         const SearchType type = J.rq.type;
         const size_t nq = J.qs.size();
         J.st.ms_gpu_first = J.group >= 0 ? ms_first : 0.0;
+        J.st.ms_setup = ms_setup;
         J.st.ms_graph = ms_graph;
         J.st.graph_reused = graph_reused ? 1 : 0;
         uint64_t& computed = computed_job[j];
