@@ -597,6 +597,20 @@ def main():
     note("c2 headline: SW 8-bit cascade, 200k DB, q = 50/500/5000")
     out = bench_search(args, workload("c2"), args.steps, args.warmup, cpu=True)
     keys = ("value", "unit", "ms_per_step", "config", "roofline", "breakdown_ms_per_step")
+    # the same workload with the cell arithmetic forced to packed int16: the
+    # default fp16 datapath holds only integers of at most 2,047 in magnitude
+    # (SW: relative to per-lane offsets), which fp16 represents and adds
+    # exactly, so both give the same bit-exact results
+    note("c2 in packed int16")
+    i16_args = argparse.Namespace(**vars(args))
+    i16_args.cell_arithmetic = 1
+    i16 = bench_search(i16_args, workload("c2"), min(args.steps, 10), 2)
+    out["int16_arithmetic"] = dict(summary(i16, ("value", "unit", "ms_per_step", "dtype")),
+                                   kernel_ms_per_step=i16["roofline"]["kernel_ms_per_step"],
+                                   frac=i16["roofline"]["frac"],
+                                   note="the headline workload with ssa_amd_set_cell_arithmetic(1): every cell in "
+                                        "packed saturating int16; the default computes the same integers in fp16, "
+                                        "exactly (bit-identical results)")
     # the metric's other parts, each on its own workload at full size
     note("c3: NW DNA both strands with CIGARs, 1M DB")
     out["nw_dna"] = summary(bench_search(args, workload("c3"), 10, 2), keys)
