@@ -153,9 +153,6 @@ hipError_t launch_init_results(hipStream_t st, PairResult* r, int64_t n, int32_t
 hipError_t launch_exact(bool wide, int npairs, hipStream_t st, const ExactPair* pairs, const QueryDesc* queries,
                         const uint8_t* qcodes, const int64_t* mat, const uint8_t* codes, int mode, int go, int ge,
                         void* bnd, int64_t* out);
-hipError_t launch_traceback_wave(int ntasks, hipStream_t st, const TraceTask* tasks, const uint8_t* codes,
-                                 const int64_t* mat, int go, int ge, uint8_t* dirs, int64_t* scratch, int32_t* ops,
-                                 TraceResult* out);
 // workgroup traceback: a hit's passes of 64 * rows_per_lane query rows (1, 2,
 // 4 or 8) spread over `waves` waves (1-16); at most traceback_wg_max_passes()
 // passes per dynamic programme

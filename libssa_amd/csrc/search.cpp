@@ -1559,7 +1559,7 @@ void gpu_traceback(Exec& gpu, const std::vector<SearchType>& types, std::vector<
     std::vector<TraceTask> tasks(al.size());
     std::vector<int64_t> dir_bytes(al.size());
     int64_t hh_total = 0, ops_total = 0;
-    bool wave_ok = true;
+    bool int32_ok = true;
     int64_t smax = 0;
     for (int i = 0; i < kDim * kDim; ++i) smax = std::max<int64_t>(smax, std::abs(matrix().s[i]));
     for (size_t k = 0; k < al.size(); ++k) {
@@ -1575,13 +1575,13 @@ void gpu_traceback(Exec& gpu, const std::vector<SearchType>& types, std::vector<
         dir_bytes[k] = ((((int64_t)t.qlen + 7) & ~(int64_t)7) * t.dlen + 8 + 7) & ~(int64_t)7;
         t.hh_off = hh_total;
         // int64 scratch of the one-thread kernel, or the int32 carries and
-        // last-row copies of the wave kernel, whichever is larger
+        // last-row copies of the workgroup kernel, whichever is larger
         hh_total += std::max<int64_t>(2 * (int64_t)std::max(t.qlen, t.dlen) + 2,
                                       (3 * (int64_t)t.dlen + 4 * (int64_t)t.qlen) / 2 + 2);
-        // the wave kernel computes in int32: every value of the hit must fit
+        // the workgroup kernel computes in int32: every value of the hit must fit
         const int64_t bound = ((int64_t)t.qlen + t.dlen + 4) * (std::abs((int64_t)cf.gap_open) +
                                                                  std::abs((int64_t)cf.gap_extend) + smax);
-        if (bound >= (int64_t)1 << 30) wave_ok = false;
+        if (bound >= (int64_t)1 << 30) int32_ok = false;
         t.ops_off = ops_total;
         t.ops_cap = t.qlen + t.dlen + 1;
         ops_total += t.ops_cap;
@@ -1616,7 +1616,7 @@ void gpu_traceback(Exec& gpu, const std::vector<SearchType>& types, std::vector<
     // launches in stream order share the direction arena; within one, a
     // workgroup launch per run of hits of one shape
     for (const auto& lr : launches) {
-        if (!wave_ok) {
+        if (!int32_ok) {
             SSA_HIP(launch_traceback((int)(lr.second - lr.first), gpu.stream(), d_tasks + lr.first, d_codes, d_mat,
                                      cf.gap_open, cf.gap_extend, d_dirs, d_hh, d_ops, d_res + lr.first));
             continue;

@@ -1,8 +1,9 @@
 // traceback_kernels.hip -- recomputes each hit and emits its CIGAR runs.
 //
-// traceback_wave_kernel: one wave per hit, int32 (the host checks that every
-// value of the hit fits).  traceback_kernel: one thread per hit in int64, for
-// hits whose values might not fit int32.
+// traceback_wg_kernel: one workgroup per hit, the query's passes spread over
+// its waves, int32 (the host checks that every value of the hit fits).
+// traceback_kernel: one thread per hit in int64, for hits whose values might
+// not fit int32.
 //
 // Both restate the reference's region
 // search (src/algo/align.c:39-150), direction matrix (src/algo/cigar.c:48-198)
@@ -210,19 +211,8 @@ __global__ __launch_bounds__(64) void traceback_kernel(const TraceTask* __restri
 }
 
 // ---------------------------------------------------------------------------
-// Wave-parallel traceback.  The three dynamic programmes of a hit (the local
-// forward and reverse region searches, the direction matrix) each run as a
-// systolic pipeline inside one wave: lane g owns kTbR consecutive query rows
-// of a pass of 64 * kTbR rows and walks the DB columns one column behind lane
-// g-1, which hands it the carries of its last row (DPP wave shift); the
-// bottom lane stores its carries for the next pass.  Each programme keeps the
-// reference's cell order for the quantities that depend on it: the first
-// strict maximum in (DB column, query row) order for the end cell, the first
-// cell reaching the score in reverse order for the start cell.
+// Helpers of the workgroup traceback below.
 // ---------------------------------------------------------------------------
-constexpr int kTbR = 8;
-constexpr int kTbRows = 64 * kTbR;
-
 __device__ __forceinline__ int32_t tb_shr(int32_t v) {
     return __builtin_amdgcn_update_dpp(0, v, 0x138, 0xf, 0xf, false);  // wave_shr:1
 }
@@ -235,338 +225,17 @@ __device__ __forceinline__ bool tb_better(int32_t v, int32_t c, int32_t r, int32
     return r < br;
 }
 
-// Region search pass (src/algo/align.c:55-83 forward, :85-129 reverse), in
-// the rows x columns of the pass's own orientation: rows run along the query
-// (index j of the reference), columns along the DB sequence (index i).
-// FWD: rows 0..Q-1 = query 0..alen-1, columns = DB 0..blen-1, zero floor,
-//      finds the first strict maximum (value, column, row).
-// !FWD: rows = query a_end..0, columns = DB b_end..0, no floor, finds the
-//      first cell in (column, row) order whose value reaches `target`.
-template <bool FWD>
-__device__ void tb_region(const int32_t* smat, const uint8_t* qc, const uint8_t* dc, int Q, int C, int32_t go,
-                          int32_t ge, int32_t target, int32_t* bnd, int32_t* stale_h, int32_t* stale_e,
-                          int stale_from, int32_t* res_v, int32_t* res_c, int32_t* res_r) {
-    const int lane = threadIdx.x;
-    int32_t bv = 0, bcol = INT32_MAX, brow = INT32_MAX;
-    bool found = false;
-    const int npass = (Q + kTbRows - 1) / kTbRows;
-    for (int p = 0; p < npass; ++p) {
-        const int r0 = p * kTbRows + lane * kTbR;
-        const int nrow = max(0, min(kTbR, Q - r0));
-        int32_t H[kTbR], E[kTbR];
-        int qcode[kTbR];
-#pragma unroll
-        for (int k = 0; k < kTbR; ++k) {
-            const int r = r0 + k;
-            qcode[k] = k < nrow ? (int)qc[FWD ? r : -r] : 0;
-            if (FWD) {
-                H[k] = 0;
-                E[k] = go;
-            } else {
-                // the reverse pass resets only query indices up to b_end; the
-                // others keep the forward pass's last values (stale_* indexed by row)
-                const bool fresh = k >= nrow || r >= stale_from;
-                H[k] = fresh ? -1 : stale_h[r];
-                E[k] = fresh ? -1 : stale_e[r];
-            }
-        }
-        int32_t oh = 0, of = 0, op = 0;  // carries out of this lane's last row at its last column
-        const int nsteps = C + 63;
-        for (int t = 0; t < nsteps; ++t) {
-            const int c = t - lane;
-            int32_t ih = tb_shr(oh), ifr = tb_shr(of), ip = tb_shr(op);
-            const bool valid = c >= 0 && c < C;
-            if (lane == 0 && valid) {
-                if (p == 0) {
-                    ih = FWD ? 0 : -1;
-                    ifr = FWD ? go : -1;
-                    ip = FWD ? 0 : (c == 0 ? 0 : -1);
-                } else {
-                    ih = bnd[3 * c];
-                    ifr = bnd[3 * c + 1];
-                    ip = bnd[3 * c + 2];
-                }
-            }
-            if (valid) {
-                const int dcode = dc[FWD ? c : -c];
-                int32_t h = ih, f = ifr, pp = ip;
-#pragma unroll
-                for (int k = 0; k < kTbR; ++k) {
-                    if (k < nrow) {
-                        f = mx(f, h + go) + ge;
-                        E[k] = mx(E[k], H[k] + go) + ge;
-                        int32_t hn = pp + smat[qcode[k] * 32 + dcode];
-                        if (FWD) hn = mx(hn, 0);
-                        hn = mx(hn, f);
-                        hn = mx(hn, E[k]);
-                        pp = H[k];
-                        H[k] = hn;
-                        h = hn;
-                        // the first cell in (column, row) order: a lane meets
-                        // the columns in order within a pass, but its later
-                        // passes revisit earlier columns (on later rows), so
-                        // an equal cell of a later pass wins when its column
-                        // comes first
-                        if (FWD) {
-                            if (hn > bv || (hn == bv && hn > 0 && c < bcol)) {
-                                bv = hn;
-                                bcol = c;
-                                brow = r0 + k;
-                            }
-                        } else if (hn >= target && (!found || c < bcol)) {
-                            found = true;
-                            bcol = c;
-                            brow = r0 + k;
-                        }
-                    }
-                }
-                oh = h;
-                of = f;
-                op = pp;
-                if (lane == 63 && p + 1 < npass) {
-                    bnd[3 * c] = h;
-                    bnd[3 * c + 1] = f;
-                    bnd[3 * c + 2] = pp;
-                }
-            }
-        }
-        if (FWD) {
-#pragma unroll
-            for (int k = 0; k < kTbR; ++k)
-                if (k < nrow) {
-                    stale_h[r0 + k] = H[k];
-                    stale_e[r0 + k] = E[k];
-                }
-        }
-        __threadfence_block();
-        __builtin_amdgcn_wave_barrier();
-    }
-    // wave reduction: FWD the best (value desc, column, row); !FWD the first (column, row)
-    if (!FWD && !found) {
-        bv = INT32_MIN;
-        bcol = INT32_MAX;
-        brow = INT32_MAX;
-    } else if (!FWD) {
-        bv = 1;
-    }
-    for (int off = 32; off > 0; off >>= 1) {
-        const int32_t v2 = __shfl_xor(bv, off), c2 = __shfl_xor(bcol, off), r2 = __shfl_xor(brow, off);
-        if (tb_better(v2, c2, r2, bv, bcol, brow)) {
-            bv = v2;
-            bcol = c2;
-            brow = r2;
-        }
-    }
-    *res_v = bv;
-    *res_c = bcol;
-    *res_r = brow;
-}
-
-// Direction matrix (src/algo/cigar.c:48-198) over rows 0..Q-1 (query) and
-// columns 0..C-1 (DB): D[rows_pad * column + row].
-__device__ void tb_directions(const int32_t* smat, const uint8_t* qc, const uint8_t* dc, int Q, int C, int rows_pad,
-                              int32_t go, int32_t ge, bool nw, int32_t* bnd, uint8_t* D) {
-    const int lane = threadIdx.x;
-    const int npass = (Q + kTbRows - 1) / kTbRows;
-    for (int p = 0; p < npass; ++p) {
-        const int r0 = p * kTbRows + lane * kTbR;
-        const int nrow = max(0, min(kTbR, Q - r0));
-        int32_t hh[kTbR], ee[kTbR];
-        int qcode[kTbR];
-#pragma unroll
-        for (int k = 0; k < kTbR; ++k) {
-            const int i = r0 + k;
-            qcode[k] = k < nrow ? (int)qc[i] : 0;
-            hh[k] = nw ? go + (i + 1) * ge : 0;
-            ee[k] = nw ? 2 * go + (i + 2) * ge : 0;
-        }
-        int32_t oh = 0, of = 0;
-        const int nsteps = C + 63;
-        for (int t = 0; t < nsteps; ++t) {
-            const int j = t - lane;
-            int32_t ih = tb_shr(oh), ifr = tb_shr(of);
-            const bool valid = j >= 0 && j < C;
-            if (lane == 0 && valid) {
-                if (p == 0) {
-                    ifr = nw ? 2 * go + (j + 2) * ge : 0;
-                    ih = nw ? (j == 0 ? 0 : go + j * ge) : 0;
-                } else {
-                    ih = bnd[2 * j];
-                    ifr = bnd[2 * j + 1];
-                }
-            }
-            if (valid) {
-                const int dcode = dc[j];
-                int32_t h = ih, f = ifr;
-                uint32_t lo = 0, hi = 0;
-#pragma unroll
-                for (int k = 0; k < kTbR; ++k) {
-                    if (k < nrow) {
-                        uint32_t d = 0;
-                        const int32_t nn = hh[k];
-                        int32_t e = ee[k];
-                        h += smat[dcode * 32 + qcode[k]];
-                        if (f > h) {
-                            d |= kGapUp;
-                            h = f;
-                        }
-                        if (e > h) {
-                            h = e;
-                            d |= kGapLeft;
-                        }
-                        if (!nw && h < 0) h = 0;
-                        hh[k] = h;
-                        h += go + ge;
-                        e += ge;
-                        f += ge;
-                        if (f > h) d |= kExtUp;
-                        else f = h;
-                        if (e > h) d |= kExtLeft;
-                        else e = h;
-                        ee[k] = e;
-                        h = nn;
-                        if (k < 4) lo |= d << (8 * k);
-                        else hi |= d << (8 * (k - 4));
-                    }
-                }
-                of = f;
-                oh = h;
-                if (nrow > 0) *reinterpret_cast<uint2*>(D + (int64_t)rows_pad * j + r0) = make_uint2(lo, hi);
-                if (lane == 63 && p + 1 < npass) {
-                    bnd[2 * j] = h;
-                    bnd[2 * j + 1] = f;
-                }
-            }
-        }
-        __threadfence_block();
-        __builtin_amdgcn_wave_barrier();
-    }
-}
-
-__global__ __launch_bounds__(64) void traceback_wave_kernel(const TraceTask* __restrict__ tasks,
-                                                            const uint8_t* __restrict__ codes,
-                                                            const int64_t* __restrict__ mat, int go, int ge,
-                                                            uint8_t* __restrict__ dirs, int64_t* __restrict__ scratch,
-                                                            int32_t* __restrict__ ops,
-                                                            TraceResult* __restrict__ out) {
-    __shared__ int32_t smat[32 * 32];
-    const int lane = threadIdx.x;
-    const TraceTask T = tasks[blockIdx.x];
-    for (int i = lane; i < 32 * 32; i += 64) smat[i] = (int32_t)mat[i];
-    __syncthreads();
-    const uint8_t* a = codes + T.q_off;  // query
-    const uint8_t* b = codes + T.d_off;  // DB sequence
-    const int alen = T.qlen, blen = T.dlen;
-    int32_t* sc = reinterpret_cast<int32_t*>(scratch + T.hh_off);
-    int32_t* bnd = sc;                   // 3 * blen carries between passes
-    int32_t* stale_h = sc + 3 * (int64_t)blen;
-    int32_t* stale_e = stale_h + alen;
-    TraceResult R;
-    R.nops = 0;
-    R.status = 0;
-    if (T.type == 0) {
-        int32_t v, c, r;
-        tb_region<true>(smat, a, b, alen, blen, go, ge, 0, bnd, stale_h, stale_e, 0, &v, &c, &r);
-        if (v <= 0) {
-            if (lane == 0) {
-                R.status = 1;
-                out[blockIdx.x] = R;
-            }
-            return;
-        }
-        R.a_end = r;
-        R.b_end = c;
-        const int32_t score = v;
-        // reverse pass over query a_end..0 and DB b_end..0: row k is query a_end - k; the
-        // reference resets query indices <= b_end, i.e. rows k >= a_end - b_end
-        int32_t v2, c2, r2;
-        const int stale_rows = (int)R.a_end - (int)R.b_end;  // rows k < stale_rows keep stale values
-        {
-            // the region routine indexes qc[-r], dc[-c] and stale_*[r]; give it the
-            // reversed stale arrays through a copy into the (now unused) carry area's tail
-            int32_t* rs_h = stale_e + alen;  // alen int32 each, inside the task's scratch
-            int32_t* rs_e = rs_h + alen;
-            for (int k = lane; k < (int)R.a_end + 1; k += 64) {
-                rs_h[k] = stale_h[R.a_end - k];
-                rs_e[k] = stale_e[R.a_end - k];
-            }
-            __threadfence_block();
-            __builtin_amdgcn_wave_barrier();
-            tb_region<false>(smat, a + R.a_end, b + R.b_end, (int)R.a_end + 1, (int)R.b_end + 1, go, ge, score, bnd,
-                             rs_h, rs_e, stale_rows > 0 ? stale_rows : 0, &v2, &c2, &r2);
-        }
-        if (v2 == INT32_MIN) {
-            if (lane == 0) {
-                R.status = 1;
-                out[blockIdx.x] = R;
-            }
-            return;
-        }
-        R.a_begin = R.a_end - r2;
-        R.b_begin = R.b_end - c2;
-    } else {
-        R.a_begin = 0;
-        R.a_end = alen - 1;
-        R.b_begin = 0;
-        R.b_end = blen - 1;
-    }
-
-    const int rows = (int)R.a_end + 1, cols = (int)R.b_end + 1;
-    const int rows_pad = (rows + 7) & ~7;
-    uint8_t* D = dirs + T.dir_off;
-    tb_directions(smat, a, b, rows, cols, rows_pad, go, ge, T.type == 1, bnd, D);
-
-    if (lane != 0) return;
-    // back-trace: runs are written from the end of the alignment to its start
-    int32_t* O = ops + T.ops_off * 2;
-    int64_t i = R.a_end, j = R.b_end;
-    char prev = 0;
-    int32_t cnt = 0, nops = 0;
-    while (i >= 0 && j >= 0 && i >= R.a_begin && j >= R.b_begin) {
-        const uint8_t d = D[(int64_t)rows_pad * j + i];
-        char op;
-        if ((d & kGapLeft) || (d & kExtLeft)) {
-            --j;
-            op = 'I';
-        } else if ((d & kGapUp) || (d & kExtUp)) {
-            --i;
-            op = 'D';
-        } else {
-            --i;
-            --j;
-            op = 'M';
-        }
-        if (op == prev) {
-            ++cnt;
-        } else {
-            if (cnt > 0) {
-                if (nops >= T.ops_cap) {
-                    R.status = 2;
-                    break;
-                }
-                O[2 * nops] = prev;
-                O[2 * nops + 1] = cnt;
-                ++nops;
-            }
-            prev = op;
-            cnt = 1;
-        }
-    }
-    if (R.status == 0 && cnt > 0) {
-        if (nops >= T.ops_cap) R.status = 2;
-        else {
-            O[2 * nops] = prev;
-            O[2 * nops + 1] = cnt;
-            ++nops;
-        }
-    }
-    R.nops = nops;
-    out[blockIdx.x] = R;
-}
-
 // ---------------------------------------------------------------------------
-// Workgroup traceback: the same three dynamic programmes as the wave kernel,
-// with the query's passes spread over the W waves of a workgroup.  A pass is
+// Workgroup traceback.  The three dynamic programmes of a hit (the local
+// forward and reverse region searches, the direction matrix) each run as a
+// systolic pipeline: lane g owns R consecutive query rows of a pass and walks
+// the DB columns one column behind lane g-1, which hands it the carries of its
+// last row (DPP wave shift); the bottom lane stores its carries for the next
+// pass.  Each programme keeps the reference's cell order for the quantities
+// that depend on it: the first strict maximum in (DB column, query row) order
+// for the end cell, the first cell reaching the score in reverse order for the
+// start cell.  The query's passes are spread over the W waves of the
+// workgroup.  A pass is
 // 64 * R query rows (lane g owns R consecutive rows) and walks the columns in
 // blocks of kTgS; pass p runs on wave p mod W, starting kTgL blocks after
 // pass p - 1 (so that the bottom lane of pass p - 1 has written a column's
@@ -574,8 +243,7 @@ __global__ __launch_bounds__(64) void traceback_wave_kernel(const TraceTask* __r
 // 63-step skew between them) and not before its wave has finished pass p - W.
 // All waves step through the same phases, one column block each, separated by
 // workgroup barriers; the carries between passes go through the hit's scratch
-// in global memory.  Every cell is computed exactly as in the wave kernel, so
-// the results (end cell, start cell, direction bits) are the same.
+// in global memory.
 // ---------------------------------------------------------------------------
 constexpr int kTgS = 16;        // columns per phase
 constexpr int kTgL = 5;         // phases between the starts of consecutive passes
@@ -624,7 +292,13 @@ __device__ void tg_reduce(int32_t& bv, int32_t& bcol, int32_t& brow, int32_t* s_
     __syncthreads();  // s_red is free again
 }
 
-// Region search (as tb_region) over Q rows and C columns.
+// Region search (src/algo/align.c:55-83 forward, :85-129 reverse) over Q rows
+// and C columns, in the pass's own orientation: rows run along the query
+// (index j of the reference), columns along the DB sequence (index i).
+// FWD: rows 0..Q-1 = query 0..alen-1, columns = DB 0..blen-1, zero floor,
+//      finds the first strict maximum (value, column, row).
+// !FWD: rows = query a_end..0, columns = DB b_end..0, no floor, finds the
+//      first cell in (column, row) order whose value reaches `target`.
 template <int R, bool FWD>
 __device__ void tg_region(const int32_t* smat, const uint8_t* qc, const uint8_t* dc, int Q, int C, int32_t go,
                           int32_t ge, int32_t target, int32_t* bnd, int32_t* stale_h, int32_t* stale_e,
@@ -744,7 +418,8 @@ __device__ void tg_region(const int32_t* smat, const uint8_t* qc, const uint8_t*
     *res_r = brow;
 }
 
-// Direction matrix (as tb_directions) over Q rows and C columns.
+// Direction matrix (src/algo/cigar.c:48-198) over rows 0..Q-1 (query) and
+// columns 0..C-1 (DB): D[rows_pad * column + row].
 template <int R>
 __device__ void tg_directions(const int32_t* smat, const uint8_t* qc, const uint8_t* dc, int Q, int C, int rows_pad,
                               int32_t go, int32_t ge, bool nw, int32_t* bnd, uint8_t* D, int* s_start) {
@@ -1001,15 +676,6 @@ hipError_t launch_traceback_wg(int rows_per_lane, int waves, int ntasks, hipStre
         case 8: hipLaunchKernelGGL(traceback_wg_kernel<8>, grid, block, 0, st, tasks, codes, mat, go, ge, dirs, scratch, ops, out); break;
         default: return hipErrorInvalidValue;
     }
-    return hipGetLastError();
-}
-
-hipError_t launch_traceback_wave(int ntasks, hipStream_t st, const TraceTask* tasks, const uint8_t* codes,
-                                 const int64_t* mat, int go, int ge, uint8_t* dirs, int64_t* scratch, int32_t* ops,
-                                 TraceResult* out) {
-    if (ntasks <= 0) return hipSuccess;
-    hipLaunchKernelGGL(traceback_wave_kernel, dim3(ntasks), dim3(64), 0, st, tasks, codes, mat, go, ge, dirs, scratch,
-                       ops, out);
     return hipGetLastError();
 }
 
