@@ -611,6 +611,10 @@ __device__ __forceinline__ void run_task(const ScoreParams& P, uint32_t gbase0, 
                                 read_part<R>(sp + colofs(wc.x, cn), pl, r / 8);
                                 read_part<R>(sp + colofs(wc.y, cn), ph, r / 8);
                             }
+                            // keep the reads here: left free, the scheduler
+                            // sinks all of a column's reads to its end, and
+                            // the next column's first rows wait for them
+                            __builtin_amdgcn_sched_barrier(0);
                         }
                         if constexpr (F16) {
                             // E carries the floor at real 0 (E >= 0 makes H >=
