@@ -321,7 +321,8 @@ __device__ __forceinline__ void run_task(const ScoreParams& P, uint32_t gbase0, 
     // byte offset of the lane's rows for column c's DB code in the staged profile
     // (the code field lies above the lane's rows and below the buffer bit of
     // gbase, so the two combine with an or)
-    auto colofs = [&gbase](uint32_t word, int c) { return (((word >> (8 * c)) & 0x3eu) << LPR) | gbase; };
+    auto colofs_at = [](uint32_t word, int c, uint32_t base) { return (((word >> (8 * c)) & 0x3eu) << LPR) | base; };
+    auto colofs = [&gbase, &colofs_at](uint32_t word, int c) { return colofs_at(word, c, gbase); };
 
     s16x2 Hl[R], El[R];
 #pragma unroll
@@ -397,19 +398,28 @@ __device__ __forceinline__ void run_task(const ScoreParams& P, uint32_t gbase0, 
         TT = (tlo > 4096 ? 0x7c00u : (v & 0xffffu)) | (thi > 4096 ? 0x7c000000u : (v & 0xffff0000u));
     };
 
+    // profile dwords of the current column (lo, hi halves), read while the
+    // column before is scored; a step's first column at the step's start, or
+    // (EARLY, 32-row lanes, whose steps are long enough to cover the global
+    // load of the next step's residues) during the last column of the step
+    // before, the first step's below
+    // (PAIR: pl holds one dword per row, both halves' scores)
+    constexpr bool EARLY = R >= 32;
+    uint32_t pl[PAIR ? R : (VP ? 1 : R / 2)], ph[PAIR || VP ? 1 : R / 2];
+    auto read_first = [&](uint2 w0) {
+        if constexpr (PAIR) {
+#pragma unroll
+            for (int q = 0; q < (R >= 8 ? R / 8 : 1); ++q) read_part_pair<R>(sp + colofs(w0.x, 0), pl, q);
+        } else if constexpr (!VP) {
+            read_rows<R>(sp + colofs(w0.x, 0), pl);
+            read_rows<R>(sp + colofs(w0.y, 0), ph);
+        }
+    };
+
     // FIRST: the top lane is in the first pass and takes the matrix border
     auto step = [&](auto first_tag, const StepIn<NB>& in, StepIn<NB>& nx) {
         constexpr bool FIRST = decltype(first_tag)::value;
-        // profile dwords of the current column (lo, hi halves); column 0's now
-        // (PAIR: pl holds one dword per row, both halves' scores)
-        uint32_t pl[PAIR ? R : (VP ? 1 : R / 2)], ph[PAIR || VP ? 1 : R / 2];
-        if constexpr (PAIR) {
-#pragma unroll
-            for (int q = 0; q < (R >= 8 ? R / 8 : 1); ++q) read_part_pair<R>(sp + colofs(in.w[0].x, 0), pl, q);
-        } else if constexpr (!VP) {
-            read_rows<R>(sp + colofs(in.w[0].x, 0), pl);
-            read_rows<R>(sp + colofs(in.w[0].y, 0), ph);
-        }
+        if constexpr (!EARLY) read_first(in.w[0]);
         const bool lane_last = pass == npass - 1;
         const bool bot_out = g == G - 1 && pass < npass - 1;  // bottom row to the next pass
         const bool score_lo = SW ? bot_lo : (lane_last && g == last_g);
@@ -417,6 +427,10 @@ __device__ __forceinline__ void run_task(const ScoreParams& P, uint32_t gbase0, 
         // the lane's next step, in this pass or (MULTI) at the start of the next
         const bool wrap = MULTI && b + NB == nbp;
         const int b_n = wrap ? 0 : b + NB, pass_n = wrap ? pass + 1 : pass;
+        // the profile buffer of the lane's next step (a lane entering its next
+        // pass reads that pass's buffer, staged before the pass's first lane
+        // entered it)
+        const uint32_t gbase_n = wrap ? gbase ^ kProfBytes : gbase;
 #pragma unroll
         for (int j = 0; j < NB; ++j) {
             if (F16) nx.toff[j] = 0u;
@@ -600,16 +614,20 @@ __device__ __forceinline__ void run_task(const ScoreParams& P, uint32_t gbase0, 
                                                                vp[2 * (r + 1 < R ? r + 1 : 0)], vsel[c]));
                             else v = pair_row(pl, ph, r + 1);
                         }
-                        // the next column's profile rows (the step's next block
-                        // after column 3), once this column's dwords are consumed
-                        if (!VP && (c < 3 || j + 1 < NB) && ((R == 4 && r == 3) || (R >= 8 && r % 8 == 7))) {
-                            const uint2 wc = c < 3 ? w : in.w[j + 1 < NB ? j + 1 : j];
+                        // the next column's profile rows (after column 3 the
+                        // step's next block, or the next step's first, whose
+                        // residues were loaded at this step's start), once
+                        // this column's dwords are consumed
+                        if (!VP && (c < 3 || j + 1 < NB || EARLY) && ((R == 4 && r == 3) || (R >= 8 && r % 8 == 7))) {
+                            const bool nstep = c == 3 && j + 1 == NB;
+                            const uint2 wc = c < 3 ? w : nstep ? nx.w[0] : in.w[j + 1 < NB ? j + 1 : j];
                             const int cn = c < 3 ? c + 1 : 0;
+                            const uint32_t base = nstep ? gbase_n : gbase;
                             if constexpr (PAIR) {
-                                read_part_pair<R>(sp + colofs(wc.x, cn), pl, r / 8);
+                                read_part_pair<R>(sp + colofs_at(wc.x, cn, base), pl, r / 8);
                             } else {
-                                read_part<R>(sp + colofs(wc.x, cn), pl, r / 8);
-                                read_part<R>(sp + colofs(wc.y, cn), ph, r / 8);
+                                read_part<R>(sp + colofs_at(wc.x, cn, base), pl, r / 8);
+                                read_part<R>(sp + colofs_at(wc.y, cn, base), ph, r / 8);
                             }
                             // keep the reads here: left free, the scheduler
                             // sinks all of a column's reads to its end, and
@@ -859,6 +877,7 @@ __device__ __forceinline__ void run_task(const ScoreParams& P, uint32_t gbase0, 
         sa.toff[j] = sb.toff[j] = 0u;
         if (g == 0 && j < nbw) sa.w[j] = wres[j * NG + k];
     }
+    if constexpr (EARLY) read_first(sa.w[0]);
     // steps with the top lane in the first pass (an even number: nbp is a
     // multiple of 2 * NB)
     const int spp = nbp / NB;  // steps per pass
