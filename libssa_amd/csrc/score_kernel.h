@@ -648,14 +648,16 @@ __device__ __forceinline__ void run_task(const ScoreParams& P, uint32_t gbase0, 
                             El[r] = c_max3<true>(eR, tt, as_v(flr));
                             if (more) fR = c_add<true>(f, RR);
                             Hl[r] = h;
-                        } else if constexpr (MODE == 4) {  // NW in fp16: no floor, no trackers
+                        } else if constexpr (MODE == 4) {
+                            // NW in fp16, no trackers, every value of cell
+                            // (i, j) held less (i + j) gap extend: E and F
+                            // extend without an add, QR is the gap open and
+                            // the profile holds the score - 2 gap extend
                             const s16x2 h = c_max3<true>(h1, f, El[r]);
-                            const s16x2 eR = c_add<true>(El[r], RR);
                             if (more) h1 = c_add<true>(Hl[r], v);  // next row's diagonal term
                             const s16x2 tt = c_add<true>(h, QR);
-                            f = c_max<true>(fR, tt);
-                            El[r] = c_max<true>(eR, tt);
-                            if (more) fR = c_add<true>(f, RR);
+                            f = c_max<true>(f, tt);
+                            El[r] = c_max<true>(El[r], tt);
                             Hl[r] = h;
                             if (CAP && r == last_r) hq = h;
                         } else {
@@ -831,6 +833,13 @@ __device__ __forceinline__ void run_task(const ScoreParams& P, uint32_t gbase0, 
                     const s16x2 fin = SW ? as_v(scout[j]) : cap;
                     sc_lo = MODE == 4 ? f16_int(as_u(fin) & 0xffffu) : lo16(fin);
                     sc_hi = MODE == 4 ? f16_int(as_u(fin) >> 16) : hi16(fin);
+                    if constexpr (MODE == 4) {
+                        // back from the shifted values: the score cell is
+                        // (qlen, column z + 1 of the pair's block blk)
+                        const int qlen = (npass - 1) * PR + last_g * R + last_r + 1;
+                        sc_lo += (qlen + 4 * blk_lo + z_lo + 1) * P.nw_shift;
+                        sc_hi += (qlen + 4 * blk_hi + z_hi + 1) * P.nw_shift;
+                    }
                 }
                 uint32_t m = 0u;
                 if (ls_lo && pair_lo >= 0) {

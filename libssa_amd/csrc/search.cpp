@@ -390,18 +390,24 @@ bool int16_exact(size_t qlen, int64_t dmax, const MatrixInfo& mi) {
     return lowest > -32768 + gap + 1 && up < 32767;
 }
 
-// True when every H, E and F value of an NW alignment of a query of qlen
-// residues with a DB entry of at most dmax residues lies within +-2047, so
-// that fp16 holds it exactly: a cell (i, j) is at least the path of two gaps
-// to it, 2 gap open + (i + j) gap extend, E and F one more gap open below
-// that (with the padding columns of a block and the borders within four
-// more extends), and at most max score * min(i, j).
+// True when every value the NW fp16 kernel (mode 4) holds for a query of
+// qlen residues and a DB entry of at most dmax residues (padding columns
+// included) lies within +-2047, so that fp16 holds it exactly.  That kernel
+// holds each H, E and F of cell (i, j) less (i + j) * gap extend (g <= 0):
+// the extension adds vanish from the recurrence (E and F take H + gap open,
+// the diagonal takes the score - 2 g, the borders are constants).  A shifted
+// H is at least the path of two gaps to its cell, 2 gap open + (i + j) g,
+// less the shift: 2 gap open; E, F and the gap candidates one gap open
+// below that, the diagonal candidate the lowest score below it; and at most
+// max score * min(i, j) - (i + j) g.
 bool nw_f16_exact(size_t qlen, int64_t dmax, const MatrixInfo& mi) {
     const Config& cf = config();
-    const int64_t go = std::abs((int64_t)cf.gap_open), ge = std::abs((int64_t)cf.gap_extend);
-    const int64_t lowest = -(3 * go + ((int64_t)qlen + dmax + 4) * ge) - std::max<int64_t>(-mi.minv, 0);
-    const int64_t up = std::max<int64_t>(mi.maxv, 0) * std::min<int64_t>((int64_t)qlen, dmax);
-    return mi.maxv <= 1024 && mi.minv >= -1024 && lowest >= -2047 && up <= 2047;
+    if (cf.gap_open > 0 || cf.gap_extend > 0) return false;  // the bounds below take penalties <= 0
+    const int64_t go = -(int64_t)cf.gap_open, ge = -(int64_t)cf.gap_extend;
+    const int64_t smax = std::max<int64_t>(mi.maxv, 0), sneg = std::max<int64_t>(-mi.minv, 0);
+    const int64_t lowest = -(3 * go) - sneg;
+    const int64_t up = smax * std::min<int64_t>((int64_t)qlen, dmax) + ((int64_t)qlen + dmax) * ge;
+    return mi.maxv <= 1024 && mi.minv >= -1024 && smax + 2 * ge <= 2047 && lowest >= -2047 && up <= 2047;
 }
 
 // result of one pair at one tier, decoded: d > 0 is a heap insert of score
@@ -942,6 +948,7 @@ void tier_prepare(Exec& ex, TierRun& R) {
     P.hb_init[0] = 0;
     for (int k = 1; k < 4; ++k) P.hb_init[k] = pack2(T.repr(T.wrap(go + k * ge)));
     for (int k = 0; k < 4; ++k) P.fb_init[k] = pack2(T.repr(T.wrap(go + (k + 1) * ge)));
+    P.nw_shift = 0;
 
     // ---- streams: cached for the whole DB, built per call for subsets ----
     if (R.sorted_all) {
@@ -985,16 +992,34 @@ void tier_prepare(Exec& ex, TierRun& R) {
             for (size_t qi = 0; qi < nq && R.f16; ++qi)
                 R.f16 = nw_f16_exact(R.qs[qi].len, 4 * (int64_t)L->max_entry_blocks, mi);
         }
-        if (R.type == kNW && R.f16) {  // the NW parameters and borders as fp16 values
-            auto cv = [](uint32_t w) { return pack2(f16_bits((int16_t)(w & 0xffffu))); };
-            P.qr = cv(P.qr);
-            P.r = cv(P.r);
+        if (R.type == kNW && R.f16) {
+            // every value of cell (i, j) less (i + j) ge (nw_f16_exact): the
+            // extension adds drop out (E and F take H + gap open, P.r = 0),
+            // the diagonal takes the score - 2 ge, and the borders are
+            // constants: H(0, j) = H(i, 0) = gap open, E(i, 1) = F(1, j) =
+            // 2 gap open, the corner 0
+            P.qr = pack2(f16_bits(go));
+            P.r = 0;
             P.zero = 0;
-            for (int c = 0; c < 4; ++c) {
-                P.hb_init[c] = cv(P.hb_init[c]);
-                P.fb_init[c] = cv(P.fb_init[c]);
+            P.hb_init[0] = 0;
+            for (int c = 1; c < 4; ++c) P.hb_init[c] = pack2(f16_bits(go));
+            for (int c = 0; c < 4; ++c) P.fb_init[c] = pack2(f16_bits(go));
+            P.nw_shift = (int32_t)ge;
+            for (size_t qi = 0; qi < nq; ++qi) {
+                const size_t len = R.qs[qi].len, rows = (size_t)npass[qi] * PR;
+                uint32_t* lb = R.lbv.data() + R.qd[qi].lb_off;
+                for (size_t i = 0; i < rows; ++i) {
+                    lb[2 * i] = i < len ? pack2(f16_bits(go)) : 0u;
+                    lb[2 * i + 1] = i < len ? pack2(f16_bits(2 * go)) : 0u;
+                }
+                // the rows of the query (the padding rows keep 0); the
+                // profile becomes fp16 at upload
+                int16_t* pr = R.prof.data() + R.qd[qi].qp_off;
+                for (int p = 0; p < npass[qi]; ++p)
+                    for (int d = 0; d < kDim; ++d)
+                        for (int r = 0; r < PR; ++r)
+                            if ((size_t)p * PR + r < len) pr[((size_t)p * kDim + d) * PR + r] -= (int16_t)(2 * ge);
             }
-            for (uint32_t& w : R.lbv) w = cv(w);
         }
     }
     // the query profile in VGPRs: forced, or the autotuner's choice for the
