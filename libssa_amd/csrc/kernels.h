@@ -79,8 +79,9 @@ struct ScoreParams {
     int32_t thr_b;        // SW: second flag threshold; NW: hmax >= thr_b
     uint32_t hb_init[4];  // NW: top boundary H of the first block's four columns
     uint32_t fb_init[4];  // NW: top boundary F of the first block's four columns
-    int32_t nw_shift;     // NW in fp16 (mode 4): the gap extend g; the kernel holds every value of
-                          // cell (i, j) less (i + j) g and adds (qlen + dlen) g back to the score
+    int32_t shift;        // the gap extend g (<= 0) of the shifted fp16 kernels, else 0: SW in fp16
+                          // windows (mode 3) holds the values of DB column j less j g, NW in fp16
+                          // (mode 4) those of cell (i, j) less (i + j) g
 };
 
 // result per pair: score in representation (SW: maximum cell, NW: last cell),

@@ -109,6 +109,11 @@ __device__ __forceinline__ s16x2 c_max(s16x2 a, s16x2 b) {
     else return vmax(a, b);
 }
 template <bool F>
+__device__ __forceinline__ s16x2 c_min(s16x2 a, s16x2 b) {
+    if constexpr (F) return of_h(__builtin_elementwise_minimum(as_h(a), as_h(b)));
+    else return vmin(a, b);
+}
+template <bool F>
 __device__ __forceinline__ s16x2 c_max3(s16x2 a, s16x2 b, s16x2 c) {
     if constexpr (F) return of_h(__builtin_elementwise_maximum(__builtin_elementwise_maximum(as_h(a), as_h(b)), as_h(c)));
     else return vmax(vmax(a, b), c);
@@ -284,6 +289,13 @@ __device__ __forceinline__ void run_task(const ScoreParams& P, uint32_t gbase0, 
     constexpr int LPR = Log2<PR>::value + (PAIR ? 1 : 0);  // log2 of a profile code row's int16 count
     constexpr uint32_t kProfBytes = 32 * PR * (PAIR ? 4 : 2);  // one staged profile buffer
     const s16x2 QR = as_v(P.qr), RR = as_v(P.r), Z = as_v(P.zero);
+    // mode 3 holds the values of DB column j (the pair's j-th) less j g (g =
+    // the gap extend, P.shift): E extends without an add, F as max(F, H +
+    // open) + g, the floor (real 0) rises by -g = e per column; EP = e
+    // packed, E4 = 4 e (a block), ei = e as an integer
+    const s16x2 EP = as_v(P.r ^ 0x80008000u);
+    const s16x2 E4 = F16 ? c_add<true>(c_add<true>(EP, EP), c_add<true>(EP, EP)) : Z;
+    const int ei = -P.shift;
     // per-half thresholds of the overflow predicates; a half whose predicate
     // cannot fire (flag taken, or no pair running) holds the neutral value
     const uint32_t TA_on = SW ? pack2d(P.thr_a - 1) : pack2d(P.thr_a);
@@ -369,10 +381,14 @@ __device__ __forceinline__ void run_task(const ScoreParams& P, uint32_t gbase0, 
     uint32_t scout[NB], scout_hi[NB];
 #pragma unroll
     for (int j = 0; j < NB; ++j) scout[j] = scout_hi[j] = F16 ? 0u : as_u(Z);
-    // mode 3: the window's offset (offset / 64 per half, packed int16), real
-    // 0 in the window (-offset, fp16), the running maximum folded at
-    // re-centring (real, per half), the threshold of the fire test (fp16)
-    uint32_t off = 0u, flr = 0u, TT = kF16Inf2;
+    // mode 3: the window's offset O (O / 64 per half, packed int16); the
+    // floor flr (real 0, fp16): j e - O for the next column j (so that E,
+    // which feeds column j, carries it), raised by e at each column's start;
+    // the running maximum folded at re-centring (real, per half); the
+    // threshold of the fire test TT = min(TW, TTt): TW the re-centring
+    // trigger, TTt the overflow thresholds, raised by 4 e at each block's
+    // start (they are tested against S at the block's end)
+    uint32_t off = 0u, flr = 0u, TT = kF16Inf2, TW = kF16Inf2, TTt = kF16Inf2;
     // mode 3: the offsets of the bottom rows in obh / obf (the lane's offset
     // at the end of block j of its last step: a pair starting in the next
     // block resets `off` before the lane below has read those rows)
@@ -383,19 +399,23 @@ __device__ __forceinline__ void run_task(const ScoreParams& P, uint32_t gbase0, 
     const int thr_a = P.thr_a, thr_b = P.thr_b;
     int kp_lo = 0, kp_hi = 0, pair_lo = -1, pair_hi = -1, blk_lo = 0, blk_hi = 0;
     int fa_lo = kNoFlag, fa_hi = kNoFlag, fb_lo = kNoFlag, fb_hi = kNoFlag;
-    // mode 3: the fire threshold of one half: the window's re-centring
-    // trigger, and the overflow thresholds not yet passed, less the offset
-    auto thr_half = [&](int pair, int fa, int fb, int o) {
-        int t = kWinTrigger;
-        if (fa == kNoFlag) t = min(t, thr_a - 1 - o);
-        if (fb == kNoFlag) t = min(t, thr_b - 1 - o);
+    // mode 3: the overflow thresholds of one half not yet passed, in the
+    // frame of S at the end of the block (column 4 blk + 4), less the offset
+    auto thr_half = [&](int pair, int fa, int fb, int o, int blk) {
+        int t = 100000;
+        if (fa == kNoFlag) t = min(t, thr_a - 1 - o + (4 * blk + 4) * ei);
+        if (fb == kNoFlag) t = min(t, thr_b - 1 - o + (4 * blk + 4) * ei);
         return pair >= 0 ? t : 100000;
+    };
+    auto f16_thr = [](int lo, int hi) {
+        const uint32_t v = f16_pair(min(lo, 4096), min(hi, 4096));
+        return (lo > 4096 ? 0x7c00u : (v & 0xffffu)) | (hi > 4096 ? 0x7c000000u : (v & 0xffff0000u));
     };
     auto set_tt = [&]() {
         const int olo = (int)(short)(off & 0xffffu) * 64, ohi = (int)(short)(off >> 16) * 64;
-        const int tlo = thr_half(pair_lo, fa_lo, fb_lo, olo), thi = thr_half(pair_hi, fa_hi, fb_hi, ohi);
-        const uint32_t v = f16_pair(min(tlo, 4096), min(thi, 4096));
-        TT = (tlo > 4096 ? 0x7c00u : (v & 0xffffu)) | (thi > 4096 ? 0x7c000000u : (v & 0xffff0000u));
+        TTt = f16_thr(thr_half(pair_lo, fa_lo, fb_lo, olo, blk_lo), thr_half(pair_hi, fa_hi, fb_hi, ohi, blk_hi));
+        TW = f16_thr(pair_lo >= 0 ? kWinTrigger : 100000, pair_hi >= 0 ? kWinTrigger : 100000);
+        TT = as_u(c_min<true>(as_v(TTt), as_v(TW)));
     };
 
     // profile dwords of the current column (lo, hi halves), read while the
@@ -453,6 +473,10 @@ __device__ __forceinline__ void run_task(const ScoreParams& P, uint32_t gbase0, 
             const uint4 t0 = in.t[2 * j], t1 = in.t[2 * j + 1];
             const bool valid = pass < npass && (unsigned)(b + j) < (unsigned)nbw;
 
+            if constexpr (F16) {  // the thresholds in the frame of this block's end
+                TTt = as_u(c_add<true>(as_v(TTt), E4));
+                TT = as_u(c_min<true>(as_v(TTt), as_v(TW)));
+            }
             // the wave-level tests read both halves' flags with one or; the
             // per-half flags are only taken inside the rare branches
             if (__any(((w.x | w.y) & kResStart) != 0)) {
@@ -475,16 +499,20 @@ __device__ __forceinline__ void run_task(const ScoreParams& P, uint32_t gbase0, 
                 TA = (m & TA_on) | (~m & TA);
                 TB = (m & TB_on) | (~m & TB);
                 if (SW) {
+                    // mode 3: H of column 0 and S in the frame of column 0
+                    // (real 0: 0), E of column 1 and the floor in that of
+                    // column 1 (real 0: e)
+                    const s16x2 e0 = F16 ? EP : Z;
 #pragma unroll
                     for (int r = 0; r < R; ++r) {
                         Hl[r] = sel(m, Z, Hl[r]);
-                        El[r] = sel(m, Z, El[r]);
+                        El[r] = sel(m, e0, El[r]);
                     }
                     S = sel(m, Z, S);
                     htop = sel(m, Z, htop);
                     if constexpr (F16) {  // a new pair starts in a window at offset 0
                         off &= ~m;
-                        flr &= ~m;
+                        flr = (m & as_u(EP)) | (~m & flr);
                         if (st_lo) s32_lo = 0;
                         if (st_hi) s32_hi = 0;
                         set_tt();
@@ -510,6 +538,15 @@ __device__ __forceinline__ void run_task(const ScoreParams& P, uint32_t gbase0, 
             // top boundary of this block: the lane above, one step earlier; the
             // group's top lane takes the previous pass's bottom row or the matrix
             // border.  tbh[3] also carries the next block's diagonal.
+            // mode 3: the floor (real 0) of the block's four columns, the
+            // border of the first pass's top lane and of a packed query's
+            // first row (in this lane's window)
+            uint32_t fz[4];
+            if constexpr (F16 && (FIRST || SEG)) {
+                fz[0] = flr;
+#pragma unroll
+                for (int c = 1; c < 4; ++c) fz[c] = as_u(c_add<true>(as_v(fz[c - 1]), EP));
+            }
             uint32_t toph[4], topf[4];
             if (!FIRST) {
                 toph[0] = t0.x;
@@ -523,8 +560,8 @@ __device__ __forceinline__ void run_task(const ScoreParams& P, uint32_t gbase0, 
             } else if (SW) {
 #pragma unroll
                 for (int c = 0; c < 4; ++c) {
-                    toph[c] = as_u(Z);
-                    topf[c] = as_u(Z);
+                    toph[c] = F16 ? fz[c] : as_u(Z);
+                    topf[c] = F16 ? fz[c] : as_u(Z);
                 }
             } else {
                 toph[0] = as_u(hb[1]);
@@ -542,8 +579,8 @@ __device__ __forceinline__ void run_task(const ScoreParams& P, uint32_t gbase0, 
             }
             if constexpr (F16) {
                 // the values above are in their source's window (the matrix
-                // border: offset 0): move them into this lane's
-                const uint32_t osrc = shift_down<G>(FIRST ? 0u : in.toff[j], ob_off[j], top_lane);
+                // border: already in this lane's): move them into this lane's
+                const uint32_t osrc = shift_down<G>(FIRST ? off : in.toff[j], ob_off[j], top_lane);
                 if (__any(osrc != off)) {
                     const int dlo = ((int)(short)(osrc & 0xffffu) - (int)(short)(off & 0xffffu)) * 64;
                     const int dhi = ((int)(short)(osrc >> 16) - (int)(short)(off >> 16)) * 64;
@@ -556,9 +593,9 @@ __device__ __forceinline__ void run_task(const ScoreParams& P, uint32_t gbase0, 
                 }
             }
             if (SEG) {  // a packed query's first row: the SW matrix border (real 0)
-                const uint32_t zb = F16 ? flr : as_u(Z);
 #pragma unroll
                 for (int c = 0; c < 4; ++c) {
+                    const uint32_t zb = F16 ? fz[c] : as_u(Z);
                     tbh[c] = (seg_top & zb) | (~seg_top & tbh[c]);
                     tbf[c] = (seg_top & zb) | (~seg_top & tbf[c]);
                 }
@@ -594,6 +631,10 @@ __device__ __forceinline__ void run_task(const ScoreParams& P, uint32_t gbase0, 
                 // it leaves is hidden by the other waves of the SIMD).
 #pragma unroll
                 for (int c = 0; c < 4; ++c) {
+                    if constexpr (F16) {  // into this column's frame; the floor of the next
+                        S = c_add<true>(S, EP);
+                        flr = as_u(c_add<true>(as_v(flr), EP));
+                    }
                     s16x2 f = as_v(tbf[c]);
                     s16x2 fR = c_add<FP>(f, RR);
                     s16x2 v;
@@ -637,16 +678,18 @@ __device__ __forceinline__ void run_task(const ScoreParams& P, uint32_t gbase0, 
                         if constexpr (F16) {
                             // E carries the floor at real 0 (E >= 0 makes H >=
                             // 0; an F below 0 never wins over that E, as with
-                            // saturation, and is not clamped)
+                            // saturation, and is not clamped).  Values of
+                            // column j less j g: the next column's E is
+                            // max(E, H + open) (QR is the gap open), the next
+                            // row's F is max(F, H + open) + g, and the
+                            // profile holds the score + e
                             const s16x2 h = c_max3<true>(h1, f, El[r]);
-                            const s16x2 eR = c_add<true>(El[r], RR);
                             if (more) h1 = c_add<true>(Hl[r], v);  // next row's diagonal term
                             const s16x2 tt = c_add<true>(h, QR);
                             if (r & 1) S = c_max3<true>(S, hp, h);
                             else hp = h;
-                            f = c_max<true>(fR, tt);
-                            El[r] = c_max3<true>(eR, tt, as_v(flr));
-                            if (more) fR = c_add<true>(f, RR);
+                            f = c_add<true>(c_max<true>(f, tt), RR);
+                            El[r] = c_max3<true>(El[r], tt, as_v(flr));
                             Hl[r] = h;
                         } else if constexpr (MODE == 4) {
                             // NW in fp16, no trackers, every value of cell
@@ -710,7 +753,11 @@ __device__ __forceinline__ void run_task(const ScoreParams& P, uint32_t gbase0, 
                                      (((blk_hi & 31) == 31) && (off >> 16) != 0u));
             if (F16 && __any(fire || due)) {
                 const int olo = (int)(short)(off & 0xffffu) * 64, ohi = (int)(short)(off >> 16) * 64;
-                const int slo = f16_int(as_u(S) & 0xffffu) + olo, shi = f16_int(as_u(S) >> 16) + ohi;
+                // S is in the frame of the block's last column, 4 blk + 4:
+                // real = held + O - (4 blk + 4) e
+                const int jlo = (4 * blk_lo + 4) * ei, jhi = (4 * blk_hi + 4) * ei;
+                const int wlo = f16_int(as_u(S) & 0xffffu), whi = f16_int(as_u(S) >> 16);
+                const int slo = wlo + olo - jlo, shi = whi + ohi - jhi;
                 if (pair_lo >= 0) {
                     const int v = max(s32_lo, slo);
                     if (v >= thr_a && fa_lo == kNoFlag) fa_lo = blk_lo;
@@ -721,8 +768,8 @@ __device__ __forceinline__ void run_task(const ScoreParams& P, uint32_t gbase0, 
                     if (v >= thr_a && fa_hi == kNoFlag) fa_hi = blk_hi;
                     if (v >= thr_b && fb_hi == kNoFlag) fb_hi = blk_hi;
                 }
-                const bool re_lo = slo - olo >= kWinTrigger || ((blk_lo & 31) == 31 && olo > 0);
-                const bool re_hi = shi - ohi >= kWinTrigger || ((blk_hi & 31) == 31 && ohi > 0);
+                const bool re_lo = wlo >= kWinTrigger || ((blk_lo & 31) == 31 && olo > 0);
+                const bool re_hi = whi >= kWinTrigger || ((blk_hi & 31) == 31 && ohi > 0);
                 if (__any(re_lo || re_hi)) {
                     // fold the running maximum, centre the window on the
                     // lane's largest H and move every value the lane holds
@@ -748,12 +795,14 @@ __device__ __forceinline__ void run_task(const ScoreParams& P, uint32_t gbase0, 
                         obh[c] = as_u(c_add<true>(as_v(obh[c]), d));
                         obf[c] = as_u(c_add<true>(as_v(obf[c]), d));
                     }
-                    const uint32_t nflr = f16_pair(-nlo, -nhi);
+                    // real 0 in the new window, in the frame of S and in that
+                    // of the next column (the floor)
+                    const uint32_t nzero = f16_pair(jlo - nlo, jhi - nhi);
                     // the running maximum since this re-centring (real 0 at
                     // least; the folded maximum keeps the rest)
                     const uint32_t rm = (re_lo ? 0x0000ffffu : 0u) | (re_hi ? 0xffff0000u : 0u);
-                    S = as_v((rm & nflr) | (~rm & as_u(c_add<true>(S, d))));
-                    flr = nflr;
+                    S = as_v((rm & nzero) | (~rm & as_u(c_add<true>(S, d))));
+                    flr = f16_pair(jlo + ei - nlo, jhi + ei - nhi);
                     off = ((uint32_t)(uint16_t)(nlo / 64)) | ((uint32_t)(uint16_t)(nhi / 64) << 16);
                     // the step's earlier bottom rows moved with the halves that
                     // re-centred (a half that started a pair in this block keeps
@@ -812,8 +861,9 @@ __device__ __forceinline__ void run_task(const ScoreParams& P, uint32_t gbase0, 
                     // real maxima (the folded one and the window's), handed
                     // down the group as int32 per half
                     const int olo = (int)(short)(off & 0xffffu) * 64, ohi = (int)(short)(off >> 16) * 64;
-                    const int rlo = max(s32_lo, f16_int(as_u(S) & 0xffffu) + olo);
-                    const int rhi = max(s32_hi, f16_int(as_u(S) >> 16) + ohi);
+                    // (S in the frame of the block's last column)
+                    const int rlo = max(s32_lo, f16_int(as_u(S) & 0xffffu) + olo - (4 * blk_lo + 4) * ei);
+                    const int rhi = max(s32_hi, f16_int(as_u(S) >> 16) + ohi - (4 * blk_hi + 4) * ei);
                     int inlo = (int)shift_down<G>(0u, scout[j], top_lane);
                     int inhi = (int)shift_down<G>(0u, scout_hi[j], top_lane);
                     if (SEG) {
@@ -837,8 +887,8 @@ __device__ __forceinline__ void run_task(const ScoreParams& P, uint32_t gbase0, 
                         // back from the shifted values: the score cell is
                         // (qlen, column z + 1 of the pair's block blk)
                         const int qlen = (npass - 1) * PR + last_g * R + last_r + 1;
-                        sc_lo += (qlen + 4 * blk_lo + z_lo + 1) * P.nw_shift;
-                        sc_hi += (qlen + 4 * blk_hi + z_hi + 1) * P.nw_shift;
+                        sc_lo += (qlen + 4 * blk_lo + z_lo + 1) * P.shift;
+                        sc_hi += (qlen + 4 * blk_hi + z_hi + 1) * P.shift;
                     }
                 }
                 uint32_t m = 0u;

@@ -371,7 +371,7 @@ bool f16_window_ok(int rows, int64_t qr, int64_t r) {
     const int64_t aqr = qr < 0 ? -qr : qr, ar = r < 0 ? -r : r;
     const int64_t step = smax + aqr;
     const int64_t spread = (int64_t)(rows - 1) * step + aqr;
-    const int64_t grow = 5 * step;
+    const int64_t grow = 5 * (step + ar);  // the kernel's values of column j less j gap extend
     const int64_t fall = aqr + 127 * ar;
     return 1536 + grow <= 2047 && 1024 - spread - fall >= -2047;
 }
@@ -939,7 +939,9 @@ void tier_prepare(Exec& ex, TierRun& R) {
         P.thr_b = T.repr(T.imax());
     }
     if (R.f16) {  // real scores in fp16 windows: real 0 at offset 0, thresholds as real maxima
-        P.qr = pack2(f16_bits(QRw));
+        // values of DB column j held less j ge: E and F take H + gap open
+        // (P.qr), F adds ge once more (P.r), the profile holds score - ge
+        P.qr = pack2(f16_bits(go));
         P.r = pack2(f16_bits(Rw));
         P.zero = 0;
         P.thr_a = 255;    // the 8-bit tier's overflow
@@ -948,7 +950,7 @@ void tier_prepare(Exec& ex, TierRun& R) {
     P.hb_init[0] = 0;
     for (int k = 1; k < 4; ++k) P.hb_init[k] = pack2(T.repr(T.wrap(go + k * ge)));
     for (int k = 0; k < 4; ++k) P.fb_init[k] = pack2(T.repr(T.wrap(go + (k + 1) * ge)));
-    P.nw_shift = 0;
+    P.shift = R.f16 ? (int32_t)Rw : 0;
 
     // ---- streams: cached for the whole DB, built per call for subsets ----
     if (R.sorted_all) {
@@ -1004,7 +1006,7 @@ void tier_prepare(Exec& ex, TierRun& R) {
             P.hb_init[0] = 0;
             for (int c = 1; c < 4; ++c) P.hb_init[c] = pack2(f16_bits(go));
             for (int c = 0; c < 4; ++c) P.fb_init[c] = pack2(f16_bits(go));
-            P.nw_shift = (int32_t)ge;
+            P.shift = (int32_t)ge;
             for (size_t qi = 0; qi < nq; ++qi) {
                 const size_t len = R.qs[qi].len, rows = (size_t)npass[qi] * PR;
                 uint32_t* lb = R.lbv.data() + R.qd[qi].lb_off;
@@ -1248,9 +1250,14 @@ void tier_prepare(Exec& ex, TierRun& R) {
     ex.upload(d_tasks, R.tasks.data(), R.tasks.size() * sizeof(ScoreTask));
     ex.upload(d_qd, R.qd.data(), R.qd.size() * sizeof(QueryDesc));
     // the fp16 kernels' profile: the 16-bit tier's scores as fp16 values (the
-    // padding rows' -32768 (SW) and 0 (NW) as well)
-    if (R.f16)
+    // padding rows' -32768 (SW) and 0 (NW) as well); SW's scores less the gap
+    // extend (the column shift of its values; NW's shift is in already)
+    if (R.f16 && R.type == kSW) {
+        const int16_t sh = (int16_t)-R.P.shift;
+        for (int16_t& x : R.prof) x = f16_of_int16(x == -32768 ? x : (int16_t)(x + sh));
+    } else if (R.f16) {
         for (int16_t& x : R.prof) x = f16_of_int16(x);
+    }
     ex.upload(d_prof, R.prof.data(), R.prof.size() * sizeof(int16_t));
     ex.upload(d_lb, R.lbv.data(), R.lbv.size() * sizeof(uint32_t));
     ex.upload(d_segs, R.segs.data(), R.segs.size() * sizeof(SegLane));
