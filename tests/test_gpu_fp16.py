@@ -80,6 +80,30 @@ def test_fp16_windows_every_shape(shape):
         ssa.set_kernel_variant(0, 0)
 
 
+@pytest.mark.parametrize("gaps", [(-10, -1), (-5, -3), (-11, -4)])
+def test_fp16_long_entries_column_shift(gaps):
+    """Mode 3 holds the values of DB column j less j gap extend, so over long
+    entries the held values rise by one extend per column even where the real
+    ones stay low: unrelated entries of 3,000-9,000 residues re-centre their
+    windows on that rise alone, and homologs that start after thousands of
+    such columns (or end thousands of columns before the entry does) score
+    across many re-centrings and 128-column re-centrings down; extends of 1,
+    3 and 4, in the 16- and 8-bit cascades, against the oracle."""
+    rnd = random.Random(4100 - 7 * gaps[1])
+    qraw = "".join(rnd.choice(AA) for _ in range(400))
+    db = ["".join(rnd.choice(AA) for _ in range(rnd.randint(3000, 9000))) for _ in range(24)]
+    for c in range(6):
+        a, b = rnd.randint(0, 100), rnd.randint(250, 400)
+        frag = "".join(x if rnd.random() < 0.85 else rnd.choice(AA) for x in qraw[a:b])
+        pre = "".join(rnd.choice(AA) for _ in range(rnd.randint(1500, 6000)))
+        post = "".join(rnd.choice(AA) for _ in range(rnd.randint(0, 3000) if c % 2 else 0))
+        db.insert(rnd.randrange(len(db)), pre + frag + post)
+    db += ["".join(rnd.choice(AA) for _ in range(rnd.randint(20, 300))) for _ in range(30)]
+    for bw in (16, 8):
+        seen = run_both(lambda: check(O.SW, bw, O.AMINOACID, 1, qraw, db, 12, ("builtin", "blosum62"), gaps))
+        assert seen == [2, 1]
+
+
 def test_fp16_scores_past_the_16_bit_tier():
     """A 13,000-residue query against its own copy (75,255: past 65,535, so
     the 16-bit overflow flag sends it to the exact tier), a prefix scoring just
