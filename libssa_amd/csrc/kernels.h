@@ -82,6 +82,9 @@ struct ScoreParams {
     int32_t shift;        // the gap extend g (<= 0) of the shifted fp16 kernels, else 0: SW in fp16
                           // windows (mode 3) holds the values of DB column j less j g, NW in fp16
                           // (mode 4) those of cell (i, j) less (i + j) g
+    uint32_t e4;          // mode 3: 4 (-g) as packed fp16 (a block's rise of the threshold frame)
+    uint32_t thr_start;   // mode 3: packed fp16 fire threshold of a half that starts a pair
+                          // (min(thr_a, thr_b) - 1 - 4 g, +inf past 2047)
 };
 
 // result per pair: score in representation (SW: maximum cell, NW: last cell),

@@ -130,6 +130,7 @@ __device__ __forceinline__ uint32_t f16_pair(int lo, int hi) {
     return __builtin_bit_cast(uint32_t, v);
 }
 constexpr uint32_t kF16Inf2 = 0x7c007c00u;  // +inf in both halves: a threshold that never fires
+constexpr uint32_t kF16Trigger2 = 0x66006600u;  // kWinTrigger (1,536) in both halves
 constexpr int kWinTrigger = 1536;           // re-centre once the window's maximum passes this
 constexpr int kWinCentre = 1024;            // the window's maximum after re-centring
 
@@ -294,7 +295,7 @@ __device__ __forceinline__ void run_task(const ScoreParams& P, uint32_t gbase0, 
     // open) + g, the floor (real 0) rises by -g = e per column; EP = e
     // packed, E4 = 4 e (a block), ei = e as an integer
     const s16x2 EP = as_v(P.r ^ 0x80008000u);
-    const s16x2 E4 = F16 ? c_add<true>(c_add<true>(EP, EP), c_add<true>(EP, EP)) : Z;
+    const s16x2 E4 = as_v(P.e4);
     const int ei = -P.shift;
     // per-half thresholds of the overflow predicates; a half whose predicate
     // cannot fire (flag taken, or no pair running) holds the neutral value
@@ -388,7 +389,7 @@ __device__ __forceinline__ void run_task(const ScoreParams& P, uint32_t gbase0, 
     // threshold of the fire test TT = min(TW, TTt): TW the re-centring
     // trigger, TTt the overflow thresholds, raised by 4 e at each block's
     // start (they are tested against S at the block's end)
-    uint32_t off = 0u, flr = 0u, TT = kF16Inf2, TW = kF16Inf2, TTt = kF16Inf2;
+    uint32_t off = 0u, flr = 0u, TW = kF16Inf2, TTt = kF16Inf2;
     // mode 3: the offsets of the bottom rows in obh / obf (the lane's offset
     // at the end of block j of its last step: a pair starting in the next
     // block resets `off` before the lane below has read those rows)
@@ -414,8 +415,18 @@ __device__ __forceinline__ void run_task(const ScoreParams& P, uint32_t gbase0, 
     auto set_tt = [&]() {
         const int olo = (int)(short)(off & 0xffffu) * 64, ohi = (int)(short)(off >> 16) * 64;
         TTt = f16_thr(thr_half(pair_lo, fa_lo, fb_lo, olo, blk_lo), thr_half(pair_hi, fa_hi, fb_hi, ohi, blk_hi));
-        TW = f16_thr(pair_lo >= 0 ? kWinTrigger : 100000, pair_hi >= 0 ? kWinTrigger : 100000);
-        TT = as_u(c_min<true>(as_v(TTt), as_v(TW)));
+        TW = (pair_lo >= 0 ? (kF16Trigger2 & 0xffffu) : 0x7c00u) | (pair_hi >= 0 ? (kF16Trigger2 & 0xffff0000u) : 0x7c000000u);
+    };
+    // the same for the halves in `mv` that start a pair in this block (no
+    // flag, offset 0, the block's end at column 4: one value per launch),
+    // the halves in `mi` that have no pair (the neutral value); the others
+    // keep theirs.  Pair starts and ends come nearly every block in a wave
+    // (8 streams per wave, each seen by its G lanes one step apart), so
+    // these stay selects.
+    auto set_tt_masked = [&](uint32_t mv, uint32_t mi) {
+        const uint32_t keep = ~(mv | mi);
+        TTt = (mv & P.thr_start) | (mi & kF16Inf2) | (keep & TTt);
+        TW = (mv & kF16Trigger2) | (mi & kF16Inf2) | (keep & TW);
     };
 
     // profile dwords of the current column (lo, hi halves), read while the
@@ -473,10 +484,7 @@ __device__ __forceinline__ void run_task(const ScoreParams& P, uint32_t gbase0, 
             const uint4 t0 = in.t[2 * j], t1 = in.t[2 * j + 1];
             const bool valid = pass < npass && (unsigned)(b + j) < (unsigned)nbw;
 
-            if constexpr (F16) {  // the thresholds in the frame of this block's end
-                TTt = as_u(c_add<true>(as_v(TTt), E4));
-                TT = as_u(c_min<true>(as_v(TTt), as_v(TW)));
-            }
+            if constexpr (F16) TTt = as_u(c_add<true>(as_v(TTt), E4));  // in the frame of this block's end
             // the wave-level tests read both halves' flags with one or; the
             // per-half flags are only taken inside the rare branches
             if (__any(((w.x | w.y) & kResStart) != 0)) {
@@ -515,7 +523,9 @@ __device__ __forceinline__ void run_task(const ScoreParams& P, uint32_t gbase0, 
                         flr = (m & as_u(EP)) | (~m & flr);
                         if (st_lo) s32_lo = 0;
                         if (st_hi) s32_hi = 0;
-                        set_tt();
+                        const uint32_t mv = (st_lo && pair_lo >= 0 ? 0x0000ffffu : 0u) |
+                                            (st_hi && pair_hi >= 0 ? 0xffff0000u : 0u);
+                        set_tt_masked(mv, m & ~mv);
                     }
                 } else {
                     const int r0 = g * R;
@@ -738,6 +748,7 @@ __device__ __forceinline__ void run_task(const ScoreParams& P, uint32_t gbase0, 
             // the exact per-half bookkeeping below only where a predicate fired
             bool fire;
             if (F16) {
+                const uint32_t TT = as_u(c_min<true>(as_v(TTt), as_v(TW)));
                 fire = as_u(c_max<true>(S, as_v(TT))) != TT;
             } else if (SW) {
                 fire = as_u(vmax(S, as_v(TA))) != TA || as_u(vmax(S, as_v(TB))) != TB;
@@ -904,7 +915,7 @@ __device__ __forceinline__ void run_task(const ScoreParams& P, uint32_t gbase0, 
                 }
                 TA = (m & TA_off) | (~m & TA);
                 TB = (m & TB_off) | (~m & TB);
-                if (F16) set_tt();
+                if (F16) set_tt_masked(0u, m);  // the finished halves run no pair
             }
             blk_lo++;
             blk_hi++;

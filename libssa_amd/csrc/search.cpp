@@ -951,6 +951,13 @@ void tier_prepare(Exec& ex, TierRun& R) {
     for (int k = 1; k < 4; ++k) P.hb_init[k] = pack2(T.repr(T.wrap(go + k * ge)));
     for (int k = 0; k < 4; ++k) P.fb_init[k] = pack2(T.repr(T.wrap(go + (k + 1) * ge)));
     P.shift = R.f16 ? (int32_t)Rw : 0;
+    P.e4 = 0;
+    P.thr_start = 0x7c007c00u;
+    if (R.f16) {  // the fire threshold a half starts a pair with (no flag, offset 0, column 4)
+        const int64_t t = std::min<int64_t>(P.thr_a, P.thr_b) - 1 - 4 * Rw;
+        P.e4 = pack2(f16_bits(-4 * Rw));
+        if (t <= 2047) P.thr_start = pack2(f16_bits(t));  // (above, the trigger at 1,536 fires first)
+    }
 
     // ---- streams: cached for the whole DB, built per call for subsets ----
     if (R.sorted_all) {
