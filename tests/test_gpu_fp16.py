@@ -104,6 +104,23 @@ def test_fp16_long_entries_column_shift(gaps):
         assert seen == [2, 1]
 
 
+@pytest.mark.parametrize("gaps", [(-5, 0), (0, -1), (-1, 0), (0, 0)])
+def test_fp16_shifted_values_edge_gaps(gaps):
+    """Gap extend 0 (the shifted kernels add -0 per column and hold unshifted
+    values) and gap open 0 (E and F take H itself): SW and NW, protein and
+    DNA, fp16 and int16, against the oracle."""
+    rnd = random.Random(515 + 10 * gaps[0] + gaps[1])
+    qraw = "".join(rnd.choice(AA) for _ in range(300))
+    db = homolog_db(rnd, qraw, 150, 1, 700, 8)
+    for t in (O.SW, O.NW):
+        run_both(lambda: check(t, 16, O.AMINOACID, 1, qraw, db, 15, ("builtin", "blosum62"), gaps))
+    dq = "".join(rnd.choice("ACGT") for _ in range(200))
+    ddb = ["".join(rnd.choice("ACGT") for _ in range(rnd.randint(50, 300))) for _ in range(200)]
+    ddb[17] = dq[20:180]
+    for t in (O.SW, O.NW):
+        run_both(lambda: check(t, 16, O.NUCLEOTIDE, 3, dq, ddb, 15, ("const", (2, -3)), gaps))
+
+
 def test_fp16_scores_past_the_16_bit_tier():
     """A 13,000-residue query against its own copy (75,255: past 65,535, so
     the 16-bit overflow flag sends it to the exact tier), a prefix scoring just
