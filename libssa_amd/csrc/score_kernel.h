@@ -119,9 +119,12 @@ __device__ __forceinline__ s16x2 c_max3(s16x2 a, s16x2 b, s16x2 c) {
     else return vmax(vmax(a, b), c);
 }
 // mode 3: an fp16 value (the half of a packed pair) as an integer (values
-// below -65536, -inf included, read as -65536)
+// below -65536, -inf included, read as -65536, values above 65536, +inf
+// included, as 65536: a half with no pair keeps raising its running maximum
+// by e per column and may reach +inf; its readings are never used, and the
+// clamp keeps the integer arithmetic on them in range)
 __device__ __forceinline__ int f16_int(uint32_t bits16) {
-    return (int)fmaxf((float)__builtin_bit_cast(_Float16, (uint16_t)bits16), -65536.0f);
+    return (int)fminf(fmaxf((float)__builtin_bit_cast(_Float16, (uint16_t)bits16), -65536.0f), 65536.0f);
 }
 // two integers as a packed fp16 pair (exact for the multiples of 64 and the
 // window values mode 3 converts)
